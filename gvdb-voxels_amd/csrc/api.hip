@@ -1,0 +1,1038 @@
+// api.hip -- host side of the C ABI (include/gvdbx.h): context, pools, the RebuildTopology
+// state machine and the launch sequences of the point path.
+//
+// Host orchestration restates VolumeGVDB (V.cpp = source/gvdb_library/src/gvdb_volume_gvdb.cpp):
+// the same passes, pool order and state transitions, with node allocation moved onto the
+// device (pool index = rank of the sorted unique key) instead of the reference's host loop.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "primitives.hpp"
+
+using namespace gvdbx;
+
+namespace {
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+enum MetaSlot { M_BBOX = 0, M_UNIQUE = 8, M_COUNT = 9, M_USED = 10, M_TOTAL = 11, M_LEV = 16, M_INTS = 64 };
+
+}  // namespace
+
+struct gvdbx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+
+  bool configured = false;
+  int levs = 0;
+  int ld[kMaxLev] = {};
+  int lgR[kMaxLev] = {};
+
+  Buf nodes[kMaxLev], clist[kMaxLev];
+  int64_t cnt[kMaxLev] = {};
+  int64_t used0 = 0;
+
+  int rebuild = 1, cur_depth = -1, depth = -1;
+  int root[3] = {0, 0, 0};
+  float posmin[3] = {0, 0, 0}, posmax[3] = {0, 0, 0};
+
+  int axis_cx = 16, axis_cy = 16, axis_cz = 1;
+  int apron = -1;
+  bool chan_on[32] = {};
+  Buf chan[32];
+  int64_t chan_slot_cap = 0;
+  int64_t nslots = 0;
+  Buf leaf_slot, slot_leaf, slot_pos;
+
+  const char* pos = nullptr;
+  int pos_off = 0, pos_stride = 12;
+  const char* vel = nullptr;
+  int vel_off = 0, vel_stride = 12;
+  const char* clr = nullptr;
+  int clr_off = 0, clr_stride = 4;
+  int64_t npts = 0;
+
+  int sc = 0, scdim = 0;
+  int64_t nsc = 0, sclen = 0;
+  bool lists_have_vel = false;
+  Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel;
+
+  Buf keys, keys_alt, flags, rank, radix_counts, tile_buf, markers, meta;
+  int* h_meta = nullptr;
+
+  int64_t topo_version = 0, atlas_version = -1;
+
+  // named-kernel timer: event pairs recorded on the stream, summed when read
+  std::string timer_name;
+  bool timer_on = false;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+int fail(gvdbx_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail((c), GVDBX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(expr)            \
+  do {                       \
+    int s_ = (expr);         \
+    if (s_ != GVDBX_OK) return s_; \
+  } while (0)
+
+// Grow-only buffer; contents are not kept.
+int ensure(gvdbx_ctx* c, Buf& b, size_t bytes) {
+  if (bytes <= b.cap) return GVDBX_OK;
+  size_t nb = b.cap ? b.cap : 256;
+  while (nb < bytes) nb += nb / 2 + 256;
+  if (b.p) HIPCHK(c, hipFree(b.p));
+  b.p = nullptr;
+  b.cap = 0;
+  hipError_t e = hipMalloc(&b.p, nb);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, GVDBX_ERR_NOMEM, "hipMalloc(%zu) failed: %s", nb, hipGetErrorString(e));
+  }
+  b.cap = nb;
+  return GVDBX_OK;
+}
+
+// Grow-only buffer keeping the first `keep` bytes (pool growth, A.cpp:168-187).
+int ensure_keep(gvdbx_ctx* c, Buf& b, size_t bytes, size_t keep) {
+  if (bytes <= b.cap) return GVDBX_OK;
+  size_t nb = b.cap ? b.cap : 256;
+  while (nb < bytes) nb *= 2;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, nb);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, GVDBX_ERR_NOMEM, "hipMalloc(%zu) failed: %s", nb, hipGetErrorString(e));
+  }
+  if (keep && b.p) HIPCHK(c, hipMemcpyAsync(p, b.p, keep, hipMemcpyDeviceToDevice, c->stream));
+  if (b.p) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(b.p));
+  }
+  b.p = p;
+  b.cap = nb;
+  return GVDBX_OK;
+}
+
+template <typename T>
+T* P(Buf& b) {
+  return (T*)b.p;
+}
+
+int res_of(const gvdbx_ctx* c, int l) { return 1 << c->ld[l]; }
+int64_t clist_per(const gvdbx_ctx* c, int l) { return (int64_t)1 << (3 * c->ld[l]); }
+
+DevTree tree_of(gvdbx_ctx* c) {
+  DevTree T;
+  memset(&T, 0, sizeof(T));
+  T.depth = c->depth;
+  T.levs = c->levs;
+  for (int l = 0; l < kMaxLev; ++l) {
+    T.ld[l] = c->ld[l];
+    T.lgR[l] = c->lgR[l];
+    T.nodes[l] = P<gvdbx_node>(c->nodes[l]);
+    T.clist[l] = P<uint64_t>(c->clist[l]);
+  }
+  T.root[0] = c->root[0];
+  T.root[1] = c->root[1];
+  T.root[2] = c->root[2];
+  return T;
+}
+
+PointsView points_of(gvdbx_ctx* c, int64_t n, const float* tr) {
+  PointsView v;
+  v.pos = c->pos;
+  v.pos_off = c->pos_off;
+  v.pos_stride = c->pos_stride;
+  v.vel = c->vel;
+  v.vel_off = c->vel_off;
+  v.vel_stride = c->vel_stride;
+  v.n = n;
+  v.tx = tr ? tr[0] : 0.f;
+  v.ty = tr ? tr[1] : 0.f;
+  v.tz = tr ? tr[2] : 0.f;
+  return v;
+}
+
+int read_meta(gvdbx_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->h_meta, c->meta.p, M_INTS * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GVDBX_OK;
+}
+
+int zero_meta(gvdbx_ctx* c) {
+  fill_u32(P<uint32_t>(c->meta), M_INTS, 0u, c->stream);
+  return GVDBX_OK;
+}
+
+// Named-kernel timer: brackets the launches of one kernel family with a pair of HIP events
+// on the context stream (no host synchronisation until gvdbx_kernel_timer_read).
+hipEvent_t next_event(gvdbx_ctx* c) {
+  if (c->ev_used == c->ev.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev.push_back(e);
+  }
+  return c->ev[c->ev_used++];
+}
+
+struct TimedRegion {
+  gvdbx_ctx* c;
+  bool on;
+  TimedRegion(gvdbx_ctx* ctx, const char* name) : c(ctx), on(ctx->timer_on && ctx->timer_name == name) {
+    if (on) {
+      hipEvent_t e = next_event(c);
+      if (e) (void)hipEventRecord(e, c->stream);
+      else on = false;
+    }
+  }
+  ~TimedRegion() {
+    if (!on) return;
+    hipEvent_t e = next_event(c);
+    if (e) (void)hipEventRecord(e, c->stream);
+  }
+};
+
+// ---------------------------------------------------------------- topology
+
+void clear_tree(gvdbx_ctx* c) {  // Clear (V.cpp:2487-2503)
+  for (int l = 0; l < c->levs; ++l) c->cnt[l] = 0;
+  c->used0 = 0;
+  c->rebuild = 1;
+}
+
+// Capacity for `add` more nodes at level l (nodes + child lists keep their contents).
+int reserve_level(gvdbx_ctx* c, int l, int64_t add) {
+  const int64_t need = c->cnt[l] + add;
+  CHK(ensure_keep(c, c->nodes[l], (size_t)need * sizeof(gvdbx_node), (size_t)c->cnt[l] * sizeof(gvdbx_node)));
+  if (l > 0)
+    CHK(ensure_keep(c, c->clist[l], (size_t)(need * clist_per(c, l)) * 8, (size_t)(c->cnt[l] * clist_per(c, l)) * 8));
+  return GVDBX_OK;
+}
+
+int bounding_box(gvdbx_ctx* c, int64_t n, const float* tr) {  // GetBoundingBox (V.cpp:5404-5467)
+  bbox_init(P<int>(c->meta), c->stream);
+  PointsView pv = points_of(c, n, tr);
+  bbox(pv, P<int>(c->meta), c->stream);
+  CHK(read_meta(c));
+  for (int a = 0; a < 3; ++a) {
+    float mn = ordered_to_float(c->h_meta[M_BBOX + a]);
+    float mx = ordered_to_float(c->h_meta[M_BBOX + 3 + a]);
+    c->posmin[a] = mn - 1.0f;
+    c->posmax[a] = mx + 2.0f;
+    if (0 > c->posmin[a]) c->posmin[a] = 0;
+    if (0 > c->posmax[a]) c->posmax[a] = 0;
+  }
+  return GVDBX_OK;
+}
+
+int determine_depth(gvdbx_ctx* c, int* root) {  // DetermineDepth (V.cpp:821-837)
+  for (int lev = 0; lev < c->levs; ++lev) {
+    const int64_t R = (int64_t)1 << c->lgR[lev];
+    bool same = true;
+    int c0[3];
+    for (int a = 0; a < 3; ++a) {
+      int64_t pm = (int32_t)c->posmin[a], px = (int32_t)c->posmax[a];
+      int64_t cm = pm / R * R, cx = px / R * R;
+      if (pm < cm) cm -= R;
+      if (px < cx) cx -= R;
+      c0[a] = (int)cm;
+      if (cm != cx) same = false;
+    }
+    if (same) {
+      root[0] = c0[0];
+      root[1] = c0[1];
+      root[2] = c0[2];
+      return lev;
+    }
+  }
+  return -1;
+}
+
+KeyFmt key_fmt(const gvdbx_ctx* c, int d) {
+  KeyFmt f;
+  f.b = c->lgR[d] - c->lgR[0];
+  int lb = 0;
+  while ((1 << lb) <= d) ++lb;
+  f.nbits = 3 * f.b + lb;
+  return f;
+}
+
+// Sort m compact keys in c->keys, unique them, and append the nodes per level in sorted
+// order (FindUniqueBrick V.cpp:843-873 + host allocation V.cpp:1308-1331 / 975-997).
+template <typename K>
+int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d) {
+  if (m <= 0) return GVDBX_OK;
+  if (m > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "%lld topology keys exceed the 32-bit sort index", (long long)m);
+  const int64_t ncnt = radix_count_ints(m);
+  CHK(ensure(c, c->radix_counts, (size_t)ncnt * 4));
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(ncnt > m ? ncnt : m) + 16) * 4));
+  CHK(ensure(c, c->flags, (size_t)m * 4));
+  CHK(ensure(c, c->rank, (size_t)m * 4));
+  int par;
+  {
+    TimedRegion tr(c, "sort");
+    if (sizeof(K) == 4)
+      par = radix_sort_u32(P<uint32_t>(c->keys), P<uint32_t>(c->keys_alt), nullptr, nullptr, m, 0, f.nbits,
+                           P<int>(c->radix_counts), P<int>(c->tile_buf), c->stream);
+    else
+      par = radix_sort_u64(P<uint64_t>(c->keys), P<uint64_t>(c->keys_alt), nullptr, nullptr, m, 0, f.nbits,
+                           P<int>(c->radix_counts), P<int>(c->tile_buf), c->stream);
+  }
+  const K* sorted = par ? P<K>(c->keys_alt) : P<K>(c->keys);
+  CHK(zero_meta(c));
+  unique_flags<K>(sorted, m, f, d, P<int>(c->flags), c->stream);
+  exclusive_scan(P<int>(c->flags), P<int>(c->rank), m, P<int>(c->tile_buf), P<int>(c->meta) + M_UNIQUE, c->stream);
+  level_hist<K>(sorted, P<int>(c->flags), m, f, P<int>(c->meta) + M_LEV, c->stream);
+  CHK(read_meta(c));
+  AllocPlan plan;
+  memset(&plan, 0, sizeof(plan));
+  int64_t acc = 0;
+  int64_t levcnt[kMaxLev] = {};
+  for (int l = 0; l < d; ++l) {
+    levcnt[l] = c->h_meta[M_LEV + l];
+    plan.levstart[l] = acc;
+    plan.base[l] = c->cnt[l];
+    acc += levcnt[l];
+    CHK(reserve_level(c, l, levcnt[l]));
+  }
+  DevTree T = tree_of(c);
+  setup_nodes<K>(T, sorted, P<int>(c->flags), P<int>(c->rank), m, f, plan, c->stream);
+  for (int l = 1; l < d; ++l)
+    if (levcnt[l])
+      fill_u64(P<uint64_t>(c->clist[l]) + c->cnt[l] * clist_per(c, l), levcnt[l] * clist_per(c, l), kUndef64,
+               c->stream);
+  for (int l = 0; l < d; ++l) c->cnt[l] += levcnt[l];
+  c->used0 += levcnt[0];
+  return GVDBX_OK;
+}
+
+void link_all(gvdbx_ctx* c, int d) {  // per-level top-down gvdbLinkBricks launches
+  DevTree T = tree_of(c);
+  for (int l = d - 1; l >= 0; --l) link_level(T, l, c->cnt[l], c->stream);
+}
+
+template <typename K>
+int full_pass(gvdbx_ctx* c, int64_t n, const float* tr, int d) {  // ActivateBricksGPU (V.cpp:1229-1413)
+  KeyFmt f = key_fmt(c, d);
+  const int64_t m = n * d;
+  CHK(ensure(c, c->keys, (size_t)m * sizeof(K)));
+  CHK(ensure(c, c->keys_alt, (size_t)m * sizeof(K)));
+  CHK(reserve_level(c, d, 1));
+  c->cnt[d] = 1;
+  DevTree T = tree_of(c);
+  setup_root(T, c->stream);
+  fill_u64(P<uint64_t>(c->clist[d]), clist_per(c, d), kUndef64, c->stream);
+  {
+    TimedRegion trg(c, "keys");
+    brick_keys<K>(T, points_of(c, n, tr), f, P<K>(c->keys), c->stream);
+  }
+  CHK(sort_unique_alloc<K>(c, m, f, d));
+  link_all(c, d);
+  return GVDBX_OK;
+}
+
+// Box pass shared by the extra (hi = 2r) and incremental (hi = r) passes; returns the
+// number of emitted keys in *m (count-then-fill on overflow).
+template <typename K>
+int box_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, float hi_mul, int d, int* markers, int64_t* m_out) {
+  KeyFmt f = key_fmt(c, d);
+  int64_t cap = c->keys.cap / sizeof(K);
+  if (cap < (int64_t)n * d) {
+    CHK(ensure(c, c->keys, (size_t)n * d * sizeof(K)));
+    cap = c->keys.cap / sizeof(K);
+  }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    CHK(zero_meta(c));
+    {
+      TimedRegion trg(c, "box");
+      box_keys<K>(tree_of(c), points_of(c, n, tr), f, r, hi_mul, P<K>(c->keys), cap, P<int>(c->meta) + M_COUNT, markers,
+                  c->stream);
+    }
+    CHK(read_meta(c));
+    const int64_t m = (uint32_t)c->h_meta[M_COUNT];
+    if (m <= cap) {
+      *m_out = m;
+      CHK(ensure(c, c->keys_alt, (size_t)(m > 0 ? m : 1) * sizeof(K)));
+      return GVDBX_OK;
+    }
+    CHK(ensure(c, c->keys, (size_t)m * sizeof(K)));
+    cap = c->keys.cap / sizeof(K);
+  }
+  return fail(c, GVDBX_ERR_CAPACITY, "extra-brick key emission did not converge");
+}
+
+template <typename K>
+int extra_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d) {  // ActivateExtraBricksGPU (V.cpp:931-1016)
+  int64_t m = 0;
+  CHK(box_pass<K>(c, n, tr, r, 2.0f, d, nullptr, &m));
+  if (m > 0) {
+    CHK(sort_unique_alloc<K>(c, m, key_fmt(c, d), d));
+    link_all(c, d);
+  }
+  return GVDBX_OK;
+}
+
+template <typename K>
+int incre_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool accum) {  // V.cpp:1020-1156
+  const int64_t total0 = c->cnt[0];
+  if (total0 == 0) return GVDBX_OK;
+  CHK(ensure(c, c->markers, (size_t)total0 * 4));
+  DevTree T = tree_of(c);
+  if (accum)
+    copy_flags_to_markers(T, total0, P<int>(c->markers), c->stream);
+  else
+    fill_u32(P<uint32_t>(c->markers), total0, 0u, c->stream);
+  int64_t m = 0;
+  CHK(box_pass<K>(c, n, tr, r, 1.0f, d, P<int>(c->markers), &m));
+  if (m == 0) return GVDBX_OK;  // no new node: topology unchanged (V.cpp:1066)
+  apply_markers(tree_of(c), P<int>(c->markers), total0, c->stream);
+  CHK(sort_unique_alloc<K>(c, m, key_fmt(c, d), d));
+  link_all(c, d);
+  T = tree_of(c);
+  delink_leaves(T, c->cnt[0], c->stream);
+  for (int l = 1; l < d; ++l) delink_level(T, l, c->cnt[l], c->stream);
+  // used count (V.cpp:1144-1147)
+  CHK(ensure(c, c->flags, (size_t)c->cnt[0] * 4));
+  leaf_flags(T, c->cnt[0], 0, P<int>(c->flags), c->stream);
+  CHK(zero_meta(c));
+  count_flags(P<int>(c->flags), c->cnt[0], P<int>(c->meta) + M_USED, c->stream);
+  CHK(read_meta(c));
+  c->used0 = c->h_meta[M_USED];
+  return GVDBX_OK;
+}
+
+int check_points(gvdbx_ctx* c, int64_t n) {
+  if (!c || !c->configured) return fail(c, GVDBX_ERR_ARG, "context not configured");
+  if (!c->pos) return fail(c, GVDBX_ERR_STATE, "no points bound (gvdbx_set_points)");
+  if (n <= 0 || n > c->npts) return fail(c, GVDBX_ERR_ARG, "point count %lld outside bound buffer (%lld)",
+                                         (long long)n, (long long)c->npts);
+  if (n > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "point count exceeds 2^31");
+  return GVDBX_OK;
+}
+
+bool wide_keys(const gvdbx_ctx* c, int d) { return key_fmt(c, d).nbits > 32; }
+
+int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accumulate, int min_depth) {
+  CHK(check_points(c, n));
+  c->topo_version++;
+  c->sc = 0;
+  CHK(bounding_box(c, n, orig));
+  int root[3];
+  int depth = determine_depth(c, root);
+  if (accumulate && depth >= 0 && depth < min_depth) depth = min_depth;
+  if (depth == 0) return fail(c, GVDBX_ERR_DEPTH0, "Points extents smaller than 1 brick");
+  if (depth < 0 || depth > 5 || depth >= c->levs)
+    return fail(c, GVDBX_ERR_DEPTH_MAX, "Point extents exceed the configured levels (depth %d)", depth);
+  if (accumulate && !c->rebuild) {  // AccumulateTopology (V.cpp:1158-1178)
+    return wide_keys(c, c->depth) ? incre_pass<uint64_t>(c, n, orig, r, c->depth, true)
+                                  : incre_pass<uint32_t>(c, n, orig, r, c->depth, true);
+  }
+  if (c->rebuild) {
+    clear_tree(c);
+  } else if (depth != c->cur_depth) {  // V.cpp:1198-1203
+    clear_tree(c);
+    c->rebuild = 1;
+    c->cur_depth = depth;
+  }
+  if (c->rebuild) {
+    c->depth = depth;
+    for (int a = 0; a < 3; ++a) c->root[a] = root[a];
+    if (wide_keys(c, depth)) {
+      CHK(full_pass<uint64_t>(c, n, orig, depth));
+      if (r >= 1) CHK(extra_pass<uint64_t>(c, n, orig, r, depth));
+    } else {
+      CHK(full_pass<uint32_t>(c, n, orig, depth));
+      if (r >= 1) CHK(extra_pass<uint32_t>(c, n, orig, r, depth));
+    }
+    c->rebuild = 0;
+    return GVDBX_OK;
+  }
+  return wide_keys(c, c->depth) ? incre_pass<uint64_t>(c, n, orig, r, c->depth, false)
+                                : incre_pass<uint32_t>(c, n, orig, r, c->depth, false);
+}
+
+int chan_ok(gvdbx_ctx* c, int chan) {
+  if (!c) return GVDBX_ERR_ARG;
+  if (chan < 0 || chan >= 32 || !c->chan_on[chan]) return fail(c, GVDBX_ERR_ARG, "channel %d does not exist", chan);
+  return GVDBX_OK;
+}
+
+int64_t brick_floats(const gvdbx_ctx* c) {
+  const int64_t br = res_of(c, 0) + 2 * c->apron;
+  return br * br * br;
+}
+
+// Channel storage for nslots bricks (AtlasResize keeps existing bricks, A.cpp:622-650).
+int reserve_channels(gvdbx_ctx* c, int64_t slots) {
+  if (slots <= c->chan_slot_cap) return GVDBX_OK;
+  int64_t ns = c->chan_slot_cap ? c->chan_slot_cap : 64;
+  while (ns < slots) ns *= 2;
+  for (int ch = 0; ch < 32; ++ch) {
+    if (!c->chan_on[ch]) continue;
+    const size_t old = (size_t)(c->chan_slot_cap * brick_floats(c)) * 4;
+    const size_t nb = (size_t)(ns * brick_floats(c)) * 4;
+    CHK(ensure_keep(c, c->chan[ch], nb, old));
+    fill_f32(P<float>(c->chan[ch]) + c->chan_slot_cap * brick_floats(c), (ns - c->chan_slot_cap) * brick_floats(c),
+             0.f, c->stream);
+  }
+  c->chan_slot_cap = ns;
+  return GVDBX_OK;
+}
+
+SubcellCfg subcell_cfg(const gvdbx_ctx* c, int sc, float r) {
+  SubcellCfg s;
+  s.sc = sc;
+  s.lgsc = 0;
+  while ((1 << s.lgsc) < sc) ++s.lgsc;
+  s.res0 = res_of(c, 0);
+  s.D = s.res0 / sc;
+  s.r = r;
+  return s;
+}
+
+int atlas_current(gvdbx_ctx* c) {
+  if (c->atlas_version != c->topo_version)
+    return fail(c, GVDBX_ERR_STATE, "UpdateAtlas must run after the topology changed");
+  return GVDBX_OK;
+}
+
+int gather_checks(gvdbx_ctx* c, int chan) {
+  CHK(chan_ok(c, chan));
+  if (c->sc <= 0) return fail(c, GVDBX_ERR_STATE, "InsertPointsSubcell must run before a gather");
+  CHK(atlas_current(c));
+  return GVDBX_OK;
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+
+extern "C" {
+
+const char* gvdbx_status_string(int s) {
+  switch (s) {
+    case GVDBX_OK: return "ok";
+    case GVDBX_ERR_ARG: return "invalid argument";
+    case GVDBX_ERR_DEPTH0: return "points extents smaller than 1 brick";
+    case GVDBX_ERR_DEPTH_MAX: return "point extents exceed the configured levels";
+    case GVDBX_ERR_CAPACITY: return "capacity exceeded";
+    case GVDBX_ERR_HIP: return "HIP runtime error";
+    case GVDBX_ERR_STATE: return "call order violated";
+    case GVDBX_ERR_NOMEM: return "out of device memory";
+    case GVDBX_ERR_DEGENERATE: return "degenerate single-brick topology";
+    default: return "unknown status";
+  }
+}
+
+int gvdbx_ctx_create(int device, void* hip_stream, gvdbx_ctx** out) {
+  if (!out) return GVDBX_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    (void)hipGetLastError();
+    return GVDBX_ERR_HIP;
+  }
+  if (hipSetDevice(device) != hipSuccess) return GVDBX_ERR_HIP;
+  gvdbx_ctx* c = new gvdbx_ctx();
+  c->device = device;
+  if (hip_stream) {
+    c->stream = (hipStream_t)hip_stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return GVDBX_ERR_HIP;
+    }
+    c->own_stream = true;
+  }
+  if (ensure(c, c->meta, M_INTS * sizeof(int)) != GVDBX_OK ||
+      hipHostMalloc((void**)&c->h_meta, M_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    gvdbx_ctx_destroy(c);
+    return GVDBX_ERR_HIP;
+  }
+  *out = c;
+  return GVDBX_OK;
+}
+
+int gvdbx_ctx_destroy(gvdbx_ctx* c) {
+  if (!c) return GVDBX_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  Buf* bufs[] = {&c->leaf_slot, &c->slot_leaf, &c->slot_pos, &c->sc_map, &c->sc_cnt, &c->sc_off, &c->sc_pos,
+                 &c->sc_nid, &c->list_pos, &c->list_vel, &c->keys, &c->keys_alt, &c->flags, &c->rank,
+                 &c->radix_counts, &c->tile_buf, &c->markers, &c->meta};
+  for (Buf* b : bufs)
+    if (b->p) (void)hipFree(b->p);
+  for (int l = 0; l < kMaxLev; ++l) {
+    if (c->nodes[l].p) (void)hipFree(c->nodes[l].p);
+    if (c->clist[l].p) (void)hipFree(c->clist[l].p);
+  }
+  for (int ch = 0; ch < 32; ++ch)
+    if (c->chan[ch].p) (void)hipFree(c->chan[ch].p);
+  if (c->h_meta) (void)hipHostFree(c->h_meta);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GVDBX_OK;
+}
+
+int gvdbx_last_error(const gvdbx_ctx* c, char* buf, int buflen) {
+  if (!c || !buf || buflen <= 0) return GVDBX_ERR_ARG;
+  snprintf(buf, (size_t)buflen, "%s", c->err.c_str());
+  return GVDBX_OK;
+}
+
+int gvdbx_synchronize(gvdbx_ctx* c) {
+  if (!c) return GVDBX_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GVDBX_OK;
+}
+
+int gvdbx_get_stream(const gvdbx_ctx* c, void** s) {
+  if (!c || !s) return GVDBX_ERR_ARG;
+  *s = (void*)c->stream;
+  return GVDBX_OK;
+}
+
+int gvdbx_configure(gvdbx_ctx* c, int levs, const int* log2dims) {  // Configure (V.cpp:2364-2434)
+  if (!c || !log2dims || levs < 2 || levs > kMaxLev) return fail(c, GVDBX_ERR_ARG, "bad level configuration");
+  int sum = 0;
+  for (int l = 0; l < levs; ++l) {
+    int v = log2dims[l] == 0 ? 1 : log2dims[l];
+    if (v < 1 || v > 8) return fail(c, GVDBX_ERR_ARG, "log2 dim %d out of range at level %d", v, l);
+    c->ld[l] = v;
+    sum += v;
+    c->lgR[l] = sum;
+  }
+  if (sum > 30) return fail(c, GVDBX_ERR_ARG, "total tree range 2^%d exceeds 32-bit coordinates", sum);
+  for (int l = levs; l < kMaxLev; ++l) {
+    c->ld[l] = 0;
+    c->lgR[l] = sum;
+  }
+  c->levs = levs;
+  c->configured = true;
+  clear_tree(c);
+  c->depth = -1;
+  c->cur_depth = -1;
+  c->nslots = 0;
+  c->sc = 0;
+  return GVDBX_OK;
+}
+
+int gvdbx_set_channel_default(gvdbx_ctx* c, int cx, int cy, int cz) {  // SetChannelDefault (V.h:390)
+  if (!c || cx <= 0 || cy <= 0 || cz <= 0) return fail(c, GVDBX_ERR_ARG, "bad channel default");
+  c->axis_cx = cx;
+  c->axis_cy = cy;
+  c->axis_cz = cz;
+  return GVDBX_OK;
+}
+
+int gvdbx_add_channel(gvdbx_ctx* c, int chan, int dtype, int apron) {  // AddChannel (V.cpp:2437-2453)
+  if (!c || !c->configured) return fail(c, GVDBX_ERR_ARG, "context not configured");
+  if (chan < 0 || chan >= 32 || dtype != GVDBX_T_FLOAT || apron < 0 || apron > res_of(c, 0))
+    return fail(c, GVDBX_ERR_ARG, "unsupported channel (chan %d dtype %d apron %d)", chan, dtype, apron);
+  if (c->apron >= 0 && apron != c->apron)
+    return fail(c, GVDBX_ERR_ARG, "all channels share one apron width (%d)", c->apron);
+  c->apron = apron;
+  if (!c->chan_on[chan]) {
+    c->chan_on[chan] = true;
+    if (c->chan_slot_cap > 0) {
+      const size_t nb = (size_t)(c->chan_slot_cap * brick_floats(c)) * 4;
+      CHK(ensure(c, c->chan[chan], nb));
+      fill_f32(P<float>(c->chan[chan]), c->chan_slot_cap * brick_floats(c), 0.f, c->stream);
+    }
+  }
+  return GVDBX_OK;
+}
+
+int gvdbx_fill_channel(gvdbx_ctx* c, int chan, float v) {  // FillChannel / ClearChannel
+  CHK(chan_ok(c, chan));
+  if (c->chan_slot_cap > 0) fill_f32(P<float>(c->chan[chan]), c->chan_slot_cap * brick_floats(c), v, c->stream);
+  return GVDBX_OK;
+}
+
+int gvdbx_set_points(gvdbx_ctx* c, int64_t n, const void* pos, int pos_off, int pos_stride, const void* vel,
+                     int vel_off, int vel_stride, const void* clr, int clr_off, int clr_stride) {
+  if (!c || !pos || n <= 0 || pos_stride < 12 || pos_off < 0 || (pos_off & 3) || (pos_stride & 3))
+    return fail(c, GVDBX_ERR_ARG, "bad position buffer");
+  if (vel && (vel_stride < 12 || vel_off < 0 || (vel_off & 3) || (vel_stride & 3)))
+    return fail(c, GVDBX_ERR_ARG, "bad velocity buffer");
+  c->npts = n;
+  c->pos = (const char*)pos;
+  c->pos_off = pos_off;
+  c->pos_stride = pos_stride;
+  c->vel = (const char*)vel;
+  c->vel_off = vel_off;
+  c->vel_stride = vel_stride;
+  c->clr = (const char*)clr;
+  c->clr_off = clr_off;
+  c->clr_stride = clr_stride;
+  return GVDBX_OK;
+}
+
+int gvdbx_rebuild_topology(gvdbx_ctx* c, int64_t n, float r, const float orig[3]) {
+  if (!c) return GVDBX_ERR_ARG;
+  return rebuild_impl(c, n, r, orig, false, 0);
+}
+
+int gvdbx_accumulate_topology(gvdbx_ctx* c, int64_t n, float r, const float orig[3], int min_depth) {
+  if (!c) return GVDBX_ERR_ARG;
+  return rebuild_impl(c, n, r, orig, true, min_depth);
+}
+
+int gvdbx_request_full_rebuild(gvdbx_ctx* c, int flag) {
+  if (!c) return GVDBX_ERR_ARG;
+  c->rebuild = flag ? 1 : 0;
+  return GVDBX_OK;
+}
+
+int gvdbx_finish_topology(gvdbx_ctx* c, float bmin[3], float bmax[3]) {  // ComputeBounds (V.cpp:1792-1816)
+  if (!c || !c->configured) return GVDBX_ERR_ARG;
+  const int64_t n = c->cnt[0];
+  if (n == 0) return fail(c, GVDBX_ERR_STATE, "no leaves");
+  gvdbx_node* h = (gvdbx_node*)malloc((size_t)n * sizeof(gvdbx_node));
+  if (!h) return GVDBX_ERR_NOMEM;
+  hipError_t e = hipMemcpyAsync(h, c->nodes[0].p, (size_t)n * sizeof(gvdbx_node), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    free(h);
+    return fail(c, GVDBX_ERR_HIP, "%s", hipGetErrorString(e));
+  }
+  const int R = res_of(c, 0);
+  float mn[3] = {(float)h[0].pos[0], (float)h[0].pos[1], (float)h[0].pos[2]};
+  float mx[3] = {mn[0], mn[1], mn[2]};
+  for (int64_t i = 0; i < n; ++i) {
+    if (!h[i].flags) continue;
+    for (int a = 0; a < 3; ++a) {
+      if (h[i].pos[a] < mn[a]) mn[a] = (float)h[i].pos[a];
+      if (h[i].pos[a] + R > mx[a]) mx[a] = (float)(h[i].pos[a] + R);
+    }
+  }
+  free(h);
+  for (int a = 0; a < 3; ++a) {
+    if (bmin) bmin[a] = mn[a];
+    if (bmax) bmax[a] = mx[a];
+  }
+  return GVDBX_OK;
+}
+
+int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
+  if (!c || !c->configured) return GVDBX_ERR_ARG;
+  if (c->apron < 0) return fail(c, GVDBX_ERR_STATE, "UpdateAtlas needs channel 0 (AddChannel)");
+  const int64_t total = c->cnt[0];
+  CHK(ensure(c, c->flags, (size_t)(total + 1) * 4));
+  CHK(ensure(c, c->rank, (size_t)(total + 1) * 4));
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(total + 1) + 16) * 4));
+  CHK(ensure(c, c->leaf_slot, (size_t)(total + 1) * 4));
+  CHK(ensure(c, c->slot_leaf, (size_t)(total + 1) * 4));
+  CHK(ensure(c, c->slot_pos, (size_t)(total + 1) * 16));
+  DevTree T = tree_of(c);
+  leaf_flags(T, total, 0, P<int>(c->flags), c->stream);
+  exclusive_scan(P<int>(c->flags), P<int>(c->rank), total, P<int>(c->tile_buf), nullptr, c->stream);
+  const int br = res_of(c, 0) + 2 * c->apron;
+  assign_slots(T, P<int>(c->flags), P<int>(c->rank), total, c->apron, br, c->axis_cx, c->axis_cy, P<int>(c->leaf_slot),
+               P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
+  c->nslots = c->used0;
+  CHK(reserve_channels(c, c->nslots));
+  c->atlas_version = c->topo_version;
+  return GVDBX_OK;
+}
+
+int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t* len) {
+  CHK(check_points(c, n));  // InsertPointsSubcell (V.cpp:4971-5092)
+  const int res0 = res_of(c, 0);
+  if (sc <= 0 || (sc & (sc - 1)) || res0 % sc || sc * sc * sc > 512)
+    return fail(c, GVDBX_ERR_ARG, "subcell size %d must be a power of two dividing %d", sc, res0);
+  if (len) *len = 0;
+  if (c->used0 == 0) return GVDBX_OK;
+  if (c->depth < 1) return fail(c, GVDBX_ERR_DEGENERATE, "no tree");
+  const SubcellCfg cfg = subcell_cfg(c, sc, r);
+  const int64_t per = (int64_t)cfg.D * cfg.D * cfg.D;
+  const int64_t total = c->cnt[0];
+  const int64_t nsc = per * c->used0;
+  if (nsc > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "subcell count exceeds 2^31");
+  CHK(ensure(c, c->sc_map, (size_t)(total + 1) * 4));
+  CHK(ensure(c, c->flags, (size_t)(total + 1) * 4));
+  // arrays sized for every pool leaf (an upper bound of the linked leaves indexed by map)
+  const int64_t nsc_cap = per * total;
+  CHK(ensure(c, c->sc_cnt, (size_t)(nsc_cap + 1) * 4));
+  CHK(ensure(c, c->sc_off, (size_t)(nsc_cap + 1) * 4));
+  CHK(ensure(c, c->sc_pos, (size_t)(nsc_cap + 1) * 12));
+  CHK(ensure(c, c->sc_nid, (size_t)(nsc_cap + 1) * 4));
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count((nsc > total ? nsc : total) + 1) + 16) * 4));
+  DevTree T = tree_of(c);
+  PointsView pv = points_of(c, n, trans);
+  leaf_flags(T, total, 1, P<int>(c->flags), c->stream);
+  exclusive_scan(P<int>(c->flags), P<int>(c->sc_map), total, P<int>(c->tile_buf), nullptr, c->stream);
+  fill_u32(P<uint32_t>(c->sc_cnt), nsc_cap, 0u, c->stream);
+  {
+    TimedRegion tr(c, "count");
+    sc_count(T, pv, cfg, P<int>(c->sc_map), P<int>(c->sc_cnt), c->stream);
+  }
+  CHK(zero_meta(c));
+  exclusive_scan(P<int>(c->sc_cnt), P<int>(c->sc_off), nsc, P<int>(c->tile_buf), P<int>(c->meta) + M_TOTAL,
+                 c->stream);
+  CHK(read_meta(c));
+  const int64_t ln = (uint32_t)c->h_meta[M_TOTAL];
+  c->sc = sc;
+  c->scdim = cfg.D;
+  c->nsc = nsc;
+  c->sclen = ln;
+  if (len) *len = ln;
+  c->lists_have_vel = c->vel != nullptr;
+  CHK(ensure(c, c->list_pos, (size_t)(ln + 1) * 12));
+  if (c->vel) CHK(ensure(c, c->list_vel, (size_t)(ln + 1) * 12));
+  fill_u32(P<uint32_t>(c->sc_cnt), nsc, 0u, c->stream);
+  {
+    TimedRegion tr(c, "insert");
+    sc_insert(T, pv, cfg, P<int>(c->sc_map), P<int>(c->sc_off), P<int>(c->sc_cnt), P<float>(c->list_pos),
+              c->vel ? P<float>(c->list_vel) : nullptr, c->stream);
+  }
+  sc_positions(T, total, cfg, P<int>(c->sc_map), P<int>(c->sc_pos), P<int>(c->sc_nid), c->stream);
+  return GVDBX_OK;
+}
+
+static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
+  CHK(gather_checks(c, chan));
+  if (c->nsc == 0) return GVDBX_OK;
+  const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
+  TimedRegion tr(c, "gather");
+  gather_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
+               P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), accum, mode, c->stream);
+  return GVDBX_OK;
+}
+
+int gvdbx_gather_density(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t len, int chan,
+                         int chan_clr, int accum) {
+  (void)n; (void)trans; (void)len; (void)chan_clr;
+  if (!c) return GVDBX_ERR_ARG;
+  if (sc != c->sc) return fail(c, GVDBX_ERR_ARG, "subcell size %d differs from the insert (%d)", sc, c->sc);
+  return gather_impl(c, r, chan, accum, 0);
+}
+
+int gvdbx_gather_levelset(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t len, int chan,
+                          int chan_clr, int accum) {
+  (void)n; (void)trans; (void)len; (void)chan_clr;
+  if (!c) return GVDBX_ERR_ARG;
+  if (sc != c->sc) return fail(c, GVDBX_ERR_ARG, "subcell size %d differs from the insert (%d)", sc, c->sc);
+  return gather_impl(c, r, chan, accum, 1);
+}
+
+int gvdbx_gather_velocity(gvdbx_ctx* c, int cx, int cy, int cz, int accum) {
+  CHK(gather_checks(c, cx));
+  CHK(chan_ok(c, cy));
+  CHK(chan_ok(c, cz));
+  if (!c->lists_have_vel) return fail(c, GVDBX_ERR_STATE, "no velocities were bound at InsertPointsSubcell");
+  if (c->nsc == 0) return GVDBX_OK;
+  const SubcellCfg cfg = subcell_cfg(c, c->sc, 0.f);
+  TimedRegion tr(c, "velocity");
+  gather_velocity(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
+                  P<float>(c->list_vel), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[cx]), P<float>(c->chan[cy]),
+                  P<float>(c->chan[cz]), accum, c->stream);
+  return GVDBX_OK;
+}
+
+int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {
+  CHK(gather_checks(c, chan));
+  if (c->nsc == 0) return GVDBX_OK;
+  const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
+  TimedRegion tr(c, "scatter");
+  scatter_reduce_init(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->leaf_slot), c->apron,
+                      P<float>(c->chan[chan]), init, c->stream);
+  scatter_reduce_levelset(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
+                          P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), c->stream);
+  return GVDBX_OK;
+}
+
+int gvdbx_update_apron(gvdbx_ctx* c, int chan, float boundval) {  // UpdateApron (V.cpp:4418-4457)
+  CHK(chan_ok(c, chan));
+  CHK(atlas_current(c));
+  if (c->apron == 0 || c->nslots == 0) return GVDBX_OK;
+  TimedRegion tr(c, "apron");
+  update_apron(tree_of(c), c->nslots, P<int4>(c->slot_pos), P<int>(c->leaf_slot), res_of(c, 0), c->apron,
+               P<float>(c->chan[chan]), boundval, c->stream);
+  return GVDBX_OK;
+}
+
+int gvdbx_sample_trilinear(gvdbx_ctx* c, int chan, const void* q, int64_t n, void* out) {
+  CHK(chan_ok(c, chan));
+  if (!q || !out || n < 0) return fail(c, GVDBX_ERR_ARG, "bad sample buffers");
+  if (c->apron < 1) return fail(c, GVDBX_ERR_STATE, "trilinear sampling needs an apron >= 1");
+  CHK(atlas_current(c));
+  if (c->depth < 0) return fail(c, GVDBX_ERR_STATE, "no topology");
+  TimedRegion tr(c, "sample");
+  sample_trilinear(tree_of(c), P<int>(c->leaf_slot), res_of(c, 0), c->apron, P<float>(c->chan[chan]),
+                   (const float*)q, n, (float*)out, c->stream);
+  return GVDBX_OK;
+}
+
+int gvdbx_get_depth(const gvdbx_ctx* c, int* depth, int32_t rootpos[3], float posmin[3], float posmax[3]) {
+  if (!c) return GVDBX_ERR_ARG;
+  if (depth) *depth = c->depth;
+  for (int a = 0; a < 3; ++a) {
+    if (rootpos) rootpos[a] = c->root[a];
+    if (posmin) posmin[a] = c->posmin[a];
+    if (posmax) posmax[a] = c->posmax[a];
+  }
+  return GVDBX_OK;
+}
+
+int gvdbx_get_node_count(const gvdbx_ctx* c, int lev, int64_t* total, int64_t* used) {
+  if (!c || lev < 0 || lev >= kMaxLev) return GVDBX_ERR_ARG;
+  if (total) *total = lev < c->levs ? c->cnt[lev] : 0;
+  if (used) *used = lev == 0 ? c->used0 : (lev < c->levs ? c->cnt[lev] : 0);
+  return GVDBX_OK;
+}
+
+static int copy_out(gvdbx_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return GVDBX_OK;
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GVDBX_OK;
+}
+
+int gvdbx_copy_nodes(gvdbx_ctx* c, int lev, gvdbx_node* out, int64_t max_nodes) {
+  if (!c || !out || lev < 0 || lev >= c->levs) return GVDBX_ERR_ARG;
+  const int64_t n = c->cnt[lev] < max_nodes ? c->cnt[lev] : max_nodes;
+  return copy_out(c, out, c->nodes[lev].p, (size_t)n * sizeof(gvdbx_node));
+}
+
+int gvdbx_copy_childlists(gvdbx_ctx* c, int lev, uint64_t* out, int64_t max_entries) {
+  if (!c || !out || lev < 1 || lev >= c->levs) return GVDBX_ERR_ARG;
+  int64_t n = c->cnt[lev] * clist_per(c, lev);
+  if (n > max_entries) n = max_entries;
+  return copy_out(c, out, c->clist[lev].p, (size_t)n * 8);
+}
+
+int gvdbx_get_num_slots(const gvdbx_ctx* c, int64_t* slots, int* brick_res) {
+  if (!c) return GVDBX_ERR_ARG;
+  if (slots) *slots = c->nslots;
+  if (brick_res) *brick_res = c->configured ? res_of(c, 0) + 2 * (c->apron < 0 ? 0 : c->apron) : 0;
+  return GVDBX_OK;
+}
+
+int gvdbx_copy_channel(gvdbx_ctx* c, int chan, float* out, int64_t max_floats) {
+  CHK(chan_ok(c, chan));
+  if (!out) return GVDBX_ERR_ARG;
+  int64_t n = c->nslots * brick_floats(c);
+  if (n > max_floats) n = max_floats;
+  return copy_out(c, out, c->chan[chan].p, (size_t)n * 4);
+}
+
+int gvdbx_get_channel_ptr(gvdbx_ctx* c, int chan, void** p, int64_t* floats) {
+  CHK(chan_ok(c, chan));
+  if (p) *p = c->chan[chan].p;
+  if (floats) *floats = c->nslots * brick_floats(c);
+  return GVDBX_OK;
+}
+
+int gvdbx_get_subcell_info(const gvdbx_ctx* c, int64_t* nsc, int64_t* len, int* sc) {
+  if (!c) return GVDBX_ERR_ARG;
+  if (nsc) *nsc = c->nsc;
+  if (len) *len = c->sclen;
+  if (sc) *sc = c->sc;
+  return GVDBX_OK;
+}
+
+int gvdbx_copy_subcell_array(gvdbx_ctx* c, int which, void* out, int64_t max_bytes) {
+  if (!c || !out) return GVDBX_ERR_ARG;
+  const void* src = nullptr;
+  int64_t bytes = 0;
+  switch (which) {
+    case 0: src = c->sc_map.p; bytes = c->cnt[0] * 4; break;
+    case 1: src = c->sc_cnt.p; bytes = c->nsc * 4; break;
+    case 2: src = c->sc_off.p; bytes = c->nsc * 4; break;
+    case 3: src = c->sc_pos.p; bytes = c->nsc * 12; break;
+    case 4: src = c->sc_nid.p; bytes = c->nsc * 4; break;
+    case 5: src = c->list_pos.p; bytes = c->sclen * 12; break;
+    case 6: src = c->lists_have_vel ? c->list_vel.p : nullptr; bytes = c->lists_have_vel ? c->sclen * 12 : 0; break;
+    default: return fail(c, GVDBX_ERR_ARG, "unknown subcell array %d", which);
+  }
+  if (bytes > max_bytes) bytes = max_bytes;
+  if (!src || bytes <= 0) return GVDBX_OK;
+  return copy_out(c, out, src, (size_t)bytes);
+}
+
+int gvdbx_dev_alloc(gvdbx_ctx* c, int64_t bytes, void** p) {
+  if (!c || !p || bytes < 0) return GVDBX_ERR_ARG;
+  hipError_t e = hipMalloc(p, (size_t)(bytes > 0 ? bytes : 1));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, GVDBX_ERR_NOMEM, "hipMalloc(%lld): %s", (long long)bytes, hipGetErrorString(e));
+  }
+  return GVDBX_OK;
+}
+
+int gvdbx_dev_free(gvdbx_ctx* c, void* p) {
+  if (!c) return GVDBX_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (p) HIPCHK(c, hipFree(p));
+  return GVDBX_OK;
+}
+
+int gvdbx_commit(gvdbx_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || bytes < 0) return GVDBX_ERR_ARG;
+  if (bytes == 0) return GVDBX_OK;
+  HIPCHK(c, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GVDBX_OK;
+}
+
+int gvdbx_retrieve(gvdbx_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || bytes < 0) return GVDBX_ERR_ARG;
+  return copy_out(c, dst, src, (size_t)bytes);
+}
+
+int gvdbx_kernel_timer(gvdbx_ctx* c, const char* kernel, int enable) {
+  if (!c) return GVDBX_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timer_on = enable && kernel;
+  c->timer_name = kernel ? kernel : "";
+  c->ev_used = 0;
+  return GVDBX_OK;
+}
+
+int gvdbx_kernel_timer_read(gvdbx_ctx* c, double* ms, int64_t* n) {
+  if (!c) return GVDBX_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double tot = 0;
+  for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+    tot += t;
+  }
+  if (ms) *ms = tot;
+  if (n) *n = (int64_t)(c->ev_used / 2);
+  c->ev_used = 0;
+  return GVDBX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// common.hpp -- shared device-side definitions of the gfx950 point-to-sparse-grid library.
+//
+// The topology is the reference's pool layout (src/gvdb_node.h:27-40, A.h:59-62): per level a
+// pool of 64-byte nodes and, for levels >= 1, a pool of child lists of res^3 uint64 entries.
+// Child list i belongs to node i of the same level (both pools grow in lockstep, V.cpp:1308-1331),
+// and every node is aligned to its level's range, so a tree descent needs one 8-byte load per
+// level and never reads a node (the reference reads node + child list per level, N.cuh:255-288).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gvdbx.h"
+
+namespace gvdbx {
+
+constexpr int kMaxLev = 10;
+constexpr uint64_t kUndef64 = 0xFFFFFFFFFFFFFFFFull;  // ID_UNDEF64
+constexpr uint64_t kUndefL = 0xFFFFFFFFull;           // ID_UNDEFL
+constexpr float kNoHit = 1.0e10f;                     // NOHIT (src/gvdb_types.h:52)
+
+// Tree description passed by value to kernels.
+struct DevTree {
+  int depth;                 // root level
+  int levs;
+  int ld[kMaxLev];           // log2 of per-level resolution
+  int lgR[kMaxLev];          // log2 of per-level range in voxels
+  int root[3];               // root corner (aligned to its range)
+  gvdbx_node* nodes[kMaxLev];
+  uint64_t* clist[kMaxLev];
+};
+
+__device__ __forceinline__ uint64_t elem0(int lev, uint64_t ndx) { return ((uint64_t)lev << 8) | (ndx << 16); }
+
+__device__ __forceinline__ bool in_root(const DevTree& T, int x, int y, int z) {
+  const int R = T.lgR[T.depth];
+  return x >= T.root[0] && y >= T.root[1] && z >= T.root[2] && ((x - T.root[0]) >> R) == 0 &&
+         ((y - T.root[1]) >> R) == 0 && ((z - T.root[2]) >> R) == 0;
+}
+
+// Child slot of position (x,y,z) inside its level-l ancestor: ((z*res + y)*res + x) of the
+// child cell (getBitPos, N.cuh:172-176).
+__device__ __forceinline__ int child_bit(const DevTree& T, int l, int x, int y, int z) {
+  const int sh = T.lgR[l - 1], ld = T.ld[l], m = (1 << ld) - 1;
+  return ((((z >> sh) & m) << ld | ((y >> sh) & m)) << ld) | ((x >> sh) & m);
+}
+
+// Index of the linked node of level stop containing (x,y,z), or -1 (positions outside the
+// root are missing).  Restates getPosParent/getPosLeafParent (N.cuh:255-348).
+__device__ __forceinline__ int64_t descend(const DevTree& T, int x, int y, int z, int stop) {
+  if (!in_root(T, x, y, z)) return -1;
+  int64_t cur = 0;
+  for (int l = T.depth; l > stop; --l) {
+    const uint64_t e = T.clist[l][((uint64_t)cur << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, x, y, z)];
+    if (e == kUndef64) return -1;
+    cur = (int64_t)(e >> 16);
+  }
+  return cur;
+}
+
+// ---- wave64 helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Monotonic float <-> int mapping for atomic min/max on floats.
+__device__ __forceinline__ int float_to_ordered(float f) {
+  int i = __float_as_int(f);
+  return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__host__ __device__ inline float ordered_to_float(int i) {
+  int b = i >= 0 ? i : i ^ 0x7FFFFFFF;
+  float f;
+  __builtin_memcpy(&f, &b, 4);
+  return f;
+}
+
+__device__ __forceinline__ float3 load_f3(const char* base, int64_t i, int off, int stride) {
+  const float* p = (const float*)(base + i * (int64_t)stride + off);
+  return make_float3(p[0], p[1], p[2]);
+}
+
+}  // namespace gvdbx

@@ -1,0 +1,731 @@
+// kernels.hip -- gfx950 kernels of the point-to-sparse-grid path.
+//
+// Each kernel cites the reference kernel whose semantics it restates
+// (P.cuh = kernels/cuda_gvdb_particles.cuh, N.cuh = kernels/cuda_gvdb_nodes.cuh,
+//  O.cuh = kernels/cuda_gvdb_operators.cuh, R.cuh = kernels/cuda_gvdb_raycast.cuh).
+#include <limits.h>
+
+#include "kernels.hpp"
+#include "primitives.hpp"
+
+namespace gvdbx {
+
+static inline int blocks_for(int64_t n, int bs, int64_t cap = 0x7FFFFFFF) {
+  int64_t g = (n + bs - 1) / bs;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+// ================================================================ bounding box
+// GetBoundingBox (V.cpp:5404-5467): min/max of p + trans over all points, as six
+// ordered-int atomics (one pass instead of split + six Thrust reductions).
+
+__global__ void k_bbox_init(int* meta) {
+  if (threadIdx.x < 3) meta[threadIdx.x] = INT_MAX;
+  else if (threadIdx.x < 6) meta[threadIdx.x] = INT_MIN;
+}
+
+__global__ void __launch_bounds__(256) k_bbox(PointsView P, int* meta) {
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * blockDim.x) {
+    float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+    float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { mn[a] = fminf(mn[a], q[a]); mx[a] = fmaxf(mx[a], q[a]); }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mn[a] = fminf(mn[a], __shfl_xor(mn[a], o, 64));
+      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o, 64));
+    }
+  __shared__ float sm[4][6];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) { sm[w][a] = mn[a]; sm[w][3 + a] = mx[a]; }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    float v = sm[0][a];
+    for (int k = 1; k < 4; ++k) v = a < 3 ? fminf(v, sm[k][a]) : fmaxf(v, sm[k][a]);
+    if (a < 3) atomicMin(&meta[a], float_to_ordered(v));
+    else atomicMax(&meta[a], float_to_ordered(v));
+  }
+}
+
+void bbox_init(int* meta, hipStream_t s) { hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, s, meta); }
+void bbox(const PointsView& P, int* meta, hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox, dim3(blocks_for(P.n, 256, 2048)), dim3(256), 0, s, P, meta);
+}
+
+// ================================================================ keys
+
+template <typename K>
+__device__ __forceinline__ K pack_key(KeyFmt f, int lev, int xr, int yr, int zr) {
+  return ((K)lev << (3 * f.b)) | ((K)xr << (2 * f.b)) | ((K)yr << f.b) | (K)zr;
+}
+
+// gvdbCalcBrickId (P.cuh:352-382): per point and level < depth the covering node of q.
+// For q >= 0, floor(q / R) (the ceilf-and-adjust form of GetCoveringNode, P.cuh:88-100,
+// with R a power of two) equals trunc(q) >> log2(R).
+template <typename K>
+__global__ void __launch_bounds__(256) k_brick_keys(DevTree T, PointsView P, KeyFmt f, K* __restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const int d = T.depth;
+  float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+  const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
+  K* out = keys + i * d;
+  if (q[0] < 0 || q[1] < 0 || q[2] < 0) {
+    for (int l = 0; l < d; ++l) out[l] = pack_key<K>(f, d, 0, 0, 0);
+    return;
+  }
+  const int iq[3] = {(int)q[0], (int)q[1], (int)q[2]};
+  for (int l = 0; l < d; ++l) {
+    const int sh = T.lgR[l];
+    out[l] = pack_key<K>(f, l, (iq[0] >> sh) - (T.root[0] >> sh), (iq[1] >> sh) - (T.root[1] >> sh),
+                         (iq[2] >> sh) - (T.root[2] >> sh));
+  }
+}
+
+template <typename K>
+void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, K* keys, hipStream_t s) {
+  hipLaunchKernelGGL(k_brick_keys<K>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, keys);
+}
+
+// Box enumeration of gvdbCalcExtraBrickId (P.cuh:467-512, hi = trunc(q) + 2r) and
+// gvdbCalcIncreExtraBrickId (P.cuh:420-465, hi = trunc(q) + r, marks found leaves).
+// Nodes outside the root are not emitted.  Emission past cap only counts.
+template <typename K>
+__global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFmt f, float r, float hi_mul,
+                                                  K* __restrict__ keys, int64_t cap, int* counter,
+                                                  int* __restrict__ markers) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+  const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
+  if (q[0] < 0 || q[1] < 0 || q[2] < 0) return;
+  int bmin[3], bmax[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float tq = (float)(int)q[a];
+    bmin[a] = (int)(tq - r);
+    bmax[a] = (int)(tq + hi_mul * r);
+  }
+  const int d = T.depth;
+  for (int l = 0; l < d; ++l) {
+    const int R = 1 << T.lgR[l];
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { lo[a] = bmin[a] / R * R; hi[a] = bmax[a] / R * R; }
+    for (int x = lo[0]; x <= hi[0]; x += R)
+      for (int y = lo[1]; y <= hi[1]; y += R)
+        for (int z = lo[2]; z <= hi[2]; z += R) {
+          if (!in_root(T, x, y, z)) continue;
+          const int64_t nd = descend(T, x, y, z, l);
+          if (nd < 0) {
+            const int sh = T.lgR[l];
+            const int k = atomicAdd(counter, 1);
+            if (k < cap)
+              keys[k] = pack_key<K>(f, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
+                                    (z >> sh) - (T.root[2] >> sh));
+          } else if (l == 0 && markers) {
+            markers[nd] = 1;
+          }
+        }
+  }
+}
+
+template <typename K>
+void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi_mul, K* keys, int64_t cap,
+              int* counter, int* markers, hipStream_t s) {
+  hipLaunchKernelGGL(k_box_keys<K>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, r, hi_mul, keys, cap,
+                     counter, markers);
+}
+
+// gvdbFindUnique (P.cuh:336-350): run starts of the sorted keys, invalid level excluded.
+template <typename K>
+__global__ void k_unique_flags(const K* __restrict__ k, int64_t m, KeyFmt f, int depth, int* __restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const K v = k[i];
+  const int lev = (int)(v >> (3 * f.b));
+  flags[i] = ((i == 0 || k[i - 1] != v) && lev < depth) ? 1 : 0;
+}
+
+template <typename K>
+void unique_flags(const K* k, int64_t m, KeyFmt f, int depth, int* flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_unique_flags<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, k, m, f, depth, flags);
+}
+
+// Per-level unique counts (the levCnt atomics of gvdbFindUnique), LDS histogram per block.
+template <typename K>
+__global__ void __launch_bounds__(256) k_level_hist(const K* __restrict__ k, const int* __restrict__ flags, int64_t m,
+                                                    KeyFmt f, int* lev_cnt) {
+  __shared__ int h[kMaxLev];
+  if (threadIdx.x < kMaxLev) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    if (flags[i]) atomicAdd(&h[(int)(k[i] >> (3 * f.b))], 1);
+  __syncthreads();
+  if (threadIdx.x < kMaxLev && h[threadIdx.x]) atomicAdd(&lev_cnt[threadIdx.x], h[threadIdx.x]);
+}
+
+template <typename K>
+void level_hist(const K* k, const int* flags, int64_t m, KeyFmt f, int* lev_cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_level_hist<K>, dim3(blocks_for(m, 256, 1024)), dim3(256), 0, s, k, flags, m, f, lev_cnt);
+}
+
+// SetupNode (V.cpp:2512-2524) for the unique keys in sorted order, appended after the
+// existing nodes of each level (PoolAlloc A.cpp:163-192); inner nodes get their child list.
+__device__ __forceinline__ void write_node(gvdbx_node* nd, int lev, int x, int y, int z, uint64_t childlist) {
+  uint4* w = (uint4*)nd;
+  uint32_t h0 = (uint32_t)lev | (1u << 8);  // lev, flags=1, prio=0, pad=0
+  w[0] = make_uint4(h0, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+  w[1] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);  // value = -1, vrange.x = 0
+  w[2] = make_uint4(0u, 0u, (uint32_t)kUndefL, 0u);               // vrange.yz, parent = ID_UNDEFL
+  w[3] = make_uint4((uint32_t)childlist, (uint32_t)(childlist >> 32), 0u, 0u);
+}
+
+template <typename K>
+__global__ void k_setup_nodes(DevTree T, const K* __restrict__ k, const int* __restrict__ flags,
+                              const int* __restrict__ rank, int64_t m, KeyFmt f, AllocPlan plan) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !flags[i]) return;
+  const K v = k[i];
+  const uint64_t mask = (1ull << f.b) - 1ull;
+  const int lev = (int)(v >> (3 * f.b));
+  const int xr = (int)((v >> (2 * f.b)) & mask), yr = (int)((v >> f.b) & mask), zr = (int)(v & mask);
+  const int sh = T.lgR[lev];
+  const int64_t idx = plan.base[lev] + (rank[i] - plan.levstart[lev]);
+  const uint64_t cl = lev > 0 ? (1ull | ((uint64_t)lev << 8) | ((uint64_t)idx << 16)) : kUndefL;
+  write_node(T.nodes[lev] + idx, lev, ((T.root[0] >> sh) + xr) << sh, ((T.root[1] >> sh) + yr) << sh,
+             ((T.root[2] >> sh) + zr) << sh, cl);
+}
+
+template <typename K>
+void setup_nodes(const DevTree& T, const K* k, const int* flags, const int* rank, int64_t m, KeyFmt f,
+                 const AllocPlan& plan, hipStream_t s) {
+  hipLaunchKernelGGL(k_setup_nodes<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, T, k, flags, rank, m, f, plan);
+}
+
+__global__ void k_setup_root(DevTree T) {
+  const int d = T.depth;
+  write_node(T.nodes[d], d, T.root[0], T.root[1], T.root[2], 1ull | ((uint64_t)d << 8));
+}
+void setup_root(const DevTree& T, hipStream_t s) { hipLaunchKernelGGL(k_setup_root, dim3(1), dim3(1), 0, s, T); }
+
+// gvdbLinkBricks (N.cuh:470-505): flagged node -> parent by descent, parent's child slot.
+__global__ void k_link_level(DevTree T, int lev, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  gvdbx_node* nd = T.nodes[lev] + i;
+  if (!nd->flags) return;
+  const int x = nd->pos[0], y = nd->pos[1], z = nd->pos[2];
+  const int64_t pi = descend(T, x, y, z, lev + 1);
+  if (pi < 0) return;
+  nd->parent = elem0(lev + 1, (uint64_t)pi);
+  T.clist[lev + 1][((uint64_t)pi << (3 * T.ld[lev + 1])) + (uint64_t)child_bit(T, lev + 1, x, y, z)] =
+      elem0(lev, (uint64_t)i);
+}
+void link_level(const DevTree& T, int lev, int64_t count, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_link_level, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, lev, count);
+}
+
+// Leaf flags <- markers (V.cpp:1075-1081).
+__global__ void k_apply_markers(DevTree T, const int* __restrict__ markers, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) T.nodes[0][i].flags = markers[i] ? 1 : 0;
+}
+void apply_markers(const DevTree& T, const int* markers, int64_t count, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_apply_markers, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, markers, count);
+}
+
+__global__ void k_flags_to_markers(DevTree T, int64_t count, int* __restrict__ markers) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) markers[i] = T.nodes[0][i].flags;
+}
+void copy_flags_to_markers(const DevTree& T, int64_t count, int* markers, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_flags_to_markers, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, count, markers);
+}
+
+// gvdbDelinkLeafBricks (N.cuh:401-429).
+__global__ void k_delink_leaves(DevTree T, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  gvdbx_node* nd = T.nodes[0] + i;
+  if (nd->flags) return;
+  if (nd->parent == kUndef64) return;
+  const int x = nd->pos[0], y = nd->pos[1], z = nd->pos[2];
+  const int64_t pi = descend(T, x, y, z, 1);
+  nd->parent = kUndef64;
+  if (pi < 0) return;
+  T.clist[1][((uint64_t)pi << (3 * T.ld[1])) + (uint64_t)child_bit(T, 1, x, y, z)] = kUndef64;
+}
+void delink_leaves(const DevTree& T, int64_t count, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_delink_leaves, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, count);
+}
+
+// gvdbDelinkBricks (N.cuh:431-467): one wave per inner node scans its child list.
+__global__ void __launch_bounds__(256) k_delink_level(DevTree T, int lev, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= count) return;
+  gvdbx_node* nd = T.nodes[lev] + i;
+  if (nd->parent == kUndef64) {
+    if (lane == 0) nd->flags = 0;
+    return;
+  }
+  const int64_t nch = 1ll << (3 * T.ld[lev]);
+  const uint64_t* cl = T.clist[lev] + ((uint64_t)i << (3 * T.ld[lev]));
+  bool any = false;
+  for (int64_t k = lane; k < nch; k += 64) any |= (cl[k] != kUndef64);
+  any = __ballot(any) != 0;
+  if (lane != 0) return;
+  if (any) { nd->flags = 1; return; }
+  nd->flags = 0;
+  const int x = nd->pos[0], y = nd->pos[1], z = nd->pos[2];
+  const int64_t pi = descend(T, x, y, z, lev + 1);
+  nd->parent = kUndef64;
+  if (pi < 0) return;
+  T.clist[lev + 1][((uint64_t)pi << (3 * T.ld[lev + 1])) + (uint64_t)child_bit(T, lev + 1, x, y, z)] = kUndef64;
+}
+void delink_level(const DevTree& T, int lev, int64_t count, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_delink_level, dim3(blocks_for(count, 4)), dim3(256), 0, s, T, lev, count);
+}
+
+// by_parent = 0: mFlags (UpdateAtlas V.cpp:2655-2661); 1: mParent != ID_UNDEF64
+// (gvdbSetFlagSubcell P.cuh:109-118).
+__global__ void k_leaf_flags(DevTree T, int64_t count, int by_parent, int* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const gvdbx_node* nd = T.nodes[0] + i;
+  out[i] = by_parent ? (nd->parent != kUndef64 ? 1 : 0) : (nd->flags ? 1 : 0);
+}
+void leaf_flags(const DevTree& T, int64_t count, int by_parent, int* out, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_leaf_flags, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, count, by_parent, out);
+}
+
+__global__ void __launch_bounds__(256) k_count_flags(const int* __restrict__ f, int64_t n, int* out) {
+  int c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    c += f[i] != 0;
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+void count_flags(const int* flags, int64_t n, int* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_count_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, flags, n, out);
+}
+
+// ================================================================ atlas
+// UpdateAtlas (V.cpp:2630-2704): slot = rank of flagged leaves; mValue = reference atlas
+// corner of the slot (getAtlasPos A.cpp:705-715).  Unflagged leaves keep their stale mValue.
+__global__ void k_assign_slots(DevTree T, const int* __restrict__ flags, const int* __restrict__ rank, int64_t count,
+                               int apron, int br, int cx, int cy, int* __restrict__ leaf_slot,
+                               int* __restrict__ slot_leaf, int4* __restrict__ slot_pos) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  if (!flags[i]) { leaf_slot[i] = -1; return; }
+  const int s = rank[i];
+  gvdbx_node* nd = T.nodes[0] + i;
+  const int a2 = cx * cy;
+  const int z = s / a2, rem = s - z * a2, y = rem / cx, x = rem - y * cx;
+  nd->value[0] = x * br + apron;
+  nd->value[1] = y * br + apron;
+  nd->value[2] = z * br + apron;
+  leaf_slot[i] = s;
+  slot_leaf[s] = (int)i;
+  slot_pos[s] = make_int4(nd->pos[0], nd->pos[1], nd->pos[2], (int)i);
+}
+void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t count, int apron, int br, int cx, int cy,
+                  int* leaf_slot, int* slot_leaf, int4* slot_pos, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_assign_slots, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, flags, rank, count, apron, br,
+                     cx, cy, leaf_slot, slot_leaf, slot_pos);
+}
+
+// ================================================================ subcells
+// gvdbCountSubcell / gvdbInsertSubcell (P.cuh:275-315, 218-273): every subcell corner from
+// trunc(q-r)/sc*sc to trunc(q+1+r)/sc*sc; leaf by descent; idx = map[leaf]*D^3 + local.
+
+template <bool kInsert>
+__global__ void __launch_bounds__(256) k_subcell(DevTree T, PointsView P, SubcellCfg c, const int* __restrict__ map,
+                                                 int* __restrict__ cnt, const int* __restrict__ off,
+                                                 float* __restrict__ list_pos, float* __restrict__ list_vel) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+  if (p.z == kNoHit) return;
+  const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
+  if (q[0] < 0 || q[1] < 0 || q[2] < 0) return;
+  float3 v = make_float3(0.f, 0.f, 0.f);
+  if (kInsert && list_vel) v = load_f3(P.vel, i, P.vel_off, P.vel_stride);
+  int lo[3], hi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float lq = q[a] - c.r;
+    float hq = q[a] + 1.0f;
+    hq = hq + c.r;
+    lo[a] = ((int)lq / c.sc) * c.sc;
+    hi[a] = ((int)hq / c.sc) * c.sc;
+  }
+  const int m = c.res0 - 1;
+  for (int x = lo[0]; x <= hi[0]; x += c.sc)
+    for (int y = lo[1]; y <= hi[1]; y += c.sc)
+      for (int z = lo[2]; z <= hi[2]; z += c.sc) {
+        const int64_t leaf = descend(T, x, y, z, 0);
+        if (leaf < 0) continue;
+        const int lx = (x & m) >> c.lgsc, ly = (y & m) >> c.lgsc, lz = (z & m) >> c.lgsc;
+        const int64_t idx = (int64_t)map[leaf] * (c.D * c.D * c.D) + (lz * c.D + ly) * c.D + lx;
+        if (!kInsert) {
+          atomicAdd(&cnt[idx], 1);
+        } else {
+          const int64_t k = (int64_t)off[idx] + atomicAdd(&cnt[idx], 1);
+          list_pos[3 * k + 0] = q[0];
+          list_pos[3 * k + 1] = q[1];
+          list_pos[3 * k + 2] = q[2];
+          if (list_vel) {
+            list_vel[3 * k + 0] = v.x;
+            list_vel[3 * k + 1] = v.y;
+            list_vel[3 * k + 2] = v.z;
+          }
+        }
+      }
+}
+
+void sc_count(const DevTree& T, const PointsView& P, SubcellCfg c, const int* map, int* cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_subcell<false>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, c, map, cnt, nullptr,
+                     nullptr, nullptr);
+}
+void sc_insert(const DevTree& T, const PointsView& P, SubcellCfg c, const int* map, const int* off, int* cursor,
+               float* list_pos, float* list_vel, hipStream_t s) {
+  hipLaunchKernelGGL(k_subcell<true>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, c, map, cursor, off,
+                     list_pos, list_vel);
+}
+
+// gvdbCalcSubcellPos (P.cuh:677-694).
+__global__ void k_sc_positions(DevTree T, int64_t leaves, SubcellCfg c, const int* __restrict__ map,
+                               int* __restrict__ sc_pos, int* __restrict__ sc_nid) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= leaves) return;
+  const gvdbx_node* nd = T.nodes[0] + i;
+  if (nd->parent == kUndef64) return;
+  const int per = c.D * c.D * c.D;
+  for (int s = 0; s < per; ++s) {
+    const int64_t k = (int64_t)map[i] * per + s;
+    sc_pos[3 * k + 0] = nd->pos[0] + (s % c.D) * c.sc;
+    sc_pos[3 * k + 1] = nd->pos[1] + ((s / c.D) % c.D) * c.sc;
+    sc_pos[3 * k + 2] = nd->pos[2] + ((s / c.D / c.D) % c.D) * c.sc;
+    sc_nid[k] = (int)i;
+  }
+}
+void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map, int* sc_pos, int* sc_nid,
+                  hipStream_t s) {
+  if (leaves <= 0) return;
+  hipLaunchKernelGGL(k_sc_positions, dim3(blocks_for(leaves, 256)), dim3(256), 0, s, T, leaves, c, map, sc_pos,
+                     sc_nid);
+}
+
+// ================================================================ gathers
+// gvdbGatherDensity / gvdbGatherLevelSet (P.cuh:696-773).  One block per subcell, one
+// thread per voxel; the subcell's list is staged through LDS in chunks.  The reductions
+// max_p(r - |p-v|) and min_p(|p-v| - r) are taken on |p-v|^2 with one sqrt at the end
+// (sqrt and the subtraction are monotonic, so the result is the same reduction).
+
+__device__ __forceinline__ int64_t atlas_voxel(int slot, int br, int lx, int ly, int lz) {
+  return (((int64_t)slot * br + lz) * br + ly) * br + lx;
+}
+
+__global__ void __launch_bounds__(512) k_gather_field(SubcellCfg c, const int* __restrict__ sc_pos,
+                                                      const int* __restrict__ sc_nid, const int* __restrict__ cnt,
+                                                      const int* __restrict__ off, const float* __restrict__ list_pos,
+                                                      const int* __restrict__ leaf_slot, int apron,
+                                                      float* __restrict__ chan, int accum, int mode) {
+  __shared__ float sx[512], sy[512], sz[512];
+  const int k = blockIdx.x;
+  const int nt = c.sc * c.sc * c.sc;
+  const int t = threadIdx.x;
+  const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
+  const float vx = (float)(sc_pos[3 * k + 0] + tx), vy = (float)(sc_pos[3 * k + 1] + ty),
+              vz = (float)(sc_pos[3 * k + 2] + tz);
+  const int n = cnt[k], o = off[k];
+  float md2 = INFINITY;
+  for (int base = 0; base < n; base += nt) {
+    const int j = base + t;
+    if (j < n) {
+      sx[t] = list_pos[3 * (int64_t)(o + j) + 0];
+      sy[t] = list_pos[3 * (int64_t)(o + j) + 1];
+      sz[t] = list_pos[3 * (int64_t)(o + j) + 2];
+    }
+    __syncthreads();
+    const int lim = min(nt, n - base);
+    for (int jj = 0; jj < lim; ++jj) {
+      const float dx = sx[jj] - vx, dy = sy[jj] - vy, dz = sz[jj] - vz;
+      md2 = fminf(md2, dx * dx + dy * dy + dz * dz);
+    }
+    __syncthreads();
+  }
+  const int br = c.res0 + 2 * apron;
+  const int m = c.res0 - 1;
+  const int slot = leaf_slot[sc_nid[k]];
+  if (slot < 0) return;
+  const int64_t vi = atlas_voxel(slot, br, ((int)vx & m) + apron, ((int)vy & m) + apron, ((int)vz & m) + apron);
+  float val;
+  if (mode == 0) {
+    val = accum ? chan[vi] : 0.0f;
+    if (n > 0) val = fmaxf(val, c.r - sqrtf(md2));
+  } else {
+    val = accum ? chan[vi] : 3.0f;
+    if (n > 0) val = fminf(val, sqrtf(md2) - c.r);
+  }
+  chan[vi] = val;
+}
+
+void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
+                  const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode,
+                  hipStream_t s) {
+  if (nsc <= 0) return;
+  hipLaunchKernelGGL(k_gather_field, dim3((unsigned)nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, sc_pos, sc_nid, cnt, off,
+                     list_pos, leaf_slot, apron, chan, accum, mode);
+}
+
+// Velocity mean per subcell: (accum ? old : 0) + sum(v), divided by (cnt + accum); the
+// colour rule of P.cuh:717-731 for float3.  Sums are exact int64 in 2^-24 units.
+__global__ void __launch_bounds__(512) k_gather_velocity(SubcellCfg c, const int* __restrict__ sc_pos,
+                                                         const int* __restrict__ sc_nid, const int* __restrict__ cnt,
+                                                         const int* __restrict__ off,
+                                                         const float* __restrict__ list_vel,
+                                                         const int* __restrict__ leaf_slot, int apron,
+                                                         float* __restrict__ cx, float* __restrict__ cy,
+                                                         float* __restrict__ cz, int accum) {
+  __shared__ long long part[8][3];
+  const int k = blockIdx.x;
+  const int nt = c.sc * c.sc * c.sc;
+  const int t = threadIdx.x;
+  const int n = cnt[k], o = off[k];
+  long long s[3] = {0, 0, 0};
+  for (int j = t; j < n; j += nt)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) s[a] += __float2ll_rn(list_vel[3 * (int64_t)(o + j) + a] * 16777216.0f);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) s[a] = wave_sum(s[a]);
+  const int nw = (nt + 63) / 64;
+  if ((t & 63) == 0)
+    for (int a = 0; a < 3; ++a) part[t >> 6][a] = s[a];
+  __syncthreads();
+  long long tot[3] = {0, 0, 0};
+  for (int w = 0; w < nw; ++w)
+    for (int a = 0; a < 3; ++a) tot[a] += part[w][a];
+  const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
+  const int br = c.res0 + 2 * apron;
+  const int m = c.res0 - 1;
+  const int slot = leaf_slot[sc_nid[k]];
+  if (slot < 0) return;
+  const int64_t vi = atlas_voxel(slot, br, ((sc_pos[3 * k] + tx) & m) + apron, ((sc_pos[3 * k + 1] + ty) & m) + apron,
+                                 ((sc_pos[3 * k + 2] + tz) & m) + apron);
+  const long long den = (long long)n + (accum ? 1 : 0);
+  float* ch[3] = {cx, cy, cz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double sum = (double)tot[a] * (1.0 / 16777216.0);
+    if (accum) sum += (double)ch[a][vi];
+    ch[a][vi] = den > 0 ? (float)(sum / (double)den) : 0.0f;
+  }
+}
+
+void gather_velocity(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
+                     const float* list_vel, const int* leaf_slot, int apron, float* cx, float* cy, float* cz,
+                     int accum, hipStream_t s) {
+  if (nsc <= 0) return;
+  hipLaunchKernelGGL(k_gather_velocity, dim3((unsigned)nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, sc_pos, sc_nid, cnt,
+                     off, list_vel, leaf_slot, apron, cx, cy, cz, accum);
+}
+
+// ScatterReduceLevelSet: init the subcell voxels, then every (point, subcell) pair
+// min-reduces |p-v|-r into the voxels with an ordered-int atomicMin.
+__global__ void k_sr_init(SubcellCfg c, const int* __restrict__ sc_pos, const int* __restrict__ sc_nid,
+                          const int* __restrict__ leaf_slot, int apron, float* __restrict__ chan, float init) {
+  const int k = blockIdx.x, t = threadIdx.x;
+  const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
+  const int br = c.res0 + 2 * apron, m = c.res0 - 1;
+  const int slot = leaf_slot[sc_nid[k]];
+  if (slot < 0) return;
+  chan[atlas_voxel(slot, br, ((sc_pos[3 * k] + tx) & m) + apron, ((sc_pos[3 * k + 1] + ty) & m) + apron,
+                   ((sc_pos[3 * k + 2] + tz) & m) + apron)] = init;
+}
+
+__global__ void __launch_bounds__(256) k_sr_scatter(SubcellCfg c, int64_t nsc, const int* __restrict__ sc_pos,
+                                                    const int* __restrict__ sc_nid, const int* __restrict__ cnt,
+                                                    const int* __restrict__ off, const float* __restrict__ list_pos,
+                                                    const int* __restrict__ leaf_slot, int apron,
+                                                    float* __restrict__ chan) {
+  // one wave per subcell, lanes over list entries; per entry all sc^3 voxels
+  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= nsc) return;
+  const int n = cnt[k], o = off[k];
+  const int br = c.res0 + 2 * apron, m = c.res0 - 1;
+  const int slot = leaf_slot[sc_nid[k]];
+  if (slot < 0) return;
+  const int bx = sc_pos[3 * k], by = sc_pos[3 * k + 1], bz = sc_pos[3 * k + 2];
+  int* ci = (int*)chan;
+  for (int j = lane; j < n; j += 64) {
+    const float px = list_pos[3 * (int64_t)(o + j)], py = list_pos[3 * (int64_t)(o + j) + 1],
+                pz = list_pos[3 * (int64_t)(o + j) + 2];
+    for (int tz = 0; tz < c.sc; ++tz)
+      for (int ty = 0; ty < c.sc; ++ty)
+        for (int tx = 0; tx < c.sc; ++tx) {
+          const float dx = px - (float)(bx + tx), dy = py - (float)(by + ty), dz = pz - (float)(bz + tz);
+          const float w = sqrtf(dx * dx + dy * dy + dz * dz) - c.r;
+          atomicMin(&ci[atlas_voxel(slot, br, ((bx + tx) & m) + apron, ((by + ty) & m) + apron,
+                                    ((bz + tz) & m) + apron)],
+                    float_to_ordered(w));
+        }
+  }
+}
+
+__global__ void k_sr_final(SubcellCfg c, const int* __restrict__ sc_pos, const int* __restrict__ sc_nid,
+                           const int* __restrict__ leaf_slot, int apron, float* __restrict__ chan) {
+  const int k = blockIdx.x, t = threadIdx.x;
+  const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
+  const int br = c.res0 + 2 * apron, m = c.res0 - 1;
+  const int slot = leaf_slot[sc_nid[k]];
+  if (slot < 0) return;
+  const int64_t vi = atlas_voxel(slot, br, ((sc_pos[3 * k] + tx) & m) + apron, ((sc_pos[3 * k + 1] + ty) & m) + apron,
+                                 ((sc_pos[3 * k + 2] + tz) & m) + apron);
+  chan[vi] = ordered_to_float(((int*)chan)[vi]);
+}
+
+__global__ void k_sr_to_ordered(SubcellCfg c, const int* __restrict__ sc_pos, const int* __restrict__ sc_nid,
+                                const int* __restrict__ leaf_slot, int apron, float* __restrict__ chan) {
+  const int k = blockIdx.x, t = threadIdx.x;
+  const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
+  const int br = c.res0 + 2 * apron, m = c.res0 - 1;
+  const int slot = leaf_slot[sc_nid[k]];
+  if (slot < 0) return;
+  const int64_t vi = atlas_voxel(slot, br, ((sc_pos[3 * k] + tx) & m) + apron, ((sc_pos[3 * k + 1] + ty) & m) + apron,
+                                 ((sc_pos[3 * k + 2] + tz) & m) + apron);
+  ((int*)chan)[vi] = float_to_ordered(chan[vi]);
+}
+
+void scatter_reduce_init(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* leaf_slot,
+                         int apron, float* chan, float init, hipStream_t s) {
+  if (nsc <= 0) return;
+  const int nt = c.sc * c.sc * c.sc;
+  hipLaunchKernelGGL(k_sr_init, dim3((unsigned)nsc), dim3(nt), 0, s, c, sc_pos, sc_nid, leaf_slot, apron, chan, init);
+}
+
+void scatter_reduce_levelset(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
+                             const int* off, const float* list_pos, const int* leaf_slot, int apron, float* chan,
+                             hipStream_t s) {
+  if (nsc <= 0) return;
+  const int nt = c.sc * c.sc * c.sc;
+  hipLaunchKernelGGL(k_sr_to_ordered, dim3((unsigned)nsc), dim3(nt), 0, s, c, sc_pos, sc_nid, leaf_slot, apron, chan);
+  hipLaunchKernelGGL(k_sr_scatter, dim3(blocks_for(nsc, 4, 1ll << 31)), dim3(256), 0, s, c, nsc, sc_pos, sc_nid, cnt,
+                     off, list_pos, leaf_slot, apron, chan);
+  hipLaunchKernelGGL(k_sr_final, dim3((unsigned)nsc), dim3(nt), 0, s, c, sc_pos, sc_nid, leaf_slot, apron, chan);
+}
+
+// ================================================================ apron
+// UpdateApron<T> (O.cuh:72-126): each apron voxel of each active brick copies the interior
+// voxel of the neighbour brick at the same world position, or boundval.  The 27 neighbour
+// slots are resolved once per brick (descent) into LDS instead of one descent per voxel.
+__global__ void __launch_bounds__(256) k_apron(DevTree T, const int4* __restrict__ slot_pos,
+                                               const int* __restrict__ leaf_slot, int res0, int apron,
+                                               float* __restrict__ chan, float boundval) {
+  __shared__ int nb[27];
+  const int s = blockIdx.x;
+  const int4 bp = slot_pos[s];
+  if (threadIdx.x < 27) {
+    const int dx = threadIdx.x % 3 - 1, dy = (threadIdx.x / 3) % 3 - 1, dz = threadIdx.x / 9 - 1;
+    const int64_t lf = descend(T, bp.x + dx * res0, bp.y + dy * res0, bp.z + dz * res0, 0);
+    nb[threadIdx.x] = lf < 0 ? -1 : leaf_slot[lf];
+  }
+  __syncthreads();
+  const int br = res0 + 2 * apron;
+  const int nvox = br * br * br;
+  for (int t = threadIdx.x; t < nvox; t += blockDim.x) {
+    const int x = t % br, y = (t / br) % br, z = t / (br * br);
+    const int gx = x - apron, gy = y - apron, gz = z - apron;
+    const int ox = gx < 0 ? -1 : (gx >= res0 ? 1 : 0), oy = gy < 0 ? -1 : (gy >= res0 ? 1 : 0),
+              oz = gz < 0 ? -1 : (gz >= res0 ? 1 : 0);
+    if ((ox | oy | oz) == 0) continue;
+    const int ns = nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)];
+    float v = boundval;
+    if (ns >= 0)
+      v = chan[atlas_voxel(ns, br, gx - ox * res0 + apron, gy - oy * res0 + apron, gz - oz * res0 + apron)];
+    chan[atlas_voxel(s, br, x, y, z)] = v;
+  }
+}
+
+void update_apron(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int apron,
+                  float* chan, float boundval, hipStream_t s) {
+  if (nslots <= 0 || apron <= 0) return;
+  hipLaunchKernelGGL(k_apron, dim3((unsigned)nslots), dim3(256), 0, s, T, slot_pos, leaf_slot, res0, apron, chan,
+                     boundval);
+}
+
+// ================================================================ trilinear sampling
+// getVolSampleF (R.cuh:116-125): leaf at the point, tex3D-style linear filter with voxel
+// v centred at v + 0.5, read from the brick's padded (apron) voxels; 0 outside leaves.
+__global__ void k_sample(DevTree T, const int* __restrict__ leaf_slot, int res0, int apron,
+                         const float* __restrict__ chan, const float* __restrict__ q, int64_t n,
+                         float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
+  if (!(px >= 0.f && py >= 0.f && pz >= 0.f)) { out[i] = 0.f; return; }
+  const int ix = (int)px, iy = (int)py, iz = (int)pz;
+  const int64_t lf = descend(T, ix, iy, iz, 0);
+  const int slot = lf < 0 ? -1 : leaf_slot[lf];
+  if (slot < 0) { out[i] = 0.f; return; }
+  const int m = ~(res0 - 1);
+  const float xb = px - (float)(ix & m) + (float)apron - 0.5f;
+  const float yb = py - (float)(iy & m) + (float)apron - 0.5f;
+  const float zb = pz - (float)(iz & m) + (float)apron - 0.5f;
+  const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
+  const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
+  const float ax = xb - fx, ay = yb - fy, az = zb - fz;
+  const int br = res0 + 2 * apron;
+  const float* b = chan + (int64_t)slot * br * br * br;
+  auto at = [&](int x, int y, int z) { return b[(z * br + y) * br + x]; };
+  const float c00 = at(x0, y0, z0) * (1.f - ax) + at(x0 + 1, y0, z0) * ax;
+  const float c10 = at(x0, y0 + 1, z0) * (1.f - ax) + at(x0 + 1, y0 + 1, z0) * ax;
+  const float c01 = at(x0, y0, z0 + 1) * (1.f - ax) + at(x0 + 1, y0, z0 + 1) * ax;
+  const float c11 = at(x0, y0 + 1, z0 + 1) * (1.f - ax) + at(x0 + 1, y0 + 1, z0 + 1) * ax;
+  const float c0 = c00 * (1.f - ay) + c10 * ay, c1 = c01 * (1.f - ay) + c11 * ay;
+  out[i] = c0 * (1.f - az) + c1 * az;
+}
+
+void sample_trilinear(const DevTree& T, const int* leaf_slot, int res0, int apron, const float* chan, const float* q,
+                      int64_t n, float* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sample, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, leaf_slot, res0, apron, chan, q, n, out);
+}
+
+// explicit instantiations
+template void brick_keys<uint32_t>(const DevTree&, const PointsView&, KeyFmt, uint32_t*, hipStream_t);
+template void brick_keys<uint64_t>(const DevTree&, const PointsView&, KeyFmt, uint64_t*, hipStream_t);
+template void box_keys<uint32_t>(const DevTree&, const PointsView&, KeyFmt, float, float, uint32_t*, int64_t, int*,
+                                 int*, hipStream_t);
+template void box_keys<uint64_t>(const DevTree&, const PointsView&, KeyFmt, float, float, uint64_t*, int64_t, int*,
+                                 int*, hipStream_t);
+template void unique_flags<uint32_t>(const uint32_t*, int64_t, KeyFmt, int, int*, hipStream_t);
+template void unique_flags<uint64_t>(const uint64_t*, int64_t, KeyFmt, int, int*, hipStream_t);
+template void level_hist<uint32_t>(const uint32_t*, const int*, int64_t, KeyFmt, int*, hipStream_t);
+template void level_hist<uint64_t>(const uint64_t*, const int*, int64_t, KeyFmt, int*, hipStream_t);
+template void setup_nodes<uint32_t>(const DevTree&, const uint32_t*, const int*, const int*, int64_t, KeyFmt,
+                                    const AllocPlan&, hipStream_t);
+template void setup_nodes<uint64_t>(const DevTree&, const uint64_t*, const int*, const int*, int64_t, KeyFmt,
+                                    const AllocPlan&, hipStream_t);
+
+}  // namespace gvdbx

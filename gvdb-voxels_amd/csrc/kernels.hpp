@@ -1,0 +1,85 @@
+// kernels.hpp -- launchers of the point-path kernels (device code in kernels.hip).
+#pragma once
+#include "common.hpp"
+
+namespace gvdbx {
+
+// Compact topology key: [lev | x_rel | y_rel | z_rel], coordinates relative to the root in
+// units of the level's range, b bits each; invalid points use lev = depth.  Ascending
+// compact order == ascending reference key order (lev<<48 | x<<32 | y<<16 | z, P.cuh:373-381).
+struct KeyFmt {
+  int b;       // bits per coordinate = lgR[depth] - lgR[0]
+  int nbits;   // total live bits
+};
+
+struct PointsView {
+  const char* pos; int pos_off, pos_stride;
+  const char* vel; int vel_off, vel_stride;
+  int64_t n;
+  float tx, ty, tz;
+};
+
+// ---- bounding box (meta[0..5] ordered ints, pre-filled by bbox_init)
+void bbox_init(int* meta, hipStream_t s);
+void bbox(const PointsView& P, int* meta, hipStream_t s);
+
+// ---- keys
+template <typename K> void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, K* keys, hipStream_t s);
+template <typename K>
+void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi_mul, K* keys, int64_t cap,
+              int* counter, int* markers, hipStream_t s);
+template <typename K> void unique_flags(const K* k, int64_t m, KeyFmt f, int depth, int* flags, hipStream_t s);
+template <typename K>
+void level_hist(const K* k, const int* flags, int64_t m, KeyFmt f, int* lev_cnt, hipStream_t s);
+
+struct AllocPlan {
+  int64_t base[kMaxLev];      // first new pool index per level
+  int64_t levstart[kMaxLev];  // first unique rank of each level
+};
+template <typename K>
+void setup_nodes(const DevTree& T, const K* k, const int* flags, const int* rank, int64_t m, KeyFmt f,
+                 const AllocPlan& plan, hipStream_t s);
+void setup_root(const DevTree& T, hipStream_t s);
+void link_level(const DevTree& T, int lev, int64_t count, hipStream_t s);
+void apply_markers(const DevTree& T, const int* markers, int64_t count, hipStream_t s);
+void delink_leaves(const DevTree& T, int64_t count, hipStream_t s);
+void delink_level(const DevTree& T, int lev, int64_t count, hipStream_t s);
+void leaf_flags(const DevTree& T, int64_t count, int by_parent, int* out, hipStream_t s);
+void count_flags(const int* flags, int64_t n, int* out, hipStream_t s);
+void copy_flags_to_markers(const DevTree& T, int64_t count, int* markers, hipStream_t s);
+
+// ---- atlas
+void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t count, int apron, int br, int cx,
+                  int cy, int* leaf_slot, int* slot_leaf, int4* slot_pos, hipStream_t s);
+
+// ---- subcells
+struct SubcellCfg {
+  int sc, lgsc, D, res0;
+  float r;
+};
+void sc_count(const DevTree& T, const PointsView& P, SubcellCfg c, const int* map, int* cnt, hipStream_t s);
+void sc_insert(const DevTree& T, const PointsView& P, SubcellCfg c, const int* map, const int* off, int* cursor,
+               float* list_pos, float* list_vel, hipStream_t s);
+void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map, int* sc_pos, int* sc_nid,
+                  hipStream_t s);
+
+// ---- gathers (mode 0 density, 1 level set)
+void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
+                  const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode,
+                  hipStream_t s);
+void gather_velocity(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
+                     const int* off, const float* list_vel, const int* leaf_slot, int apron, float* cx, float* cy,
+                     float* cz, int accum, hipStream_t s);
+void scatter_reduce_init(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* leaf_slot,
+                         int apron, float* chan, float init, hipStream_t s);
+void scatter_reduce_levelset(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
+                             const int* off, const float* list_pos, const int* leaf_slot, int apron, float* chan,
+                             hipStream_t s);
+
+// ---- apron and sampling
+void update_apron(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0,
+                  int apron, float* chan, float boundval, hipStream_t s);
+void sample_trilinear(const DevTree& T, const int* leaf_slot, int res0, int apron, const float* chan,
+                      const float* q, int64_t n, float* out, hipStream_t s);
+
+}  // namespace gvdbx

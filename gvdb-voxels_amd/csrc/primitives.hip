@@ -1,0 +1,250 @@
+// primitives.hip -- wave64 scan and LSD radix sort for gfx950.
+//
+// Replace the reference's three-level Blelloch scan (prefixSum/prefixFixup, P.cuh:856-902,
+// driven by V.cpp:4760-4819) and the Thrust uint64 sort (gvdbDeviceRadixSort,
+// src/gvdb_cutils.cu:11-16).  Both are spin-free multi-kernel designs: no inter-block waits.
+#include "primitives.hpp"
+
+namespace gvdbx {
+
+// ------------------------------------------------------------------ fills
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+static int grid_for(int64_t n, int block, int cap = 65536) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+void fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<uint32_t>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+void fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<uint64_t>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+void fill_f32(float* p, int64_t n, float v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+// ------------------------------------------------------------------ scan
+
+// Exclusive scan of a 256-thread block's values; returns the block total in *total.
+__device__ __forceinline__ int block_exclusive_scan_256(int v, int* total) {
+  __shared__ int ws[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[wave] = x;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    int s = ws[w];
+    pre += (w < wave) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + x - v;
+}
+
+// Tile = 4 rounds x 256 threads x int4 = 4096 ints.
+__global__ void __launch_bounds__(256) k_scan_reduce(const int* __restrict__ in, int64_t n, int* __restrict__ tile_sums) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int s = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t i = base + (int64_t)r * 1024 + threadIdx.x * 4;
+    if (i + 3 < n) {
+      int4 v = *(const int4*)(in + i);
+      s += v.x + v.y + v.z + v.w;
+    } else {
+      for (int k = 0; k < 4; ++k)
+        if (i + k < n) s += in[i + k];
+    }
+  }
+  s = wave_sum(s);
+  __shared__ int ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// Single block: exclusive scan of tile sums in place; grand total to *total.
+__global__ void __launch_bounds__(256) k_scan_tiles(int* __restrict__ sums, int64_t ntiles, int* __restrict__ total) {
+  int carry = 0;
+  for (int64_t base = 0; base < ntiles; base += 1024) {
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int64_t i = base + threadIdx.x * 4 + k;
+      v[k] = i < ntiles ? sums[i] : 0;
+    }
+    int t = v[0] + v[1] + v[2] + v[3];
+    int tot;
+    int pre = block_exclusive_scan_256(t, &tot) + carry;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int64_t i = base + threadIdx.x * 4 + k;
+      if (i < ntiles) sums[i] = pre;
+      pre += v[k];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void __launch_bounds__(256) k_scan_down(const int* __restrict__ in, int* __restrict__ out, int64_t n,
+                                                   const int* __restrict__ tile_prefix) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int carry = tile_prefix[blockIdx.x];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t i = base + (int64_t)r * 1024 + threadIdx.x * 4;
+    int v[4];
+    if (i + 3 < n) {
+      int4 q = *(const int4*)(in + i);
+      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+      for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? in[i + k] : 0;
+    }
+    int tot;
+    int pre = block_exclusive_scan_256(v[0] + v[1] + v[2] + v[3], &tot) + carry;
+    int o[4];
+    o[0] = pre; o[1] = pre + v[0]; o[2] = o[1] + v[1]; o[3] = o[2] + v[2];
+    if (i + 3 < n) {
+      *(int4*)(out + i) = make_int4(o[0], o[1], o[2], o[3]);
+    } else {
+      for (int k = 0; k < 4; ++k)
+        if (i + k < n) out[i + k] = o[k];
+    }
+    carry += tot;
+  }
+}
+
+int64_t scan_tile_count(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+void exclusive_scan(const int* in, int* out, int64_t n, int* tile_buf, int* total, hipStream_t s) {
+  if (n <= 0) {
+    if (total) fill_u32((uint32_t*)total, 1, 0u, s);
+    return;
+  }
+  const int64_t nt = scan_tile_count(n);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(256), 0, s, in, n, tile_buf);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, s, tile_buf, nt, total);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nt), dim3(256), 0, s, in, out, n, tile_buf);
+}
+
+// ------------------------------------------------------------------ radix sort
+
+template <typename K>
+__global__ void __launch_bounds__(256) k_radix_up(const K* __restrict__ keys, int64_t n, int shift,
+                                                 int* __restrict__ counts, int ntiles) {
+  __shared__ int hist[256];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+#pragma unroll 4
+  for (int r = 0; r < kRadixTile / 256; ++r) {
+    const int64_t i = base + r * 256 + threadIdx.x;
+    if (i < n) atomicAdd(&hist[(int)((keys[i] >> shift) & 255)], 1);
+  }
+  __syncthreads();
+  counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+template <typename K, bool kHasVal>
+__global__ void __launch_bounds__(256) k_radix_down(const K* __restrict__ kin, K* __restrict__ kout,
+                                                   const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
+                                                   int64_t n, int shift, const int* __restrict__ offsets,
+                                                   int ntiles) {
+  __shared__ int base_of[256];
+  __shared__ int whist[4][256];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  base_of[tid] = offsets[(int64_t)tid * ntiles + blockIdx.x];
+  const uint64_t lt = lanemask_lt();
+  const int64_t tbase = (int64_t)blockIdx.x * kRadixTile;
+  for (int r = 0; r < kRadixTile / 256; ++r) {
+    const int64_t i = tbase + r * 256 + tid;
+    const bool valid = i < n;
+    K k = valid ? kin[i] : (K)0;
+    uint32_t v = 0;
+    if (kHasVal && valid) v = vin[i];
+    const int d = (int)((k >> shift) & 255);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) whist[w][tid] = 0;
+    __syncthreads();
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const int rank = __popcll(peers & lt);
+    if (valid && rank == 0) whist[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int off = base_of[d] + rank;
+      for (int w = 0; w < wave; ++w) off += whist[w][d];
+      kout[off] = k;
+      if (kHasVal) vout[off] = v;
+    }
+    __syncthreads();
+    base_of[tid] += whist[0][tid] + whist[1][tid] + whist[2][tid] + whist[3][tid];
+    __syncthreads();
+  }
+}
+
+int64_t radix_count_ints(int64_t n) { return 256 * ((n + kRadixTile - 1) / kRadixTile); }
+
+template <typename K>
+static int radix_sort_impl(K* keys, K* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int begin_bit,
+                           int end_bit, int* counts, int* tile_buf, hipStream_t s) {
+  if (n <= 1) return 0;
+  const int ntiles = (int)((n + kRadixTile - 1) / kRadixTile);
+  const int64_t ncnt = 256LL * ntiles;
+  int parity = 0;
+  K* kin = keys;
+  K* kout = keys_alt;
+  uint32_t* vin = vals;
+  uint32_t* vout = vals_alt;
+  for (int shift = begin_bit; shift < end_bit; shift += 8) {
+    hipLaunchKernelGGL(k_radix_up<K>, dim3(ntiles), dim3(256), 0, s, kin, n, shift, counts, ntiles);
+    exclusive_scan(counts, counts, ncnt, tile_buf, nullptr, s);
+    if (vals)
+      hipLaunchKernelGGL((k_radix_down<K, true>), dim3(ntiles), dim3(256), 0, s, kin, kout, vin, vout, n, shift, counts,
+                         ntiles);
+    else
+      hipLaunchKernelGGL((k_radix_down<K, false>), dim3(ntiles), dim3(256), 0, s, kin, kout, nullptr, nullptr, n, shift,
+                         counts, ntiles);
+    K* t = kin; kin = kout; kout = t;
+    uint32_t* tv = vin; vin = vout; vout = tv;
+    parity ^= 1;
+  }
+  return parity;
+}
+
+int radix_sort_u32(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int begin_bit,
+                   int end_bit, int* counts, int* tile_buf, hipStream_t s) {
+  return radix_sort_impl<uint32_t>(keys, keys_alt, vals, vals_alt, n, begin_bit, end_bit, counts, tile_buf, s);
+}
+int radix_sort_u64(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int begin_bit,
+                   int end_bit, int* counts, int* tile_buf, hipStream_t s) {
+  return radix_sort_impl<uint64_t>(keys, keys_alt, vals, vals_alt, n, begin_bit, end_bit, counts, tile_buf, s);
+}
+
+}  // namespace gvdbx

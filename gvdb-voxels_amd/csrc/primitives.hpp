@@ -1,0 +1,28 @@
+// primitives.hpp -- scan / radix sort / fill launchers (device code in primitives.hip).
+#pragma once
+#include "common.hpp"
+
+namespace gvdbx {
+
+constexpr int kScanTile = 4096;
+constexpr int kRadixTile = 4096;
+
+void fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+void fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
+void fill_f32(float* p, int64_t n, float v, hipStream_t s);
+
+// Exclusive scan of n int32 values (in may equal out).  tile_buf holds scan_tile_count(n)
+// ints; *total (device) receives the sum when non-null.
+int64_t scan_tile_count(int64_t n);
+void exclusive_scan(const int* in, int* out, int64_t n, int* tile_buf, int* total, hipStream_t s);
+
+// Stable LSD radix sort over bits [begin_bit, end_bit) in 8-bit digits, ping-ponging between
+// keys and keys_alt (vals/vals_alt optional).  Returns 0 if the result is in keys, 1 if in
+// keys_alt.  counts holds radix_count_ints(n) ints, tile_buf scan_tile_count of that.
+int64_t radix_count_ints(int64_t n);
+int radix_sort_u32(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int begin_bit,
+                   int end_bit, int* counts, int* tile_buf, hipStream_t s);
+int radix_sort_u64(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int begin_bit,
+                   int end_bit, int* counts, int* tile_buf, hipStream_t s);
+
+}  // namespace gvdbx

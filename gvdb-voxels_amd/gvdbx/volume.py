@@ -1,0 +1,228 @@
+"""VolumeGVDB: the reference's point-path methods (V.h:321-558) over the gvdbx C ABI.
+
+Method names, argument order and meaning follow the reference VolumeGVDB
+(source/gvdb_library/src/gvdb_volume_gvdb.h).  Failures raise GvdbxError where the
+reference would print and exit (cudaCheck, A.cpp:1006-1034) or print and return
+(RebuildTopology depth checks, V.cpp:1187-1194).
+"""
+import ctypes as C
+
+import numpy as np
+
+from .lib import NODE_DTYPE, DeviceArray, GvdbxError, fptr, load_library
+
+T_FLOAT = 3
+
+
+class VolumeGVDB:
+    def __init__(self, device=0, stream=None):
+        self.L = load_library()
+        h = C.c_void_p()
+        code = self.L.gvdbx_ctx_create(device, stream, C.byref(h))
+        if code != 0:
+            raise GvdbxError(code, "gvdbx_ctx_create")
+        self.h = h.value
+        self.log2dims = None
+        self.apron = None
+        self._points = None
+
+    # ------------------------------------------------------------ plumbing
+    def _chk(self, code, what):
+        if code != 0:
+            buf = C.create_string_buffer(512)
+            if self.h:
+                self.L.gvdbx_last_error(self.h, buf, 512)
+            raise GvdbxError(code, what, buf.value.decode(errors="replace"))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gvdbx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        self._chk(self.L.gvdbx_synchronize(self.h), "synchronize")
+
+    def alloc(self, nbytes):
+        return DeviceArray(self, nbytes)
+
+    def to_device(self, arr):
+        return DeviceArray.from_numpy(self, arr)
+
+    # ------------------------------------------------------------ configuration
+    def Configure(self, q4, q3, q2, q1, q0):
+        """Configure(q4..q0) as in V.cpp:2364: the last argument is the leaf level."""
+        self.configure_levels([q0, q1, q2, q3, q4])
+
+    def configure_levels(self, log2dims):
+        ld = np.ascontiguousarray(log2dims, dtype=np.int32)
+        self._chk(self.L.gvdbx_configure(self.h, len(ld), ld.ctypes.data), "configure")
+        self.log2dims = tuple(int(max(x, 1)) for x in ld)
+
+    def SetChannelDefault(self, cx, cy, cz):
+        self._chk(self.L.gvdbx_set_channel_default(self.h, cx, cy, cz), "set_channel_default")
+
+    def AddChannel(self, chan, dtype=T_FLOAT, apron=1):
+        self._chk(self.L.gvdbx_add_channel(self.h, chan, dtype, apron), "add_channel")
+        self.apron = apron
+
+    def FillChannel(self, chan, value):
+        self._chk(self.L.gvdbx_fill_channel(self.h, chan, float(value)), "fill_channel")
+
+    def ClearChannel(self, chan):
+        self.FillChannel(chan, 0.0)
+
+    # ------------------------------------------------------------ points
+    def SetPoints(self, pos, vel=None, clr=None, n=None, pos_off=0, pos_stride=12, vel_off=0, vel_stride=12,
+                  clr_off=0, clr_stride=4):
+        """SetPoints + SetDataGPU: device buffers (DeviceArray or raw pointer ints)."""
+        ptr = lambda a: None if a is None else (a.ptr if isinstance(a, DeviceArray) else int(a))
+        if n is None:
+            n = pos.nbytes // pos_stride
+        self._points = (pos, vel, clr)
+        self.npts = n
+        self._chk(self.L.gvdbx_set_points(self.h, n, ptr(pos), pos_off, pos_stride, ptr(vel), vel_off, vel_stride,
+                                          ptr(clr), clr_off, clr_stride), "set_points")
+
+    # ------------------------------------------------------------ topology
+    def RebuildTopology(self, n, radius, orig=(0, 0, 0)):
+        self._chk(self.L.gvdbx_rebuild_topology(self.h, n, radius, fptr(orig)), "RebuildTopology")
+
+    def AccumulateTopology(self, n, radius, orig=(0, 0, 0), min_depth=0):
+        self._chk(self.L.gvdbx_accumulate_topology(self.h, n, radius, fptr(orig), min_depth), "AccumulateTopology")
+
+    def RequestFullRebuild(self, flag=True):
+        self._chk(self.L.gvdbx_request_full_rebuild(self.h, int(bool(flag))), "RequestFullRebuild")
+
+    def FinishTopology(self):
+        bmin = np.zeros(3, np.float32)
+        bmax = np.zeros(3, np.float32)
+        fp = C.POINTER(C.c_float)
+        self._chk(self.L.gvdbx_finish_topology(self.h, bmin.ctypes.data_as(fp), bmax.ctypes.data_as(fp)),
+                  "FinishTopology")
+        return bmin, bmax
+
+    def UpdateAtlas(self):
+        self._chk(self.L.gvdbx_update_atlas(self.h), "UpdateAtlas")
+
+    # ------------------------------------------------------------ subcells / gathers
+    def InsertPointsSubcell(self, subcell, n, radius, trans=(0, 0, 0)):
+        ln = C.c_int64(0)
+        self._chk(self.L.gvdbx_insert_points_subcell(self.h, subcell, n, radius, fptr(trans), C.byref(ln)),
+                  "InsertPointsSubcell")
+        return ln.value
+
+    def GatherDensity(self, subcell, n, radius, trans, sc_len, chan, chan_clr=-1, accum=False):
+        self._chk(self.L.gvdbx_gather_density(self.h, subcell, n, radius, fptr(trans), sc_len, chan, chan_clr,
+                                              int(bool(accum))), "GatherDensity")
+
+    def GatherLevelSet(self, subcell, n, radius, trans, sc_len, chan, chan_clr=-1, accum=False):
+        self._chk(self.L.gvdbx_gather_levelset(self.h, subcell, n, radius, fptr(trans), sc_len, chan, chan_clr,
+                                               int(bool(accum))), "GatherLevelSet")
+
+    def GatherVelocity(self, chan_x, chan_y, chan_z, accum=False):
+        self._chk(self.L.gvdbx_gather_velocity(self.h, chan_x, chan_y, chan_z, int(bool(accum))), "GatherVelocity")
+
+    def ScatterReduceLevelSet(self, radius, chan, init=3.0):
+        self._chk(self.L.gvdbx_scatter_reduce_levelset(self.h, radius, chan, init), "ScatterReduceLevelSet")
+
+    def UpdateApron(self, chan, boundval=0.0):
+        self._chk(self.L.gvdbx_update_apron(self.h, chan, float(boundval)), "UpdateApron")
+
+    def SampleTrilinear(self, chan, q_dev, n, out_dev):
+        self._chk(self.L.gvdbx_sample_trilinear(self.h, chan, q_dev.ptr, n, out_dev.ptr), "SampleTrilinear")
+
+    # ------------------------------------------------------------ introspection
+    def depth(self):
+        d = C.c_int(0)
+        root = np.zeros(3, np.int32)
+        pmin = np.zeros(3, np.float32)
+        pmax = np.zeros(3, np.float32)
+        self._chk(self.L.gvdbx_get_depth(self.h, C.byref(d), root.ctypes.data, pmin.ctypes.data, pmax.ctypes.data),
+                  "get_depth")
+        return d.value, root, pmin, pmax
+
+    def node_count(self, lev):
+        t = C.c_int64(0)
+        u = C.c_int64(0)
+        self._chk(self.L.gvdbx_get_node_count(self.h, lev, C.byref(t), C.byref(u)), "get_node_count")
+        return t.value, u.value
+
+    def getNumTotalNodes(self, lev):
+        return self.node_count(lev)[0]
+
+    def getNumUsedNodes(self, lev):
+        return self.node_count(lev)[1]
+
+    def get_nodes(self, lev):
+        n = self.node_count(lev)[0]
+        out = np.zeros(n, NODE_DTYPE)
+        if n:
+            self._chk(self.L.gvdbx_copy_nodes(self.h, lev, out.ctypes.data, n), "copy_nodes")
+        return out
+
+    def get_childlists(self, lev):
+        n = self.node_count(lev)[0]
+        per = 1 << (3 * self.log2dims[lev])
+        out = np.zeros((n, per), np.uint64)
+        if n:
+            self._chk(self.L.gvdbx_copy_childlists(self.h, lev, out.ctypes.data, n * per), "copy_childlists")
+        return out
+
+    def num_slots(self):
+        s = C.c_int64(0)
+        br = C.c_int(0)
+        self._chk(self.L.gvdbx_get_num_slots(self.h, C.byref(s), C.byref(br)), "get_num_slots")
+        return s.value, br.value
+
+    def get_channel(self, chan):
+        """Brick-linear channel as (slots, br, br, br) indexed [slot, z, y, x]."""
+        s, br = self.num_slots()
+        out = np.zeros(s * br ** 3, np.float32)
+        if out.size:
+            self._chk(self.L.gvdbx_copy_channel(self.h, chan, out.ctypes.data, out.size), "copy_channel")
+        return out.reshape(s, br, br, br)
+
+    def channel_ptr(self, chan):
+        p = C.c_void_p()
+        n = C.c_int64(0)
+        self._chk(self.L.gvdbx_get_channel_ptr(self.h, chan, C.byref(p), C.byref(n)), "get_channel_ptr")
+        return p.value, n.value
+
+    def subcell_info(self):
+        nsc = C.c_int64(0)
+        ln = C.c_int64(0)
+        sc = C.c_int(0)
+        self._chk(self.L.gvdbx_get_subcell_info(self.h, C.byref(nsc), C.byref(ln), C.byref(sc)), "subcell_info")
+        return nsc.value, ln.value, sc.value
+
+    def get_subcells(self):
+        nsc, ln, sc = self.subcell_info()
+        total = self.node_count(0)[0]
+
+        def grab(which, n, dtype, shape=None):
+            a = np.zeros(n, dtype)
+            if a.nbytes:
+                self._chk(self.L.gvdbx_copy_subcell_array(self.h, which, a.ctypes.data, a.nbytes), "copy_subcell")
+            return a.reshape(shape) if shape else a
+
+        return dict(
+            map=grab(0, total, np.int32), cnt=grab(1, nsc, np.int32), off=grab(2, nsc, np.int32),
+            pos=grab(3, nsc * 3, np.int32, (nsc, 3)), nid=grab(4, nsc, np.int32),
+            list_pos=grab(5, ln * 3, np.float32, (ln, 3)), list_vel=grab(6, ln * 3, np.float32, (ln, 3)),
+        )
+
+    # ------------------------------------------------------------ timing
+    def kernel_timer(self, name, enable=True):
+        self._chk(self.L.gvdbx_kernel_timer(self.h, name.encode() if name else None, int(enable)), "kernel_timer")
+
+    def kernel_timer_read(self):
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        self._chk(self.L.gvdbx_kernel_timer_read(self.h, C.byref(ms), C.byref(n)), "kernel_timer_read")
+        return ms.value, n.value

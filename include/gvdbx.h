@@ -1,0 +1,144 @@
+/*
+ * gvdbx.h -- C ABI of the MI355X (gfx950) point-to-sparse-grid library.
+ *
+ * Drop-in replacement for the point path of NVIDIA GVDB Voxels' VolumeGVDB
+ * (V.cpp = source/gvdb_library/src/gvdb_volume_gvdb.cpp, V.h = .../gvdb_volume_gvdb.h).
+ * Each entry point names the reference method it replaces.  The C++ facade in
+ * gvdb-voxels_amd/gvdbx/volume_gvdb.hpp keeps the reference's method names and
+ * signatures on top of these calls.
+ *
+ * ABI rules
+ *   - every function returns int status (GVDBX_OK = 0); none exits or throws;
+ *   - point buffers are caller-owned device pointers (SetDataGPU semantics, V.cpp:4872-4880);
+ *     topology pools, atlas channels and subcell lists are library-owned;
+ *   - work is enqueued on the context's HIP stream; functions that must return a
+ *     count to the host (the reference's int& outputs and DtoH copies) synchronise that
+ *     stream, as the reference does;
+ *   - one context per host thread; calls on one context must not overlap.
+ *
+ * Topology layout (bit-identical to the reference pools):
+ *   node pool of level l: gvdbx_node[count_l] (64 B, src/gvdb_node.h:27-40);
+ *   child-list pool of level l >= 1: uint64[count_l][res_l^3], entries are
+ *   Elem(0, l-1, index) = (index << 16) | ((l-1) << 8) or 0xFFFFFFFFFFFFFFFF (A.h:59-62).
+ * Atlas layout (replaces the reference 3D CUarray atlas, A.cpp:528-560):
+ *   channel c: float[slots][br][br][br], br = res0 + 2*apron, brick-linear, slot s is
+ *   the s-th flagged leaf in pool order; leaf.value holds the reference atlas corner of s.
+ */
+#ifndef GVDBX_H
+#define GVDBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GVDBX_ABI_VERSION 1
+
+enum gvdbx_status {
+  GVDBX_OK = 0,
+  GVDBX_ERR_ARG = 1,          /* bad argument / unconfigured context */
+  GVDBX_ERR_DEPTH0 = 2,       /* "Points extents smaller than 1 brick" (V.cpp:1187-1190) */
+  GVDBX_ERR_DEPTH_MAX = 3,    /* "Point extents exceed 5 levels" (V.cpp:1191-1194) */
+  GVDBX_ERR_CAPACITY = 4,     /* a size exceeds a 32-bit index or a fixed limit */
+  GVDBX_ERR_HIP = 5,          /* a HIP runtime call failed */
+  GVDBX_ERR_STATE = 6,        /* call order violated (e.g. gather before insert) */
+  GVDBX_ERR_NOMEM = 7,        /* device allocation failed */
+  GVDBX_ERR_DEGENERATE = 8    /* the whole topology is one brick at level 0 */
+};
+
+enum gvdbx_dtype { GVDBX_T_FLOAT = 3 };   /* T_FLOAT (src/gvdb_types.h) */
+
+/* Node, same field order and size as the reference Node (src/gvdb_node.h:27-40). */
+typedef struct gvdbx_node {
+  uint8_t lev, flags, prio, pad;
+  int32_t pos[3];
+  int32_t value[3];
+  float vrange[3];
+  uint64_t parent;
+  uint64_t childlist;
+  uint64_t mask;
+} gvdbx_node;
+
+typedef struct gvdbx_ctx gvdbx_ctx;
+
+/* ---- context: SetCudaDevice + Initialize (V.cpp:240-334, 2320-2361) */
+int gvdbx_ctx_create(int device, void* hip_stream, gvdbx_ctx** out);
+int gvdbx_ctx_destroy(gvdbx_ctx* ctx);
+const char* gvdbx_status_string(int status);
+int gvdbx_last_error(const gvdbx_ctx* ctx, char* buf, int buflen);
+int gvdbx_synchronize(gvdbx_ctx* ctx);
+int gvdbx_get_stream(const gvdbx_ctx* ctx, void** hip_stream);
+
+/* ---- configuration: Configure (V.cpp:2364-2434), SetChannelDefault (V.h:390),
+ *      AddChannel (V.cpp:2437-2453), FillChannel/ClearChannel (V.cpp:2456-2471, 1596-1603) */
+int gvdbx_configure(gvdbx_ctx* ctx, int levs, const int* log2dims);
+int gvdbx_set_channel_default(gvdbx_ctx* ctx, int cx, int cy, int cz);
+int gvdbx_add_channel(gvdbx_ctx* ctx, int chan, int dtype, int apron);
+int gvdbx_fill_channel(gvdbx_ctx* ctx, int chan, float value);
+
+/* ---- points: SetPoints + SetDataGPU (V.cpp:4900-4908, 4872-4880); byte offsets and strides */
+int gvdbx_set_points(gvdbx_ctx* ctx, int64_t n,
+                     const void* pos, int pos_off, int pos_stride,
+                     const void* vel, int vel_off, int vel_stride,
+                     const void* clr, int clr_off, int clr_stride);
+
+/* ---- topology: RebuildTopology (V.cpp:1181-1218), AccumulateTopology (V.cpp:1158-1178),
+ *      RequestFullRebuild (V.h:432), FinishTopology (V.cpp:1579-1593), UpdateAtlas (V.cpp:2630-2704) */
+int gvdbx_rebuild_topology(gvdbx_ctx* ctx, int64_t n, float radius, const float orig[3]);
+int gvdbx_accumulate_topology(gvdbx_ctx* ctx, int64_t n, float radius, const float orig[3], int min_depth);
+int gvdbx_request_full_rebuild(gvdbx_ctx* ctx, int flag);
+int gvdbx_finish_topology(gvdbx_ctx* ctx, float bmin[3], float bmax[3]);
+int gvdbx_update_atlas(gvdbx_ctx* ctx);
+
+/* ---- subcells and gathers: InsertPointsSubcell (V.cpp:4971-5092),
+ *      GatherDensity (V.cpp:5318-5344), GatherLevelSet (V.cpp:5348-5373) */
+int gvdbx_insert_points_subcell(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, const float trans[3],
+                                int64_t* sc_pnts_len);
+int gvdbx_gather_density(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, const float trans[3],
+                         int64_t sc_pnts_len, int chan, int chan_clr, int accum);
+int gvdbx_gather_levelset(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, const float trans[3],
+                          int64_t sc_pnts_len, int chan, int chan_clr, int accum);
+/* Velocity channels: the reference colour-mean rule (P.cuh:717-731) applied to the binned
+ * float3 velocities (stored by gvdbInsertSubcell, P.cuh:266-267, read by no reference gather).
+ * Sums are exact 64-bit fixed point (2^-24 units), so the result is order independent. */
+int gvdbx_gather_velocity(gvdbx_ctx* ctx, int chan_x, int chan_y, int chan_z, int accum);
+/* ScatterReduceLevelSet: absent from the reference; defined as the GatherLevelSet field
+ * (P.cuh:736-773) produced by scattering each (point, subcell) pair and min-reducing. */
+int gvdbx_scatter_reduce_levelset(gvdbx_ctx* ctx, float radius, int chan, float init);
+
+/* ---- apron: UpdateApron (V.cpp:4411-4457, kernel O.cuh:72-126) */
+int gvdbx_update_apron(gvdbx_ctx* ctx, int chan, float boundval);
+
+/* ---- trilinear sampling: getTrilinear/getVolSampleF (R.cuh:104-126), voxel v at v+0.5,
+ *      0 outside active bricks.  q: device float3[n]; out: device float[n]. */
+int gvdbx_sample_trilinear(gvdbx_ctx* ctx, int chan, const void* q, int64_t n, void* out);
+
+/* ---- introspection (getNumTotalNodes/getNumUsedNodes/getNode V.h:637-641, getAux) */
+int gvdbx_get_depth(const gvdbx_ctx* ctx, int* depth, int32_t rootpos[3], float posmin[3], float posmax[3]);
+int gvdbx_get_node_count(const gvdbx_ctx* ctx, int lev, int64_t* total, int64_t* used);
+int gvdbx_copy_nodes(gvdbx_ctx* ctx, int lev, gvdbx_node* host_out, int64_t max_nodes);
+int gvdbx_copy_childlists(gvdbx_ctx* ctx, int lev, uint64_t* host_out, int64_t max_entries);
+int gvdbx_get_num_slots(const gvdbx_ctx* ctx, int64_t* slots, int* brick_res);
+int gvdbx_copy_channel(gvdbx_ctx* ctx, int chan, float* host_out, int64_t max_floats);
+int gvdbx_get_channel_ptr(gvdbx_ctx* ctx, int chan, void** dev_ptr, int64_t* floats);
+int gvdbx_get_subcell_info(const gvdbx_ctx* ctx, int64_t* num_subcells, int64_t* list_len, int* subcell);
+/* which: 0 map(int32 per leaf) 1 cnt 2 off 3 pos(int32x3) 4 nid 5 list pos(float x3) 6 list vel(float x3) */
+int gvdbx_copy_subcell_array(gvdbx_ctx* ctx, int which, void* host_out, int64_t max_bytes);
+
+/* ---- device memory helpers: AllocData/CommitData/RetrieveData (V.cpp:4822-4860, A.cpp:263-298) */
+int gvdbx_dev_alloc(gvdbx_ctx* ctx, int64_t bytes, void** dev_ptr);
+int gvdbx_dev_free(gvdbx_ctx* ctx, void* dev_ptr);
+int gvdbx_commit(gvdbx_ctx* ctx, void* dev_dst, const void* host_src, int64_t bytes);
+int gvdbx_retrieve(gvdbx_ctx* ctx, void* host_dst, const void* dev_src, int64_t bytes);
+
+/* ---- timing of one named kernel with HIP events on the context stream.
+ *      kernel: "gather", "insert", "count", "apron", "keys", "sort" ... ; enable=0 stops. */
+int gvdbx_kernel_timer(gvdbx_ctx* ctx, const char* kernel, int enable);
+int gvdbx_kernel_timer_read(gvdbx_ctx* ctx, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
