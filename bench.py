@@ -1,0 +1,181 @@
+"""Benchmark of the point-to-sparse-grid step on one MI355X.
+
+Workload (the SPH-style step): 4,194,304 particles on a jittered lattice filling the
+lower 40% of a 256^3 box, velocities ~N(0,1), dt = 0.5, seed 1.  One step is
+  RebuildTopology(r=1) + UpdateAtlas + InsertPointsSubcell(sc=4, r=1)
+  + GatherDensity(ch0) + velocity means (ch1-3) + UpdateApron(ch0..3)
+on the positions of that step (p_k = p_{k-1} + v*dt, precomputed and resident in HBM
+before timing).  The rebuild state machine runs as in the reference: steps 0 and 1 are
+full rebuilds (inside the warmup), later steps are incremental.
+
+Prints one JSON line with the throughput in Mpoints/s, the HBM roofline of the dominant
+kernel (HIP events on the library stream around that kernel) and a CPU baseline (the
+oracle restatement on the host, one thread).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gvdb-voxels_amd"))
+
+HBM_PEAK_GBS = 8000.0
+N_POINTS = 4_194_304
+DT = np.float32(0.5)
+RADIUS = 1.0
+SUBCELL = 4
+
+
+def step_positions(p0, vel, count):
+    out = [p0]
+    cur = p0
+    for _ in range(count - 1):
+        cur = (cur + vel * DT).astype(np.float32)
+        out.append(cur)
+    return out
+
+
+def run_step(v, dpos, dvel, n):
+    v.SetPoints(dpos, dvel, n=n)
+    v.RebuildTopology(n, RADIUS)
+    v.UpdateAtlas()
+    ln = v.InsertPointsSubcell(SUBCELL, n, RADIUS)
+    v.GatherDensity(SUBCELL, n, RADIUS, (0, 0, 0), ln, 0)
+    v.GatherVelocity(1, 2, 3)
+    v.UpdateApron()  # all four channels, boundary value 0
+    return ln
+
+
+def algorithmic_bytes(n, s_len, nsc, bricks, chans=4, vel=True):
+    """Compulsory HBM bytes of one insert+gather+apron step (lower bound):
+    points read once, binned copies written and read once, subcell count+offset,
+    interior written, apron read+written."""
+    return (n * (12 + 12 * vel) + s_len * (24 + 24 * vel) + 8 * nsc + 4 * chans * bricks * 512
+            + 8 * chans * bricks * 488)
+
+
+def gather_bytes(s_len, nsc, bricks):
+    """Algorithmic bytes of one GatherDensity launch: list positions read, per-subcell
+    count/offset/corner/leaf read, interior voxels written."""
+    return s_len * 12 + nsc * (4 + 4 + 12 + 4) + bricks * 512 * 4
+
+
+def cpu_baseline(p, vel, sample):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ps = p[:sample]
+    vs = vel[:sample]
+    t0 = time.perf_counter()
+    t = O.Tree((3, 3, 3, 3, 5))
+    for c in range(4):
+        t.add_channel(c)
+    t.rebuild_topology(ps, RADIUS)
+    t.update_atlas()
+    t.insert_points_subcell(SUBCELL, ps, RADIUS)
+    t.gather_density(RADIUS, 0)
+    t.gather_velocity(vs, (1, 2, 3))
+    for c in range(4):
+        t.update_apron(c, 0.0)
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    O.cpu_activate_space(ps)
+    dt_act = time.perf_counter() - t1
+    return {"value": round(sample / dt / 1e6, 4), "unit": "Mpoints/s", "cores": 1, "kind": "port",
+            "sample": f"first {sample} points of step 0 (full rebuild + atlas + subcell insert + density + "
+                      f"3 velocity means + 4 aprons), oracle C restatement, 1 thread, {dt:.2f} s",
+            "reference_cpu_activate_space_Mpoints_s": round(sample / dt_act / 1e6, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--points", type=int, default=N_POINTS)
+    ap.add_argument("--cpu-sample", type=int, default=N_POINTS)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", default="gather", help="kernel family timed for the roofline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+    if args.warmup < 2:
+        args.warmup = 2  # the two full rebuilds of the reference state machine stay outside timing
+
+    from gvdbx import VolumeGVDB, synth
+
+    p0, vel = synth.sph_block(n=args.points, seed=1)
+    n = len(p0)
+    total = args.warmup + args.steps
+    poss = step_positions(p0, vel, total)
+
+    v = VolumeGVDB(0)
+    v.Configure(5, 3, 3, 3, 3)
+    v.SetChannelDefault(16, 16, 1)
+    for c in range(4):
+        v.AddChannel(c, apron=1)
+    dvel = v.to_device(vel)
+    dposs = [v.to_device(p) for p in poss]
+
+    for k in range(args.warmup):
+        run_step(v, dposs[k], dvel, n)
+    v.synchronize()
+
+    # timed region: K steps, kernel-family events recorded on the library stream
+    v.kernel_timer(args.kernel, True)
+    t0 = time.perf_counter()
+    lens = []
+    for k in range(args.warmup, total):
+        lens.append(run_step(v, dposs[k], dvel, n))
+    v.synchronize()
+    t1 = time.perf_counter()
+    k_ms, k_launches = v.kernel_timer_read()
+    v.kernel_timer(None, False)
+
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = n / (ms_per_step * 1e-3) / 1e6
+    nsc, s_len, _ = v.subcell_info()
+    bricks = v.getNumUsedNodes(0)
+    step_bytes = algorithmic_bytes(n, s_len, nsc, bricks)
+    kb = gather_bytes(s_len, nsc, bricks) if args.kernel == "gather" else None
+    k_avg_ms = k_ms / max(k_launches, 1)
+    achieved = kb / (k_avg_ms * 1e-3) / 1e9 if kb else None
+
+    out = {
+        "metric": "Mpoints/s insert+gather+apron (SPH step, 4M particles)",
+        "value": round(value, 2),
+        "unit": "Mpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered lattice, N(0,1) velocities)",
+        "config": {"workload": "sph_step_4M", "points": n, "box": 256, "fill": 0.4, "dt": 0.5,
+                   "radius": RADIUS, "subcell": SUBCELL, "log2dims": [3, 3, 3, 3, 5], "channels": 4,
+                   "apron": 1, "rebuild": "reference state machine (incremental after 2 full)",
+                   "bricks_last_step": int(bricks), "subcell_entries_last_step": int(s_len)},
+        "step_algorithmic_bytes": int(step_bytes),
+        "step_hbm_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": args.kernel,
+                     "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None, "avg_launch_ms": round(k_avg_ms, 5), "launches": int(k_launches),
+                     "algorithmic_bytes_per_launch": int(kb) if kb else None},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(poss[0], vel, min(args.cpu_sample, n))
+    v.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

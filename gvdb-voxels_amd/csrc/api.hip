@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "binning.hpp"
 #include "kernels.hpp"
 #include "primitives.hpp"
 
@@ -52,7 +53,7 @@ struct gvdbx_ctx {
   Buf chan[32];
   int64_t chan_slot_cap = 0;
   int64_t nslots = 0;
-  Buf leaf_slot, slot_leaf, slot_pos;
+  Buf leaf_slot, slot_leaf, slot_pos, nbr;
 
   const char* pos = nullptr;
   int pos_off = 0, pos_stride = 12;
@@ -65,7 +66,10 @@ struct gvdbx_ctx {
   int sc = 0, scdim = 0;
   int64_t nsc = 0, sclen = 0;
   bool lists_have_vel = false;
-  Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel;
+  bool store_vel_lists = false;
+  bool have_vsum = false;
+  Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
+  Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb;
 
   Buf keys, keys_alt, flags, rank, radix_counts, tile_buf, markers, meta;
   int* h_meta = nullptr;
@@ -239,9 +243,9 @@ int reserve_level(gvdbx_ctx* c, int l, int64_t add) {
 }
 
 int bounding_box(gvdbx_ctx* c, int64_t n, const float* tr) {  // GetBoundingBox (V.cpp:5404-5467)
-  bbox_init(P<int>(c->meta), c->stream);
+  CHK(ensure(c, c->bbox_part, (size_t)kBBoxParts * 6 * sizeof(float)));
   PointsView pv = points_of(c, n, tr);
-  bbox(pv, P<int>(c->meta), c->stream);
+  bbox_two_pass(pv, P<float>(c->bbox_part), P<int>(c->meta), c->stream);
   CHK(read_meta(c));
   for (int a = 0; a < 3; ++a) {
     float mn = ordered_to_float(c->h_meta[M_BBOX + a]);
@@ -586,9 +590,10 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
   if (!c) return GVDBX_ERR_ARG;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  Buf* bufs[] = {&c->leaf_slot, &c->slot_leaf, &c->slot_pos, &c->sc_map, &c->sc_cnt, &c->sc_off, &c->sc_pos,
-                 &c->sc_nid, &c->list_pos, &c->list_vel, &c->keys, &c->keys_alt, &c->flags, &c->rank,
-                 &c->radix_counts, &c->tile_buf, &c->markers, &c->meta};
+  Buf* bufs[] = {&c->leaf_slot, &c->slot_leaf, &c->slot_pos, &c->nbr, &c->sc_map, &c->sc_cnt, &c->sc_off, &c->sc_pos,
+                 &c->sc_nid, &c->list_pos, &c->list_vel, &c->map_leaf, &c->vsum, &c->hkeys, &c->hkeys_alt,
+                 &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->keys, &c->keys_alt, &c->flags,
+                 &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (int l = 0; l < kMaxLev; ++l) {
@@ -761,6 +766,8 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   assign_slots(T, P<int>(c->flags), P<int>(c->rank), total, c->apron, br, c->axis_cx, c->axis_cy, P<int>(c->leaf_slot),
                P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
   c->nslots = c->used0;
+  CHK(ensure(c, c->nbr, (size_t)(c->nslots * 27 + 1) * 4));
+  neighbor_table(T, c->nslots, P<int4>(c->slot_pos), P<int>(c->leaf_slot), res_of(c, 0), P<int>(c->nbr), c->stream);
   CHK(reserve_channels(c, c->nslots));
   c->atlas_version = c->topo_version;
   return GVDBX_OK;
@@ -771,6 +778,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   const int res0 = res_of(c, 0);
   if (sc <= 0 || (sc & (sc - 1)) || res0 % sc || sc * sc * sc > 512)
     return fail(c, GVDBX_ERR_ARG, "subcell size %d must be a power of two dividing %d", sc, res0);
+  if (!(r >= 0.0f) || r > 1.0e6f) return fail(c, GVDBX_ERR_ARG, "radius %g out of range", (double)r);
   if (len) *len = 0;
   if (c->used0 == 0) return GVDBX_OK;
   if (c->depth < 1) return fail(c, GVDBX_ERR_DEGENERATE, "no tree");
@@ -778,45 +786,116 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   const int64_t per = (int64_t)cfg.D * cfg.D * cfg.D;
   const int64_t total = c->cnt[0];
   const int64_t nsc = per * c->used0;
-  if (nsc > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "subcell count exceeds 2^31");
-  CHK(ensure(c, c->sc_map, (size_t)(total + 1) * 4));
-  CHK(ensure(c, c->flags, (size_t)(total + 1) * 4));
+  const int nmap = (int)c->used0;
+  if (nsc > INT32_MAX || total > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "subcell count exceeds 2^31");
   // arrays sized for every pool leaf (an upper bound of the linked leaves indexed by map)
   const int64_t nsc_cap = per * total;
+  CHK(ensure(c, c->sc_map, (size_t)(total + 1) * 4));
+  CHK(ensure(c, c->map_leaf, (size_t)(total + 1) * 4));
+  CHK(ensure(c, c->flags, (size_t)(total + 1) * 4));
   CHK(ensure(c, c->sc_cnt, (size_t)(nsc_cap + 1) * 4));
   CHK(ensure(c, c->sc_off, (size_t)(nsc_cap + 1) * 4));
   CHK(ensure(c, c->sc_pos, (size_t)(nsc_cap + 1) * 12));
   CHK(ensure(c, c->sc_nid, (size_t)(nsc_cap + 1) * 4));
-  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count((nsc > total ? nsc : total) + 1) + 16) * 4));
+  const bool with_vel = c->vel != nullptr;
+  if (with_vel) CHK(ensure(c, c->vsum, (size_t)(nsc_cap + 1) * 24));
+  CHK(ensure(c, c->hkeys, (size_t)n * 4));
+  CHK(ensure(c, c->hkeys_alt, (size_t)n * 4));
+  CHK(ensure(c, c->hvals, (size_t)n * 4));
+  CHK(ensure(c, c->hvals_alt, (size_t)n * 4));
+  CHK(ensure(c, c->ranges, (size_t)(total + 2) * 4));
+  const int64_t ncnt = radix_count_ints(n);
+  CHK(ensure(c, c->radix_counts, (size_t)ncnt * 4));
+  int64_t big = nsc_cap > total ? nsc_cap : total;
+  if (ncnt > big) big = ncnt;
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(big + 1) + 16) * 4));
   DevTree T = tree_of(c);
   PointsView pv = points_of(c, n, trans);
+
+  // subcell map (gvdbSetFlagSubcell + PrefixSum, P.cuh:109-118) and corners (P.cuh:677-694)
   leaf_flags(T, total, 1, P<int>(c->flags), c->stream);
   exclusive_scan(P<int>(c->flags), P<int>(c->sc_map), total, P<int>(c->tile_buf), nullptr, c->stream);
-  fill_u32(P<uint32_t>(c->sc_cnt), nsc_cap, 0u, c->stream);
+  fill_u32(P<uint32_t>(c->map_leaf), total + 1, 0xFFFFFFFFu, c->stream);
+  sc_positions(T, total, cfg, P<int>(c->sc_map), P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->map_leaf),
+               c->stream);
+
+  // sort points by home brick
+  uint32_t* skeys;
+  uint32_t* svals;
+  {
+    TimedRegion tr(c, "home");
+    home_keys(T, pv, P<int>(c->sc_map), nmap, P<uint32_t>(c->hkeys), P<uint32_t>(c->hvals), c->stream);
+    int bits = 1;
+    while (((int64_t)1 << bits) <= nmap) ++bits;
+    const int par = radix_sort_u32(P<uint32_t>(c->hkeys), P<uint32_t>(c->hkeys_alt), P<uint32_t>(c->hvals),
+                                   P<uint32_t>(c->hvals_alt), n, 0, bits, P<int>(c->radix_counts),
+                                   P<int>(c->tile_buf), c->stream);
+    skeys = par ? P<uint32_t>(c->hkeys_alt) : P<uint32_t>(c->hkeys);
+    svals = par ? P<uint32_t>(c->hvals_alt) : P<uint32_t>(c->hvals);
+    key_ranges(skeys, n, nmap, P<int>(c->ranges), c->stream);
+  }
+  const int W = cfg.D + 2;
+  const int* tail = P<int>(c->ranges) + nmap;  // points whose own leaf is missing, and invalid ones
+  CHK(ensure(c, c->cnt_in, (size_t)(nsc_cap + 1) * 4));
+  CHK(ensure(c, c->ocnt, (size_t)(nsc_cap + 1) * 4));
+  CHK(ensure(c, c->ocur, (size_t)(nsc_cap + 1) * 4));
+  CHK(ensure(c, c->wgb, (size_t)((int64_t)nmap * W * W * W + 1) * 4));
+  BinArgs A;
+  A.T = T;
+  A.P = pv;
+  A.c = cfg;
+  A.W = W;
+  A.map = P<int>(c->sc_map);
+  A.map_leaf = P<int>(c->map_leaf);
+  A.svals = svals;
+  A.start = P<int>(c->ranges);
+  A.cnt_in = P<int>(c->cnt_in);
+  A.ocnt = P<int>(c->ocnt);
+  A.ocur = P<int>(c->ocur);
+  A.wgb = P<int>(c->wgb);
+  A.off = P<int>(c->sc_off);
+  A.list_pos = nullptr;
+  A.list_vel = nullptr;
+  A.vsum = with_vel ? P<unsigned long long>(c->vsum) : nullptr;
+
+  fill_u32(P<uint32_t>(c->cnt_in), nsc, 0u, c->stream);
+  fill_u32(P<uint32_t>(c->ocnt), nsc, 0u, c->stream);
+  if (with_vel) fill_u64(P<uint64_t>(c->vsum), nsc * 3, 0ull, c->stream);
   {
     TimedRegion tr(c, "count");
-    sc_count(T, pv, cfg, P<int>(c->sc_map), P<int>(c->sc_cnt), c->stream);
+    bin_count(A, nmap, c->stream);
+    bin_points_count(A, tail, c->stream);
   }
+  sum_counts(P<int>(c->cnt_in), P<int>(c->ocnt), P<int>(c->sc_cnt), nsc, c->stream);
   CHK(zero_meta(c));
   exclusive_scan(P<int>(c->sc_cnt), P<int>(c->sc_off), nsc, P<int>(c->tile_buf), P<int>(c->meta) + M_TOTAL,
                  c->stream);
   CHK(read_meta(c));
   const int64_t ln = (uint32_t)c->h_meta[M_TOTAL];
+  if (ln > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length exceeds 2^31");
   c->sc = sc;
   c->scdim = cfg.D;
   c->nsc = nsc;
   c->sclen = ln;
   if (len) *len = ln;
-  c->lists_have_vel = c->vel != nullptr;
+  c->lists_have_vel = with_vel && c->store_vel_lists;
+  c->have_vsum = with_vel;
   CHK(ensure(c, c->list_pos, (size_t)(ln + 1) * 12));
-  if (c->vel) CHK(ensure(c, c->list_vel, (size_t)(ln + 1) * 12));
-  fill_u32(P<uint32_t>(c->sc_cnt), nsc, 0u, c->stream);
+  if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(ln + 1) * 12));
+  A.list_pos = P<float>(c->list_pos);
+  A.list_vel = c->lists_have_vel ? P<float>(c->list_vel) : nullptr;
+  fill_u32(P<uint32_t>(c->ocur), nsc, 0u, c->stream);
   {
     TimedRegion tr(c, "insert");
-    sc_insert(T, pv, cfg, P<int>(c->sc_map), P<int>(c->sc_off), P<int>(c->sc_cnt), P<float>(c->list_pos),
-              c->vel ? P<float>(c->list_vel) : nullptr, c->stream);
+    bin_insert(A, nmap, c->stream);
+    bin_points_insert(A, tail, c->stream);
   }
-  sc_positions(T, total, cfg, P<int>(c->sc_map), P<int>(c->sc_pos), P<int>(c->sc_nid), c->stream);
+  return GVDBX_OK;
+}
+
+int gvdbx_set_subcell_options(gvdbx_ctx* c, int store_velocity_lists) {
+  if (!c) return GVDBX_ERR_ARG;
+  c->store_vel_lists = store_velocity_lists != 0;
   return GVDBX_OK;
 }
 
@@ -825,8 +904,14 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
   if (c->nsc == 0) return GVDBX_OK;
   const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
   TimedRegion tr(c, "gather");
-  gather_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
-               P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), accum, mode, c->stream);
+  if (mode == 0 && !accum && cfg.res0 <= 8)
+    density_brick(cfg, (int)c->used0, c->nsc, c->sclen, P<int>(c->map_leaf), P<int>(c->sc_off),
+                  P<float>(c->list_pos), P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]), c->apron,
+                  P<float>(c->chan[chan]), c->stream);
+  else
+    gather_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
+                 P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), accum, mode,
+                 c->stream);
   return GVDBX_OK;
 }
 
@@ -850,13 +935,18 @@ int gvdbx_gather_velocity(gvdbx_ctx* c, int cx, int cy, int cz, int accum) {
   CHK(gather_checks(c, cx));
   CHK(chan_ok(c, cy));
   CHK(chan_ok(c, cz));
-  if (!c->lists_have_vel) return fail(c, GVDBX_ERR_STATE, "no velocities were bound at InsertPointsSubcell");
+  if (!c->have_vsum) return fail(c, GVDBX_ERR_STATE, "no velocities were bound at InsertPointsSubcell");
   if (c->nsc == 0) return GVDBX_OK;
   const SubcellCfg cfg = subcell_cfg(c, c->sc, 0.f);
   TimedRegion tr(c, "velocity");
-  gather_velocity(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
-                  P<float>(c->list_vel), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[cx]), P<float>(c->chan[cy]),
-                  P<float>(c->chan[cz]), accum, c->stream);
+  if ((int64_t)cfg.D * cfg.D * cfg.D <= 512)
+    velocity_brick(cfg, (int)c->used0, P<int>(c->map_leaf), P<int>(c->sc_cnt), P<unsigned long long>(c->vsum),
+                   P<int>(c->leaf_slot), c->apron, P<float>(c->chan[cx]), P<float>(c->chan[cy]),
+                   P<float>(c->chan[cz]), accum, c->stream);
+  else
+    velocity_subcell(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt),
+                     P<unsigned long long>(c->vsum), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[cx]),
+                     P<float>(c->chan[cy]), P<float>(c->chan[cz]), accum, c->stream);
   return GVDBX_OK;
 }
 
@@ -872,14 +962,27 @@ int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {
   return GVDBX_OK;
 }
 
-int gvdbx_update_apron(gvdbx_ctx* c, int chan, float boundval) {  // UpdateApron (V.cpp:4418-4457)
-  CHK(chan_ok(c, chan));
+int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* boundvals, int nchan) {
+  if (!c || !chans || nchan < 0) return GVDBX_ERR_ARG;  // UpdateApron (V.cpp:4411-4457)
+  for (int i = 0; i < nchan; ++i) CHK(chan_ok(c, chans[i]));
   CHK(atlas_current(c));
   if (c->apron == 0 || c->nslots == 0) return GVDBX_OK;
   TimedRegion tr(c, "apron");
-  update_apron(tree_of(c), c->nslots, P<int4>(c->slot_pos), P<int>(c->leaf_slot), res_of(c, 0), c->apron,
-               P<float>(c->chan[chan]), boundval, c->stream);
+  for (int i0 = 0; i0 < nchan; i0 += kMaxApronChannels) {
+    ApronChannels ac;
+    ac.n = 0;
+    for (int i = i0; i < nchan && ac.n < kMaxApronChannels; ++i) {
+      ac.ptr[ac.n] = P<float>(c->chan[chans[i]]);
+      ac.bound[ac.n] = boundvals ? boundvals[i] : 0.0f;
+      ac.n++;
+    }
+    apron_multi(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
+  }
   return GVDBX_OK;
+}
+
+int gvdbx_update_apron(gvdbx_ctx* c, int chan, float boundval) {
+  return gvdbx_update_apron_channels(c, &chan, &boundval, 1);
 }
 
 int gvdbx_sample_trilinear(gvdbx_ctx* c, int chan, const void* q, int64_t n, void* out) {

@@ -1,0 +1,669 @@
+// binning.hip -- subcell binning by home brick, and the gathers that read its results.
+//
+// Semantics restated from gvdbCountSubcell / gvdbInsertSubcell (P.cuh:218-315): a point q
+// enters every subcell whose corner lies in [trunc(q-r)/sc*sc, trunc(q+1+r)/sc*sc] and whose
+// leaf exists.  The reference issues one global atomic per (point, subcell) pair, in point
+// order.  Here the points are first sorted by home brick (LSD radix sort of the home key),
+// then one workgroup per brick counts / inserts the pairs of its own points into LDS counters
+// covering the brick's subcells plus a one-subcell shell, and flushes them with one global
+// atomic per touched subcell (at most 8 workgroups touch a subcell).  Velocity sums for the
+// subcell means are exact 64-bit fixed point, so their value is independent of order.
+#include <limits.h>
+
+#include "binning.hpp"
+#include "primitives.hpp"
+
+namespace gvdbx {
+
+static inline int nblocks(int64_t n, int bs) {
+  int64_t g = (n + bs - 1) / bs;
+  return (int)(g < 1 ? 1 : (g > 0x7FFFFFFF ? 0x7FFFFFFF : g));
+}
+
+constexpr float kVelScale = 16777216.0f;  // 2^24 fixed-point units
+__device__ __forceinline__ long long vel_fixed(float v) { return __float2ll_rn(v * kVelScale); }
+
+// Subcell corner range of a point (the exact FP32 operation order of P.cuh:238-243).
+__device__ __forceinline__ void subcell_range(const float q[3], float r, int sc, int lo[3], int hi[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float lq = q[a] - r;
+    float hq = q[a] + 1.0f;
+    hq = hq + r;
+    lo[a] = ((int)lq / sc) * sc;
+    hi[a] = ((int)hq / sc) * sc;
+  }
+}
+
+// ------------------------------------------------------------------ bounding box
+__global__ void __launch_bounds__(256) k_bbox_partial(PointsView P, float* __restrict__ part) {
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+    const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { mn[a] = fminf(mn[a], q[a]); mx[a] = fmaxf(mx[a], q[a]); }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mn[a] = fminf(mn[a], __shfl_xor(mn[a], o, 64));
+      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o, 64));
+    }
+  __shared__ float sm[4][6];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) { sm[w][a] = mn[a]; sm[w][3 + a] = mx[a]; }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    float v = sm[0][a];
+    for (int k = 1; k < 4; ++k) v = a < 3 ? fminf(v, sm[k][a]) : fmaxf(v, sm[k][a]);
+    part[blockIdx.x * 6 + a] = v;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_bbox_final(const float* __restrict__ part, int nparts, int* meta) {
+  const int lane = threadIdx.x;
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = lane; b < nparts; b += 64)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fminf(mn[a], part[b * 6 + a]);
+      mx[a] = fmaxf(mx[a], part[b * 6 + 3 + a]);
+    }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mn[a] = fminf(mn[a], __shfl_xor(mn[a], o, 64));
+      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o, 64));
+    }
+  if (lane < 3) meta[lane] = float_to_ordered(mn[lane]);
+  else if (lane < 6) meta[lane] = float_to_ordered(mx[lane - 3]);
+}
+
+void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s) {
+  const int nb = nblocks(P.n, 256) < kBBoxParts ? nblocks(P.n, 256) : kBBoxParts;
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, s, P, part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, part, nb, meta);
+}
+
+// ------------------------------------------------------------------ home keys
+// Key = subcell-map index of the leaf holding trunc(q); points that are invalid, NOHIT or
+// whose own leaf is missing get the sentinel nmap and are binned by the per-point path.
+__global__ void __launch_bounds__(256) k_home_keys(DevTree T, PointsView P, const int* __restrict__ map, int nmap,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+  uint32_t key = (uint32_t)nmap;
+  if (p.z != kNoHit) {
+    const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
+    if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
+      const int64_t lf = descend(T, (int)q[0], (int)q[1], (int)q[2], 0);
+      if (lf >= 0) key = (uint32_t)map[lf];
+    }
+  }
+  keys[i] = key;
+  vals[i] = (uint32_t)i;
+}
+
+void home_keys(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, uint32_t* vals,
+               hipStream_t s) {
+  hipLaunchKernelGGL(k_home_keys, dim3(nblocks(P.n, 256)), dim3(256), 0, s, T, P, map, nmap, keys, vals);
+}
+
+// start[k] = first sorted position with key >= k, for k in [0, nmap + 1].
+__global__ void k_ranges(const uint32_t* __restrict__ skeys, int64_t n, int nmap, int* __restrict__ start) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = (int)skeys[i];
+  const int kp = i == 0 ? -1 : (int)skeys[i - 1];
+  for (int kk = kp + 1; kk <= k; ++kk) start[kk] = (int)i;
+  if (i == n - 1)
+    for (int kk = k + 1; kk <= nmap + 1; ++kk) start[kk] = (int)n;
+}
+
+void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStream_t s) {
+  hipLaunchKernelGGL(k_ranges, dim3(nblocks(n, 256)), dim3(256), 0, s, skeys, n, nmap, start);
+}
+
+// ------------------------------------------------------------------ per-brick binning
+// Window: the brick's D^3 subcells plus one subcell on every side, W = D + 2 per axis; a point
+// of the brick reaches outside the window only when r + 1 > sc (those pairs go to global
+// memory by tree descent).  Counting: when D <= 2 every point adds itself once to a
+// (home subcell, box class) bin -- per axis its subcell range is [home + dl, home + dh] with
+// dl in {-1, 0}, dh in {0, 1}, 64 classes -- and each window cell then sums the bins whose box
+// covers it; otherwise pairs are counted one by one in LDS.  The brick's per-cell totals
+// reserve a contiguous range of each subcell list (wgb), so the insert needs no global atomics.
+
+struct BrickCtx {
+  int bx, by, bz, W, D, D3;
+  const int* nb;
+  // window cell of a corner, or -1 outside the window
+  __device__ __forceinline__ int window_of(int x, int y, int z, int lgsc) const {
+    const int sx = ((x - bx) >> lgsc) + 1, sy = ((y - by) >> lgsc) + 1, sz = ((z - bz) >> lgsc) + 1;
+    if ((unsigned)sx >= (unsigned)W || (unsigned)sy >= (unsigned)W || (unsigned)sz >= (unsigned)W) return -1;
+    return (sz * W + sy) * W + sx;
+  }
+  // global subcell of a window cell, or -1 when its brick does not exist
+  __device__ __forceinline__ int64_t global_of(int w) const {
+    const int sx = w % W - 1, sy = (w / W) % W - 1, sz = w / (W * W) - 1;
+    const int ox = sx < 0 ? -1 : (sx >= D ? 1 : 0), oy = sy < 0 ? -1 : (sy >= D ? 1 : 0),
+              oz = sz < 0 ? -1 : (sz >= D ? 1 : 0);
+    const int b = nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)];
+    if (b < 0) return -1;
+    return (int64_t)b * D3 + ((sz - oz * D) * D + (sy - oy * D)) * D + (sx - ox * D);
+  }
+};
+
+__device__ __forceinline__ int64_t direct_subcell(const BinArgs& A, int x, int y, int z) {
+  const int64_t lf = descend(A.T, x, y, z, 0);
+  if (lf < 0) return -1;
+  const int msk = A.c.res0 - 1, D = A.c.D;
+  return (int64_t)A.map[lf] * (D * D * D) + ((((z & msk) >> A.c.lgsc) * D + ((y & msk) >> A.c.lgsc)) * D) +
+         ((x & msk) >> A.c.lgsc);
+}
+
+__device__ __forceinline__ void add_vel_global(unsigned long long* vsum, int64_t g, const long long fv[3]) {
+  atomicAdd(&vsum[3 * g + 0], (unsigned long long)fv[0]);
+  atomicAdd(&vsum[3 * g + 1], (unsigned long long)fv[1]);
+  atomicAdd(&vsum[3 * g + 2], (unsigned long long)fv[2]);
+}
+
+// Loads the 27 neighbour subcell-map indices; returns false for an unused map entry.
+__device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, BrickCtx& B) {
+  const int leaf = A.map_leaf[m];
+  if (leaf < 0) return false;
+  const gvdbx_node* nd = A.T.nodes[0] + leaf;
+  B.bx = nd->pos[0];
+  B.by = nd->pos[1];
+  B.bz = nd->pos[2];
+  B.W = A.W;
+  B.D = A.c.D;
+  B.D3 = B.D * B.D * B.D;
+  B.nb = nb;
+  if (threadIdx.x < 27) {
+    const int t = threadIdx.x;
+    const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+    const int64_t lf = descend(A.T, B.bx + dx * A.c.res0, B.by + dy * A.c.res0, B.bz + dz * A.c.res0, 0);
+    nb[t] = lf < 0 ? -1 : A.map[lf];
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int use_bins) {
+  extern __shared__ unsigned long long smem64[];
+  __shared__ int nb[27];
+  BrickCtx B;
+  const int m = blockIdx.x;
+  if (!brick_setup(A, m, nb, B)) return;
+  const int W = A.W, W3 = W * W * W, D = B.D, D3 = B.D3;
+  const int NB = use_bins ? D3 * 64 : 0;
+  const bool vel = A.vsum != nullptr && A.P.vel != nullptr;
+  // layout: [cvs 3*W3 u64][bvs 3*NB u64][ccnt W3 int][bcnt NB int]
+  unsigned long long* cvs = smem64;
+  unsigned long long* bvs = smem64 + 3 * W3;
+  int* ccnt = (int*)(smem64 + 3 * W3 + 3 * NB);
+  int* bcnt = ccnt + W3;
+  for (int t = threadIdx.x; t < W3; t += blockDim.x) {
+    ccnt[t] = 0;
+    cvs[3 * t] = cvs[3 * t + 1] = cvs[3 * t + 2] = 0ull;
+  }
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    bcnt[t] = 0;
+    bvs[3 * t] = bvs[3 * t + 1] = bvs[3 * t + 2] = 0ull;
+  }
+  __syncthreads();
+  const SubcellCfg c = A.c;
+  const int beg = A.start[m], end = A.start[m + 1];
+  for (int j = beg + threadIdx.x; j < end; j += blockDim.x) {
+    const int64_t i = A.svals[j];
+    const float3 p = load_f3(A.P.pos, i, A.P.pos_off, A.P.pos_stride);
+    const float q[3] = {p.x + A.P.tx, p.y + A.P.ty, p.z + A.P.tz};
+    long long fv[3] = {0, 0, 0};
+    if (vel) {
+      const float3 v = load_f3(A.P.vel, i, A.P.vel_off, A.P.vel_stride);
+      fv[0] = vel_fixed(v.x);
+      fv[1] = vel_fixed(v.y);
+      fv[2] = vel_fixed(v.z);
+    }
+    int lo[3], hi[3];
+    subcell_range(q, c.r, c.sc, lo, hi);
+    if (use_bins) {
+      const int hx = ((int)q[0] - B.bx) >> c.lgsc, hy = ((int)q[1] - B.by) >> c.lgsc, hz = ((int)q[2] - B.bz) >> c.lgsc;
+      const int dlx = ((lo[0] - B.bx) >> c.lgsc) - hx, dhx = ((hi[0] - B.bx) >> c.lgsc) - hx;
+      const int dly = ((lo[1] - B.by) >> c.lgsc) - hy, dhy = ((hi[1] - B.by) >> c.lgsc) - hy;
+      const int dlz = ((lo[2] - B.bz) >> c.lgsc) - hz, dhz = ((hi[2] - B.bz) >> c.lgsc) - hz;
+      if (dlx >= -1 && dlx <= 0 && dhx >= 0 && dhx <= 1 && dly >= -1 && dly <= 0 && dhy >= 0 && dhy <= 1 &&
+          dlz >= -1 && dlz <= 0 && dhz >= 0 && dhz <= 1) {
+        const int cls = (((dlz + 1) + 2 * dhz) * 4 + ((dly + 1) + 2 * dhy)) * 4 + ((dlx + 1) + 2 * dhx);
+        const int bin = ((hz * D + hy) * D + hx) * 64 + cls;
+        atomicAdd(&bcnt[bin], 1);
+        if (vel) {
+          atomicAdd(&bvs[3 * bin + 0], (unsigned long long)fv[0]);
+          atomicAdd(&bvs[3 * bin + 1], (unsigned long long)fv[1]);
+          atomicAdd(&bvs[3 * bin + 2], (unsigned long long)fv[2]);
+        }
+        continue;
+      }
+    }
+    for (int x = lo[0]; x <= hi[0]; x += c.sc)
+      for (int y = lo[1]; y <= hi[1]; y += c.sc)
+        for (int z = lo[2]; z <= hi[2]; z += c.sc) {
+          const int w = B.window_of(x, y, z, c.lgsc);
+          if (w >= 0) {
+            atomicAdd(&ccnt[w], 1);
+            if (vel) {
+              atomicAdd(&cvs[3 * w + 0], (unsigned long long)fv[0]);
+              atomicAdd(&cvs[3 * w + 1], (unsigned long long)fv[1]);
+              atomicAdd(&cvs[3 * w + 2], (unsigned long long)fv[2]);
+            }
+          } else {
+            const int64_t g = direct_subcell(A, x, y, z);
+            if (g < 0) continue;
+            atomicAdd(&A.ocnt[g], 1);
+            if (vel) add_vel_global(A.vsum, g, fv);
+          }
+        }
+  }
+  __syncthreads();
+  // scatter every populated (home, class) bin into the window cells its box covers
+  for (int bin = threadIdx.x; bin < NB; bin += blockDim.x) {
+    const int bc = bcnt[bin];
+    if (!bc) continue;
+    const int h = bin >> 6, cls = bin & 63;
+    const int hx = h % D, hy = (h / D) % D, hz = h / (D * D);
+    const int cx = cls & 3, cy = (cls >> 2) & 3, cz = cls >> 4;
+    // class c = (dl + 1) + 2*dh  ->  range [h + dl, h + dh]
+    const int x0 = hx + (cx & 1) - 1, x1 = hx + (cx >> 1);
+    const int y0 = hy + (cy & 1) - 1, y1 = hy + (cy >> 1);
+    const int z0 = hz + (cz & 1) - 1, z1 = hz + (cz >> 1);
+    const unsigned long long v0 = vel ? bvs[3 * bin] : 0ull, v1 = vel ? bvs[3 * bin + 1] : 0ull,
+                             v2 = vel ? bvs[3 * bin + 2] : 0ull;
+    for (int z = z0; z <= z1; ++z)
+      for (int y = y0; y <= y1; ++y)
+        for (int x = x0; x <= x1; ++x) {
+          const int w = ((z + 1) * W + (y + 1)) * W + (x + 1);
+          atomicAdd(&ccnt[w], bc);
+          if (vel) {
+            atomicAdd(&cvs[3 * w + 0], v0);
+            atomicAdd(&cvs[3 * w + 1], v1);
+            atomicAdd(&cvs[3 * w + 2], v2);
+          }
+        }
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < W3; w += blockDim.x) {
+    const int n = ccnt[w];
+    const unsigned long long vs[3] = {cvs[3 * w], cvs[3 * w + 1], cvs[3 * w + 2]};
+    int base = -1;
+    if (n > 0) {
+      const int64_t g = B.global_of(w);
+      if (g >= 0) {
+        base = atomicAdd(&A.cnt_in[g], n);
+        if (vel) {
+          atomicAdd(&A.vsum[3 * g + 0], vs[0]);
+          atomicAdd(&A.vsum[3 * g + 1], vs[1]);
+          atomicAdd(&A.vsum[3 * g + 2], vs[2]);
+        }
+      }
+    }
+    A.wgb[(int64_t)m * W3 + w] = base;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_bin_insert(BinArgs A) {
+  extern __shared__ unsigned long long smem64[];
+  __shared__ int nb[27];
+  BrickCtx B;
+  const int m = blockIdx.x;
+  if (!brick_setup(A, m, nb, B)) return;
+  const int W = A.W, W3 = W * W * W;
+  int* lbase = (int*)smem64;
+  int* lcnt = lbase + W3;
+  __syncthreads();
+  for (int w = threadIdx.x; w < W3; w += blockDim.x) {
+    const int b = A.wgb[(int64_t)m * W3 + w];
+    lbase[w] = -1;
+    if (b >= 0) {
+      const int64_t g = B.global_of(w);
+      if (g >= 0) lbase[w] = A.off[g] + b;
+    }
+    lcnt[w] = 0;
+  }
+  __syncthreads();
+  const SubcellCfg c = A.c;
+  const int beg = A.start[m], end = A.start[m + 1];
+  for (int j = beg + threadIdx.x; j < end; j += blockDim.x) {
+    const int64_t i = A.svals[j];
+    const float3 p = load_f3(A.P.pos, i, A.P.pos_off, A.P.pos_stride);
+    const float q[3] = {p.x + A.P.tx, p.y + A.P.ty, p.z + A.P.tz};
+    float3 v = make_float3(0.f, 0.f, 0.f);
+    if (A.list_vel) v = load_f3(A.P.vel, i, A.P.vel_off, A.P.vel_stride);
+    int lo[3], hi[3];
+    subcell_range(q, c.r, c.sc, lo, hi);
+    for (int x = lo[0]; x <= hi[0]; x += c.sc)
+      for (int y = lo[1]; y <= hi[1]; y += c.sc)
+        for (int z = lo[2]; z <= hi[2]; z += c.sc) {
+          const int w = B.window_of(x, y, z, c.lgsc);
+          int64_t k;
+          if (w >= 0) {
+            const int base = lbase[w];
+            if (base < 0) continue;
+            k = (int64_t)base + atomicAdd(&lcnt[w], 1);
+          } else {
+            const int64_t g = direct_subcell(A, x, y, z);
+            if (g < 0) continue;
+            k = (int64_t)A.off[g] + A.cnt_in[g] + atomicAdd(&A.ocur[g], 1);
+          }
+          A.list_pos[3 * k + 0] = q[0];
+          A.list_pos[3 * k + 1] = q[1];
+          A.list_pos[3 * k + 2] = q[2];
+          if (A.list_vel) {
+            A.list_vel[3 * k + 0] = v.x;
+            A.list_vel[3 * k + 1] = v.y;
+            A.list_vel[3 * k + 2] = v.z;
+          }
+        }
+  }
+}
+
+int bin_uses_class_bins(int D) { return D <= 2 ? 1 : 0; }
+
+size_t bin_count_smem(int W, int D) {
+  const int W3 = W * W * W;
+  const int NB = bin_uses_class_bins(D) ? D * D * D * 64 : 0;
+  return (size_t)(W3 + NB) * 24 + (size_t)(W3 + NB) * 4 + 8;
+}
+
+void bin_count(const BinArgs& A, int nmap, hipStream_t s) {
+  if (nmap <= 0) return;
+  hipLaunchKernelGGL(k_bin_count, dim3(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A,
+                     bin_uses_class_bins(A.c.D));
+}
+
+void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
+  if (nmap <= 0) return;
+  const size_t sm = (size_t)2 * A.W * A.W * A.W * 4 + 8;
+  hipLaunchKernelGGL(k_bin_insert, dim3(nmap), dim3(256), sm, s, A);
+}
+
+__global__ void k_sum_counts(const int* __restrict__ a, const int* __restrict__ b, int* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
+
+void sum_counts(const int* a, const int* b, int* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sum_counts, dim3(nblocks(n, 256)), dim3(256), 0, s, a, b, out, n);
+}
+
+// ------------------------------------------------------------------ per-point path
+// Points from sorted position *begin on (the sentinel tail: points whose own leaf is missing,
+// and invalid ones) are binned one pair at a time with global atomics.
+template <bool kInsert>
+__global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __restrict__ begin) {
+  const int64_t b0 = *begin;
+  const SubcellCfg c = A.c;
+  const bool vel = A.vsum != nullptr && A.P.vel != nullptr;
+  for (int64_t j = b0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < A.P.n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = (int64_t)A.svals[j];
+    const float3 p = load_f3(A.P.pos, i, A.P.pos_off, A.P.pos_stride);
+    if (p.z == kNoHit) continue;
+    const float q[3] = {p.x + A.P.tx, p.y + A.P.ty, p.z + A.P.tz};
+    if (q[0] < 0 || q[1] < 0 || q[2] < 0) continue;
+    float3 v = make_float3(0.f, 0.f, 0.f);
+    long long fv[3] = {0, 0, 0};
+    if (A.P.vel && (vel || A.list_vel)) {
+      v = load_f3(A.P.vel, i, A.P.vel_off, A.P.vel_stride);
+      fv[0] = vel_fixed(v.x);
+      fv[1] = vel_fixed(v.y);
+      fv[2] = vel_fixed(v.z);
+    }
+    int lo[3], hi[3];
+    subcell_range(q, c.r, c.sc, lo, hi);
+    for (int x = lo[0]; x <= hi[0]; x += c.sc)
+      for (int y = lo[1]; y <= hi[1]; y += c.sc)
+        for (int z = lo[2]; z <= hi[2]; z += c.sc) {
+          const int64_t g = direct_subcell(A, x, y, z);
+          if (g < 0) continue;
+          if (!kInsert) {
+            atomicAdd(&A.ocnt[g], 1);
+            if (vel) add_vel_global(A.vsum, g, fv);
+            continue;
+          }
+          const int64_t k = (int64_t)A.off[g] + A.cnt_in[g] + atomicAdd(&A.ocur[g], 1);
+          A.list_pos[3 * k + 0] = q[0];
+          A.list_pos[3 * k + 1] = q[1];
+          A.list_pos[3 * k + 2] = q[2];
+          if (A.list_vel) {
+            A.list_vel[3 * k + 0] = v.x;
+            A.list_vel[3 * k + 1] = v.y;
+            A.list_vel[3 * k + 2] = v.z;
+          }
+        }
+  }
+}
+
+void bin_points_count(const BinArgs& A, const int* begin, hipStream_t s) {
+  const int nb = nblocks(A.P.n, 256) < 2048 ? nblocks(A.P.n, 256) : 2048;
+  hipLaunchKernelGGL(k_bin_points<false>, dim3(nb), dim3(256), 0, s, A, begin);
+}
+
+void bin_points_insert(const BinArgs& A, const int* begin, hipStream_t s) {
+  const int nb = nblocks(A.P.n, 256) < 2048 ? nblocks(A.P.n, 256) : 2048;
+  hipLaunchKernelGGL(k_bin_points<true>, dim3(nb), dim3(256), 0, s, A, begin);
+}
+
+// ------------------------------------------------------------------ gathers
+
+__device__ __forceinline__ int64_t atlas_index(int slot, int br, int lx, int ly, int lz) {
+  return (((int64_t)slot * br + lz) * br + ly) * br + lx;
+}
+
+// GatherDensity without accumulation (P.cuh:696-733): val = max(0, max_p (r - |p - v|)) over
+// the points listed for the voxel's subcell.  Only points within r of a voxel can raise it
+// above 0, so each list entry visits the voxels of its own subcell inside its r-ball (a
+// 0.01-voxel margin keeps rounding at the ball edge inclusive) and min-reduces |p-v|^2 in
+// LDS.  One workgroup per brick: the brick's D^3 subcell lists are contiguous, and the brick
+// interior is written with row-contiguous stores.
+__global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int64_t nsc, int64_t len,
+                                                      const int* __restrict__ map_leaf, const int* __restrict__ off,
+                                                      const float* __restrict__ list_pos,
+                                                      const int* __restrict__ leaf_slot,
+                                                      const gvdbx_node* __restrict__ nodes0, int apron,
+                                                      float* __restrict__ chan) {
+  __shared__ unsigned int d2[512];
+  __shared__ int soff[513];
+  const int m = blockIdx.x;
+  const int leaf = map_leaf[m];
+  if (leaf < 0) return;
+  const int D = c.D, D3 = D * D * D, res0 = c.res0, nv = res0 * res0 * res0;
+  const int64_t k0 = (int64_t)m * D3;
+  for (int t = threadIdx.x; t < nv; t += blockDim.x) d2[t] = 0x7F800000u;
+  for (int s = threadIdx.x; s <= D3; s += blockDim.x) soff[s] = (k0 + s < nsc) ? off[k0 + s] : (int)len;
+  __syncthreads();
+  const int bx = nodes0[leaf].pos[0], by = nodes0[leaf].pos[1], bz = nodes0[leaf].pos[2];
+  const float rr = c.r + 0.01f;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int s = wave; s < D3; s += nw) {  // one wave per subcell: its list entries over the lanes
+    const int sx0 = (s % D) * c.sc, sy0 = ((s / D) % D) * c.sc, sz0 = (s / (D * D)) * c.sc;
+    for (int j = soff[s] + lane; j < soff[s + 1]; j += 64) {
+      const float gx = list_pos[3 * (int64_t)j], gy = list_pos[3 * (int64_t)j + 1], gz = list_pos[3 * (int64_t)j + 2];
+      const float px = gx - (float)bx, py = gy - (float)by, pz = gz - (float)bz;
+      const int x0 = max(sx0, (int)ceilf(px - rr)), x1 = min(sx0 + c.sc - 1, (int)floorf(px + rr));
+      const int y0 = max(sy0, (int)ceilf(py - rr)), y1 = min(sy0 + c.sc - 1, (int)floorf(py + rr));
+      const int z0 = max(sz0, (int)ceilf(pz - rr)), z1 = min(sz0 + c.sc - 1, (int)floorf(pz + rr));
+      for (int z = z0; z <= z1; ++z)
+        for (int y = y0; y <= y1; ++y)
+          for (int x = x0; x <= x1; ++x) {
+            const float d = dist2(gx - (float)(bx + x), gy - (float)(by + y), gz - (float)(bz + z));
+            atomicMin(&d2[(z * res0 + y) * res0 + x], __float_as_uint(d));
+          }
+    }
+  }
+  __syncthreads();
+  const int slot = leaf_slot[leaf];
+  if (slot < 0) return;
+  const int br = res0 + 2 * apron;
+  float* b = chan + (int64_t)slot * br * br * br;
+  for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+    const int x = t % res0, y = (t / res0) % res0, z = t / (res0 * res0);
+    const unsigned int u = d2[t];
+    const float val = u == 0x7F800000u ? 0.0f : fmaxf(0.0f, c.r - sqrtf(__uint_as_float(u)));
+    b[((z + apron) * br + (y + apron)) * br + (x + apron)] = val;
+  }
+}
+
+void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
+                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, float* chan,
+                   hipStream_t s) {
+  if (nmap <= 0) return;
+  hipLaunchKernelGGL(k_density_brick, dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off, list_pos, leaf_slot,
+                     nodes0, apron, chan);
+}
+
+// Velocity means from the fixed-point sums: (accum ? old : 0) + sum, over (cnt + accum); the
+// colour rule of P.cuh:717-731 for float3; an empty subcell without accumulation gives 0.
+// One workgroup per brick; the means are formed once per subcell, then broadcast.
+__global__ void __launch_bounds__(256) k_velocity_brick(SubcellCfg c, const int* __restrict__ map_leaf,
+                                                       const int* __restrict__ cnt,
+                                                       const unsigned long long* __restrict__ vsum,
+                                                       const int* __restrict__ leaf_slot, int apron,
+                                                       float* __restrict__ cx, float* __restrict__ cy,
+                                                       float* __restrict__ cz, int accum) {
+  __shared__ double mean[512][3];
+  __shared__ double den_s[512];
+  const int m = blockIdx.x;
+  const int leaf = map_leaf[m];
+  if (leaf < 0) return;
+  const int slot = leaf_slot[leaf];
+  if (slot < 0) return;
+  const int D = c.D, D3 = D * D * D, res0 = c.res0, nv = res0 * res0 * res0;
+  const int64_t k0 = (int64_t)m * D3;
+  for (int s = threadIdx.x; s < D3; s += blockDim.x) {
+    const long long den = (long long)cnt[k0 + s] + (accum ? 1 : 0);
+    den_s[s] = (double)den;
+    for (int a = 0; a < 3; ++a) mean[s][a] = (double)(long long)vsum[3 * (k0 + s) + a] * (1.0 / kVelScale);
+  }
+  __syncthreads();
+  const int br = res0 + 2 * apron;
+  const int64_t base = (int64_t)slot * br * br * br;
+  float* ch[3] = {cx, cy, cz};
+  for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+    const int x = t % res0, y = (t / res0) % res0, z = t / (res0 * res0);
+    const int s = ((z >> c.lgsc) * D + (y >> c.lgsc)) * D + (x >> c.lgsc);
+    const int64_t vi = base + ((z + apron) * br + (y + apron)) * br + (x + apron);
+    const double den = den_s[s];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double v = mean[s][a];
+      if (accum) v += (double)ch[a][vi];
+      ch[a][vi] = den > 0 ? (float)(v / den) : 0.0f;
+    }
+  }
+}
+
+void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* vsum,
+                    const int* leaf_slot, int apron, float* cx, float* cy, float* cz, int accum, hipStream_t s) {
+  if (nmap <= 0) return;
+  hipLaunchKernelGGL(k_velocity_brick, dim3(nmap), dim3(256), 0, s, c, map_leaf, cnt, vsum, leaf_slot, apron, cx, cy,
+                     cz, accum);
+}
+
+// Per-subcell form of the velocity means, for bricks with more than 512 subcells.
+__global__ void k_velocity_subcell(SubcellCfg c, const int* __restrict__ sc_pos, const int* __restrict__ sc_nid,
+                                   const int* __restrict__ cnt, const unsigned long long* __restrict__ vsum,
+                                   const int* __restrict__ leaf_slot, int apron, float* __restrict__ cx,
+                                   float* __restrict__ cy, float* __restrict__ cz, int accum) {
+  const int64_t k = blockIdx.x;
+  const int slot = leaf_slot[sc_nid[k]];
+  if (slot < 0) return;
+  const double den = (double)((long long)cnt[k] + (accum ? 1 : 0));
+  double sum[3];
+  for (int a = 0; a < 3; ++a) sum[a] = (double)(long long)vsum[3 * k + a] * (1.0 / kVelScale);
+  const int br = c.res0 + 2 * apron, msk = c.res0 - 1, nv = c.sc * c.sc * c.sc;
+  const int bx = sc_pos[3 * k], by = sc_pos[3 * k + 1], bz = sc_pos[3 * k + 2];
+  float* ch[3] = {cx, cy, cz};
+  for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+    const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
+    const int64_t vi = (((int64_t)slot * br + (((bz + tz) & msk) + apron)) * br + (((by + ty) & msk) + apron)) * br +
+                       (((bx + tx) & msk) + apron);
+    for (int a = 0; a < 3; ++a) {
+      double v = sum[a];
+      if (accum) v += (double)ch[a][vi];
+      ch[a][vi] = den > 0 ? (float)(v / den) : 0.0f;
+    }
+  }
+}
+
+void velocity_subcell(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
+                      const unsigned long long* vsum, const int* leaf_slot, int apron, float* cx, float* cy,
+                      float* cz, int accum, hipStream_t s) {
+  if (nsc <= 0) return;
+  const int nv = c.sc * c.sc * c.sc;
+  hipLaunchKernelGGL(k_velocity_subcell, dim3((unsigned)nsc), dim3(nv < 64 ? 64 : nv), 0, s, c, sc_pos, sc_nid, cnt,
+                     vsum, leaf_slot, apron, cx, cy, cz, accum);
+}
+
+// ------------------------------------------------------------------ apron
+// UpdateApron<T> (O.cuh:72-126): each apron voxel of each active brick copies the interior
+// voxel of the neighbour brick at the same world position, or boundval.  Neighbour slots
+// come from a table built once per topology (the reference resolves them per voxel by a
+// full tree descent); one launch covers several channels.
+__global__ void k_neighbor_table(DevTree T, int64_t nslots, const int4* __restrict__ slot_pos,
+                                 const int* __restrict__ leaf_slot, int res0, int* __restrict__ nbr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots * 27) return;
+  const int64_t s = i / 27;
+  const int k = (int)(i % 27);
+  const int4 bp = slot_pos[s];
+  const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+  const int64_t lf = descend(T, bp.x + dx * res0, bp.y + dy * res0, bp.z + dz * res0, 0);
+  nbr[i] = lf < 0 ? -1 : leaf_slot[lf];
+}
+
+void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int* nbr,
+                    hipStream_t s) {
+  if (nslots <= 0) return;
+  hipLaunchKernelGGL(k_neighbor_table, dim3(nblocks(nslots * 27, 256)), dim3(256), 0, s, T, nslots, slot_pos,
+                     leaf_slot, res0, nbr);
+}
+
+__global__ void __launch_bounds__(256) k_apron_multi(int64_t nslots, const int* __restrict__ nbr, int res0, int apron,
+                                                    ApronChannels ch) {
+  __shared__ int nb[27];
+  const int64_t s = blockIdx.x;
+  if (threadIdx.x < 27) nb[threadIdx.x] = nbr[s * 27 + threadIdx.x];
+  __syncthreads();
+  const int br = res0 + 2 * apron;
+  const int nvox = br * br * br;
+  const int64_t bv = (int64_t)br * br * br;
+  for (int t = threadIdx.x; t < nvox; t += blockDim.x) {
+    const int x = t % br, y = (t / br) % br, z = t / (br * br);
+    const int gx = x - apron, gy = y - apron, gz = z - apron;
+    const int ox = gx < 0 ? -1 : (gx >= res0 ? 1 : 0), oy = gy < 0 ? -1 : (gy >= res0 ? 1 : 0),
+              oz = gz < 0 ? -1 : (gz >= res0 ? 1 : 0);
+    if ((ox | oy | oz) == 0) continue;
+    const int ns = nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)];
+    const int64_t src = ns < 0 ? -1 : (int64_t)ns * bv +
+                                          (((gz - oz * res0 + apron) * br) + (gy - oy * res0 + apron)) * br +
+                                          (gx - ox * res0 + apron);
+    const int64_t dst = s * bv + t;
+    for (int c = 0; c < ch.n; ++c) {
+      float* p = ch.ptr[c];
+      p[dst] = src < 0 ? ch.bound[c] : p[src];
+    }
+  }
+}
+
+void apron_multi(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s) {
+  if (nslots <= 0 || apron <= 0 || ch.n <= 0) return;
+  hipLaunchKernelGGL(k_apron_multi, dim3((unsigned)nslots), dim3(256), 0, s, nslots, nbr, res0, apron, ch);
+}
+
+}  // namespace gvdbx

@@ -1,0 +1,61 @@
+// binning.hpp -- launchers of the home-brick subcell binning and its gathers (binning.hip).
+#pragma once
+#include "kernels.hpp"
+
+namespace gvdbx {
+
+constexpr int kBBoxParts = 1024;
+
+// two-pass bounding box: per-block partials, then one block writes meta[0..5] (ordered ints)
+void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s);
+
+void home_keys(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, uint32_t* vals,
+               hipStream_t s);
+void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStream_t s);
+
+struct BinArgs {
+  DevTree T;
+  PointsView P;
+  SubcellCfg c;
+  int W;
+  const int* map;
+  const int* map_leaf;
+  const uint32_t* svals;
+  const int* start;
+  int* cnt_in;   // per subcell: pairs placed through brick windows
+  int* ocnt;     // per subcell: pairs placed one by one (outside windows, orphans)
+  int* ocur;     // insert cursor of the one-by-one pairs
+  int* wgb;      // [nmap][W^3]: this brick's offset inside cnt_in, or -1
+  const int* off;
+  float* list_pos;
+  float* list_vel;
+  unsigned long long* vsum;
+};
+
+void bin_count(const BinArgs& A, int nmap, hipStream_t s);
+void bin_insert(const BinArgs& A, int nmap, hipStream_t s);
+void bin_points_count(const BinArgs& A, const int* begin, hipStream_t s);
+void bin_points_insert(const BinArgs& A, const int* begin, hipStream_t s);
+void sum_counts(const int* a, const int* b, int* out, int64_t n, hipStream_t s);
+
+void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
+                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, float* chan,
+                   hipStream_t s);
+void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* vsum,
+                    const int* leaf_slot, int apron, float* cx, float* cy, float* cz, int accum, hipStream_t s);
+
+void velocity_subcell(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
+                      const unsigned long long* vsum, const int* leaf_slot, int apron, float* cx, float* cy,
+                      float* cz, int accum, hipStream_t s);
+
+constexpr int kMaxApronChannels = 8;
+struct ApronChannels {
+  int n;
+  float* ptr[kMaxApronChannels];
+  float bound[kMaxApronChannels];
+};
+void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int* nbr,
+                    hipStream_t s);
+void apron_multi(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s);
+
+}  // namespace gvdbx

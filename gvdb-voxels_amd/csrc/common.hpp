@@ -58,6 +58,10 @@ __device__ __forceinline__ int64_t descend(const DevTree& T, int x, int y, int z
   return cur;
 }
 
+// |d|^2 with a fixed operation order, so every kernel that reduces distances (gathers,
+// scatter-reduce) rounds identically.
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
+
 // ---- wave64 helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }

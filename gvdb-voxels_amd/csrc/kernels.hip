@@ -17,49 +17,6 @@ static inline int blocks_for(int64_t n, int bs, int64_t cap = 0x7FFFFFFF) {
   return (int)g;
 }
 
-// ================================================================ bounding box
-// GetBoundingBox (V.cpp:5404-5467): min/max of p + trans over all points, as six
-// ordered-int atomics (one pass instead of split + six Thrust reductions).
-
-__global__ void k_bbox_init(int* meta) {
-  if (threadIdx.x < 3) meta[threadIdx.x] = INT_MAX;
-  else if (threadIdx.x < 6) meta[threadIdx.x] = INT_MIN;
-}
-
-__global__ void __launch_bounds__(256) k_bbox(PointsView P, int* meta) {
-  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * blockDim.x) {
-    float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
-    float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { mn[a] = fminf(mn[a], q[a]); mx[a] = fmaxf(mx[a], q[a]); }
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      mn[a] = fminf(mn[a], __shfl_xor(mn[a], o, 64));
-      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o, 64));
-    }
-  __shared__ float sm[4][6];
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-    for (int a = 0; a < 3; ++a) { sm[w][a] = mn[a]; sm[w][3 + a] = mx[a]; }
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    const int a = threadIdx.x;
-    float v = sm[0][a];
-    for (int k = 1; k < 4; ++k) v = a < 3 ? fminf(v, sm[k][a]) : fmaxf(v, sm[k][a]);
-    if (a < 3) atomicMin(&meta[a], float_to_ordered(v));
-    else atomicMax(&meta[a], float_to_ordered(v));
-  }
-}
-
-void bbox_init(int* meta, hipStream_t s) { hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, s, meta); }
-void bbox(const PointsView& P, int* meta, hipStream_t s) {
-  hipLaunchKernelGGL(k_bbox, dim3(blocks_for(P.n, 256, 2048)), dim3(256), 0, s, P, meta);
-}
-
 // ================================================================ keys
 
 template <typename K>
@@ -97,6 +54,10 @@ void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, K* keys, hipStr
 
 // Box enumeration of gvdbCalcExtraBrickId (P.cuh:467-512, hi = trunc(q) + 2r) and
 // gvdbCalcIncreExtraBrickId (P.cuh:420-465, hi = trunc(q) + r, marks found leaves).
+// The reference walks the box once per level and descends for every node.  The level-l box
+// nodes are exactly the level-l ancestors of the leaf-level box nodes, so here each leaf-level
+// node is descended once: a found leaf is marked, and when the descent stops at level l the
+// missing chain (levels l-1 .. 0) is emitted (duplicates vanish in the unique step).
 // Nodes outside the root are not emitted.  Emission past cap only counts.
 template <typename K>
 __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFmt f, float r, float hi_mul,
@@ -107,35 +68,38 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
   float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
   const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
   if (q[0] < 0 || q[1] < 0 || q[2] < 0) return;
-  int bmin[3], bmax[3];
+  int lo[3], hi[3];
+  const int R0 = 1 << T.lgR[0];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float tq = (float)(int)q[a];
-    bmin[a] = (int)(tq - r);
-    bmax[a] = (int)(tq + hi_mul * r);
+    lo[a] = (int)(tq - r) / R0 * R0;
+    hi[a] = (int)(tq + hi_mul * r) / R0 * R0;
   }
   const int d = T.depth;
-  for (int l = 0; l < d; ++l) {
-    const int R = 1 << T.lgR[l];
-    int lo[3], hi[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { lo[a] = bmin[a] / R * R; hi[a] = bmax[a] / R * R; }
-    for (int x = lo[0]; x <= hi[0]; x += R)
-      for (int y = lo[1]; y <= hi[1]; y += R)
-        for (int z = lo[2]; z <= hi[2]; z += R) {
-          if (!in_root(T, x, y, z)) continue;
-          const int64_t nd = descend(T, x, y, z, l);
-          if (nd < 0) {
-            const int sh = T.lgR[l];
-            const int k = atomicAdd(counter, 1);
-            if (k < cap)
-              keys[k] = pack_key<K>(f, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
-                                    (z >> sh) - (T.root[2] >> sh));
-          } else if (l == 0 && markers) {
-            markers[nd] = 1;
-          }
+  for (int x = lo[0]; x <= hi[0]; x += R0)
+    for (int y = lo[1]; y <= hi[1]; y += R0)
+      for (int z = lo[2]; z <= hi[2]; z += R0) {
+        if (!in_root(T, x, y, z)) continue;
+        int64_t cur = 0;
+        int miss = -1;  // highest missing level along the chain
+        for (int l = d; l > 0; --l) {
+          const uint64_t e = T.clist[l][((uint64_t)cur << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, x, y, z)];
+          if (e == kUndef64) { miss = l - 1; break; }
+          cur = (int64_t)(e >> 16);
         }
-  }
+        if (miss < 0) {
+          if (markers && markers[cur] == 0) markers[cur] = 1;
+          continue;
+        }
+        for (int l = miss; l >= 0; --l) {
+          const int sh = T.lgR[l];
+          const int k = atomicAdd(counter, 1);
+          if (k < cap)
+            keys[k] = pack_key<K>(f, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
+                                  (z >> sh) - (T.root[2] >> sh));
+        }
+      }
 }
 
 template <typename K>
@@ -354,71 +318,16 @@ void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t c
 }
 
 // ================================================================ subcells
-// gvdbCountSubcell / gvdbInsertSubcell (P.cuh:275-315, 218-273): every subcell corner from
-// trunc(q-r)/sc*sc to trunc(q+1+r)/sc*sc; leaf by descent; idx = map[leaf]*D^3 + local.
-
-template <bool kInsert>
-__global__ void __launch_bounds__(256) k_subcell(DevTree T, PointsView P, SubcellCfg c, const int* __restrict__ map,
-                                                 int* __restrict__ cnt, const int* __restrict__ off,
-                                                 float* __restrict__ list_pos, float* __restrict__ list_vel) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
-  if (p.z == kNoHit) return;
-  const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
-  if (q[0] < 0 || q[1] < 0 || q[2] < 0) return;
-  float3 v = make_float3(0.f, 0.f, 0.f);
-  if (kInsert && list_vel) v = load_f3(P.vel, i, P.vel_off, P.vel_stride);
-  int lo[3], hi[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float lq = q[a] - c.r;
-    float hq = q[a] + 1.0f;
-    hq = hq + c.r;
-    lo[a] = ((int)lq / c.sc) * c.sc;
-    hi[a] = ((int)hq / c.sc) * c.sc;
-  }
-  const int m = c.res0 - 1;
-  for (int x = lo[0]; x <= hi[0]; x += c.sc)
-    for (int y = lo[1]; y <= hi[1]; y += c.sc)
-      for (int z = lo[2]; z <= hi[2]; z += c.sc) {
-        const int64_t leaf = descend(T, x, y, z, 0);
-        if (leaf < 0) continue;
-        const int lx = (x & m) >> c.lgsc, ly = (y & m) >> c.lgsc, lz = (z & m) >> c.lgsc;
-        const int64_t idx = (int64_t)map[leaf] * (c.D * c.D * c.D) + (lz * c.D + ly) * c.D + lx;
-        if (!kInsert) {
-          atomicAdd(&cnt[idx], 1);
-        } else {
-          const int64_t k = (int64_t)off[idx] + atomicAdd(&cnt[idx], 1);
-          list_pos[3 * k + 0] = q[0];
-          list_pos[3 * k + 1] = q[1];
-          list_pos[3 * k + 2] = q[2];
-          if (list_vel) {
-            list_vel[3 * k + 0] = v.x;
-            list_vel[3 * k + 1] = v.y;
-            list_vel[3 * k + 2] = v.z;
-          }
-        }
-      }
-}
-
-void sc_count(const DevTree& T, const PointsView& P, SubcellCfg c, const int* map, int* cnt, hipStream_t s) {
-  hipLaunchKernelGGL(k_subcell<false>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, c, map, cnt, nullptr,
-                     nullptr, nullptr);
-}
-void sc_insert(const DevTree& T, const PointsView& P, SubcellCfg c, const int* map, const int* off, int* cursor,
-               float* list_pos, float* list_vel, hipStream_t s) {
-  hipLaunchKernelGGL(k_subcell<true>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, c, map, cursor, off,
-                     list_pos, list_vel);
-}
+// (count / insert live in binning.hip)
 
 // gvdbCalcSubcellPos (P.cuh:677-694).
 __global__ void k_sc_positions(DevTree T, int64_t leaves, SubcellCfg c, const int* __restrict__ map,
-                               int* __restrict__ sc_pos, int* __restrict__ sc_nid) {
+                               int* __restrict__ sc_pos, int* __restrict__ sc_nid, int* __restrict__ map_leaf) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= leaves) return;
   const gvdbx_node* nd = T.nodes[0] + i;
   if (nd->parent == kUndef64) return;
+  map_leaf[map[i]] = (int)i;
   const int per = c.D * c.D * c.D;
   for (int s = 0; s < per; ++s) {
     const int64_t k = (int64_t)map[i] * per + s;
@@ -429,10 +338,10 @@ __global__ void k_sc_positions(DevTree T, int64_t leaves, SubcellCfg c, const in
   }
 }
 void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map, int* sc_pos, int* sc_nid,
-                  hipStream_t s) {
+                  int* map_leaf, hipStream_t s) {
   if (leaves <= 0) return;
   hipLaunchKernelGGL(k_sc_positions, dim3(blocks_for(leaves, 256)), dim3(256), 0, s, T, leaves, c, map, sc_pos,
-                     sc_nid);
+                     sc_nid, map_leaf);
 }
 
 // ================================================================ gathers
@@ -470,7 +379,7 @@ __global__ void __launch_bounds__(512) k_gather_field(SubcellCfg c, const int* _
     const int lim = min(nt, n - base);
     for (int jj = 0; jj < lim; ++jj) {
       const float dx = sx[jj] - vx, dy = sy[jj] - vy, dz = sz[jj] - vz;
-      md2 = fminf(md2, dx * dx + dy * dy + dz * dz);
+      md2 = fminf(md2, dist2(dx, dy, dz));
     }
     __syncthreads();
   }
@@ -496,58 +405,6 @@ void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_ni
   if (nsc <= 0) return;
   hipLaunchKernelGGL(k_gather_field, dim3((unsigned)nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, sc_pos, sc_nid, cnt, off,
                      list_pos, leaf_slot, apron, chan, accum, mode);
-}
-
-// Velocity mean per subcell: (accum ? old : 0) + sum(v), divided by (cnt + accum); the
-// colour rule of P.cuh:717-731 for float3.  Sums are exact int64 in 2^-24 units.
-__global__ void __launch_bounds__(512) k_gather_velocity(SubcellCfg c, const int* __restrict__ sc_pos,
-                                                         const int* __restrict__ sc_nid, const int* __restrict__ cnt,
-                                                         const int* __restrict__ off,
-                                                         const float* __restrict__ list_vel,
-                                                         const int* __restrict__ leaf_slot, int apron,
-                                                         float* __restrict__ cx, float* __restrict__ cy,
-                                                         float* __restrict__ cz, int accum) {
-  __shared__ long long part[8][3];
-  const int k = blockIdx.x;
-  const int nt = c.sc * c.sc * c.sc;
-  const int t = threadIdx.x;
-  const int n = cnt[k], o = off[k];
-  long long s[3] = {0, 0, 0};
-  for (int j = t; j < n; j += nt)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) s[a] += __float2ll_rn(list_vel[3 * (int64_t)(o + j) + a] * 16777216.0f);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) s[a] = wave_sum(s[a]);
-  const int nw = (nt + 63) / 64;
-  if ((t & 63) == 0)
-    for (int a = 0; a < 3; ++a) part[t >> 6][a] = s[a];
-  __syncthreads();
-  long long tot[3] = {0, 0, 0};
-  for (int w = 0; w < nw; ++w)
-    for (int a = 0; a < 3; ++a) tot[a] += part[w][a];
-  const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
-  const int br = c.res0 + 2 * apron;
-  const int m = c.res0 - 1;
-  const int slot = leaf_slot[sc_nid[k]];
-  if (slot < 0) return;
-  const int64_t vi = atlas_voxel(slot, br, ((sc_pos[3 * k] + tx) & m) + apron, ((sc_pos[3 * k + 1] + ty) & m) + apron,
-                                 ((sc_pos[3 * k + 2] + tz) & m) + apron);
-  const long long den = (long long)n + (accum ? 1 : 0);
-  float* ch[3] = {cx, cy, cz};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    double sum = (double)tot[a] * (1.0 / 16777216.0);
-    if (accum) sum += (double)ch[a][vi];
-    ch[a][vi] = den > 0 ? (float)(sum / (double)den) : 0.0f;
-  }
-}
-
-void gather_velocity(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
-                     const float* list_vel, const int* leaf_slot, int apron, float* cx, float* cy, float* cz,
-                     int accum, hipStream_t s) {
-  if (nsc <= 0) return;
-  hipLaunchKernelGGL(k_gather_velocity, dim3((unsigned)nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, sc_pos, sc_nid, cnt,
-                     off, list_vel, leaf_slot, apron, cx, cy, cz, accum);
 }
 
 // ScatterReduceLevelSet: init the subcell voxels, then every (point, subcell) pair
@@ -585,7 +442,7 @@ __global__ void __launch_bounds__(256) k_sr_scatter(SubcellCfg c, int64_t nsc, c
       for (int ty = 0; ty < c.sc; ++ty)
         for (int tx = 0; tx < c.sc; ++tx) {
           const float dx = px - (float)(bx + tx), dy = py - (float)(by + ty), dz = pz - (float)(bz + tz);
-          const float w = sqrtf(dx * dx + dy * dy + dz * dz) - c.r;
+          const float w = sqrtf(dist2(dx, dy, dz)) - c.r;
           atomicMin(&ci[atlas_voxel(slot, br, ((bx + tx) & m) + apron, ((by + ty) & m) + apron,
                                     ((bz + tz) & m) + apron)],
                     float_to_ordered(w));
@@ -633,45 +490,6 @@ void scatter_reduce_levelset(SubcellCfg c, int64_t nsc, const int* sc_pos, const
   hipLaunchKernelGGL(k_sr_scatter, dim3(blocks_for(nsc, 4, 1ll << 31)), dim3(256), 0, s, c, nsc, sc_pos, sc_nid, cnt,
                      off, list_pos, leaf_slot, apron, chan);
   hipLaunchKernelGGL(k_sr_final, dim3((unsigned)nsc), dim3(nt), 0, s, c, sc_pos, sc_nid, leaf_slot, apron, chan);
-}
-
-// ================================================================ apron
-// UpdateApron<T> (O.cuh:72-126): each apron voxel of each active brick copies the interior
-// voxel of the neighbour brick at the same world position, or boundval.  The 27 neighbour
-// slots are resolved once per brick (descent) into LDS instead of one descent per voxel.
-__global__ void __launch_bounds__(256) k_apron(DevTree T, const int4* __restrict__ slot_pos,
-                                               const int* __restrict__ leaf_slot, int res0, int apron,
-                                               float* __restrict__ chan, float boundval) {
-  __shared__ int nb[27];
-  const int s = blockIdx.x;
-  const int4 bp = slot_pos[s];
-  if (threadIdx.x < 27) {
-    const int dx = threadIdx.x % 3 - 1, dy = (threadIdx.x / 3) % 3 - 1, dz = threadIdx.x / 9 - 1;
-    const int64_t lf = descend(T, bp.x + dx * res0, bp.y + dy * res0, bp.z + dz * res0, 0);
-    nb[threadIdx.x] = lf < 0 ? -1 : leaf_slot[lf];
-  }
-  __syncthreads();
-  const int br = res0 + 2 * apron;
-  const int nvox = br * br * br;
-  for (int t = threadIdx.x; t < nvox; t += blockDim.x) {
-    const int x = t % br, y = (t / br) % br, z = t / (br * br);
-    const int gx = x - apron, gy = y - apron, gz = z - apron;
-    const int ox = gx < 0 ? -1 : (gx >= res0 ? 1 : 0), oy = gy < 0 ? -1 : (gy >= res0 ? 1 : 0),
-              oz = gz < 0 ? -1 : (gz >= res0 ? 1 : 0);
-    if ((ox | oy | oz) == 0) continue;
-    const int ns = nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)];
-    float v = boundval;
-    if (ns >= 0)
-      v = chan[atlas_voxel(ns, br, gx - ox * res0 + apron, gy - oy * res0 + apron, gz - oz * res0 + apron)];
-    chan[atlas_voxel(s, br, x, y, z)] = v;
-  }
-}
-
-void update_apron(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int apron,
-                  float* chan, float boundval, hipStream_t s) {
-  if (nslots <= 0 || apron <= 0) return;
-  hipLaunchKernelGGL(k_apron, dim3((unsigned)nslots), dim3(256), 0, s, T, slot_pos, leaf_slot, res0, apron, chan,
-                     boundval);
 }
 
 // ================================================================ trilinear sampling

@@ -19,10 +19,6 @@ struct PointsView {
   float tx, ty, tz;
 };
 
-// ---- bounding box (meta[0..5] ordered ints, pre-filled by bbox_init)
-void bbox_init(int* meta, hipStream_t s);
-void bbox(const PointsView& P, int* meta, hipStream_t s);
-
 // ---- keys
 template <typename K> void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, K* keys, hipStream_t s);
 template <typename K>
@@ -57,19 +53,13 @@ struct SubcellCfg {
   int sc, lgsc, D, res0;
   float r;
 };
-void sc_count(const DevTree& T, const PointsView& P, SubcellCfg c, const int* map, int* cnt, hipStream_t s);
-void sc_insert(const DevTree& T, const PointsView& P, SubcellCfg c, const int* map, const int* off, int* cursor,
-               float* list_pos, float* list_vel, hipStream_t s);
 void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map, int* sc_pos, int* sc_nid,
-                  hipStream_t s);
+                  int* map_leaf, hipStream_t s);
 
 // ---- gathers (mode 0 density, 1 level set)
 void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
                   const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode,
                   hipStream_t s);
-void gather_velocity(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
-                     const int* off, const float* list_vel, const int* leaf_slot, int apron, float* cx, float* cy,
-                     float* cz, int accum, hipStream_t s);
 void scatter_reduce_init(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* leaf_slot,
                          int apron, float* chan, float init, hipStream_t s);
 void scatter_reduce_levelset(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
@@ -77,8 +67,6 @@ void scatter_reduce_levelset(SubcellCfg c, int64_t nsc, const int* sc_pos, const
                              hipStream_t s);
 
 // ---- apron and sampling
-void update_apron(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0,
-                  int apron, float* chan, float boundval, hipStream_t s);
 void sample_trilinear(const DevTree& T, const int* leaf_slot, int res0, int apron, const float* chan,
                       const float* q, int64_t n, float* out, hipStream_t s);
 

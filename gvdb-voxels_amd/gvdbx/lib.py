@@ -48,8 +48,10 @@ SIGNATURES = {
     "gvdbx_gather_density": [_P, _I, _I64, _F, _PF, _I64, _I, _I, _I],
     "gvdbx_gather_levelset": [_P, _I, _I64, _F, _PF, _I64, _I, _I, _I],
     "gvdbx_gather_velocity": [_P, _I, _I, _I, _I],
+    "gvdbx_set_subcell_options": [_P, _I],
     "gvdbx_scatter_reduce_levelset": [_P, _F, _I, _F],
     "gvdbx_update_apron": [_P, _I, _F],
+    "gvdbx_update_apron_channels": [_P, _P, _P, _I],
     "gvdbx_sample_trilinear": [_P, _I, _P, _I64, _P],
     "gvdbx_get_depth": [_P, C.POINTER(_I), _P, _P, _P],
     "gvdbx_get_node_count": [_P, _I, C.POINTER(_I64), C.POINTER(_I64)],

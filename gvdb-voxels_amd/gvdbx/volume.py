@@ -25,6 +25,7 @@ class VolumeGVDB:
         self.log2dims = None
         self.apron = None
         self._points = None
+        self._channels = []
 
     # ------------------------------------------------------------ plumbing
     def _chk(self, code, what):
@@ -70,6 +71,8 @@ class VolumeGVDB:
     def AddChannel(self, chan, dtype=T_FLOAT, apron=1):
         self._chk(self.L.gvdbx_add_channel(self.h, chan, dtype, apron), "add_channel")
         self.apron = apron
+        if chan not in self._channels:
+            self._channels.append(chan)
 
     def FillChannel(self, chan, value):
         self._chk(self.L.gvdbx_fill_channel(self.h, chan, float(value)), "fill_channel")
@@ -128,11 +131,25 @@ class VolumeGVDB:
     def GatherVelocity(self, chan_x, chan_y, chan_z, accum=False):
         self._chk(self.L.gvdbx_gather_velocity(self.h, chan_x, chan_y, chan_z, int(bool(accum))), "GatherVelocity")
 
+    def SetSubcellOptions(self, store_velocity_lists=False):
+        self._chk(self.L.gvdbx_set_subcell_options(self.h, int(bool(store_velocity_lists))), "set_subcell_options")
+
     def ScatterReduceLevelSet(self, radius, chan, init=3.0):
         self._chk(self.L.gvdbx_scatter_reduce_levelset(self.h, radius, chan, init), "ScatterReduceLevelSet")
 
-    def UpdateApron(self, chan, boundval=0.0):
+    def UpdateApron(self, chan=None, boundval=0.0):
+        """UpdateApron(chan, boundval); UpdateApron() covers every channel (V.cpp:4411-4415)."""
+        if chan is None:
+            self.UpdateApronChannels(self._channels, [boundval] * len(self._channels))
+            return
         self._chk(self.L.gvdbx_update_apron(self.h, chan, float(boundval)), "UpdateApron")
+
+    def UpdateApronChannels(self, chans, boundvals):
+        ch = np.ascontiguousarray(chans, np.int32)
+        bv = np.ascontiguousarray(boundvals, np.float32)
+        assert len(ch) == len(bv)
+        self._chk(self.L.gvdbx_update_apron_channels(self.h, ch.ctypes.data, bv.ctypes.data, len(ch)),
+                  "UpdateApronChannels")
 
     def SampleTrilinear(self, chan, q_dev, n, out_dev):
         self._chk(self.L.gvdbx_sample_trilinear(self.h, chan, q_dev.ptr, n, out_dev.ptr), "SampleTrilinear")

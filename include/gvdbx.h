@@ -104,12 +104,18 @@ int gvdbx_gather_levelset(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, 
  * float3 velocities (stored by gvdbInsertSubcell, P.cuh:266-267, read by no reference gather).
  * Sums are exact 64-bit fixed point (2^-24 units), so the result is order independent. */
 int gvdbx_gather_velocity(gvdbx_ctx* ctx, int chan_x, int chan_y, int chan_z, int accum);
+/* Whether InsertPointsSubcell also stores the binned velocities (AUX_SUBCELL_PNT_VEL,
+ * P.cuh:266-267).  Off by default: no gather reads them; the velocity means come from the
+ * per-subcell sums accumulated during the insert. */
+int gvdbx_set_subcell_options(gvdbx_ctx* ctx, int store_velocity_lists);
 /* ScatterReduceLevelSet: absent from the reference; defined as the GatherLevelSet field
  * (P.cuh:736-773) produced by scattering each (point, subcell) pair and min-reducing. */
 int gvdbx_scatter_reduce_levelset(gvdbx_ctx* ctx, float radius, int chan, float init);
 
-/* ---- apron: UpdateApron (V.cpp:4411-4457, kernel O.cuh:72-126) */
+/* ---- apron: UpdateApron (V.cpp:4411-4457, kernel O.cuh:72-126); the channel-list form
+ *      covers the all-channel UpdateApron() overload (V.cpp:4411-4415) in one launch. */
 int gvdbx_update_apron(gvdbx_ctx* ctx, int chan, float boundval);
+int gvdbx_update_apron_channels(gvdbx_ctx* ctx, const int* chans, const float* boundvals, int nchan);
 
 /* ---- trilinear sampling: getTrilinear/getVolSampleF (R.cuh:104-126), voxel v at v+0.5,
  *      0 outside active bricks.  q: device float3[n]; out: device float[n]. */

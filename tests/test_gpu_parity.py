@@ -129,3 +129,65 @@ def test_depth_errors_on_gpu():
     with pytest.raises(GvdbxError) as e:
         v.GatherDensity(4, 2, 1.0, (0, 0, 0), 0, 0)
     assert e.value.code in (1, 6)
+
+
+@pytest.mark.parametrize("radius,sc", [(2.5, 2), (3.5, 4), (1.0, 1)])
+def test_window_overflow_paths(oracle_mod, radius, sc):
+    """r + 1 > sc: pairs outside the per-brick LDS window take the global path."""
+    U.log(f"case window r={radius} sc={sc}")
+    p = rand_points(3000, 21, 40, 120)
+    v, t = U.new_volume(), U.new_tree(oracle_mod)
+    U.run_pipeline(v, t, p, radius, sc, density=False, levelset=True, apron_bound=3.0)
+
+
+def test_orphan_points_and_velocity_lists(oracle_mod):
+    """Insert points whose own brick is missing (topology built from a subset) and keep the
+    optional binned velocities."""
+    U.log("case orphans")
+    g = np.random.default_rng(4)
+    a = rand_points(4000, 22, 40, 100)
+    b = g.uniform(20, 130, (3000, 3)).astype(np.float32)
+    allp = np.concatenate([a, b])
+    vel = g.standard_normal(allp.shape).astype(np.float32)
+    v, t = U.new_volume(chans=(0, 1, 2, 3)), U.new_tree(oracle_mod, chans=(0, 1, 2, 3))
+    v.SetSubcellOptions(store_velocity_lists=True)
+    dp = v.to_device(allp)
+    dv = v.to_device(vel)
+    v.SetPoints(dp, dv, n=len(a))
+    v.RebuildTopology(len(a), 1.0)
+    t.rebuild_topology(a, 1.0)
+    v.UpdateAtlas()
+    t.update_atlas()
+    U.compare_topology(v, t)
+    v.SetPoints(dp, dv, n=len(allp))
+    ln = v.InsertPointsSubcell(4, len(allp), 1.0)
+    assert ln == t.insert_points_subcell(4, allp, 1.0)
+    gs = U.compare_subcells(v, t)
+    o = t.subcells()
+    seg = np.repeat(np.arange(len(o["cnt"])), o["cnt"])
+    key_g = np.lexsort((gs["list_vel"][:, 2], gs["list_vel"][:, 1], gs["list_vel"][:, 0], seg))
+    want_vel = vel[o["list"]]
+    key_o = np.lexsort((want_vel[:, 2], want_vel[:, 1], want_vel[:, 0], seg))
+    assert np.array_equal(gs["list_vel"][key_g], want_vel[key_o])
+    v.GatherDensity(4, len(allp), 1.0, (0, 0, 0), ln, 0)
+    t.gather_density(1.0, 0)
+    v.GatherVelocity(1, 2, 3)
+    t.gather_velocity(vel, (1, 2, 3))
+    for c in range(4):
+        U.compare_channel(v, t, c, interior_only=True)
+
+
+def test_accumulate_gathers(oracle_mod):
+    """accum = True reads the channel back (GatherDensity / GatherLevelSet / velocity means)."""
+    U.log("case accumulate")
+    p = rand_points(3000, 23, 40, 100)
+    vel = np.random.default_rng(5).standard_normal(p.shape).astype(np.float32)
+    v, t = U.new_volume(chans=(0, 1, 2, 3)), U.new_tree(oracle_mod, chans=(0, 1, 2, 3))
+    U.run_pipeline(v, t, p, 1.0, 4, vel=vel, chans_vel=(1, 2, 3))
+    ln = v.subcell_info()[1]
+    v.GatherDensity(4, len(p), 1.5, (0, 0, 0), ln, 0, accum=True)
+    t.gather_density(1.5, 0, accum=True)
+    v.GatherVelocity(1, 2, 3, accum=True)
+    t.gather_velocity(vel, (1, 2, 3), accum=True)
+    for c in range(4):
+        U.compare_channel(v, t, c, interior_only=True)
