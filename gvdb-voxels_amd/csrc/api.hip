@@ -50,6 +50,9 @@ struct gvdbx_ctx {
   int axis_cx = 16, axis_cy = 16, axis_cz = 1;
   int apron = -1;
   bool chan_on[32] = {};
+  // apron voxels that have a neighbour brick hold current copies of its interior (set by the
+  // gathers that write the apron and by full apron passes; cleared by every other writer)
+  bool nbr_apron_ok[32] = {};
   Buf chan[32];
   int64_t chan_slot_cap = 0;
   int64_t nslots = 0;
@@ -69,7 +72,7 @@ struct gvdbx_ctx {
   bool store_vel_lists = false;
   bool have_vsum = false;
   Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
-  Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb;
+  Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb, nbm, spos, svel;
 
   Buf keys, keys_alt, flags, rank, radix_counts, tile_buf, markers, meta;
   int* h_meta = nullptr;
@@ -242,11 +245,16 @@ int reserve_level(gvdbx_ctx* c, int l, int64_t add) {
   return GVDBX_OK;
 }
 
-int bounding_box(gvdbx_ctx* c, int64_t n, const float* tr) {  // GetBoundingBox (V.cpp:5404-5467)
+// GetBoundingBox (V.cpp:5404-5467): launch; the result lands in meta[0..5]
+int bounding_box_launch(gvdbx_ctx* c, int64_t n, const float* tr) {
   CHK(ensure(c, c->bbox_part, (size_t)kBBoxParts * 6 * sizeof(float)));
   PointsView pv = points_of(c, n, tr);
   bbox_two_pass(pv, P<float>(c->bbox_part), P<int>(c->meta), c->stream);
-  CHK(read_meta(c));
+  return GVDBX_OK;
+}
+
+// posmin/posmax from the host copy of meta (min - 1 and max + 2, clamped at 0)
+void bounding_box_finish(gvdbx_ctx* c) {
   for (int a = 0; a < 3; ++a) {
     float mn = ordered_to_float(c->h_meta[M_BBOX + a]);
     float mx = ordered_to_float(c->h_meta[M_BBOX + 3 + a]);
@@ -255,7 +263,6 @@ int bounding_box(gvdbx_ctx* c, int64_t n, const float* tr) {  // GetBoundingBox 
     if (0 > c->posmin[a]) c->posmin[a] = 0;
     if (0 > c->posmax[a]) c->posmax[a] = 0;
   }
-  return GVDBX_OK;
 }
 
 int determine_depth(gvdbx_ctx* c, int* root) {  // DetermineDepth (V.cpp:821-837)
@@ -367,21 +374,31 @@ int full_pass(gvdbx_ctx* c, int64_t n, const float* tr, int d) {  // ActivateBri
 // Box pass shared by the extra (hi = 2r) and incremental (hi = r) passes; returns the
 // number of emitted keys in *m (count-then-fill on overflow).
 template <typename K>
-int box_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, float hi_mul, int d, int* markers, int64_t* m_out) {
-  KeyFmt f = key_fmt(c, d);
-  int64_t cap = c->keys.cap / sizeof(K);
-  if (cap < (int64_t)n * d) {
-    CHK(ensure(c, c->keys, (size_t)n * d * sizeof(K)));
-    cap = c->keys.cap / sizeof(K);
-  }
+int box_capacity(gvdbx_ctx* c, int64_t n, int d, int64_t* cap) {
+  if ((int64_t)(c->keys.cap / sizeof(K)) < (int64_t)n * d) CHK(ensure(c, c->keys, (size_t)n * d * sizeof(K)));
+  *cap = c->keys.cap / sizeof(K);
+  return GVDBX_OK;
+}
+
+template <typename K>
+void box_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, float hi_mul, int d, int* markers, int64_t cap) {
+  TimedRegion trg(c, "box");
+  box_keys<K>(tree_of(c), points_of(c, n, tr), key_fmt(c, d), r, hi_mul, P<K>(c->keys), cap, P<int>(c->meta) + M_COUNT,
+              markers, c->stream);
+}
+
+// prelaunched: the first attempt was enqueued before the caller's meta read-back.
+template <typename K>
+int box_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, float hi_mul, int d, int* markers, int64_t* m_out,
+             bool prelaunched = false) {
+  int64_t cap = 0;
+  CHK(box_capacity<K>(c, n, d, &cap));
   for (int attempt = 0; attempt < 2; ++attempt) {
-    CHK(zero_meta(c));
-    {
-      TimedRegion trg(c, "box");
-      box_keys<K>(tree_of(c), points_of(c, n, tr), f, r, hi_mul, P<K>(c->keys), cap, P<int>(c->meta) + M_COUNT, markers,
-                  c->stream);
+    if (!(prelaunched && attempt == 0)) {
+      CHK(zero_meta(c));
+      box_launch<K>(c, n, tr, r, hi_mul, d, markers, cap);
+      CHK(read_meta(c));
     }
-    CHK(read_meta(c));
     const int64_t m = (uint32_t)c->h_meta[M_COUNT];
     if (m <= cap) {
       *m_out = m;
@@ -405,23 +422,40 @@ int extra_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d) {  // A
   return GVDBX_OK;
 }
 
+// Markers and the incremental box pass (hi = trunc(q) + r); launched before the read-back
+// that also returns the bounding box.
 template <typename K>
-int incre_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool accum) {  // V.cpp:1020-1156
+int incre_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool accum) {
   const int64_t total0 = c->cnt[0];
-  if (total0 == 0) return GVDBX_OK;
   CHK(ensure(c, c->markers, (size_t)total0 * 4));
   DevTree T = tree_of(c);
   if (accum)
     copy_flags_to_markers(T, total0, P<int>(c->markers), c->stream);
   else
     fill_u32(P<uint32_t>(c->markers), total0, 0u, c->stream);
+  int64_t cap = 0;
+  CHK(box_capacity<K>(c, n, d, &cap));
+  box_launch<K>(c, n, tr, r, 1.0f, d, P<int>(c->markers), cap);
+  return GVDBX_OK;
+}
+
+template <typename K>
+int incre_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool accum,
+               bool prelaunched = false) {  // V.cpp:1020-1156
+  const int64_t total0 = c->cnt[0];
+  if (total0 == 0) return GVDBX_OK;
+  if (!prelaunched) {
+    CHK(zero_meta(c));
+    CHK(incre_launch<K>(c, n, tr, r, d, accum));
+    CHK(read_meta(c));
+  }
   int64_t m = 0;
-  CHK(box_pass<K>(c, n, tr, r, 1.0f, d, P<int>(c->markers), &m));
+  CHK(box_pass<K>(c, n, tr, r, 1.0f, d, P<int>(c->markers), &m, true));
   if (m == 0) return GVDBX_OK;  // no new node: topology unchanged (V.cpp:1066)
   apply_markers(tree_of(c), P<int>(c->markers), total0, c->stream);
   CHK(sort_unique_alloc<K>(c, m, key_fmt(c, d), d));
   link_all(c, d);
-  T = tree_of(c);
+  DevTree T = tree_of(c);
   delink_leaves(T, c->cnt[0], c->stream);
   for (int l = 1; l < d; ++l) delink_level(T, l, c->cnt[l], c->stream);
   // used count (V.cpp:1144-1147)
@@ -449,7 +483,18 @@ int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accum
   CHK(check_points(c, n));
   c->topo_version++;
   c->sc = 0;
-  CHK(bounding_box(c, n, orig));
+  // One read-back returns the bounding box and, when the tree is expected to stay at its
+  // depth, the key count of the speculatively launched incremental box pass (it only writes
+  // keys and markers, so it is simply dropped when a full rebuild follows).
+  const bool spec = !c->rebuild && c->depth >= 1 && c->cnt[0] > 0;
+  CHK(zero_meta(c));
+  CHK(bounding_box_launch(c, n, orig));
+  if (spec) {
+    if (wide_keys(c, c->depth)) CHK(incre_launch<uint64_t>(c, n, orig, r, c->depth, accumulate));
+    else CHK(incre_launch<uint32_t>(c, n, orig, r, c->depth, accumulate));
+  }
+  CHK(read_meta(c));
+  bounding_box_finish(c);
   int root[3];
   int depth = determine_depth(c, root);
   if (accumulate && depth >= 0 && depth < min_depth) depth = min_depth;
@@ -457,8 +502,8 @@ int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accum
   if (depth < 0 || depth > 5 || depth >= c->levs)
     return fail(c, GVDBX_ERR_DEPTH_MAX, "Point extents exceed the configured levels (depth %d)", depth);
   if (accumulate && !c->rebuild) {  // AccumulateTopology (V.cpp:1158-1178)
-    return wide_keys(c, c->depth) ? incre_pass<uint64_t>(c, n, orig, r, c->depth, true)
-                                  : incre_pass<uint32_t>(c, n, orig, r, c->depth, true);
+    return wide_keys(c, c->depth) ? incre_pass<uint64_t>(c, n, orig, r, c->depth, true, spec)
+                                  : incre_pass<uint32_t>(c, n, orig, r, c->depth, true, spec);
   }
   if (c->rebuild) {
     clear_tree(c);
@@ -480,8 +525,8 @@ int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accum
     c->rebuild = 0;
     return GVDBX_OK;
   }
-  return wide_keys(c, c->depth) ? incre_pass<uint64_t>(c, n, orig, r, c->depth, false)
-                                : incre_pass<uint32_t>(c, n, orig, r, c->depth, false);
+  return wide_keys(c, c->depth) ? incre_pass<uint64_t>(c, n, orig, r, c->depth, false, spec)
+                                : incre_pass<uint32_t>(c, n, orig, r, c->depth, false, spec);
 }
 
 int chan_ok(gvdbx_ctx* c, int chan) {
@@ -592,7 +637,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   Buf* bufs[] = {&c->leaf_slot, &c->slot_leaf, &c->slot_pos, &c->nbr, &c->sc_map, &c->sc_cnt, &c->sc_off, &c->sc_pos,
                  &c->sc_nid, &c->list_pos, &c->list_vel, &c->map_leaf, &c->vsum, &c->hkeys, &c->hkeys_alt,
-                 &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->keys, &c->keys_alt, &c->flags,
+                 &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
                  &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
@@ -681,6 +726,7 @@ int gvdbx_add_channel(gvdbx_ctx* c, int chan, int dtype, int apron) {  // AddCha
 int gvdbx_fill_channel(gvdbx_ctx* c, int chan, float v) {  // FillChannel / ClearChannel
   CHK(chan_ok(c, chan));
   if (c->chan_slot_cap > 0) fill_f32(P<float>(c->chan[chan]), c->chan_slot_cap * brick_floats(c), v, c->stream);
+  c->nbr_apron_ok[chan] = true;
   return GVDBX_OK;
 }
 
@@ -766,6 +812,7 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   assign_slots(T, P<int>(c->flags), P<int>(c->rank), total, c->apron, br, c->axis_cx, c->axis_cy, P<int>(c->leaf_slot),
                P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
   c->nslots = c->used0;
+  for (int ch = 0; ch < 32; ++ch) c->nbr_apron_ok[ch] = false;
   CHK(ensure(c, c->nbr, (size_t)(c->nslots * 27 + 1) * 4));
   neighbor_table(T, c->nslots, P<int4>(c->slot_pos), P<int>(c->leaf_slot), res_of(c, 0), P<int>(c->nbr), c->stream);
   CHK(reserve_channels(c, c->nslots));
@@ -840,6 +887,11 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   CHK(ensure(c, c->ocnt, (size_t)(nsc_cap + 1) * 4));
   CHK(ensure(c, c->ocur, (size_t)(nsc_cap + 1) * 4));
   CHK(ensure(c, c->wgb, (size_t)((int64_t)nmap * W * W * W + 1) * 4));
+  CHK(ensure(c, c->nbm, (size_t)((int64_t)nmap * 27 + 1) * 4));
+  CHK(ensure(c, c->spos, (size_t)(n + 1) * 16));
+  if (with_vel) CHK(ensure(c, c->svel, (size_t)(n + 1) * 16));
+  gather_sorted(pv, svals, n, P<float4>(c->spos), with_vel ? P<float4>(c->svel) : nullptr, c->stream);
+  fill_u32(P<uint32_t>(c->nbm), (int64_t)nmap * 27, 0xFFFFFFFFu, c->stream);
   BinArgs A;
   A.T = T;
   A.P = pv;
@@ -854,11 +906,13 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   A.ocur = P<int>(c->ocur);
   A.wgb = P<int>(c->wgb);
   A.off = P<int>(c->sc_off);
+  A.spos = P<float4>(c->spos);
+  A.svel = with_vel ? P<float4>(c->svel) : nullptr;
+  A.nbm = P<int>(c->nbm);
   A.list_pos = nullptr;
   A.list_vel = nullptr;
   A.vsum = with_vel ? P<unsigned long long>(c->vsum) : nullptr;
 
-  fill_u32(P<uint32_t>(c->cnt_in), nsc, 0u, c->stream);
   fill_u32(P<uint32_t>(c->ocnt), nsc, 0u, c->stream);
   if (with_vel) fill_u64(P<uint64_t>(c->vsum), nsc * 3, 0ull, c->stream);
   {
@@ -866,22 +920,31 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
     bin_count(A, nmap, c->stream);
     bin_points_count(A, tail, c->stream);
   }
-  sum_counts(P<int>(c->cnt_in), P<int>(c->ocnt), P<int>(c->sc_cnt), nsc, c->stream);
+  cell_totals(A, nsc, P<int>(c->sc_cnt), c->stream);
   CHK(zero_meta(c));
   exclusive_scan(P<int>(c->sc_cnt), P<int>(c->sc_off), nsc, P<int>(c->tile_buf), P<int>(c->meta) + M_TOTAL,
                  c->stream);
-  CHK(read_meta(c));
-  const int64_t ln = (uint32_t)c->h_meta[M_TOTAL];
-  if (ln > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length exceeds 2^31");
+  cell_bases(A, nsc, c->stream);
   c->sc = sc;
   c->scdim = cfg.D;
   c->nsc = nsc;
-  c->sclen = ln;
-  if (len) *len = ln;
   c->lists_have_vel = with_vel && c->store_vel_lists;
   c->have_vsum = with_vel;
-  CHK(ensure(c, c->list_pos, (size_t)(ln + 1) * 12));
-  if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(ln + 1) * 12));
+  // Lists sized for the largest possible length when that bound is modest, so the insert is
+  // enqueued before the single length read-back; otherwise read the length first.
+  int64_t per_axis = (int64_t)((2.0f * r + 1.0f) / (float)sc) + 2;
+  const int64_t bound = per_axis * per_axis * per_axis * n;
+  bool early = (size_t)(bound + 1) * 12 <= (c->list_pos.cap > ((size_t)1 << 30) ? c->list_pos.cap : ((size_t)1 << 30));
+  int64_t ln = -1;
+  if (early) {
+    CHK(ensure(c, c->list_pos, (size_t)(bound + 1) * 12));
+    if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(bound + 1) * 12));
+  } else {
+    CHK(read_meta(c));
+    ln = (uint32_t)c->h_meta[M_TOTAL];
+    CHK(ensure(c, c->list_pos, (size_t)(ln + 1) * 12));
+    if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(ln + 1) * 12));
+  }
   A.list_pos = P<float>(c->list_pos);
   A.list_vel = c->lists_have_vel ? P<float>(c->list_vel) : nullptr;
   fill_u32(P<uint32_t>(c->ocur), nsc, 0u, c->stream);
@@ -890,6 +953,13 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
     bin_insert(A, nmap, c->stream);
     bin_points_insert(A, tail, c->stream);
   }
+  if (early) {
+    CHK(read_meta(c));
+    ln = (uint32_t)c->h_meta[M_TOTAL];
+  }
+  if (ln > INT32_MAX || ln > bound) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);
+  c->sclen = ln;
+  if (len) *len = ln;
   return GVDBX_OK;
 }
 
@@ -904,14 +974,18 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
   if (c->nsc == 0) return GVDBX_OK;
   const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
   TimedRegion tr(c, "gather");
-  if (mode == 0 && !accum && cfg.res0 <= 8)
+  const int br = cfg.res0 + 2 * c->apron;
+  if (mode == 0 && !accum && cfg.res0 <= 8 && br * br * br <= kMaxPaddedVox) {
     density_brick(cfg, (int)c->used0, c->nsc, c->sclen, P<int>(c->map_leaf), P<int>(c->sc_off),
                   P<float>(c->list_pos), P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]), c->apron,
-                  P<float>(c->chan[chan]), c->stream);
-  else
+                  c->apron > 0 ? P<int>(c->nbr) : nullptr, P<float>(c->chan[chan]), c->stream);
+    c->nbr_apron_ok[chan] = c->apron > 0;
+  } else {
     gather_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
                  P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), accum, mode,
                  c->stream);
+    c->nbr_apron_ok[chan] = false;
+  }
   return GVDBX_OK;
 }
 
@@ -939,14 +1013,19 @@ int gvdbx_gather_velocity(gvdbx_ctx* c, int cx, int cy, int cz, int accum) {
   if (c->nsc == 0) return GVDBX_OK;
   const SubcellCfg cfg = subcell_cfg(c, c->sc, 0.f);
   TimedRegion tr(c, "velocity");
-  if ((int64_t)cfg.D * cfg.D * cfg.D <= 512)
+  bool apron_done = false;
+  if ((int64_t)cfg.D * cfg.D * cfg.D <= 512) {
+    const bool with_apron = c->apron > 0 && !accum;
     velocity_brick(cfg, (int)c->used0, P<int>(c->map_leaf), P<int>(c->sc_cnt), P<unsigned long long>(c->vsum),
-                   P<int>(c->leaf_slot), c->apron, P<float>(c->chan[cx]), P<float>(c->chan[cy]),
-                   P<float>(c->chan[cz]), accum, c->stream);
-  else
+                   P<int>(c->leaf_slot), c->apron, with_apron ? P<int>(c->nbr) : nullptr, P<float>(c->chan[cx]),
+                   P<float>(c->chan[cy]), P<float>(c->chan[cz]), accum, c->stream);
+    apron_done = with_apron;
+  } else {
     velocity_subcell(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt),
                      P<unsigned long long>(c->vsum), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[cx]),
                      P<float>(c->chan[cy]), P<float>(c->chan[cz]), accum, c->stream);
+  }
+  c->nbr_apron_ok[cx] = c->nbr_apron_ok[cy] = c->nbr_apron_ok[cz] = apron_done;
   return GVDBX_OK;
 }
 
@@ -959,6 +1038,7 @@ int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {
                       P<float>(c->chan[chan]), init, c->stream);
   scatter_reduce_levelset(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
                           P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), c->stream);
+  c->nbr_apron_ok[chan] = false;
   return GVDBX_OK;
 }
 
@@ -968,16 +1048,27 @@ int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* bou
   CHK(atlas_current(c));
   if (c->apron == 0 || c->nslots == 0) return GVDBX_OK;
   TimedRegion tr(c, "apron");
-  for (int i0 = 0; i0 < nchan; i0 += kMaxApronChannels) {
+  // channels whose neighbour copies are current only need the boundary value; the others
+  // take the full neighbour pull (UpdateApron<T>, O.cuh:72-126)
+  for (int pass = 0; pass < 2; ++pass) {
     ApronChannels ac;
     ac.n = 0;
-    for (int i = i0; i < nchan && ac.n < kMaxApronChannels; ++i) {
+    for (int i = 0; i < nchan; ++i) {
+      if (c->nbr_apron_ok[chans[i]] != (pass == 0)) continue;
       ac.ptr[ac.n] = P<float>(c->chan[chans[i]]);
       ac.bound[ac.n] = boundvals ? boundvals[i] : 0.0f;
-      ac.n++;
+      if (++ac.n == kMaxApronChannels || i == nchan - 1) {
+        if (pass == 0) apron_bound(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
+        else apron_multi(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
+        ac.n = 0;
+      }
     }
-    apron_multi(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
+    if (ac.n) {
+      if (pass == 0) apron_bound(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
+      else apron_multi(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
+    }
   }
+  for (int i = 0; i < nchan; ++i) c->nbr_apron_ok[chans[i]] = true;
   return GVDBX_OK;
 }
 
@@ -1052,6 +1143,7 @@ int gvdbx_copy_channel(gvdbx_ctx* c, int chan, float* out, int64_t max_floats) {
 
 int gvdbx_get_channel_ptr(gvdbx_ctx* c, int chan, void** p, int64_t* floats) {
   CHK(chan_ok(c, chan));
+  c->nbr_apron_ok[chan] = false;  // the caller may write through the pointer
   if (p) *p = c->chan[chan].p;
   if (floats) *floats = c->nslots * brick_floats(c);
   return GVDBX_OK;

@@ -174,16 +174,16 @@ __device__ __forceinline__ void add_vel_global(unsigned long long* vsum, int64_t
 }
 
 // Loads the 27 neighbour subcell-map indices; returns false for an unused map entry.
-__device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, BrickCtx& B) {
+__device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, BrickCtx& B, int D, int W) {
   const int leaf = A.map_leaf[m];
   if (leaf < 0) return false;
   const gvdbx_node* nd = A.T.nodes[0] + leaf;
   B.bx = nd->pos[0];
   B.by = nd->pos[1];
   B.bz = nd->pos[2];
-  B.W = A.W;
-  B.D = A.c.D;
-  B.D3 = B.D * B.D * B.D;
+  B.W = W;
+  B.D = D;
+  B.D3 = D * D * D;
   B.nb = nb;
   if (threadIdx.x < 27) {
     const int t = threadIdx.x;
@@ -194,13 +194,14 @@ __device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, Br
   return true;
 }
 
+template <int kD>
 __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int use_bins) {
   extern __shared__ unsigned long long smem64[];
   __shared__ int nb[27];
   BrickCtx B;
   const int m = blockIdx.x;
-  if (!brick_setup(A, m, nb, B)) return;
-  const int W = A.W, W3 = W * W * W, D = B.D, D3 = B.D3;
+  const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W, D3 = D * D * D;
+  if (!brick_setup(A, m, nb, B, D, W)) return;
   const int NB = use_bins ? D3 * 64 : 0;
   const bool vel = A.vsum != nullptr && A.P.vel != nullptr;
   // layout: [cvs 3*W3 u64][bvs 3*NB u64][ccnt W3 int][bcnt NB int]
@@ -220,12 +221,11 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int use_bins) {
   const SubcellCfg c = A.c;
   const int beg = A.start[m], end = A.start[m + 1];
   for (int j = beg + threadIdx.x; j < end; j += blockDim.x) {
-    const int64_t i = A.svals[j];
-    const float3 p = load_f3(A.P.pos, i, A.P.pos_off, A.P.pos_stride);
-    const float q[3] = {p.x + A.P.tx, p.y + A.P.ty, p.z + A.P.tz};
+    const float4 p = A.spos[j];
+    const float q[3] = {p.x, p.y, p.z};
     long long fv[3] = {0, 0, 0};
     if (vel) {
-      const float3 v = load_f3(A.P.vel, i, A.P.vel_off, A.P.vel_stride);
+      const float4 v = A.svel[j];
       fv[0] = vel_fixed(v.x);
       fv[1] = vel_fixed(v.y);
       fv[2] = vel_fixed(v.z);
@@ -296,53 +296,48 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int use_bins) {
         }
   }
   __syncthreads();
+  if (threadIdx.x < 27) A.nbm[(int64_t)m * 27 + threadIdx.x] = nb[threadIdx.x];
   for (int w = threadIdx.x; w < W3; w += blockDim.x) {
     const int n = ccnt[w];
-    const unsigned long long vs[3] = {cvs[3 * w], cvs[3 * w + 1], cvs[3 * w + 2]};
-    int base = -1;
+    int keep = 0;
     if (n > 0) {
       const int64_t g = B.global_of(w);
       if (g >= 0) {
-        base = atomicAdd(&A.cnt_in[g], n);
+        keep = n;
         if (vel) {
-          atomicAdd(&A.vsum[3 * g + 0], vs[0]);
-          atomicAdd(&A.vsum[3 * g + 1], vs[1]);
-          atomicAdd(&A.vsum[3 * g + 2], vs[2]);
+          atomicAdd(&A.vsum[3 * g + 0], cvs[3 * w]);
+          atomicAdd(&A.vsum[3 * g + 1], cvs[3 * w + 1]);
+          atomicAdd(&A.vsum[3 * g + 2], cvs[3 * w + 2]);
         }
       }
     }
-    A.wgb[(int64_t)m * W3 + w] = base;
+    A.wgb[(int64_t)m * W3 + w] = keep;  // this brick's count; cell_bases turns it into a list offset
   }
 }
 
+template <int kD>
 __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A) {
   extern __shared__ unsigned long long smem64[];
   __shared__ int nb[27];
   BrickCtx B;
   const int m = blockIdx.x;
-  if (!brick_setup(A, m, nb, B)) return;
-  const int W = A.W, W3 = W * W * W;
+  const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W;
+  if (!brick_setup(A, m, nb, B, D, W)) return;
   int* lbase = (int*)smem64;
   int* lcnt = lbase + W3;
   __syncthreads();
   for (int w = threadIdx.x; w < W3; w += blockDim.x) {
-    const int b = A.wgb[(int64_t)m * W3 + w];
-    lbase[w] = -1;
-    if (b >= 0) {
-      const int64_t g = B.global_of(w);
-      if (g >= 0) lbase[w] = A.off[g] + b;
-    }
+    lbase[w] = B.global_of(w) >= 0 ? A.wgb[(int64_t)m * W3 + w] : -1;
     lcnt[w] = 0;
   }
   __syncthreads();
   const SubcellCfg c = A.c;
   const int beg = A.start[m], end = A.start[m + 1];
   for (int j = beg + threadIdx.x; j < end; j += blockDim.x) {
-    const int64_t i = A.svals[j];
-    const float3 p = load_f3(A.P.pos, i, A.P.pos_off, A.P.pos_stride);
-    const float q[3] = {p.x + A.P.tx, p.y + A.P.ty, p.z + A.P.tz};
-    float3 v = make_float3(0.f, 0.f, 0.f);
-    if (A.list_vel) v = load_f3(A.P.vel, i, A.P.vel_off, A.P.vel_stride);
+    const float4 p = A.spos[j];
+    const float q[3] = {p.x, p.y, p.z};
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (A.list_vel) v = A.svel[j];
     int lo[3], hi[3];
     subcell_range(q, c.r, c.sc, lo, hi);
     for (int x = lo[0]; x <= hi[0]; x += c.sc)
@@ -381,14 +376,20 @@ size_t bin_count_smem(int W, int D) {
 
 void bin_count(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
-  hipLaunchKernelGGL(k_bin_count, dim3(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A,
-                     bin_uses_class_bins(A.c.D));
+  if (A.c.D == 2)
+    hipLaunchKernelGGL(k_bin_count<2>, dim3(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A, 1);
+  else
+    hipLaunchKernelGGL(k_bin_count<0>, dim3(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A,
+                       bin_uses_class_bins(A.c.D));
 }
 
 void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
   const size_t sm = (size_t)2 * A.W * A.W * A.W * 4 + 8;
-  hipLaunchKernelGGL(k_bin_insert, dim3(nmap), dim3(256), sm, s, A);
+  if (A.c.D == 2)
+    hipLaunchKernelGGL(k_bin_insert<2>, dim3(nmap), dim3(256), sm, s, A);
+  else
+    hipLaunchKernelGGL(k_bin_insert<0>, dim3(nmap), dim3(256), sm, s, A);
 }
 
 __global__ void k_sum_counts(const int* __restrict__ a, const int* __restrict__ b, int* __restrict__ out, int64_t n) {
@@ -399,6 +400,71 @@ __global__ void k_sum_counts(const int* __restrict__ a, const int* __restrict__ 
 void sum_counts(const int* a, const int* b, int* out, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_sum_counts, dim3(nblocks(n, 256)), dim3(256), 0, s, a, b, out, n);
+}
+
+// Sorted copies of the points (coalesced reads for the per-brick kernels).
+__global__ void k_gather_sorted(PointsView P, const uint32_t* __restrict__ svals, int64_t n, float4* __restrict__ spos,
+                                float4* __restrict__ svel) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = svals[j];
+  const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+  spos[j] = make_float4(p.x + P.tx, p.y + P.ty, p.z + P.tz, __uint_as_float(i));
+  if (svel) {
+    const float3 v = load_f3(P.vel, i, P.vel_off, P.vel_stride);
+    svel[j] = make_float4(v.x, v.y, v.z, 0.f);
+  }
+}
+
+void gather_sorted(const PointsView& P, const uint32_t* svals, int64_t n, float4* spos, float4* svel, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_sorted, dim3(nblocks(n, 256)), dim3(256), 0, s, P, svals, n, spos, svel);
+}
+
+// Walks the bricks whose window reaches subcell g = n*D^3 + l: per axis the brick itself and,
+// for l on the first/last layer, the neighbour below/above (window cell l - delta*D).
+template <bool kBases, int kD>
+__global__ void k_cell_walk(BinArgs A, int64_t nsc, int* __restrict__ cnt) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nsc) return;
+  const int D = kD ? kD : A.c.D, D3 = D * D * D, W = D + 2, W3 = W * W * W;
+  const int64_t n = g / D3;
+  const int l = (int)(g - n * D3);
+  const int lx = l % D, ly = (l / D) % D, lz = l / (D * D);
+  int p = 0;
+  int base = 0;
+  if (kBases) base = A.off[g];
+  for (int dz = (lz == 0 ? -1 : 0); dz <= (lz == D - 1 ? 1 : 0); ++dz)
+    for (int dy = (ly == 0 ? -1 : 0); dy <= (ly == D - 1 ? 1 : 0); ++dy)
+      for (int dx = (lx == 0 ? -1 : 0); dx <= (lx == D - 1 ? 1 : 0); ++dx) {
+        const int b = A.nbm[n * 27 + (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+        if (b < 0) continue;
+        const int w = ((lz - dz * D + 1) * W + (ly - dy * D + 1)) * W + (lx - dx * D + 1);
+        int* cell = A.wgb + (int64_t)b * W3 + w;
+        const int c = *cell;
+        if (kBases) *cell = base + p;
+        p += c;
+      }
+  if (!kBases) {
+    A.cnt_in[g] = p;
+    cnt[g] = p + A.ocnt[g];
+  }
+}
+
+void cell_totals(const BinArgs& A, int64_t nsc, int* cnt, hipStream_t s) {
+  if (nsc <= 0) return;
+  if (A.c.D == 2)
+    hipLaunchKernelGGL((k_cell_walk<false, 2>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt);
+  else
+    hipLaunchKernelGGL((k_cell_walk<false, 0>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt);
+}
+
+void cell_bases(const BinArgs& A, int64_t nsc, hipStream_t s) {
+  if (nsc <= 0) return;
+  if (A.c.D == 2)
+    hipLaunchKernelGGL((k_cell_walk<true, 2>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, nullptr);
+  else
+    hipLaunchKernelGGL((k_cell_walk<true, 0>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, nullptr);
 }
 
 // ------------------------------------------------------------------ per-point path
@@ -469,110 +535,232 @@ __device__ __forceinline__ int64_t atlas_index(int slot, int br, int lx, int ly,
 // the points listed for the voxel's subcell.  Only points within r of a voxel can raise it
 // above 0, so each list entry visits the voxels of its own subcell inside its r-ball (a
 // 0.01-voxel margin keeps rounding at the ball edge inclusive) and min-reduces |p-v|^2 in
-// LDS.  One workgroup per brick: the brick's D^3 subcell lists are contiguous, and the brick
-// interior is written with row-contiguous stores.
+// LDS.  One workgroup per brick, one wave per subcell.
+//
+// The apron is produced here too: every point within r of an apron voxel g lies in the list
+// of the brick subcell next to g (the list box [q-r, q+1+r] reaches it), so extending the
+// ball push of boundary subcells by the apron width yields exactly the value the neighbour
+// brick computes for g.  Apron voxels whose neighbour brick exists are written with it; the
+// others keep waiting for UpdateApron's boundary value.
+template <int kRes, int kApron, int kD, bool kWithApron>
 __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int64_t nsc, int64_t len,
                                                       const int* __restrict__ map_leaf, const int* __restrict__ off,
                                                       const float* __restrict__ list_pos,
                                                       const int* __restrict__ leaf_slot,
-                                                      const gvdbx_node* __restrict__ nodes0, int apron,
-                                                      float* __restrict__ chan) {
-  __shared__ unsigned int d2[512];
+                                                      const gvdbx_node* __restrict__ nodes0, int apron_rt,
+                                                      const int* __restrict__ nbr, float* __restrict__ chan) {
+  __shared__ unsigned int d2[kMaxPaddedVox];
   __shared__ int soff[513];
+  __shared__ int nb[27];
+  // geometry: compile-time for the default 8^3 bricks / apron 1 / 2^3 subcells, else runtime
+  const int res0 = kRes ? kRes : c.res0;
+  const int apron = kRes ? kApron : apron_rt;
+  const int D = kRes ? kD : c.D;
+  const int sc = kRes ? kRes / kD : c.sc;
+  const int a = kWithApron ? apron : 0;   // apron width computed here
+  const int br = res0 + 2 * apron;        // channel brick layout
+  const int bw = res0 + 2 * a;            // computed region
+  const int D3 = D * D * D;
   const int m = blockIdx.x;
   const int leaf = map_leaf[m];
   if (leaf < 0) return;
-  const int D = c.D, D3 = D * D * D, res0 = c.res0, nv = res0 * res0 * res0;
+  const int slot = leaf_slot[leaf];
+  const int nv = bw * bw * bw;
   const int64_t k0 = (int64_t)m * D3;
   for (int t = threadIdx.x; t < nv; t += blockDim.x) d2[t] = 0x7F800000u;
   for (int s = threadIdx.x; s <= D3; s += blockDim.x) soff[s] = (k0 + s < nsc) ? off[k0 + s] : (int)len;
+  if (threadIdx.x < 27) nb[threadIdx.x] = (kWithApron && slot >= 0) ? nbr[(int64_t)slot * 27 + threadIdx.x] : -1;
   __syncthreads();
   const int bx = nodes0[leaf].pos[0], by = nodes0[leaf].pos[1], bz = nodes0[leaf].pos[2];
-  const float rr = c.r + 0.01f;
+  // Voxels v with |p - v| < r lie in the integer cube [floor(p) - R + 1, floor(p) + R], R = ceil(r);
+  // voxels at distance >= r cannot raise max(0, r - |p - v|), so the cube is exact.
+  const int R = (int)ceilf(c.r);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   for (int s = wave; s < D3; s += nw) {  // one wave per subcell: its list entries over the lanes
-    const int sx0 = (s % D) * c.sc, sy0 = ((s / D) % D) * c.sc, sz0 = (s / (D * D)) * c.sc;
-    for (int j = soff[s] + lane; j < soff[s + 1]; j += 64) {
+    const int ix = s % D, iy = (s / D) % D, iz = s / (D * D);
+    // voxel range of the subcell, extended into the apron on the brick's boundary
+    const int lx0 = ix * sc - (ix == 0 ? a : 0), lx1 = ix * sc + sc - 1 + (ix == D - 1 ? a : 0);
+    const int ly0 = iy * sc - (iy == 0 ? a : 0), ly1 = iy * sc + sc - 1 + (iy == D - 1 ? a : 0);
+    const int lz0 = iz * sc - (iz == 0 ? a : 0), lz1 = iz * sc + sc - 1 + (iz == D - 1 ? a : 0);
+    const int je = soff[s + 1];
+    for (int j = soff[s] + lane; j < je; j += 64) {
       const float gx = list_pos[3 * (int64_t)j], gy = list_pos[3 * (int64_t)j + 1], gz = list_pos[3 * (int64_t)j + 2];
-      const float px = gx - (float)bx, py = gy - (float)by, pz = gz - (float)bz;
-      const int x0 = max(sx0, (int)ceilf(px - rr)), x1 = min(sx0 + c.sc - 1, (int)floorf(px + rr));
-      const int y0 = max(sy0, (int)ceilf(py - rr)), y1 = min(sy0 + c.sc - 1, (int)floorf(py + rr));
-      const int z0 = max(sz0, (int)ceilf(pz - rr)), z1 = min(sz0 + c.sc - 1, (int)floorf(pz + rr));
-      for (int z = z0; z <= z1; ++z)
-        for (int y = y0; y <= y1; ++y)
-          for (int x = x0; x <= x1; ++x) {
-            const float d = dist2(gx - (float)(bx + x), gy - (float)(by + y), gz - (float)(bz + z));
-            atomicMin(&d2[(z * res0 + y) * res0 + x], __float_as_uint(d));
-          }
+      const int fx = (int)gx - bx, fy = (int)gy - by, fz = (int)gz - bz;
+      if (R == 1) {
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+          for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+              const int x = fx + dx, y = fy + dy, z = fz + dz;
+              if (x < lx0 || x > lx1 || y < ly0 || y > ly1 || z < lz0 || z > lz1) continue;
+              const float d = dist2(gx - (float)(bx + x), gy - (float)(by + y), gz - (float)(bz + z));
+              atomicMin(&d2[((z + a) * bw + (y + a)) * bw + (x + a)], __float_as_uint(d));
+            }
+      } else {
+        const int x0 = max(lx0, fx - R + 1), x1 = min(lx1, fx + R);
+        const int y0 = max(ly0, fy - R + 1), y1 = min(ly1, fy + R);
+        const int z0 = max(lz0, fz - R + 1), z1 = min(lz1, fz + R);
+        for (int z = z0; z <= z1; ++z)
+          for (int y = y0; y <= y1; ++y)
+            for (int x = x0; x <= x1; ++x) {
+              const float d = dist2(gx - (float)(bx + x), gy - (float)(by + y), gz - (float)(bz + z));
+              atomicMin(&d2[((z + a) * bw + (y + a)) * bw + (x + a)], __float_as_uint(d));
+            }
+      }
     }
   }
   __syncthreads();
-  const int slot = leaf_slot[leaf];
   if (slot < 0) return;
-  const int br = res0 + 2 * apron;
-  float* b = chan + (int64_t)slot * br * br * br;
+  float* bptr = chan + (int64_t)slot * br * br * br;
   for (int t = threadIdx.x; t < nv; t += blockDim.x) {
-    const int x = t % res0, y = (t / res0) % res0, z = t / (res0 * res0);
+    const int zy = t / bw;
+    const int x = t - zy * bw - a, z = zy / bw - a, y = zy - (zy / bw) * bw - a;
+    if (kWithApron) {
+      const int ox = x < 0 ? -1 : (x >= res0 ? 1 : 0), oy = y < 0 ? -1 : (y >= res0 ? 1 : 0),
+                oz = z < 0 ? -1 : (z >= res0 ? 1 : 0);
+      if ((ox | oy | oz) != 0 && nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] < 0) continue;
+    }
     const unsigned int u = d2[t];
     const float val = u == 0x7F800000u ? 0.0f : fmaxf(0.0f, c.r - sqrtf(__uint_as_float(u)));
-    b[((z + apron) * br + (y + apron)) * br + (x + apron)] = val;
+    bptr[((z + apron) * br + (y + apron)) * br + (x + apron)] = val;
   }
 }
 
 void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
-                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, float* chan,
-                   hipStream_t s) {
+                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, const int* nbr,
+                   float* chan, hipStream_t s) {
   if (nmap <= 0) return;
-  hipLaunchKernelGGL(k_density_brick, dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off, list_pos, leaf_slot,
-                     nodes0, apron, chan);
+  const bool fixed = c.res0 == 8 && c.D == 2 && apron == 1;
+  if (fixed && nbr)
+    hipLaunchKernelGGL((k_density_brick<8, 1, 2, true>), dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off,
+                       list_pos, leaf_slot, nodes0, apron, nbr, chan);
+  else if (fixed)
+    hipLaunchKernelGGL((k_density_brick<8, 1, 2, false>), dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off,
+                       list_pos, leaf_slot, nodes0, apron, nbr, chan);
+  else if (nbr)
+    hipLaunchKernelGGL((k_density_brick<0, 0, 0, true>), dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off,
+                       list_pos, leaf_slot, nodes0, apron, nbr, chan);
+  else
+    hipLaunchKernelGGL((k_density_brick<0, 0, 0, false>), dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off,
+                       list_pos, leaf_slot, nodes0, apron, nbr, chan);
 }
 
 // Velocity means from the fixed-point sums: (accum ? old : 0) + sum, over (cnt + accum); the
 // colour rule of P.cuh:717-731 for float3; an empty subcell without accumulation gives 0.
-// One workgroup per brick; the means are formed once per subcell, then broadcast.
+// One workgroup per brick.  Without accumulation the apron voxels whose neighbour exists get
+// the mean of the neighbour subcell holding them (slot order equals subcell-map order).
+__device__ __forceinline__ float mean_of(const int* cnt, const unsigned long long* vsum, int64_t k, int a, int acc,
+                                         float old) {
+  const double den = (double)((long long)cnt[k] + acc);
+  double v = (double)(long long)vsum[3 * k + a] * (1.0 / kVelScale);
+  if (acc) v += (double)old;
+  return den > 0 ? (float)(v / den) : 0.0f;
+}
+
+template <int kRes, int kApron, int kD>
 __global__ void __launch_bounds__(256) k_velocity_brick(SubcellCfg c, const int* __restrict__ map_leaf,
                                                        const int* __restrict__ cnt,
                                                        const unsigned long long* __restrict__ vsum,
-                                                       const int* __restrict__ leaf_slot, int apron,
-                                                       float* __restrict__ cx, float* __restrict__ cy,
-                                                       float* __restrict__ cz, int accum) {
-  __shared__ double mean[512][3];
-  __shared__ double den_s[512];
+                                                       const int* __restrict__ leaf_slot, int apron_rt,
+                                                       const int* __restrict__ nbr, float* __restrict__ cx,
+                                                       float* __restrict__ cy, float* __restrict__ cz, int accum) {
+  __shared__ int nb[27];
+  __shared__ float mean[kMaxWindowCells][3];
+  const int res0 = kRes ? kRes : c.res0;
+  const int apron = kRes ? kApron : apron_rt;
+  const int D = kRes ? kD : c.D;
+  const int lgsc = kRes ? (kD == 2 ? 2 : c.lgsc) : c.lgsc;
   const int m = blockIdx.x;
   const int leaf = map_leaf[m];
   if (leaf < 0) return;
   const int slot = leaf_slot[leaf];
   if (slot < 0) return;
-  const int D = c.D, D3 = D * D * D, res0 = c.res0, nv = res0 * res0 * res0;
-  const int64_t k0 = (int64_t)m * D3;
-  for (int s = threadIdx.x; s < D3; s += blockDim.x) {
-    const long long den = (long long)cnt[k0 + s] + (accum ? 1 : 0);
-    den_s[s] = (double)den;
-    for (int a = 0; a < 3; ++a) mean[s][a] = (double)(long long)vsum[3 * (k0 + s) + a] * (1.0 / kVelScale);
-  }
+  const int D3 = D * D * D;
+  const int a = (nbr && !accum) ? apron : 0;
+  if (threadIdx.x < 27) nb[threadIdx.x] = a ? nbr[(int64_t)slot * 27 + threadIdx.x] : -1;
   __syncthreads();
+  // means of the window subcells (own D^3 plus the one-subcell shell when the apron is written)
+  const int W = D + 2, W3 = W * W * W;
+  if (!accum) {
+    for (int w = threadIdx.x; w < W3; w += blockDim.x) {
+      const int sx = w % W - 1, sy = (w / W) % W - 1, sz = w / (W * W) - 1;
+      const int ox = sx < 0 ? -1 : (sx >= D ? 1 : 0), oy = sy < 0 ? -1 : (sy >= D ? 1 : 0),
+                oz = sz < 0 ? -1 : (sz >= D ? 1 : 0);
+      const int b = (ox | oy | oz) == 0 ? m : nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)];
+      if (b < 0) continue;
+      const int64_t k = (int64_t)b * D3 + ((sz - oz * D) * D + (sy - oy * D)) * D + (sx - ox * D);
+      for (int q = 0; q < 3; ++q) mean[w][q] = mean_of(cnt, vsum, k, q, 0, 0.0f);
+    }
+    __syncthreads();
+  }
   const int br = res0 + 2 * apron;
+  const int bw = res0 + 2 * a;  // the region written
+  const int nv = bw * bw * bw;
   const int64_t base = (int64_t)slot * br * br * br;
   float* ch[3] = {cx, cy, cz};
   for (int t = threadIdx.x; t < nv; t += blockDim.x) {
-    const int x = t % res0, y = (t / res0) % res0, z = t / (res0 * res0);
-    const int s = ((z >> c.lgsc) * D + (y >> c.lgsc)) * D + (x >> c.lgsc);
+    const int zy = t / bw;
+    const int x = t - zy * bw - a, z = zy / bw - a, y = zy - (zy / bw) * bw - a;
+    const int ox = x < 0 ? -1 : (x >= res0 ? 1 : 0), oy = y < 0 ? -1 : (y >= res0 ? 1 : 0),
+              oz = z < 0 ? -1 : (z >= res0 ? 1 : 0);
+    if ((ox | oy | oz) != 0 && nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] < 0) continue;
     const int64_t vi = base + ((z + apron) * br + (y + apron)) * br + (x + apron);
-    const double den = den_s[s];
+    if (!accum) {
+      const int w = (((z >> lgsc) + 1) * W + ((y >> lgsc) + 1)) * W + ((x >> lgsc) + 1);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      double v = mean[s][a];
-      if (accum) v += (double)ch[a][vi];
-      ch[a][vi] = den > 0 ? (float)(v / den) : 0.0f;
+      for (int q = 0; q < 3; ++q) ch[q][vi] = mean[w][q];
+    } else {
+      const int64_t k = (int64_t)m * D3 + ((z >> lgsc) * D + (y >> lgsc)) * D + (x >> lgsc);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) ch[q][vi] = mean_of(cnt, vsum, k, q, 1, ch[q][vi]);
     }
   }
 }
 
 void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* vsum,
-                    const int* leaf_slot, int apron, float* cx, float* cy, float* cz, int accum, hipStream_t s) {
+                    const int* leaf_slot, int apron, const int* nbr, float* cx, float* cy, float* cz, int accum,
+                    hipStream_t s) {
   if (nmap <= 0) return;
-  hipLaunchKernelGGL(k_velocity_brick, dim3(nmap), dim3(256), 0, s, c, map_leaf, cnt, vsum, leaf_slot, apron, cx, cy,
-                     cz, accum);
+  if (c.res0 == 8 && c.D == 2 && apron == 1)
+    hipLaunchKernelGGL((k_velocity_brick<8, 1, 2>), dim3(nmap), dim3(256), 0, s, c, map_leaf, cnt, vsum, leaf_slot,
+                       apron, nbr, cx, cy, cz, accum);
+  else
+    hipLaunchKernelGGL((k_velocity_brick<0, 0, 0>), dim3(nmap), dim3(256), 0, s, c, map_leaf, cnt, vsum, leaf_slot,
+                       apron, nbr, cx, cy, cz, accum);
+}
+
+// Boundary value into the apron voxels that have no neighbour brick (the part of
+// UpdateApron left once the neighbour copies are current).
+__global__ void __launch_bounds__(256) k_apron_bound(int64_t nslots, const int* __restrict__ nbr, int res0, int apron,
+                                                    ApronChannels ch) {
+  __shared__ int nb[27];
+  __shared__ int any;
+  const int64_t s = blockIdx.x;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  if (threadIdx.x < 27) {
+    nb[threadIdx.x] = nbr[s * 27 + threadIdx.x];
+    if (nb[threadIdx.x] < 0) atomicOr(&any, 1);
+  }
+  __syncthreads();
+  if (!any) return;
+  const int br = res0 + 2 * apron;
+  const int nvox = br * br * br;
+  const int64_t bv = (int64_t)nvox;
+  for (int t = threadIdx.x; t < nvox; t += blockDim.x) {
+    const int x = t % br - apron, y = (t / br) % br - apron, z = t / (br * br) - apron;
+    const int ox = x < 0 ? -1 : (x >= res0 ? 1 : 0), oy = y < 0 ? -1 : (y >= res0 ? 1 : 0),
+              oz = z < 0 ? -1 : (z >= res0 ? 1 : 0);
+    if ((ox | oy | oz) == 0 || nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] >= 0) continue;
+    for (int c = 0; c < ch.n; ++c) ch.ptr[c][s * bv + t] = ch.bound[c];
+  }
+}
+
+void apron_bound(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s) {
+  if (nslots <= 0 || apron <= 0 || ch.n <= 0) return;
+  hipLaunchKernelGGL(k_apron_bound, dim3((unsigned)nslots), dim3(256), 0, s, nslots, nbr, res0, apron, ch);
 }
 
 // Per-subcell form of the velocity means, for bricks with more than 512 subcells.

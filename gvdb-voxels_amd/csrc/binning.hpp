@@ -27,6 +27,9 @@ struct BinArgs {
   int* ocur;     // insert cursor of the one-by-one pairs
   int* wgb;      // [nmap][W^3]: this brick's offset inside cnt_in, or -1
   const int* off;
+  const float4* spos;   // sorted positions (q = p + trans; w = point index bits)
+  const float4* svel;   // sorted velocities, or null
+  int* nbm;             // [nmap][27] neighbour subcell-map indices, written by the count kernel
   float* list_pos;
   float* list_vel;
   unsigned long long* vsum;
@@ -37,12 +40,21 @@ void bin_insert(const BinArgs& A, int nmap, hipStream_t s);
 void bin_points_count(const BinArgs& A, const int* begin, hipStream_t s);
 void bin_points_insert(const BinArgs& A, const int* begin, hipStream_t s);
 void sum_counts(const int* a, const int* b, int* out, int64_t n, hipStream_t s);
+void gather_sorted(const PointsView& P, const uint32_t* svals, int64_t n, float4* spos, float4* svel, hipStream_t s);
+// per subcell: cnt_in = sum of the brick-window counts of the (up to 8) bricks that reach it,
+// cnt = cnt_in + ocnt; then, after the scan, each window's slice base = off + prefix
+void cell_totals(const BinArgs& A, int64_t nsc, int* cnt, hipStream_t s);
+void cell_bases(const BinArgs& A, int64_t nsc, hipStream_t s);
 
+constexpr int kMaxWindowCells = 1000;  // (D + 2)^3 for D <= 8
+constexpr int kMaxPaddedVox = 1728;  // (8 + 2*2)^3: brick-window kernels handle res0 <= 8, apron <= 2
+// nbr == nullptr: interior only; else apron voxels with a neighbour brick are written as well
 void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
-                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, float* chan,
-                   hipStream_t s);
+                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, const int* nbr,
+                   float* chan, hipStream_t s);
 void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* vsum,
-                    const int* leaf_slot, int apron, float* cx, float* cy, float* cz, int accum, hipStream_t s);
+                    const int* leaf_slot, int apron, const int* nbr, float* cx, float* cy, float* cz, int accum,
+                    hipStream_t s);
 
 void velocity_subcell(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
                       const unsigned long long* vsum, const int* leaf_slot, int apron, float* cx, float* cy,
@@ -57,5 +69,6 @@ struct ApronChannels {
 void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int* nbr,
                     hipStream_t s);
 void apron_multi(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s);
+void apron_bound(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s);
 
 }  // namespace gvdbx

@@ -191,3 +191,31 @@ def test_accumulate_gathers(oracle_mod):
     t.gather_velocity(vel, (1, 2, 3), accum=True)
     for c in range(4):
         U.compare_channel(v, t, c, interior_only=True)
+
+
+def test_apron_state_transitions(oracle_mod):
+    """Fused apron (gather-written neighbour copies) + later boundary values, a second
+    UpdateApron with another boundary value, and the full pull after a fill / level set."""
+    U.log("case apron states")
+    p = rand_points(5000, 31, 40, 110)
+    vel = np.random.default_rng(6).standard_normal(p.shape).astype(np.float32)
+    v, t = U.new_volume(chans=(0, 1, 2, 3)), U.new_tree(oracle_mod, chans=(0, 1, 2, 3))
+    U.run_pipeline(v, t, p, 1.0, 4, vel=vel, chans_vel=(1, 2, 3))
+    v.UpdateApron(0, 5.0)
+    t.update_apron(0, 5.0)
+    U.compare_channel(v, t, 0)
+    v.UpdateApron()  # all channels, boundary 0
+    for c in range(4):
+        t.update_apron(c, 0.0)
+        U.compare_channel(v, t, c)
+    v.FillChannel(2, 1.5)
+    t.fill_channel(2, 1.5)
+    v.UpdateApron(2, -2.0)
+    t.update_apron(2, -2.0)
+    U.compare_channel(v, t, 2)
+    ln = v.subcell_info()[1]
+    v.GatherLevelSet(4, len(p), 1.0, (0, 0, 0), ln, 0)
+    t.gather_levelset(1.0, 0)
+    v.UpdateApron(0, 3.0)
+    t.update_apron(0, 3.0)
+    U.compare_channel(v, t, 0)
