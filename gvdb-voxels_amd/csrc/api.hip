@@ -14,6 +14,7 @@
 #include "binning.hpp"
 #include "kernels.hpp"
 #include "primitives.hpp"
+#include "splat.hpp"
 
 using namespace gvdbx;
 
@@ -73,6 +74,7 @@ struct gvdbx_ctx {
   bool have_vsum = false;
   Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
   Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb, nbm, spos, svel;
+  Buf sp_keys, sp_keys_alt, sp_vals, sp_vals_alt, sp_start;  // splat: points sorted by (brick, region)
 
   Buf keys, keys_alt, flags, rank, radix_counts, tile_buf, markers, meta;
   int* h_meta = nullptr;
@@ -1085,6 +1087,90 @@ int gvdbx_sample_trilinear(gvdbx_ctx* c, int chan, const void* q, int64_t n, voi
   TimedRegion tr(c, "sample");
   sample_trilinear(tree_of(c), P<int>(c->leaf_slot), res_of(c, 0), c->apron, P<float>(c->chan[chan]),
                    (const float*)q, n, (float*)out, c->stream);
+  return GVDBX_OK;
+}
+
+namespace {
+
+int field_checks(gvdbx_ctx* c, int chan, const void* q, int64_t n) {
+  CHK(chan_ok(c, chan));
+  if (!q || n < 0) return fail(c, GVDBX_ERR_ARG, "bad point buffers");
+  if (n > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "more than 2^31 points");
+  CHK(atlas_current(c));
+  if (c->depth < 0) return fail(c, GVDBX_ERR_STATE, "no topology");
+  return GVDBX_OK;
+}
+
+// Trilinear splat of (q, mass) into the interior of chan (sorted pull, see splat.hip).
+int splat_impl(gvdbx_ctx* c, int chan, const float* q, const float* mass, int64_t n, int accum) {
+  if (!splat_brick_supported(c->ld[0]))
+    return fail(c, GVDBX_ERR_CAPACITY, "splat supports leaf bricks of 2^1..2^4 voxels");
+  if (c->nslots * 27 + 2 > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "too many bricks for the splat keys");
+  if (c->nslots == 0) return GVDBX_OK;
+  const uint32_t sentinel = (uint32_t)(c->nslots * 27);
+  CHK(ensure(c, c->sp_keys, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->sp_keys_alt, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->sp_vals, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->sp_vals_alt, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->sp_start, (size_t)(sentinel + 2) * 4));
+  const int64_t ncnt = radix_count_ints(n);
+  CHK(ensure(c, c->radix_counts, (size_t)(ncnt + 1) * 4));
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(ncnt + 1) + 16) * 4));
+  DevTree T = tree_of(c);
+  splat_keys(T, q, n, P<int>(c->leaf_slot), c->ld[0], sentinel, P<uint32_t>(c->sp_keys), P<uint32_t>(c->sp_vals),
+             c->stream);
+  int bits = 1;
+  while (((int64_t)1 << bits) <= (int64_t)sentinel) ++bits;
+  const uint32_t* skeys = P<uint32_t>(c->sp_keys);
+  const uint32_t* svals = P<uint32_t>(c->sp_vals);
+  if (n > 0) {
+    const int par = radix_sort_u32(P<uint32_t>(c->sp_keys), P<uint32_t>(c->sp_keys_alt), P<uint32_t>(c->sp_vals),
+                                   P<uint32_t>(c->sp_vals_alt), n, 0, bits, P<int>(c->radix_counts),
+                                   P<int>(c->tile_buf), c->stream);
+    if (par) {
+      skeys = P<uint32_t>(c->sp_keys_alt);
+      svals = P<uint32_t>(c->sp_vals_alt);
+    }
+    key_ranges(skeys, n, (int)sentinel, P<int>(c->sp_start), c->stream);
+  } else {
+    fill_u32(P<uint32_t>(c->sp_start), sentinel + 2, 0u, c->stream);
+  }
+  splat_brick(c->nslots, P<int>(c->sp_start), svals, q, mass, P<int>(c->nbr), P<int4>(c->slot_pos), c->ld[0],
+              c->apron, P<float>(c->chan[chan]), accum, c->stream);
+  c->nbr_apron_ok[chan] = false;
+  return GVDBX_OK;
+}
+
+}  // namespace
+
+int gvdbx_splat_trilinear(gvdbx_ctx* c, int chan, const void* q, const void* mass, int64_t n, int accum) {
+  CHK(field_checks(c, chan, q, n));
+  TimedRegion tr(c, "splat");
+  return splat_impl(c, chan, (const float*)q, (const float*)mass, n, accum);
+}
+
+int gvdbx_splat_backward(gvdbx_ctx* c, int chan_grad, const void* q, const void* mass, int64_t n, void* dq,
+                         void* dmass) {
+  CHK(field_checks(c, chan_grad, q, n));
+  if (!splat_brick_supported(c->ld[0]))
+    return fail(c, GVDBX_ERR_CAPACITY, "splat supports leaf bricks of 2^1..2^4 voxels");
+  TimedRegion tr(c, "splat_backward");
+  splat_backward(tree_of(c), P<int>(c->leaf_slot), P<int>(c->nbr), c->ld[0], c->apron, P<float>(c->chan[chan_grad]),
+                 (const float*)q, (const float*)mass, n, (float*)dq, (float*)dmass, c->stream);
+  return GVDBX_OK;
+}
+
+int gvdbx_sample_trilinear_backward(gvdbx_ctx* c, int chan, const void* q, const void* dout, int64_t n, void* dq,
+                                    int chan_grad, int accum_grad) {
+  CHK(field_checks(c, chan, q, n));
+  if (!dout) return fail(c, GVDBX_ERR_ARG, "dout is null");
+  if (c->apron < 1) return fail(c, GVDBX_ERR_STATE, "trilinear sampling needs an apron >= 1");
+  if (chan_grad >= 0) CHK(chan_ok(c, chan_grad));
+  TimedRegion tr(c, "sample_backward");
+  if (dq)
+    sample_backward(tree_of(c), P<int>(c->leaf_slot), res_of(c, 0), c->apron, P<float>(c->chan[chan]),
+                    (const float*)q, (const float*)dout, n, (float*)dq, c->stream);
+  if (chan_grad >= 0) CHK(splat_impl(c, chan_grad, (const float*)q, (const float*)dout, n, accum_grad));
   return GVDBX_OK;
 }
 

@@ -154,6 +154,35 @@ class VolumeGVDB:
     def SampleTrilinear(self, chan, q_dev, n, out_dev):
         self._chk(self.L.gvdbx_sample_trilinear(self.h, chan, q_dev.ptr, n, out_dev.ptr), "SampleTrilinear")
 
+    # ------------------------------------------------------------ splat and adjoints
+    @staticmethod
+    def _ptr(d):
+        return None if d is None else d.ptr
+
+    def SplatTrilinear(self, chan, q_dev, n, mass_dev=None, accum=False):
+        """Trilinear mass splat (transpose of SampleTrilinear) into chan's interior."""
+        self._chk(self.L.gvdbx_splat_trilinear(self.h, chan, q_dev.ptr, self._ptr(mass_dev), n, int(accum)),
+                  "SplatTrilinear")
+
+    def SplatBackward(self, chan_grad, q_dev, n, mass_dev=None, dq_dev=None, dmass_dev=None):
+        """dL/dq and dL/dmass of the splat from dL/drho held in chan_grad."""
+        self._chk(self.L.gvdbx_splat_backward(self.h, chan_grad, q_dev.ptr, self._ptr(mass_dev), n, self._ptr(dq_dev),
+                                              self._ptr(dmass_dev)), "SplatBackward")
+
+    def SampleTrilinearBackward(self, chan, q_dev, dout_dev, n, dq_dev=None, chan_grad=-1, accum_grad=False):
+        """dL/dq of SampleTrilinear and (chan_grad >= 0) dL/dfield = splat of dout."""
+        self._chk(self.L.gvdbx_sample_trilinear_backward(self.h, chan, q_dev.ptr, dout_dev.ptr, n, self._ptr(dq_dev),
+                                                         chan_grad, int(accum_grad)), "SampleTrilinearBackward")
+
+    def set_channel(self, chan, data):
+        """Overwrite a channel (float[slots][br][br][br]) from host memory."""
+        ptr, floats = C.c_void_p(), C.c_int64(0)
+        self._chk(self.L.gvdbx_get_channel_ptr(self.h, chan, C.byref(ptr), C.byref(floats)), "get_channel_ptr")
+        a = np.ascontiguousarray(data, np.float32).reshape(-1)
+        if a.size > floats.value:
+            raise ValueError(f"channel holds {floats.value} floats, got {a.size}")
+        self._chk(self.L.gvdbx_commit(self.h, ptr.value, a.ctypes.data, a.nbytes), "commit")
+
     # ------------------------------------------------------------ introspection
     def depth(self):
         d = C.c_int(0)

@@ -121,6 +121,23 @@ int gvdbx_update_apron_channels(gvdbx_ctx* ctx, const int* chans, const float* b
  *      0 outside active bricks.  q: device float3[n]; out: device float[n]. */
 int gvdbx_sample_trilinear(gvdbx_ctx* ctx, int chan, const void* q, int64_t n, void* out);
 
+/* ---- trilinear splat and adjoints (no reference counterpart: the transpose of the sampler
+ *      above, R.cuh:104-126).  splat: chan interior (+)= sum_i m_i w_g(q_i) over the 8 voxels
+ *      g of floor(q - 0.5) that lie in active bricks, points in no brick skipped; mass NULL =
+ *      unit masses.  Deterministic: points sorted by (brick, region), exact 64-bit fixed-point
+ *      sums per brick.  The apron is not written (UpdateApron afterwards). */
+int gvdbx_splat_trilinear(gvdbx_ctx* ctx, int chan, const void* q, const void* mass, int64_t n, int accum);
+/* Backward of the splat given G = dL/drho in chan_grad's interior: dmass_i = sum_g w_g(q_i) G(g),
+ * dq_i = m_i sum_g grad w_g(q_i) G(g).  dq: device float3[n] or NULL; dmass: float[n] or NULL. */
+int gvdbx_splat_backward(gvdbx_ctx* ctx, int chan_grad, const void* q, const void* mass, int64_t n, void* dq,
+                         void* dmass);
+/* Backward of gvdbx_sample_trilinear given dout = dL/dsample: dq_i = dout_i * grad sample(q_i)
+ * (same padded-brick filter, dq NULL to skip) and, if chan_grad >= 0, chan_grad interior (+)=
+ * splat of dout (the field gradient; exact when the apron holds neighbour copies and a zero
+ * boundary value). */
+int gvdbx_sample_trilinear_backward(gvdbx_ctx* ctx, int chan, const void* q, const void* dout, int64_t n, void* dq,
+                                    int chan_grad, int accum_grad);
+
 /* ---- introspection (getNumTotalNodes/getNumUsedNodes/getNode V.h:637-641, getAux) */
 int gvdbx_get_depth(const gvdbx_ctx* ctx, int* depth, int32_t rootpos[3], float posmin[3], float posmax[3]);
 int gvdbx_get_node_count(const gvdbx_ctx* ctx, int lev, int64_t* total, int64_t* used);

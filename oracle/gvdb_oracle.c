@@ -807,6 +807,40 @@ static void tri_corners(const float* q, int32_t* g0, double* al) {
   }
 }
 
+/* Backward of sampling with respect to the query points: dq_i = dout_i * grad sample(q_i),
+ * the derivative of the same padded-brick trilinear filter (0 where the sample is 0).  The
+ * backward with respect to the field is the splat of dout (orc_splat_trilinear). */
+int orc_sample_backward(const orc_tree* t, int chan, const float* q, const double* dout, int64_t n, double* dq) {
+  if (!t->chan[chan]) return ORC_ERR_STATE;
+  int a = t->apron;
+  const double* ch = t->chan[chan];
+  for (int64_t i = 0; i < n; i++) {
+    dq[3 * i] = dq[3 * i + 1] = dq[3 * i + 2] = 0;
+    int64_t f = leaf_at(t, q + 3 * i);
+    if (f < 0 || t->leaf_slot[f] < 0) continue;
+    const orc_node* nd = &t->nodes[0][f];
+    int i0[3]; double al[3];
+    for (int c = 0; c < 3; c++) {
+      double x = (double)q[3 * i + c] - nd->pos[c] + a - 0.5;
+      double fl = floor(x);
+      i0[c] = (int)fl;
+      al[c] = x - fl;
+    }
+    double gx = 0, gy = 0, gz = 0;
+    for (int dz = 0; dz < 2; dz++)
+      for (int dy = 0; dy < 2; dy++)
+        for (int dx = 0; dx < 2; dx++) {
+          double v = ch[atlas_index(t, t->leaf_slot[f], i0[0] + dx, i0[1] + dy, i0[2] + dz)];
+          double wx = dx ? al[0] : 1 - al[0], wy = dy ? al[1] : 1 - al[1], wz = dz ? al[2] : 1 - al[2];
+          gx += (dx ? 1.0 : -1.0) * wy * wz * v;
+          gy += wx * (dy ? 1.0 : -1.0) * wz * v;
+          gz += wx * wy * (dz ? 1.0 : -1.0) * v;
+        }
+    dq[3 * i] = dout[i] * gx; dq[3 * i + 1] = dout[i] * gy; dq[3 * i + 2] = dout[i] * gz;
+  }
+  return ORC_OK;
+}
+
 /* Trilinear splat rho(g) += m * w_g(q) into interior voxels of the leaves holding the
  * corners (the transpose of sampling over the global voxel lattice). */
 int orc_splat_trilinear(orc_tree* t, int chan, const float* q, const float* mass, int64_t n) {

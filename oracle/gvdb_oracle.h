@@ -111,6 +111,8 @@ int orc_sample_trilinear(const orc_tree* t, int chan, const float* q, int64_t n,
 int orc_splat_trilinear(orc_tree* t, int chan, const float* q, const float* mass, int64_t n);
 int orc_splat_backward(const orc_tree* t, int chan_grad, const float* q, const float* mass, int64_t n,
                        double* dq, double* dmass);
+/* Sampling backward with respect to the points: dq = dout * grad sample(q). */
+int orc_sample_backward(const orc_tree* t, int chan, const float* q, const double* dout, int64_t n, double* dq);
 
 /* Leaf lookup at integer voxel position (tree descent, -1 if none). */
 int64_t orc_find_leaf(const orc_tree* t, const int32_t* p);

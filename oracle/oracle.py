@@ -74,6 +74,7 @@ def lib():
         L.orc_sample_trilinear.argtypes = [P, C.c_int, P, i64, P]
         L.orc_splat_trilinear.argtypes = [P, C.c_int, P, P, i64]
         L.orc_splat_backward.argtypes = [P, C.c_int, P, P, i64, P, P]
+        L.orc_sample_backward.argtypes = [P, C.c_int, P, P, i64, P]
         L.orc_find_leaf.restype = i64
         L.orc_find_leaf.argtypes = [P, P]
         L.orc_cpu_activate_space.restype = i64
@@ -271,6 +272,13 @@ class Tree:
         _chk(lib().orc_splat_backward(self.h, chan_grad, _ptr(q), _ptr(m), len(q), _ptr(dq), _ptr(dm)),
              "splat_backward")
         return dq, dm
+
+    def sample_backward(self, chan, q, dout):
+        q = np.ascontiguousarray(q, np.float32)
+        d = np.ascontiguousarray(dout, np.float64)
+        dq = np.zeros((len(q), 3))
+        _chk(lib().orc_sample_backward(self.h, chan, _ptr(q), _ptr(d), len(q), _ptr(dq)), "sample_backward")
+        return dq
 
 
 def cpu_activate_space(pos, log2dims=(3, 3, 3, 3, 5), trans=(0, 0, 0)):

@@ -222,3 +222,37 @@ def test_splat_backward_matches_finite_differences(oracle_mod):
         mm[i] -= np.float32(0.01)
         fd = (loss(qs, mp) - loss(qs, mm)) / (float(mp[i]) - float(mm[i]))
         assert abs(fd - dm[i]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_sample_backward_matches_finite_differences(oracle_mod):
+    """dq of sampling = dout * grad of the sampled field: FP64 central differences of
+    orc_sample_trilinear over a random field whose aprons hold neighbour copies and a
+    non-zero boundary value."""
+    import ctypes as C
+    t, p = _affine_tree(oracle_mod)
+    g = np.random.default_rng(6)
+    f = g.standard_normal(t.channel(0).shape)
+    ptr = oracle_mod.lib().orc_channel(t.h, 0)
+    C.memmove(ptr, f.ctypes.data, f.nbytes)
+    t.update_apron(0, 0.7)
+    qs = g.uniform(35, 85, (400, 3)).astype(np.float32)
+    dout = g.standard_normal(400)
+    dq = t.sample_backward(0, qs, dout)
+    eps = np.float32(1e-3)
+    checked = 0
+    for a in range(3):
+        qp, qm = qs.copy(), qs.copy()
+        qp[:, a] += eps
+        qm[:, a] -= eps
+        # skip points whose perturbation crosses a voxel-centre plane (the filter has a kink there)
+        same = np.floor(qp[:, a].astype(np.float64) - 0.5) == np.floor(qm[:, a].astype(np.float64) - 0.5)
+        same &= np.floor(qp[:, a]) == np.floor(qm[:, a])
+        fd = (t.sample_trilinear(0, qp) - t.sample_trilinear(0, qm)) / (qp[:, a].astype(np.float64) - qm[:, a])
+        want = dout * fd
+        err = np.abs(want - dq[:, a])[same] / np.maximum(np.abs(want[same]), 1.0)
+        assert err.max() < 1e-9, err.max()
+        checked += same.sum()
+    assert checked > 1000
+    assert (np.abs(dq) > 0).sum() > 500
+    outside = np.array([[1.0, 1.0, 1.0], [500.0, 3.0, 3.0]], np.float32)  # no leaf: zero gradient
+    assert not t.sample_backward(0, outside, np.ones(2)).any()
