@@ -90,6 +90,98 @@ def cpu_baseline(p, vel, sample):
             "reference_cpu_activate_space_Mpoints_s": round(sample / dt_act / 1e6, 3)}
 
 
+def adjoint_bench(reps):
+    """The 2,097,152-point adjoint workload: forward splat of the points (masses U(0.5, 1.5))
+    against the backward gather of dL/drho into dL/dq and dL/dm, on the union topology of the
+    points and a second draw (the target).  Inputs resident in HBM; wall clock over `reps`
+    synchronised repetitions of each."""
+    from gvdbx import VolumeGVDB, synth
+    a, ma = synth.uniform_massive(seed=4)
+    b, _ = synth.uniform_massive(seed=4 + 1000)
+    v = VolumeGVDB(0)
+    v.Configure(5, 3, 3, 3, 3)
+    v.SetChannelDefault(16, 16, 1)
+    for c in range(2):
+        v.AddChannel(c, apron=1)
+    allp = np.concatenate([a, b])
+    dall = v.to_device(allp)
+    v.SetPoints(dall, n=len(allp))
+    v.RebuildTopology(len(allp), RADIUS)
+    v.UpdateAtlas()
+    n = len(a)
+    da, dm = v.to_device(a), v.to_device(ma)
+    s, br = v.num_slots()
+    G = np.random.default_rng(0).standard_normal((s, br, br, br)).astype(np.float32) * np.float32(1e-7)
+    v.set_channel(1, G)
+    gq, gm = v.alloc(12 * n), v.alloc(4 * n)
+    for _ in range(2):
+        v.SplatTrilinear(0, da, n, dm)
+        v.SplatBackward(1, da, n, dm, gq, gm)
+    v.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        v.SplatTrilinear(0, da, n, dm)
+    v.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        v.SplatBackward(1, da, n, dm, gq, gm)
+    v.synchronize()
+    t2 = time.perf_counter()
+    ev = {}
+    for name in ("splat", "splat_sort", "splat_brick"):
+        v.kernel_timer(name, True)
+        for _ in range(reps):
+            v.SplatTrilinear(0, da, n, dm)
+        v.synchronize()
+        k_ms, k_n = v.kernel_timer_read()
+        v.kernel_timer(None, False)
+        ev[name] = k_ms / max(k_n, 1)
+    v.kernel_timer("splat_backward", True)
+    for _ in range(reps):
+        v.SplatBackward(1, da, n, dm, gq, gm)
+    v.synchronize()
+    k_ms, k_n = v.kernel_timer_read()
+    v.kernel_timer(None, False)
+    ev["splat_backward"] = k_ms / max(k_n, 1)
+    fwd = (t1 - t0) * 1e3 / reps
+    bwd = (t2 - t1) * 1e3 / reps
+    v.close()
+    # forward algorithmic bytes: positions + masses read, interior of every brick written
+    fbytes = n * 16 + int(s) * 512 * 4
+    return {"workload": "adjoint_2M (2,097,152 points U(0,256)^3, masses U(0.5,1.5), r=1 union topology)",
+            "bricks": int(s), "forward_ms": round(fwd, 4), "backward_ms": round(bwd, 4),
+            "backward_over_forward": round(bwd / fwd, 3),
+            "forward_Mpoints_s": round(n / (fwd * 1e-3) / 1e6, 1),
+            "event_ms": {k: round(x, 4) for k, x in ev.items()},
+            "forward_algorithmic_bytes": fbytes,
+            "forward_hbm_frac": round(fbytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def calibrate_copy():
+    """A plain device-to-device copy of 256 MiB, three times: the reference reading of the HBM
+    counters (FETCH_SIZE / WRITE_SIZE) that scales the kernel's counter values to bytes.
+    Uses torch only (the library is not loaded in this mode)."""
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": n * 4, "copies": 3}))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the roofline kernel from the committed counter summary
+    (profiles/pmc_traffic.json: FETCH_SIZE and WRITE_SIZE of the same workload, each scaled
+    by its reading for a copy of known size), or None."""
+    f = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(f):
+        return None
+    e = json.load(open(f)).get("kernels", {}).get(kernel)
+    return int(e["bytes_per_launch"]) if e else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,8 +190,13 @@ def main():
     ap.add_argument("--points", type=int, default=N_POINTS)
     ap.add_argument("--cpu-sample", type=int, default=N_POINTS)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--kernel", default="gather", help="kernel family timed for the roofline")
+    ap.add_argument("--calibrate-copy", action="store_true", help="only run the counter-calibration copy")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not implemented)"}))
@@ -168,12 +265,16 @@ def main():
         "roofline": {"bound": "hbm", "kernel": args.kernel,
                      "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "avg_launch_ms": round(k_avg_ms, 5), "launches": int(k_launches),
+                     "traffic": measured_traffic("k_density_brick") if args.kernel == "gather" else None,
+                     "avg_launch_ms": round(k_avg_ms, 5), "launches": int(k_launches),
                      "algorithmic_bytes_per_launch": int(kb) if kb else None},
     }
+    del dposs
+    v.close()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench(max(args.steps, 5))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(poss[0], vel, min(args.cpu_sample, n))
-    v.close()
     print(json.dumps(out))
 
 

@@ -74,7 +74,12 @@ struct gvdbx_ctx {
   bool have_vsum = false;
   Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
   Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb, nbm, spos, svel;
-  Buf sp_keys, sp_keys_alt, sp_vals, sp_vals_alt, sp_start;  // splat: points sorted by (brick, region)
+  Buf sp_cnt, sp_start, sp_keyrank, sp_payload, sp_perm;  // splat: points counting-sorted by (brick, region)
+  // brick order of the last splat (a permutation of its n points), reused by the splat
+  // backward on the same point buffer for locality
+  const void* sp_order_q = nullptr;
+  int64_t sp_order_n = 0;
+  const uint32_t* sp_order = nullptr;
 
   Buf keys, keys_alt, flags, rank, radix_counts, tile_buf, markers, meta;
   int* h_meta = nullptr;
@@ -1108,35 +1113,30 @@ int splat_impl(gvdbx_ctx* c, int chan, const float* q, const float* mass, int64_
   if (c->nslots * 27 + 2 > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "too many bricks for the splat keys");
   if (c->nslots == 0) return GVDBX_OK;
   const uint32_t sentinel = (uint32_t)(c->nslots * 27);
-  CHK(ensure(c, c->sp_keys, (size_t)(n + 1) * 4));
-  CHK(ensure(c, c->sp_keys_alt, (size_t)(n + 1) * 4));
-  CHK(ensure(c, c->sp_vals, (size_t)(n + 1) * 4));
-  CHK(ensure(c, c->sp_vals_alt, (size_t)(n + 1) * 4));
-  CHK(ensure(c, c->sp_start, (size_t)(sentinel + 2) * 4));
-  const int64_t ncnt = radix_count_ints(n);
-  CHK(ensure(c, c->radix_counts, (size_t)(ncnt + 1) * 4));
-  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(ncnt + 1) + 16) * 4));
-  DevTree T = tree_of(c);
-  splat_keys(T, q, n, P<int>(c->leaf_slot), c->ld[0], sentinel, P<uint32_t>(c->sp_keys), P<uint32_t>(c->sp_vals),
-             c->stream);
-  int bits = 1;
-  while (((int64_t)1 << bits) <= (int64_t)sentinel) ++bits;
-  const uint32_t* skeys = P<uint32_t>(c->sp_keys);
-  const uint32_t* svals = P<uint32_t>(c->sp_vals);
-  if (n > 0) {
-    const int par = radix_sort_u32(P<uint32_t>(c->sp_keys), P<uint32_t>(c->sp_keys_alt), P<uint32_t>(c->sp_vals),
-                                   P<uint32_t>(c->sp_vals_alt), n, 0, bits, P<int>(c->radix_counts),
-                                   P<int>(c->tile_buf), c->stream);
-    if (par) {
-      skeys = P<uint32_t>(c->sp_keys_alt);
-      svals = P<uint32_t>(c->sp_vals_alt);
-    }
-    key_ranges(skeys, n, (int)sentinel, P<int>(c->sp_start), c->stream);
-  } else {
-    fill_u32(P<uint32_t>(c->sp_start), sentinel + 2, 0u, c->stream);
+  const int64_t nkeys = (int64_t)sentinel + 1;
+  CHK(ensure(c, c->sp_cnt, (size_t)(nkeys + 1) * 4));
+  CHK(ensure(c, c->sp_start, (size_t)(nkeys + 1) * 4));
+  CHK(ensure(c, c->sp_keyrank, (size_t)(n + 1) * 8));
+  CHK(ensure(c, c->sp_payload, (size_t)(n + 1) * 16));
+  CHK(ensure(c, c->sp_perm, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(nkeys + 1) + 16) * 4));
+  {
+    TimedRegion tr(c, "splat_sort");
+    fill_u32(P<uint32_t>(c->sp_cnt), nkeys, 0u, c->stream);
+    splat_count(tree_of(c), q, n, P<int>(c->leaf_slot), c->ld[0], sentinel, P<uint32_t>(c->sp_cnt),
+                P<uint2>(c->sp_keyrank), c->stream);
+    exclusive_scan(P<int>(c->sp_cnt), P<int>(c->sp_start), nkeys, P<int>(c->tile_buf), nullptr, c->stream);
+    splat_place(q, mass, n, P<uint2>(c->sp_keyrank), P<int>(c->sp_start), P<float4>(c->sp_payload),
+                P<uint32_t>(c->sp_perm), c->stream);
   }
-  splat_brick(c->nslots, P<int>(c->sp_start), svals, q, mass, P<int>(c->nbr), P<int4>(c->slot_pos), c->ld[0],
-              c->apron, P<float>(c->chan[chan]), accum, c->stream);
+  {
+    TimedRegion tr(c, "splat_brick");
+    splat_brick(c->nslots, P<int>(c->sp_start), P<float4>(c->sp_payload), P<int>(c->nbr), P<int4>(c->slot_pos),
+                c->ld[0], c->apron, P<float>(c->chan[chan]), accum, c->stream);
+  }
+  c->sp_order_q = n > 0 ? q : nullptr;
+  c->sp_order_n = n;
+  c->sp_order = n > 0 ? P<uint32_t>(c->sp_perm) : nullptr;
   c->nbr_apron_ok[chan] = false;
   return GVDBX_OK;
 }
@@ -1155,8 +1155,9 @@ int gvdbx_splat_backward(gvdbx_ctx* c, int chan_grad, const void* q, const void*
   if (!splat_brick_supported(c->ld[0]))
     return fail(c, GVDBX_ERR_CAPACITY, "splat supports leaf bricks of 2^1..2^4 voxels");
   TimedRegion tr(c, "splat_backward");
+  const uint32_t* order = (c->sp_order && c->sp_order_q == q && c->sp_order_n == n) ? c->sp_order : nullptr;
   splat_backward(tree_of(c), P<int>(c->leaf_slot), P<int>(c->nbr), c->ld[0], c->apron, P<float>(c->chan[chan_grad]),
-                 (const float*)q, (const float*)mass, n, (float*)dq, (float*)dmass, c->stream);
+                 (const float*)q, (const float*)mass, n, order, (float*)dq, (float*)dmass, c->stream);
   return GVDBX_OK;
 }
 

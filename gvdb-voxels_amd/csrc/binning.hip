@@ -115,19 +115,22 @@ void home_keys(const DevTree& T, const PointsView& P, const int* map, int nmap, 
   hipLaunchKernelGGL(k_home_keys, dim3(nblocks(P.n, 256)), dim3(256), 0, s, T, P, map, nmap, keys, vals);
 }
 
-// start[k] = first sorted position with key >= k, for k in [0, nmap + 1].
+// start[k] = first sorted position with key >= k, for k in [0, nmap + 1]: one lower-bound
+// search per key, so the work per thread is log2(n) whatever the gaps between keys.
 __global__ void k_ranges(const uint32_t* __restrict__ skeys, int64_t n, int nmap, int* __restrict__ start) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int k = (int)skeys[i];
-  const int kp = i == 0 ? -1 : (int)skeys[i - 1];
-  for (int kk = kp + 1; kk <= k; ++kk) start[kk] = (int)i;
-  if (i == n - 1)
-    for (int kk = k + 1; kk <= nmap + 1; ++kk) start[kk] = (int)n;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > (int64_t)nmap + 1) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)skeys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  start[k] = (int)lo;
 }
 
 void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStream_t s) {
-  hipLaunchKernelGGL(k_ranges, dim3(nblocks(n, 256)), dim3(256), 0, s, skeys, n, nmap, start);
+  hipLaunchKernelGGL(k_ranges, dim3(nblocks((int64_t)nmap + 2, 256)), dim3(256), 0, s, skeys, n, nmap, start);
 }
 
 // ------------------------------------------------------------------ per-brick binning

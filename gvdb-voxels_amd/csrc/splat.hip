@@ -5,15 +5,23 @@
 // c = floor(q - 0.5), for every g inside an active brick; points whose own voxel floor(q) lies
 // in no brick are skipped, as the sampler returns 0 there.
 //
-// Forward (deterministic, no device-scope atomics):
-//   1. k_splat_keys: key = slot(home brick) * 27 + region, where region records per axis whether
+// Forward (deterministic, no device-scope atomics on the field):
+//   1. k_splat_count: key = slot(home brick) * 27 + region, where region records per axis whether
 //      the point's dual cell pokes into the lower (0) or upper (2) neighbour brick or neither (1);
-//   2. stable radix sort of (key, point index) and start[] of every key;
-//   3. k_splat_brick: one workgroup per brick pulls its own points plus exactly those points of
-//      the 26 neighbours whose region reaches it, and sums m * w in LDS as 64-bit integers with a
+//      points in no brick get the last key.  Each point takes a rank inside its key from a
+//      wave-aggregated atomic counter (a counting sort: the order inside a key is free);
+//   2. exclusive scan of the counts = start[] of every key;
+//   3. k_splat_place: (q, m) of each point written as float4 at start[key] + rank, and the
+//      permutation (sorted position -> point) kept for the backward;
+//   4. k_splat_brick: one wave per brick pulls its own points plus exactly those points of the
+//      26 neighbours whose region reaches it, and sums m * w in LDS as 64-bit integers with a
 //      per-brick power-of-two scale taken from max |m| -- integer sums are order independent, so
-//      the FP32 result is bit-reproducible; the interior is written once, no global atomics.
-// Backward of the splat (gather): dL/dm_i = sum_g w_g(q_i) G(g), dL/dq_i = m_i sum_g grad w_g G(g).
+//      the FP32 result is bit-reproducible whatever the order inside a key; the interior is
+//      written once.
+// Backward of the splat (gather): dL/dm_i = sum_g w_g(q_i) G(g), dL/dq_i = m_i sum_g grad w_g G(g),
+// one thread per point, visiting the points in the brick order of the last forward splat of the
+// same buffer when there is one (any permutation gives the same result; brick order keeps
+// the reads of G local).
 // Backward of sampling: dL/dq_i = dout_i * grad sample(q_i) (same padded-brick filter as the
 // sampler); dL/dfield = splat of dout (the forward above).
 #include "splat.hpp"
@@ -30,26 +38,56 @@ __device__ __forceinline__ int region_of(int l, float frac, int res) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) k_splat_keys(DevTree T, const float* __restrict__ q, int64_t n,
-                                                    const int* __restrict__ leaf_slot, int lgres, uint32_t sentinel,
-                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__device__ __forceinline__ uint32_t splat_key(const DevTree& T, float px, float py, float pz,
+                                              const int* __restrict__ leaf_slot, int lgres, uint32_t sentinel) {
+  if (!(px >= 0.f && py >= 0.f && pz >= 0.f)) return sentinel;
+  const int ix = (int)px, iy = (int)py, iz = (int)pz;
+  const int64_t lf = descend(T, ix, iy, iz, 0);
+  const int slot = lf < 0 ? -1 : leaf_slot[lf];
+  if (slot < 0) return sentinel;
+  const int res = 1 << lgres, m = res - 1;
+  const int rx = region_of(ix & m, px - (float)ix, res), ry = region_of(iy & m, py - (float)iy, res),
+            rz = region_of(iz & m, pz - (float)iz, res);
+  return (uint32_t)slot * 27u + (uint32_t)(rz * 9 + ry * 3 + rx);
+}
+
+// key and rank of every point; lanes of a wave that share a key take one atomic together
+__global__ void __launch_bounds__(256) k_splat_count(DevTree T, const float* __restrict__ q, int64_t n,
+                                                     const int* __restrict__ leaf_slot, int lgres, uint32_t sentinel,
+                                                     uint32_t* __restrict__ cnt, uint2* __restrict__ keyrank) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  const uint32_t key = live ? splat_key(T, q[3 * i], q[3 * i + 1], q[3 * i + 2], leaf_slot, lgres, sentinel) : 0u;
+  const int lane = lane_id();
+  uint64_t pending = __ballot(live);
+  int leader = 0, pos = 0, size = 0;
+  while (pending) {
+    const int l = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t kl = __shfl(key, l, 64);
+    const uint64_t grp = __ballot(live && key == kl) & pending;
+    if (live && key == kl) {
+      leader = l;
+      pos = __popcll(grp & lanemask_lt());
+      size = __popcll(grp);
+    }
+    pending &= ~grp;
+  }
+  uint32_t old = 0;
+  if (live && lane == leader) old = atomicAdd(&cnt[key], (uint32_t)size);
+  const uint32_t base = __shfl(old, leader, 64);
+  if (live) keyrank[i] = make_uint2(key, base + (uint32_t)pos);
+}
+
+__global__ void __launch_bounds__(256) k_splat_place(const float* __restrict__ q, const float* __restrict__ mass,
+                                                     int64_t n, const uint2* __restrict__ keyrank,
+                                                     const int* __restrict__ start, float4* __restrict__ payload,
+                                                     uint32_t* __restrict__ perm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
-  uint32_t key = sentinel;
-  if (px >= 0.f && py >= 0.f && pz >= 0.f) {
-    const int ix = (int)px, iy = (int)py, iz = (int)pz;
-    const int64_t lf = descend(T, ix, iy, iz, 0);
-    const int slot = lf < 0 ? -1 : leaf_slot[lf];
-    if (slot >= 0) {
-      const int res = 1 << lgres, m = res - 1;
-      const int rx = region_of(ix & m, px - (float)ix, res), ry = region_of(iy & m, py - (float)iy, res),
-                rz = region_of(iz & m, pz - (float)iz, res);
-      key = (uint32_t)slot * 27u + (uint32_t)(rz * 9 + ry * 3 + rx);
-    }
-  }
-  keys[i] = key;
-  vals[i] = (uint32_t)i;
+  const uint2 kr = keyrank[i];
+  const int64_t dst = (int64_t)start[kr.x] + kr.y;
+  payload[dst] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], mass ? mass[i] : 1.f);
+  perm[dst] = (uint32_t)i;
 }
 
 // Per axis, the 5 (neighbour offset, region) pairs whose points reach this brick.
@@ -59,102 +97,97 @@ __device__ __forceinline__ void axis_combo(int k, int& d, int& r) {
 }
 
 template <int kLg>
-__global__ void __launch_bounds__(256) k_splat_brick(const int* __restrict__ start, const uint32_t* __restrict__ svals,
-                                                    const float* __restrict__ q, const float* __restrict__ mass,
-                                                    const int* __restrict__ nbr, const int4* __restrict__ slot_pos,
-                                                    int apron, float* __restrict__ chan, int accum) {
+__global__ void __launch_bounds__(64) k_splat_brick(const int* __restrict__ start, const float4* __restrict__ payload,
+                                                   const int* __restrict__ nbr, const int4* __restrict__ slot_pos,
+                                                   int apron, float* __restrict__ chan, int accum) {
   constexpr int kRes = 1 << kLg, kVox = kRes * kRes * kRes, kRanges = 125;
   __shared__ long long acc[kVox];
-  __shared__ int rbeg[kRanges];
-  __shared__ int rpre[kRanges + 1];
-  __shared__ unsigned int mbits;
-  const int s = blockIdx.x, tid = threadIdx.x;
-  for (int v = tid; v < kVox; v += blockDim.x) acc[v] = 0;
-  if (tid == 0) mbits = 0u;
-  // range table: 125 (direction, region) key ranges, prefix of their lengths by wave 0
-  int len = 0, beg = 0;
-  if (tid < kRanges) {
+  __shared__ int rbeg[kRanges];      // non-empty (direction, region) ranges: first sorted position
+  __shared__ int rpre[kRanges + 1];  // their starts in the brick's virtual point list
+  const int s = blockIdx.x, lane = threadIdx.x;  // one wave per brick
+  for (int v = lane; v < kVox; v += 64) acc[v] = 0;
+  const int nbv = lane < 27 ? nbr[(int64_t)s * 27 + lane] : -1;
+  // lane l owns ranges l and l + 64; empty ones are dropped by ballot compaction
+  int beg[2] = {0, 0}, len[2] = {0, 0};
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int r = lane + 64 * t;
     int dx, dy, dz, rx, ry, rz;
-    axis_combo(tid % 5, dx, rx);
-    axis_combo((tid / 5) % 5, dy, ry);
-    axis_combo(tid / 25, dz, rz);
-    const int nb = nbr[(int64_t)s * 27 + (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-    if (nb >= 0) {
+    axis_combo(r % 5, dx, rx);
+    axis_combo((r / 5) % 5, dy, ry);
+    axis_combo((r / 25) % 5, dz, rz);
+    const int nb = __shfl(nbv, (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1), 64);
+    if (r < kRanges && nb >= 0) {
       const int64_t k = (int64_t)nb * 27 + rz * 9 + ry * 3 + rx;
-      beg = start[k];
-      len = start[k + 1] - beg;
+      beg[t] = start[k];
+      len[t] = start[k + 1] - beg[t];
     }
-    rbeg[tid] = beg;
   }
-  if (tid == 0) rpre[0] = 0;
-  if (tid < kRanges) rpre[tid + 1] = len;
-  __syncthreads();
-  if (tid < 64) {
-    // wave 0 scans the lengths, two ranges per lane: lane l holds ranges 2l and 2l + 1,
-    // whose lengths sit in rpre[2l + 1] and rpre[2l + 2] (indices up to kRanges exist)
-    const int ia = 2 * tid + 1, ib = 2 * tid + 2;
-    const int a = ia <= kRanges ? rpre[ia] : 0, b = ib <= kRanges ? rpre[ib] : 0;
-    int incl = a + b;
+  int base = 0, nr = 0;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int incl = len[t];
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-      const int t = __shfl_up(incl, o, 64);
-      if (tid >= o) incl += t;
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
     }
-    if (ia <= kRanges) rpre[ia] = incl - b;
-    if (ib <= kRanges) rpre[ib] = incl;
+    const uint64_t m = __ballot(len[t] > 0);
+    if (len[t] > 0) {
+      const int e = nr + __popcll(m & lanemask_lt());
+      rbeg[e] = beg[t];
+      rpre[e] = base + incl - len[t];
+    }
+    nr += __popcll(m);
+    base += __shfl(incl, 63, 64);
   }
+  const int total = base;
+  if (lane == 0) rpre[nr] = total;
   __syncthreads();
-  const int total = rpre[kRanges];
-  if (total == 0) {
-    // no point reaches this brick
-    const int br = kRes + 2 * apron;
-    float* dst = chan + (int64_t)s * br * br * br;
+  const int br = kRes + 2 * apron;
+  float* dst = chan + (int64_t)s * br * br * br;
+  if (total == 0) {  // no point reaches this brick
     if (!accum)
-      for (int v = tid; v < kVox; v += blockDim.x) {
+      for (int v = lane; v < kVox; v += 64) {
         const int x = v & (kRes - 1), y = (v >> kLg) & (kRes - 1), z = v >> (2 * kLg);
         dst[((z + apron) * br + (y + apron)) * br + (x + apron)] = 0.f;
       }
     return;
   }
-  auto point_of = [&](int j) -> uint32_t {
-    int lo = 0, hi = kRanges - 1;  // last range with rpre[r] <= j
+  auto point_of = [&](int j) -> float4 {
+    int lo = 0, hi = nr - 1;  // last range starting at or before j
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (rpre[mid] <= j) lo = mid;
       else hi = mid - 1;
     }
-    return svals[rbeg[lo] + (j - rpre[lo])];
+    return payload[rbeg[lo] + (j - rpre[lo])];
   };
-  // per-brick fixed-point scale: every voxel receives at most `total` terms of |m w| <= max|m|
-  if (mass) {
-    unsigned int mb = 0u;
-    for (int j = tid; j < total; j += blockDim.x) {
-      const unsigned int b = __float_as_uint(fabsf(mass[point_of(j)]));
-      mb = b > mb ? b : mb;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const unsigned int t = __shfl_xor(mb, o, 64);
-      mb = t > mb ? t : mb;
-    }
-    if ((tid & 63) == 0) atomicMax(&mbits, mb);
+  // per-brick fixed-point scale: each voxel receives at most `total` terms of |m w| <= max|m|;
+  // the lane's first point stays in registers between the two passes
+  const float4 p0 = lane < total ? point_of(lane) : make_float4(0.f, 0.f, 0.f, 0.f);
+  unsigned int mb = __float_as_uint(fabsf(p0.w));
+  for (int j = lane + 64; j < total; j += 64) {
+    const unsigned int b = __float_as_uint(fabsf(point_of(j).w));
+    mb = b > mb ? b : mb;
   }
-  __syncthreads();
-  const double maxm = mass ? (double)__uint_as_float(mbits) : 1.0;
-  const double bound = maxm * (double)total;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned int u = __shfl_xor(mb, o, 64);
+    mb = u > mb ? u : mb;
+  }
+  const double bound = (double)__uint_as_float(mb) * (double)total;
   int e = 0;
   if (bound > 0.0) (void)frexp(bound, &e);  // bound < 2^e
   const int S = 62 - e;                     // every partial sum stays below 2^62
   const int4 bp = slot_pos[s];
-  for (int j = tid; j < total; j += blockDim.x) {
-    const uint32_t i = point_of(j);
-    const double m = mass ? (double)mass[i] : 1.0;
-    const double x = (double)q[3 * (int64_t)i] - 0.5, y = (double)q[3 * (int64_t)i + 1] - 0.5,
-                 z = (double)q[3 * (int64_t)i + 2] - 0.5;
+  for (int j = lane; j < total; j += 64) {
+    const float4 p = j == lane ? p0 : point_of(j);
+    const double x = (double)p.x - 0.5, y = (double)p.y - 0.5, z = (double)p.z - 0.5;
     const double fx = floor(x), fy = floor(y), fz = floor(z);
     const int cx = (int)fx - bp.x, cy = (int)fy - bp.y, cz = (int)fz - bp.z;
     const double ax = x - fx, ay = y - fy, az = z - fz;
-    const double ms = ldexp(m, S);
+    const double ms = ldexp((double)p.w, S);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
@@ -166,9 +199,7 @@ __global__ void __launch_bounds__(256) k_splat_brick(const int* __restrict__ sta
     }
   }
   __syncthreads();
-  const int br = kRes + 2 * apron;
-  float* dst = chan + (int64_t)s * br * br * br;
-  for (int v = tid; v < kVox; v += blockDim.x) {
+  for (int v = lane; v < kVox; v += 64) {
     const int x = v & (kRes - 1), y = (v >> kLg) & (kRes - 1), z = v >> (2 * kLg);
     float* o = dst + ((z + apron) * br + (y + apron)) * br + (x + apron);
     const double val = ldexp((double)acc[v], -S);
@@ -181,9 +212,11 @@ __global__ void __launch_bounds__(256) k_splat_backward(DevTree T, const int* __
                                                        const int* __restrict__ nbr, int lgres, int apron,
                                                        const float* __restrict__ chan, const float* __restrict__ q,
                                                        const float* __restrict__ mass, int64_t n,
-                                                       float* __restrict__ dq, float* __restrict__ dm) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                                                       const uint32_t* __restrict__ order, float* __restrict__ dq,
+                                                       float* __restrict__ dm) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int64_t i = order ? (int64_t)order[j] : j;  // a permutation of [0, n): brick order for locality
   const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
   double s = 0, gx = 0, gy = 0, gz = 0;
   const double m = mass ? (double)mass[i] : 1.0;
@@ -262,32 +295,40 @@ __global__ void __launch_bounds__(256) k_sample_backward(DevTree T, const int* _
   dq[3 * i + 2] = (float)(d * gz);
 }
 
-void splat_keys(const DevTree& T, const float* q, int64_t n, const int* leaf_slot, int lgres, uint32_t sentinel,
-                uint32_t* keys, uint32_t* vals, hipStream_t s) {
+void splat_count(const DevTree& T, const float* q, int64_t n, const int* leaf_slot, int lgres, uint32_t sentinel,
+                 uint32_t* cnt, uint2* keyrank, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_splat_keys, dim3(nblocks(n, 256)), dim3(256), 0, s, T, q, n, leaf_slot, lgres, sentinel, keys,
-                     vals);
+  hipLaunchKernelGGL(k_splat_count, dim3(nblocks(n, 256)), dim3(256), 0, s, T, q, n, leaf_slot, lgres, sentinel, cnt,
+                     keyrank);
+}
+
+void splat_place(const float* q, const float* mass, int64_t n, const uint2* keyrank, const int* start,
+                 float4* payload, uint32_t* perm, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_splat_place, dim3(nblocks(n, 256)), dim3(256), 0, s, q, mass, n, keyrank, start, payload,
+                     perm);
 }
 
 bool splat_brick_supported(int lgres) { return lgres >= 1 && lgres <= 4; }
 
-void splat_brick(int64_t nslots, const int* start, const uint32_t* svals, const float* q, const float* mass,
-                 const int* nbr, const int4* slot_pos, int lgres, int apron, float* chan, int accum, hipStream_t s) {
+void splat_brick(int64_t nslots, const int* start, const float4* payload, const int* nbr, const int4* slot_pos,
+                 int lgres, int apron, float* chan, int accum, hipStream_t s) {
   if (nslots <= 0) return;
-  const dim3 g((unsigned)nslots), b(256);
+  const dim3 g((unsigned)nslots), b(64);
   switch (lgres) {
-    case 1: hipLaunchKernelGGL(k_splat_brick<1>, g, b, 0, s, start, svals, q, mass, nbr, slot_pos, apron, chan, accum); break;
-    case 2: hipLaunchKernelGGL(k_splat_brick<2>, g, b, 0, s, start, svals, q, mass, nbr, slot_pos, apron, chan, accum); break;
-    case 3: hipLaunchKernelGGL(k_splat_brick<3>, g, b, 0, s, start, svals, q, mass, nbr, slot_pos, apron, chan, accum); break;
-    default: hipLaunchKernelGGL(k_splat_brick<4>, g, b, 0, s, start, svals, q, mass, nbr, slot_pos, apron, chan, accum); break;
+    case 1: hipLaunchKernelGGL(k_splat_brick<1>, g, b, 0, s, start, payload, nbr, slot_pos, apron, chan, accum); break;
+    case 2: hipLaunchKernelGGL(k_splat_brick<2>, g, b, 0, s, start, payload, nbr, slot_pos, apron, chan, accum); break;
+    case 3: hipLaunchKernelGGL(k_splat_brick<3>, g, b, 0, s, start, payload, nbr, slot_pos, apron, chan, accum); break;
+    default: hipLaunchKernelGGL(k_splat_brick<4>, g, b, 0, s, start, payload, nbr, slot_pos, apron, chan, accum); break;
   }
 }
 
 void splat_backward(const DevTree& T, const int* leaf_slot, const int* nbr, int lgres, int apron, const float* chan,
-                    const float* q, const float* mass, int64_t n, float* dq, float* dm, hipStream_t s) {
+                    const float* q, const float* mass, int64_t n, const uint32_t* order, float* dq, float* dm,
+                    hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_splat_backward, dim3(nblocks(n, 256)), dim3(256), 0, s, T, leaf_slot, nbr, lgres, apron, chan,
-                     q, mass, n, dq, dm);
+                     q, mass, n, order, dq, dm);
 }
 
 void sample_backward(const DevTree& T, const int* leaf_slot, int res0, int apron, const float* chan, const float* q,

@@ -4,15 +4,21 @@
 
 namespace gvdbx {
 
-// key = slot * 27 + region of the home brick, sentinel for points in no brick
-void splat_keys(const DevTree& T, const float* q, int64_t n, const int* leaf_slot, int lgres, uint32_t sentinel,
-                uint32_t* keys, uint32_t* vals, hipStream_t s);
+// Counting sort of the points by key = slot * 27 + region of the home brick (sentinel = points
+// in no brick): cnt[sentinel + 1] must be zero on entry; keyrank[i] = (key, rank inside key).
+void splat_count(const DevTree& T, const float* q, int64_t n, const int* leaf_slot, int lgres, uint32_t sentinel,
+                 uint32_t* cnt, uint2* keyrank, hipStream_t s);
+// payload[start[key] + rank] = (q, m) (m = 1 without masses), perm[same] = point index
+void splat_place(const float* q, const float* mass, int64_t n, const uint2* keyrank, const int* start,
+                 float4* payload, uint32_t* perm, hipStream_t s);
 bool splat_brick_supported(int lgres);
-// start: [27 * nslots + 2] range starts of the sorted keys; mass may be null (unit masses)
-void splat_brick(int64_t nslots, const int* start, const uint32_t* svals, const float* q, const float* mass,
-                 const int* nbr, const int4* slot_pos, int lgres, int apron, float* chan, int accum, hipStream_t s);
+// start: [27 * nslots + 1] key starts (exclusive scan of the counts)
+void splat_brick(int64_t nslots, const int* start, const float4* payload, const int* nbr, const int4* slot_pos,
+                 int lgres, int apron, float* chan, int accum, hipStream_t s);
+// order: a permutation of [0, n) to visit the points in (locality only), or null
 void splat_backward(const DevTree& T, const int* leaf_slot, const int* nbr, int lgres, int apron, const float* chan,
-                    const float* q, const float* mass, int64_t n, float* dq, float* dm, hipStream_t s);
+                    const float* q, const float* mass, int64_t n, const uint32_t* order, float* dq, float* dm,
+                    hipStream_t s);
 void sample_backward(const DevTree& T, const int* leaf_slot, int res0, int apron, const float* chan, const float* q,
                      const float* dout, int64_t n, float* dq, hipStream_t s);
 
