@@ -157,6 +157,63 @@ def adjoint_bench(reps):
             "forward_hbm_frac": round(fbytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4  # wave64 FP32 VALU issue rate of MI355X (no packed math)
+LEVELSET_VALU_PER_PAIR = 6.5  # VALU wave-instructions per (list entry, 4^3 subcell) in the level-set kernel
+
+
+def levelset_bench(reps):
+    """The level-set workload: 16,777,216 points on 64 sphere surfaces in 1024^3, full
+    RebuildTopology(r=2) + UpdateAtlas + InsertPointsSubcell(4, r=2) + ScatterReduceLevelSet
+    (r=2, init 3) + UpdateApron(3) + 1,048,576 trilinear queries; per-stage wall clock with a
+    synchronisation after each stage, best of `reps` passes."""
+    from gvdbx import VolumeGVDB, synth
+    p = synth.sphere_surfaces()
+    qs = synth.uniform_queries()
+    n = len(p)
+    v = VolumeGVDB(0)
+    v.Configure(5, 3, 3, 3, 3)
+    v.SetChannelDefault(16, 16, 1)
+    v.AddChannel(0, apron=1)
+    d, dq = v.to_device(p), v.to_device(qs)
+    out = v.alloc(4 * len(qs))
+    stages = ("rebuild", "atlas", "insert", "scatter_reduce", "apron", "sample")
+    best = {k: float("inf") for k in stages}
+    for _ in range(reps + 1):
+        t = [time.perf_counter()]
+        v.SetPoints(d, n=n)
+        v.RequestFullRebuild(True)
+        v.RebuildTopology(n, 2.0)
+        v.synchronize()
+        t.append(time.perf_counter())
+        v.UpdateAtlas()
+        v.synchronize()
+        t.append(time.perf_counter())
+        ln = v.InsertPointsSubcell(4, n, 2.0)
+        v.synchronize()
+        t.append(time.perf_counter())
+        v.ScatterReduceLevelSet(2.0, 0, 3.0)
+        v.synchronize()
+        t.append(time.perf_counter())
+        v.UpdateApron(0, 3.0)
+        v.synchronize()
+        t.append(time.perf_counter())
+        v.SampleTrilinear(0, dq, len(qs), out)
+        v.synchronize()
+        t.append(time.perf_counter())
+        for i, k in enumerate(stages):
+            best[k] = min(best[k], (t[i + 1] - t[i]) * 1e3)
+    bricks = v.getNumUsedNodes(0)
+    v.close()
+    total = sum(best.values()) - best["sample"]
+    sr = best["scatter_reduce"] * 1e-3
+    return {"workload": "levelset_16M (16,777,216 points on 64 spheres in 1024^3, r=2, sc=4)",
+            "bricks": int(bricks), "subcell_entries": int(ln), "ms": {k: round(x, 4) for k, x in best.items()},
+            "points_to_sdf_ms": round(total, 4), "Mpoints_s": round(n / (total * 1e-3) / 1e6, 1),
+            "pair_evals_per_s": round(64 * ln / sr, 1),
+            "scatter_valu_frac": round(LEVELSET_VALU_PER_PAIR * ln / sr / VALU_WAVE_INSTR_PER_S, 4),
+            "queries_Mper_s": round(len(qs) / (best["sample"] * 1e-3) / 1e6, 1)}
+
+
 def calibrate_copy():
     """A plain device-to-device copy of 256 MiB, three times: the reference reading of the HBM
     counters (FETCH_SIZE / WRITE_SIZE) that scales the kernel's counter values to bytes.
@@ -191,6 +248,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=N_POINTS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-levelset", action="store_true")
     ap.add_argument("--kernel", default="gather", help="kernel family timed for the roofline")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the counter-calibration copy")
     args = ap.parse_args()
@@ -273,6 +331,8 @@ def main():
     v.close()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(max(args.steps, 5))
+    if not args.no_levelset:
+        out["levelset"] = levelset_bench(3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(poss[0], vel, min(args.cpu_sample, n))
     print(json.dumps(out))

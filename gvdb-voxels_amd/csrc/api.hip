@@ -82,6 +82,7 @@ struct gvdbx_ctx {
   const uint32_t* sp_order = nullptr;
 
   Buf keys, keys_alt, flags, rank, radix_counts, tile_buf, markers, meta;
+  Buf box_region, box_cnt, box_off;  // per-workgroup slices of the box-pass keys
   int* h_meta = nullptr;
 
   int64_t topo_version = 0, atlas_version = -1;
@@ -378,44 +379,69 @@ int full_pass(gvdbx_ctx* c, int64_t n, const float* tr, int d) {  // ActivateBri
   return GVDBX_OK;
 }
 
-// Box pass shared by the extra (hi = 2r) and incremental (hi = r) passes; returns the
-// number of emitted keys in *m (count-then-fill on overflow).
-template <typename K>
-int box_capacity(gvdbx_ctx* c, int64_t n, int d, int64_t* cap) {
-  if ((int64_t)(c->keys.cap / sizeof(K)) < (int64_t)n * d) CHK(ensure(c, c->keys, (size_t)n * d * sizeof(K)));
-  *cap = c->keys.cap / sizeof(K);
+// Box pass shared by the extra (hi = 2r) and incremental (hi = r) passes.  Keys go to
+// per-workgroup slices sized by the per-point bound (per axis at most
+// floor(((1 + hi_mul) r + 1) / res0) + 2 leaf nodes, each emitting at most `depth` keys), in
+// launches of at most 4 GiB of slices; each launch is packed into c->keys after its count is read.
+struct BoxGeom {
+  int64_t per_point = 0, chunk = 0;
+};
+
+int box_geom(gvdbx_ctx* c, int64_t n, float r, float hi_mul, int d, size_t ksize, BoxGeom* g) {
+  const double R0 = (double)(1 << c->lgR[0]);
+  const int64_t per_axis = (int64_t)(((1.0 + hi_mul) * (double)r + 1.0) / R0) + 2;
+  if (per_axis > 64) return fail(c, GVDBX_ERR_CAPACITY, "radius %g too large for the box pass", (double)r);
+  g->per_point = per_axis * per_axis * per_axis * d;
+  const int64_t budget = (int64_t)4 << 30;
+  int64_t chunk = budget / (g->per_point * (int64_t)ksize) / 256 * 256;
+  if (chunk < 256) chunk = 256;
+  g->chunk = chunk < n ? chunk : n;
   return GVDBX_OK;
 }
 
+// Enqueue the box kernel for points [start, start + cnt) and the scan of its slice counts
+// (total -> meta[M_COUNT]).
 template <typename K>
-void box_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, float hi_mul, int d, int* markers, int64_t cap) {
+int box_launch(gvdbx_ctx* c, int64_t start, int64_t cnt, const float* tr, float r, float hi_mul, int d, int* markers,
+               const BoxGeom& g) {
+  const int64_t nb = (cnt + 255) / 256;
+  CHK(ensure(c, c->box_region, (size_t)nb * 256 * g.per_point * sizeof(K)));
+  CHK(ensure(c, c->box_cnt, (size_t)(nb + 1) * 4));
+  CHK(ensure(c, c->box_off, (size_t)(nb + 1) * 4));
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(nb + 1) + 16) * 4));
   TimedRegion trg(c, "box");
-  box_keys<K>(tree_of(c), points_of(c, n, tr), key_fmt(c, d), r, hi_mul, P<K>(c->keys), cap, P<int>(c->meta) + M_COUNT,
+  PointsView pv = points_of(c, cnt, tr);
+  pv.pos += start * (int64_t)pv.pos_stride;
+  box_keys<K>(tree_of(c), pv, key_fmt(c, d), r, hi_mul, P<K>(c->box_region), 256 * g.per_point, P<int>(c->box_cnt),
               markers, c->stream);
+  exclusive_scan(P<int>(c->box_cnt), P<int>(c->box_off), nb, P<int>(c->tile_buf), P<int>(c->meta) + M_COUNT,
+                 c->stream);
+  return GVDBX_OK;
 }
 
-// prelaunched: the first attempt was enqueued before the caller's meta read-back.
+// prelaunched: the launch of the first chunk was enqueued before the caller's meta read-back.
 template <typename K>
 int box_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, float hi_mul, int d, int* markers, int64_t* m_out,
              bool prelaunched = false) {
-  int64_t cap = 0;
-  CHK(box_capacity<K>(c, n, d, &cap));
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    if (!(prelaunched && attempt == 0)) {
-      CHK(zero_meta(c));
-      box_launch<K>(c, n, tr, r, hi_mul, d, markers, cap);
+  BoxGeom g;
+  CHK(box_geom(c, n, r, hi_mul, d, sizeof(K), &g));
+  int64_t total = 0;
+  for (int64_t start = 0; start < n; start += g.chunk) {
+    const int64_t cnt = n - start < g.chunk ? n - start : g.chunk;
+    if (!(prelaunched && start == 0)) {
+      CHK(box_launch<K>(c, start, cnt, tr, r, hi_mul, d, markers, g));
       CHK(read_meta(c));
     }
     const int64_t m = (uint32_t)c->h_meta[M_COUNT];
-    if (m <= cap) {
-      *m_out = m;
-      CHK(ensure(c, c->keys_alt, (size_t)(m > 0 ? m : 1) * sizeof(K)));
-      return GVDBX_OK;
-    }
-    CHK(ensure(c, c->keys, (size_t)m * sizeof(K)));
-    cap = c->keys.cap / sizeof(K);
+    if (total + m > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "more than 2^31 box-pass keys");
+    CHK(ensure_keep(c, c->keys, (size_t)(total + m + 1) * sizeof(K), (size_t)total * sizeof(K)));
+    compact_regions<K>(P<K>(c->box_region), 256 * g.per_point, P<int>(c->box_cnt), P<int>(c->box_off),
+                       (cnt + 255) / 256, P<K>(c->keys) + total, c->stream);
+    total += m;
   }
-  return fail(c, GVDBX_ERR_CAPACITY, "extra-brick key emission did not converge");
+  CHK(ensure(c, c->keys_alt, (size_t)(total > 0 ? total : 1) * sizeof(K)));
+  *m_out = total;
+  return GVDBX_OK;
 }
 
 template <typename K>
@@ -440,9 +466,9 @@ int incre_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool 
     copy_flags_to_markers(T, total0, P<int>(c->markers), c->stream);
   else
     fill_u32(P<uint32_t>(c->markers), total0, 0u, c->stream);
-  int64_t cap = 0;
-  CHK(box_capacity<K>(c, n, d, &cap));
-  box_launch<K>(c, n, tr, r, 1.0f, d, P<int>(c->markers), cap);
+  BoxGeom g;
+  CHK(box_geom(c, n, r, 1.0f, d, sizeof(K), &g));
+  if (n > 0) CHK(box_launch<K>(c, 0, g.chunk, tr, r, 1.0f, d, P<int>(c->markers), g));
   return GVDBX_OK;
 }
 
@@ -988,9 +1014,14 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
                   c->apron > 0 ? P<int>(c->nbr) : nullptr, P<float>(c->chan[chan]), c->stream);
     c->nbr_apron_ok[chan] = c->apron > 0;
   } else {
-    gather_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
-                 P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), accum, mode,
-                 c->stream);
+    if (mode == 1)
+      levelset_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
+                     P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), accum, 3.0f,
+                     c->stream);
+    else
+      gather_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
+                   P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), accum, mode, 0.0f,
+                   c->stream);
     c->nbr_apron_ok[chan] = false;
   }
   return GVDBX_OK;
@@ -1041,10 +1072,10 @@ int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {
   if (c->nsc == 0) return GVDBX_OK;
   const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
   TimedRegion tr(c, "scatter");
-  scatter_reduce_init(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->leaf_slot), c->apron,
-                      P<float>(c->chan[chan]), init, c->stream);
-  scatter_reduce_levelset(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
-                          P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), c->stream);
+  // the insert left the (point, subcell) pairs sorted by subcell; each voxel min-reduces the
+  // segment of its subcell, starting from init
+  levelset_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
+                 P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), 0, init, c->stream);
   c->nbr_apron_ok[chan] = false;
   return GVDBX_OK;
 }

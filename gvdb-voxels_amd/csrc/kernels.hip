@@ -58,55 +58,87 @@ void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, K* keys, hipStr
 // nodes are exactly the level-l ancestors of the leaf-level box nodes, so here each leaf-level
 // node is descended once: a found leaf is marked, and when the descent stops at level l the
 // missing chain (levels l-1 .. 0) is emitted (duplicates vanish in the unique step).
-// Nodes outside the root are not emitted.  Emission past cap only counts.
+// Nodes outside the root are not emitted.
+// Emission needs no device-scope atomic: each workgroup ranks its keys with an LDS counter into
+// its own slice of `region` (cap_b = 256 * the per-point bound), writes its count, and a scan
+// plus k_compact_regions pack the slices.
 template <typename K>
 __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFmt f, float r, float hi_mul,
-                                                  K* __restrict__ keys, int64_t cap, int* counter,
+                                                  K* __restrict__ region, int64_t cap_b, int* __restrict__ blockcnt,
                                                   int* __restrict__ markers) {
+  __shared__ int bcount;
+  if (threadIdx.x == 0) bcount = 0;
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
-  const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
-  if (q[0] < 0 || q[1] < 0 || q[2] < 0) return;
-  int lo[3], hi[3];
-  const int R0 = 1 << T.lgR[0];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float tq = (float)(int)q[a];
-    lo[a] = (int)(tq - r) / R0 * R0;
-    hi[a] = (int)(tq + hi_mul * r) / R0 * R0;
+  K* out = region + (int64_t)blockIdx.x * cap_b;
+  float q[3] = {-1.f, -1.f, -1.f};
+  if (i < P.n) {
+    const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+    q[0] = p.x + P.tx;
+    q[1] = p.y + P.ty;
+    q[2] = p.z + P.tz;
   }
-  const int d = T.depth;
-  for (int x = lo[0]; x <= hi[0]; x += R0)
-    for (int y = lo[1]; y <= hi[1]; y += R0)
-      for (int z = lo[2]; z <= hi[2]; z += R0) {
-        if (!in_root(T, x, y, z)) continue;
-        int64_t cur = 0;
-        int miss = -1;  // highest missing level along the chain
-        for (int l = d; l > 0; --l) {
-          const uint64_t e = T.clist[l][((uint64_t)cur << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, x, y, z)];
-          if (e == kUndef64) { miss = l - 1; break; }
-          cur = (int64_t)(e >> 16);
+  if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
+    int lo[3], hi[3];
+    const int R0 = 1 << T.lgR[0];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float tq = (float)(int)q[a];
+      lo[a] = (int)(tq - r) / R0 * R0;
+      hi[a] = (int)(tq + hi_mul * r) / R0 * R0;
+    }
+    const int d = T.depth;
+    for (int x = lo[0]; x <= hi[0]; x += R0)
+      for (int y = lo[1]; y <= hi[1]; y += R0)
+        for (int z = lo[2]; z <= hi[2]; z += R0) {
+          if (!in_root(T, x, y, z)) continue;
+          int64_t cur = 0;
+          int miss = -1;  // highest missing level along the chain
+          for (int l = d; l > 0; --l) {
+            const uint64_t e = T.clist[l][((uint64_t)cur << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, x, y, z)];
+            if (e == kUndef64) { miss = l - 1; break; }
+            cur = (int64_t)(e >> 16);
+          }
+          if (miss < 0) {
+            if (markers && markers[cur] == 0) markers[cur] = 1;
+            continue;
+          }
+          const int k = atomicAdd(&bcount, miss + 1);
+          for (int l = miss; l >= 0; --l) {
+            const int sh = T.lgR[l];
+            out[k + miss - l] = pack_key<K>(f, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
+                                            (z >> sh) - (T.root[2] >> sh));
+          }
         }
-        if (miss < 0) {
-          if (markers && markers[cur] == 0) markers[cur] = 1;
-          continue;
-        }
-        for (int l = miss; l >= 0; --l) {
-          const int sh = T.lgR[l];
-          const int k = atomicAdd(counter, 1);
-          if (k < cap)
-            keys[k] = pack_key<K>(f, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
-                                  (z >> sh) - (T.root[2] >> sh));
-        }
-      }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) blockcnt[blockIdx.x] = bcount;
 }
 
 template <typename K>
-void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi_mul, K* keys, int64_t cap,
-              int* counter, int* markers, hipStream_t s) {
-  hipLaunchKernelGGL(k_box_keys<K>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, r, hi_mul, keys, cap,
-                     counter, markers);
+void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi_mul, K* region, int64_t cap_b,
+              int* blockcnt, int* markers, hipStream_t s) {
+  if (P.n <= 0) return;
+  hipLaunchKernelGGL(k_box_keys<K>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, r, hi_mul, region, cap_b,
+                     blockcnt, markers);
+}
+
+template <typename K>
+__global__ void __launch_bounds__(256) k_compact_regions(const K* __restrict__ region, int64_t cap_b,
+                                                         const int* __restrict__ blockcnt,
+                                                         const int* __restrict__ blockoff, K* __restrict__ out) {
+  const int64_t b = blockIdx.x;
+  const int n = blockcnt[b];
+  const int64_t o = blockoff[b];
+  for (int j = threadIdx.x; j < n; j += blockDim.x) out[o + j] = region[b * cap_b + j];
+}
+
+template <typename K>
+void compact_regions(const K* region, int64_t cap_b, const int* blockcnt, const int* blockoff, int64_t nblocks, K* out,
+                     hipStream_t s) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_compact_regions<K>, dim3((unsigned)nblocks), dim3(256), 0, s, region, cap_b, blockcnt,
+                     blockoff, out);
 }
 
 // gvdbFindUnique (P.cuh:336-350): run starts of the sorted keys, invalid level excluded.
@@ -358,7 +390,7 @@ __global__ void __launch_bounds__(512) k_gather_field(SubcellCfg c, const int* _
                                                       const int* __restrict__ sc_nid, const int* __restrict__ cnt,
                                                       const int* __restrict__ off, const float* __restrict__ list_pos,
                                                       const int* __restrict__ leaf_slot, int apron,
-                                                      float* __restrict__ chan, int accum, int mode) {
+                                                      float* __restrict__ chan, int accum, int mode, float init) {
   __shared__ float sx[512], sy[512], sz[512];
   const int k = blockIdx.x;
   const int nt = c.sc * c.sc * c.sc;
@@ -393,103 +425,69 @@ __global__ void __launch_bounds__(512) k_gather_field(SubcellCfg c, const int* _
     val = accum ? chan[vi] : 0.0f;
     if (n > 0) val = fmaxf(val, c.r - sqrtf(md2));
   } else {
-    val = accum ? chan[vi] : 3.0f;
+    val = accum ? chan[vi] : init;
     if (n > 0) val = fminf(val, sqrtf(md2) - c.r);
   }
   chan[vi] = val;
 }
 
 void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
-                  const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode,
+                  const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode, float init,
                   hipStream_t s) {
   if (nsc <= 0) return;
   hipLaunchKernelGGL(k_gather_field, dim3((unsigned)nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, sc_pos, sc_nid, cnt, off,
-                     list_pos, leaf_slot, apron, chan, accum, mode);
+                     list_pos, leaf_slot, apron, chan, accum, mode, init);
 }
 
-// ScatterReduceLevelSet: init the subcell voxels, then every (point, subcell) pair
-// min-reduces |p-v|-r into the voxels with an ordered-int atomicMin.
-__global__ void k_sr_init(SubcellCfg c, const int* __restrict__ sc_pos, const int* __restrict__ sc_nid,
-                          const int* __restrict__ leaf_slot, int apron, float* __restrict__ chan, float init) {
-  const int k = blockIdx.x, t = threadIdx.x;
-  const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
-  const int br = c.res0 + 2 * apron, m = c.res0 - 1;
-  const int slot = leaf_slot[sc_nid[k]];
-  if (slot < 0) return;
-  chan[atlas_voxel(slot, br, ((sc_pos[3 * k] + tx) & m) + apron, ((sc_pos[3 * k + 1] + ty) & m) + apron,
-                   ((sc_pos[3 * k + 2] + tz) & m) + apron)] = init;
-}
-
-__global__ void __launch_bounds__(256) k_sr_scatter(SubcellCfg c, int64_t nsc, const int* __restrict__ sc_pos,
-                                                    const int* __restrict__ sc_nid, const int* __restrict__ cnt,
-                                                    const int* __restrict__ off, const float* __restrict__ list_pos,
-                                                    const int* __restrict__ leaf_slot, int apron,
-                                                    float* __restrict__ chan) {
-  // one wave per subcell, lanes over list entries; per entry all sc^3 voxels
-  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+// Level set over 4^3 subcells (GatherLevelSet, P.cuh:736-773, and ScatterReduceLevelSet, which is
+// the same segmented min over the (point, subcell) pairs the insert sorted by subcell): one wave
+// per subcell, lane = voxel, the subcell's pair segment read with wave-uniform (scalar) loads.
+// Each lane keeps min |p - v|^2 and takes one square root at the end: sqrtf and the
+// subtraction of r are monotone, so sqrtf(min d2) - r equals the per-pair min of
+// sqrtf(d2) - r bit for bit.
+__global__ void __launch_bounds__(256) k_levelset_sc4(SubcellCfg c, int64_t nsc, const int* __restrict__ sc_pos,
+                                                     const int* __restrict__ sc_nid, const int* __restrict__ cnt,
+                                                     const int* __restrict__ off, const float* __restrict__ list_pos,
+                                                     const int* __restrict__ leaf_slot, int apron,
+                                                     float* __restrict__ chan, int accum, float init) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t k = (int64_t)blockIdx.x * 4 + wave;
   if (k >= nsc) return;
-  const int n = cnt[k], o = off[k];
-  const int br = c.res0 + 2 * apron, m = c.res0 - 1;
-  const int slot = leaf_slot[sc_nid[k]];
-  if (slot < 0) return;
+  const int lane = threadIdx.x & 63;
   const int bx = sc_pos[3 * k], by = sc_pos[3 * k + 1], bz = sc_pos[3 * k + 2];
-  int* ci = (int*)chan;
-  for (int j = lane; j < n; j += 64) {
-    const float px = list_pos[3 * (int64_t)(o + j)], py = list_pos[3 * (int64_t)(o + j) + 1],
-                pz = list_pos[3 * (int64_t)(o + j) + 2];
-    for (int tz = 0; tz < c.sc; ++tz)
-      for (int ty = 0; ty < c.sc; ++ty)
-        for (int tx = 0; tx < c.sc; ++tx) {
-          const float dx = px - (float)(bx + tx), dy = py - (float)(by + ty), dz = pz - (float)(bz + tz);
-          const float w = sqrtf(dist2(dx, dy, dz)) - c.r;
-          atomicMin(&ci[atlas_voxel(slot, br, ((bx + tx) & m) + apron, ((by + ty) & m) + apron,
-                                    ((bz + tz) & m) + apron)],
-                    float_to_ordered(w));
-        }
+  const float vx = (float)(bx + (lane & 3)), vy = (float)(by + ((lane >> 2) & 3)), vz = (float)(bz + (lane >> 4));
+  const int n = cnt[k];
+  const float* __restrict__ lp = list_pos + 3 * (int64_t)off[k];
+  float m0 = INFINITY, m1 = INFINITY;
+  int j = 0;
+  for (; j + 4 <= n; j += 4) {
+    const float* p = lp + 3 * j;
+    m0 = fminf(m0, dist2(p[0] - vx, p[1] - vy, p[2] - vz));
+    m1 = fminf(m1, dist2(p[3] - vx, p[4] - vy, p[5] - vz));
+    m0 = fminf(m0, dist2(p[6] - vx, p[7] - vy, p[8] - vz));
+    m1 = fminf(m1, dist2(p[9] - vx, p[10] - vy, p[11] - vz));
   }
-}
-
-__global__ void k_sr_final(SubcellCfg c, const int* __restrict__ sc_pos, const int* __restrict__ sc_nid,
-                           const int* __restrict__ leaf_slot, int apron, float* __restrict__ chan) {
-  const int k = blockIdx.x, t = threadIdx.x;
-  const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
-  const int br = c.res0 + 2 * apron, m = c.res0 - 1;
+  for (; j < n; ++j) m0 = fminf(m0, dist2(lp[3 * j] - vx, lp[3 * j + 1] - vy, lp[3 * j + 2] - vz));
+  const float md2 = fminf(m0, m1);
   const int slot = leaf_slot[sc_nid[k]];
   if (slot < 0) return;
-  const int64_t vi = atlas_voxel(slot, br, ((sc_pos[3 * k] + tx) & m) + apron, ((sc_pos[3 * k + 1] + ty) & m) + apron,
-                                 ((sc_pos[3 * k + 2] + tz) & m) + apron);
-  chan[vi] = ordered_to_float(((int*)chan)[vi]);
-}
-
-__global__ void k_sr_to_ordered(SubcellCfg c, const int* __restrict__ sc_pos, const int* __restrict__ sc_nid,
-                                const int* __restrict__ leaf_slot, int apron, float* __restrict__ chan) {
-  const int k = blockIdx.x, t = threadIdx.x;
-  const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
   const int br = c.res0 + 2 * apron, m = c.res0 - 1;
-  const int slot = leaf_slot[sc_nid[k]];
-  if (slot < 0) return;
-  const int64_t vi = atlas_voxel(slot, br, ((sc_pos[3 * k] + tx) & m) + apron, ((sc_pos[3 * k + 1] + ty) & m) + apron,
-                                 ((sc_pos[3 * k + 2] + tz) & m) + apron);
-  ((int*)chan)[vi] = float_to_ordered(chan[vi]);
+  const int64_t vi = atlas_voxel(slot, br, ((int)vx & m) + apron, ((int)vy & m) + apron, ((int)vz & m) + apron);
+  float val = accum ? chan[vi] : init;
+  if (n > 0) val = fminf(val, sqrtf(md2) - c.r);
+  chan[vi] = val;
 }
 
-void scatter_reduce_init(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* leaf_slot,
-                         int apron, float* chan, float init, hipStream_t s) {
+void levelset_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
+                    const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, float init,
+                    hipStream_t s) {
   if (nsc <= 0) return;
-  const int nt = c.sc * c.sc * c.sc;
-  hipLaunchKernelGGL(k_sr_init, dim3((unsigned)nsc), dim3(nt), 0, s, c, sc_pos, sc_nid, leaf_slot, apron, chan, init);
-}
-
-void scatter_reduce_levelset(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
-                             const int* off, const float* list_pos, const int* leaf_slot, int apron, float* chan,
-                             hipStream_t s) {
-  if (nsc <= 0) return;
-  const int nt = c.sc * c.sc * c.sc;
-  hipLaunchKernelGGL(k_sr_to_ordered, dim3((unsigned)nsc), dim3(nt), 0, s, c, sc_pos, sc_nid, leaf_slot, apron, chan);
-  hipLaunchKernelGGL(k_sr_scatter, dim3(blocks_for(nsc, 4, 1ll << 31)), dim3(256), 0, s, c, nsc, sc_pos, sc_nid, cnt,
-                     off, list_pos, leaf_slot, apron, chan);
-  hipLaunchKernelGGL(k_sr_final, dim3((unsigned)nsc), dim3(nt), 0, s, c, sc_pos, sc_nid, leaf_slot, apron, chan);
+  if (c.sc != 4) {
+    gather_field(c, nsc, sc_pos, sc_nid, cnt, off, list_pos, leaf_slot, apron, chan, accum, 1, init, s);
+    return;
+  }
+  hipLaunchKernelGGL(k_levelset_sc4, dim3(blocks_for(nsc, 4, 1ll << 31)), dim3(256), 0, s, c, nsc, sc_pos, sc_nid,
+                     cnt, off, list_pos, leaf_slot, apron, chan, accum, init);
 }
 
 // ================================================================ trilinear sampling
@@ -537,6 +535,10 @@ template void box_keys<uint32_t>(const DevTree&, const PointsView&, KeyFmt, floa
                                  int*, hipStream_t);
 template void box_keys<uint64_t>(const DevTree&, const PointsView&, KeyFmt, float, float, uint64_t*, int64_t, int*,
                                  int*, hipStream_t);
+template void compact_regions<uint32_t>(const uint32_t*, int64_t, const int*, const int*, int64_t, uint32_t*,
+                                        hipStream_t);
+template void compact_regions<uint64_t>(const uint64_t*, int64_t, const int*, const int*, int64_t, uint64_t*,
+                                        hipStream_t);
 template void unique_flags<uint32_t>(const uint32_t*, int64_t, KeyFmt, int, int*, hipStream_t);
 template void unique_flags<uint64_t>(const uint64_t*, int64_t, KeyFmt, int, int*, hipStream_t);
 template void level_hist<uint32_t>(const uint32_t*, const int*, int64_t, KeyFmt, int*, hipStream_t);

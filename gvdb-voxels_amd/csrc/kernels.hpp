@@ -21,9 +21,15 @@ struct PointsView {
 
 // ---- keys
 template <typename K> void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, K* keys, hipStream_t s);
+// Box-node keys of the extra / incremental passes: block b of 256 points writes its keys to
+// region[b * cap_b ...] and their number to blockcnt[b]; compact_regions packs the slices at
+// blockoff (exclusive scan of blockcnt).  cap_b must hold 256 * the per-point bound.
 template <typename K>
-void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi_mul, K* keys, int64_t cap,
-              int* counter, int* markers, hipStream_t s);
+void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi_mul, K* region, int64_t cap_b,
+              int* blockcnt, int* markers, hipStream_t s);
+template <typename K>
+void compact_regions(const K* region, int64_t cap_b, const int* blockcnt, const int* blockoff, int64_t nblocks, K* out,
+                     hipStream_t s);
 template <typename K> void unique_flags(const K* k, int64_t m, KeyFmt f, int depth, int* flags, hipStream_t s);
 template <typename K>
 void level_hist(const K* k, const int* flags, int64_t m, KeyFmt f, int* lev_cnt, hipStream_t s);
@@ -56,15 +62,14 @@ struct SubcellCfg {
 void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map, int* sc_pos, int* sc_nid,
                   int* map_leaf, hipStream_t s);
 
-// ---- gathers (mode 0 density, 1 level set)
+// ---- gathers (mode 0 density, 1 level set starting from `init`)
 void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
-                  const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode,
+                  const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode, float init,
                   hipStream_t s);
-void scatter_reduce_init(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* leaf_slot,
-                         int apron, float* chan, float init, hipStream_t s);
-void scatter_reduce_levelset(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
-                             const int* off, const float* list_pos, const int* leaf_slot, int apron, float* chan,
-                             hipStream_t s);
+// level set (GatherLevelSet and ScatterReduceLevelSet): segmented min over the sorted pairs
+void levelset_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
+                    const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, float init,
+                    hipStream_t s);
 
 // ---- apron and sampling
 void sample_trilinear(const DevTree& T, const int* leaf_slot, int res0, int apron, const float* chan,
