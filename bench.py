@@ -24,6 +24,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gvdb-voxels_amd"))
 
 HBM_PEAK_GBS = 8000.0
+
+
+def progress(msg):
+    """Stage marker on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 N_POINTS = 4_194_304
 DT = np.float32(0.5)
 RADIUS = 1.0
@@ -214,6 +219,66 @@ def levelset_bench(reps):
             "queries_Mper_s": round(len(qs) / (best["sample"] * 1e-3) / 1e6, 1)}
 
 
+def batched_bench(reps):
+    """The batched workload: 256 Gaussian-blob clouds with log-uniform sizes in
+    [4,096, 1,048,576] (8 blobs each, sigma U(4, 32), centres U(96, 416)^3, seed 5), each with
+    its own grid.  Brick and subcell size come from the autotuner; then one step (full
+    RebuildTopology(r=1) + UpdateAtlas + InsertPointsSubcell + GatherDensity + UpdateApron) for
+    all clouds as one batch against the same step run cloud by cloud (one context per cloud,
+    created and warmed up beforehand).  Best of `reps` synchronised passes each."""
+    from gvdbx import autotune, synth
+    cs = synth.blob_clouds()
+    counts = [len(c) for c in cs]
+    allp = np.concatenate(cs)
+    n = len(allp)
+    mc = autotune.max_valid_coord(cs)
+    tuned = autotune.tune(lambda v: v.to_device(allp), counts, mc, radius=RADIUS, reps=2, log=progress)
+    leaf, sc = tuned["leaf_log2"], tuned["subcell"]
+
+    v = autotune.new_batch_volume(leaf)
+    d = v.to_device(allp)
+    autotune.bind(v, leaf, d, counts, RADIUS, mc)
+    best_b = float("inf")
+    for _ in range(reps + 1):
+        v.synchronize()
+        t0 = time.perf_counter()
+        v.RequestFullRebuild(True)
+        autotune.run_batch(v, n, RADIUS, sc)
+        v.synchronize()
+        best_b = min(best_b, (time.perf_counter() - t0) * 1e3)
+    bricks = v.getNumUsedNodes(0)
+    v.close()
+    del d
+
+    vols, devs = [], []
+    for c in cs:
+        w = autotune.new_batch_volume(leaf)
+        dc = w.to_device(c)
+        w.SetPoints(dc, n=len(c))
+        vols.append(w)
+        devs.append(dc)
+    best_p = float("inf")
+    for _ in range(reps + 1):
+        for w in vols:
+            w.synchronize()
+        t0 = time.perf_counter()
+        for w, c in zip(vols, cs):
+            w.RequestFullRebuild(True)
+            autotune.run_batch(w, len(c), RADIUS, sc)
+        for w in vols:
+            w.synchronize()
+        best_p = min(best_p, (time.perf_counter() - t0) * 1e3)
+    del devs
+    for w in vols:
+        w.close()
+    return {"workload": "batched_256 (256 blob clouds, sizes log-uniform in [4096, 1048576], own grid each, r=1)",
+            "points": n, "clouds": len(cs), "bricks": int(bricks), "autotune": tuned,
+            "batched_ms": round(best_b, 3), "per_cloud_ms": round(best_p, 3),
+            "batched_Mpoints_s": round(n / (best_b * 1e-3) / 1e6, 1),
+            "per_cloud_Mpoints_s": round(n / (best_p * 1e-3) / 1e6, 1),
+            "speedup": round(best_p / best_b, 3)}
+
+
 def calibrate_copy():
     """A plain device-to-device copy of 256 MiB, three times: the reference reading of the HBM
     counters (FETCH_SIZE / WRITE_SIZE) that scales the kernel's counter values to bytes.
@@ -249,6 +314,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-levelset", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--kernel", default="gather", help="kernel family timed for the roofline")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the counter-calibration copy")
     args = ap.parse_args()
@@ -329,12 +395,19 @@ def main():
     }
     del dposs
     v.close()
+    progress(f"sph step done: {ms_per_step:.4f} ms")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(max(args.steps, 5))
+        progress("adjoint done")
     if not args.no_levelset:
         out["levelset"] = levelset_bench(3)
+        progress("level set done")
+    if not args.no_batched:
+        out["batched"] = batched_bench(2)
+        progress("batched done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(poss[0], vel, min(args.cpu_sample, n))
+        progress("cpu baseline done")
     print(json.dumps(out))
 
 

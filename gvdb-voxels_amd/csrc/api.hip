@@ -67,6 +67,13 @@ struct gvdbx_ctx {
   int clr_off = 0, clr_stride = 4;
   int64_t npts = 0;
 
+  // batched clouds (gvdbx_batch_set_points): cloud k occupies grid cell k of 2^cell_lg voxels
+  bool batch = false;
+  int ncloud = 0, cell_lg = 0, grid[3] = {0, 0, 0}, batch_depth = 0;
+  float batch_rmax = 0.f;
+  std::vector<int64_t> cloud_off;
+  Buf cloud_id, cloud_off_dev;
+
   int sc = 0, scdim = 0;
   int64_t nsc = 0, sclen = 0;
   bool lists_have_vel = false;
@@ -194,6 +201,10 @@ PointsView points_of(gvdbx_ctx* c, int64_t n, const float* tr) {
   v.tx = tr ? tr[0] : 0.f;
   v.ty = tr ? tr[1] : 0.f;
   v.tz = tr ? tr[2] : 0.f;
+  v.cloud = c->batch ? P<uint32_t>(c->cloud_id) : nullptr;
+  v.cell_lg = c->cell_lg;
+  v.grid_x = c->grid[0] > 0 ? c->grid[0] : 1;
+  v.grid_y = c->grid[1] > 0 ? c->grid[1] : 1;
   return v;
 }
 
@@ -412,6 +423,7 @@ int box_launch(gvdbx_ctx* c, int64_t start, int64_t cnt, const float* tr, float 
   TimedRegion trg(c, "box");
   PointsView pv = points_of(c, cnt, tr);
   pv.pos += start * (int64_t)pv.pos_stride;
+  if (pv.cloud) pv.cloud += start;
   box_keys<K>(tree_of(c), pv, key_fmt(c, d), r, hi_mul, P<K>(c->box_region), 256 * g.per_point, P<int>(c->box_cnt),
               markers, c->stream);
   exclusive_scan(P<int>(c->box_cnt), P<int>(c->box_off), nb, P<int>(c->tile_buf), P<int>(c->meta) + M_COUNT,
@@ -520,16 +532,33 @@ int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accum
   // depth, the key count of the speculatively launched incremental box pass (it only writes
   // keys and markers, so it is simply dropped when a full rebuild follows).
   const bool spec = !c->rebuild && c->depth >= 1 && c->cnt[0] > 0;
+  if (c->batch) {
+    if (r > c->batch_rmax) return fail(c, GVDBX_ERR_ARG, "radius %g exceeds the batch guard radius %g", (double)r,
+                                       (double)c->batch_rmax);
+    if (orig && (orig[0] != 0.f || orig[1] != 0.f || orig[2] != 0.f))
+      return fail(c, GVDBX_ERR_ARG, "batched clouds take no translation");
+  }
   CHK(zero_meta(c));
-  CHK(bounding_box_launch(c, n, orig));
+  if (!c->batch) CHK(bounding_box_launch(c, n, orig));
   if (spec) {
     if (wide_keys(c, c->depth)) CHK(incre_launch<uint64_t>(c, n, orig, r, c->depth, accumulate));
     else CHK(incre_launch<uint32_t>(c, n, orig, r, c->depth, accumulate));
   }
-  CHK(read_meta(c));
-  bounding_box_finish(c);
-  int root[3];
-  int depth = determine_depth(c, root);
+  int root[3] = {0, 0, 0};
+  int depth;
+  if (c->batch) {
+    // the merged root sits at the origin, one level above the grid of cloud cells
+    if (spec) CHK(read_meta(c));
+    depth = c->batch_depth;
+    for (int a = 0; a < 3; ++a) {
+      c->posmin[a] = 0.f;
+      c->posmax[a] = (float)((int64_t)c->grid[a] << c->cell_lg);
+    }
+  } else {
+    CHK(read_meta(c));
+    bounding_box_finish(c);
+    depth = determine_depth(c, root);
+  }
   if (accumulate && depth >= 0 && depth < min_depth) depth = min_depth;
   if (depth == 0) return fail(c, GVDBX_ERR_DEPTH0, "Points extents smaller than 1 brick");
   if (depth < 0 || depth > 5 || depth >= c->levs)
@@ -598,6 +627,7 @@ SubcellCfg subcell_cfg(const gvdbx_ctx* c, int sc, float r) {
   s.res0 = res_of(c, 0);
   s.D = s.res0 / sc;
   s.r = r;
+  s.cmask = c->batch ? (1 << c->cell_lg) - 1 : 0x7FFFFFFF;
   return s;
 }
 
@@ -671,7 +701,9 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
   Buf* bufs[] = {&c->leaf_slot, &c->slot_leaf, &c->slot_pos, &c->nbr, &c->sc_map, &c->sc_cnt, &c->sc_off, &c->sc_pos,
                  &c->sc_nid, &c->list_pos, &c->list_vel, &c->map_leaf, &c->vsum, &c->hkeys, &c->hkeys_alt,
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
-                 &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta};
+                 &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta, &c->box_region, &c->box_cnt,
+                 &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->cloud_id,
+                 &c->cloud_off_dev};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (int l = 0; l < kMaxLev; ++l) {
@@ -769,6 +801,12 @@ int gvdbx_set_points(gvdbx_ctx* c, int64_t n, const void* pos, int pos_off, int 
     return fail(c, GVDBX_ERR_ARG, "bad position buffer");
   if (vel && (vel_stride < 12 || vel_off < 0 || (vel_off & 3) || (vel_stride & 3)))
     return fail(c, GVDBX_ERR_ARG, "bad velocity buffer");
+  if (c->batch) {  // leaving the batched layout: the next rebuild starts over
+    c->batch = false;
+    c->ncloud = 0;
+    clear_tree(c);
+    c->cur_depth = -1;
+  }
   c->npts = n;
   c->pos = (const char*)pos;
   c->pos_off = pos_off;
@@ -779,6 +817,82 @@ int gvdbx_set_points(gvdbx_ctx* c, int64_t n, const void* pos, int pos_off, int 
   c->clr = (const char*)clr;
   c->clr_off = clr_off;
   c->clr_stride = clr_stride;
+  return GVDBX_OK;
+}
+
+int gvdbx_batch_set_points(gvdbx_ctx* c, int ncloud, const int64_t* counts, const void* pos, int pos_off,
+                           int pos_stride, const void* vel, int vel_off, int vel_stride, int cell_log2,
+                           float max_radius) {
+  if (!c || !c->configured) return fail(c, GVDBX_ERR_ARG, "context not configured");
+  if (ncloud < 1 || ncloud > 65536 || !counts) return fail(c, GVDBX_ERR_ARG, "bad cloud count %d", ncloud);
+  // radii up to 1 keep every extra-pass box inside the cloud's own root (its box
+  // [trunc(q) - r, trunc(q) + 2r] lies within the bounding box margins -1 / +2), so a batched
+  // cloud gets exactly the bricks it gets alone
+  if (!(max_radius >= 0.f) || max_radius > 1.f) return fail(c, GVDBX_ERR_ARG, "batch radius must lie in [0, 1]");
+  // batches of leaf bricks below 8^3 are switched off: such a batch faulted the GPU once
+  // and the cause is not identified yet
+  if (res_of(c, 0) < 8) return fail(c, GVDBX_ERR_ARG, "batched clouds need leaf bricks of 8^3 or more");
+  int cell_lev = -1;
+  for (int l = 1; l < c->levs - 1; ++l)
+    if (c->lgR[l] == cell_log2) cell_lev = l;
+  if (cell_lev < 0) return fail(c, GVDBX_ERR_ARG, "cell 2^%d is not the range of an inner level", cell_log2);
+  std::vector<int64_t> off(ncloud + 1, 0);
+  for (int k = 0; k < ncloud; ++k) {
+    if (counts[k] < 0) return fail(c, GVDBX_ERR_ARG, "negative size of cloud %d", k);
+    off[k + 1] = off[k] + counts[k];
+  }
+  const int64_t n = off[ncloud];
+  if (n <= 0 || n > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "batched point count %lld", (long long)n);
+  int g = 1;
+  while ((int64_t)g * g * g < ncloud) ++g;
+  const int grid[3] = {g, g, (int)((ncloud + (int64_t)g * g - 1) / ((int64_t)g * g))};
+  const int64_t extent = (int64_t)(grid[0] > grid[2] ? grid[0] : grid[2]) << cell_log2;
+  int depth = -1;
+  for (int l = cell_lev + 1; l < c->levs && depth < 0; ++l)
+    if (((int64_t)1 << c->lgR[l]) >= extent) depth = l;
+  if (depth < 0) return fail(c, GVDBX_ERR_CAPACITY, "%d clouds of 2^%d voxels exceed the configured levels", ncloud,
+                             cell_log2);
+  const bool same = c->batch && c->ncloud == ncloud && c->cell_lg == cell_log2 && c->cloud_off == off;
+  c->batch = false;  // rebinding the points; the layout check below decides whether the tree restarts
+  CHK(gvdbx_set_points(c, n, pos, pos_off, pos_stride, vel, vel_off, vel_stride, nullptr, 0, 4));
+  CHK(ensure(c, c->cloud_id, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->cloud_off_dev, (size_t)(ncloud + 1) * 8));
+  c->cloud_off = off;
+  HIPCHK(c, hipMemcpyAsync(c->cloud_off_dev.p, c->cloud_off.data(), (ncloud + 1) * 8, hipMemcpyHostToDevice,
+                           c->stream));
+  // valid coordinates must stay below the cell's last brick row minus the reach of the box
+  // passes (trunc(q) + 2r) and the bounding-box margin, so no brick, subcell list or apron of
+  // one cloud can touch another cloud's cell
+  const float limit = (float)((1 << cell_log2) - res_of(c, 0)) - 2.f * max_radius - 3.f;
+  CHK(zero_meta(c));
+  PointsView pv = points_of(c, n, nullptr);
+  batch_ids(pv, P<int64_t>(c->cloud_off_dev), ncloud, limit, P<uint32_t>(c->cloud_id), P<int>(c->meta) + M_COUNT,
+            c->stream);
+  CHK(read_meta(c));
+  if (c->h_meta[M_COUNT]) {
+    c->ncloud = 0;
+    clear_tree(c);
+    c->cur_depth = -1;
+    return fail(c, GVDBX_ERR_CAPACITY, "a cloud reaches beyond %g voxels of its 2^%d cell", (double)limit, cell_log2);
+  }
+  c->batch = true;
+  c->ncloud = ncloud;
+  c->cell_lg = cell_log2;
+  for (int a = 0; a < 3; ++a) c->grid[a] = grid[a];
+  c->batch_depth = depth;
+  c->batch_rmax = max_radius;
+  if (!same) {  // a new layout: the next rebuild starts over
+    clear_tree(c);
+    c->cur_depth = -1;
+  }
+  return GVDBX_OK;
+}
+
+int gvdbx_batch_get_layout(const gvdbx_ctx* c, int* ncloud, int* cell_log2, int grid[3]) {
+  if (!c) return GVDBX_ERR_ARG;
+  if (ncloud) *ncloud = c->batch ? c->ncloud : 0;
+  if (cell_log2) *cell_log2 = c->cell_lg;
+  for (int a = 0; a < 3 && grid; ++a) grid[a] = c->batch ? c->grid[a] : 0;
   return GVDBX_OK;
 }
 
@@ -855,6 +969,12 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
 
 int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t* len) {
   CHK(check_points(c, n));  // InsertPointsSubcell (V.cpp:4971-5092)
+  if (c->batch) {
+    if (r > c->batch_rmax) return fail(c, GVDBX_ERR_ARG, "radius %g exceeds the batch guard radius %g", (double)r,
+                                       (double)c->batch_rmax);
+    if (trans && (trans[0] != 0.f || trans[1] != 0.f || trans[2] != 0.f))
+      return fail(c, GVDBX_ERR_ARG, "batched clouds take no translation");
+  }
   const int res0 = res_of(c, 0);
   if (sc <= 0 || (sc & (sc - 1)) || res0 % sc || sc * sc * sc > 512)
     return fail(c, GVDBX_ERR_ARG, "subcell size %d must be a power of two dividing %d", sc, res0);
@@ -1116,6 +1236,7 @@ int gvdbx_update_apron(gvdbx_ctx* c, int chan, float boundval) {
 
 int gvdbx_sample_trilinear(gvdbx_ctx* c, int chan, const void* q, int64_t n, void* out) {
   CHK(chan_ok(c, chan));
+  if (c->batch) return fail(c, GVDBX_ERR_STATE, "sampling takes world points: not for batched clouds");
   if (!q || !out || n < 0) return fail(c, GVDBX_ERR_ARG, "bad sample buffers");
   if (c->apron < 1) return fail(c, GVDBX_ERR_STATE, "trilinear sampling needs an apron >= 1");
   CHK(atlas_current(c));
@@ -1130,6 +1251,7 @@ namespace {
 
 int field_checks(gvdbx_ctx* c, int chan, const void* q, int64_t n) {
   CHK(chan_ok(c, chan));
+  if (c->batch) return fail(c, GVDBX_ERR_STATE, "splat and sampling take world points: not for batched clouds");
   if (!q || n < 0) return fail(c, GVDBX_ERR_ARG, "bad point buffers");
   if (n > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "more than 2^31 points");
   CHK(atlas_current(c));

@@ -102,7 +102,8 @@ __global__ void __launch_bounds__(256) k_home_keys(DevTree T, PointsView P, cons
   if (p.z != kNoHit) {
     const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
     if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
-      const int64_t lf = descend(T, (int)q[0], (int)q[1], (int)q[2], 0);
+      const int3 o = cloud_origin(P, i);
+      const int64_t lf = descend(T, (int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z, 0);
       if (lf >= 0) key = (uint32_t)map[lf];
     }
   }
@@ -113,6 +114,30 @@ __global__ void __launch_bounds__(256) k_home_keys(DevTree T, PointsView P, cons
 void home_keys(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, uint32_t* vals,
                hipStream_t s) {
   hipLaunchKernelGGL(k_home_keys, dim3(nblocks(P.n, 256)), dim3(256), 0, s, T, P, map, nmap, keys, vals);
+}
+
+// Batched clouds: cloud index of every point (off = exclusive prefix of the cloud sizes) and
+// a flag when a valid point (all coordinates >= 0) reaches `limit` in some coordinate, i.e.
+// leaves the part of its cell that keeps clouds apart.
+__global__ void __launch_bounds__(256) k_batch_ids(PointsView P, const int64_t* __restrict__ off, int ncloud,
+                                                   float limit, uint32_t* __restrict__ cloud, int* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  int lo = 0, hi = ncloud - 1;  // last cloud with off[k] <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  cloud[i] = (uint32_t)lo;
+  const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+  if (p.x >= 0.f && p.y >= 0.f && p.z >= 0.f && (p.x >= limit || p.y >= limit || p.z >= limit)) *flag = 1;
+}
+
+void batch_ids(const PointsView& P, const int64_t* off, int ncloud, float limit, uint32_t* cloud, int* flag,
+               hipStream_t s) {
+  if (P.n <= 0) return;
+  hipLaunchKernelGGL(k_batch_ids, dim3(nblocks(P.n, 256)), dim3(256), 0, s, P, off, ncloud, limit, cloud, flag);
 }
 
 // start[k] = first sorted position with key >= k, for k in [0, nmap + 1]: one lower-bound
@@ -143,7 +168,8 @@ void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStrea
 // reserve a contiguous range of each subcell list (wgb), so the insert needs no global atomics.
 
 struct BrickCtx {
-  int bx, by, bz, W, D, D3;
+  int bx, by, bz, W, D, D3;  // brick corner in its cloud's frame
+  int ox, oy, oz;            // the cloud's cell origin
   const int* nb;
   // window cell of a corner, or -1 outside the window
   __device__ __forceinline__ int window_of(int x, int y, int z, int lgsc) const {
@@ -162,7 +188,11 @@ struct BrickCtx {
   }
 };
 
-__device__ __forceinline__ int64_t direct_subcell(const BinArgs& A, int x, int y, int z) {
+// subcell of cloud-frame corner (x, y, z) of the cloud at origin (ox, oy, oz), or -1
+__device__ __forceinline__ int64_t direct_subcell(const BinArgs& A, int x, int y, int z, int ox, int oy, int oz) {
+  x += ox;
+  y += oy;
+  z += oz;
   const int64_t lf = descend(A.T, x, y, z, 0);
   if (lf < 0) return -1;
   const int msk = A.c.res0 - 1, D = A.c.D;
@@ -181,9 +211,13 @@ __device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, Br
   const int leaf = A.map_leaf[m];
   if (leaf < 0) return false;
   const gvdbx_node* nd = A.T.nodes[0] + leaf;
-  B.bx = nd->pos[0];
-  B.by = nd->pos[1];
-  B.bz = nd->pos[2];
+  const int gx = nd->pos[0], gy = nd->pos[1], gz = nd->pos[2];
+  B.bx = gx & A.c.cmask;
+  B.by = gy & A.c.cmask;
+  B.bz = gz & A.c.cmask;
+  B.ox = gx - B.bx;
+  B.oy = gy - B.by;
+  B.oz = gz - B.bz;
   B.W = W;
   B.D = D;
   B.D3 = D * D * D;
@@ -191,7 +225,7 @@ __device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, Br
   if (threadIdx.x < 27) {
     const int t = threadIdx.x;
     const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-    const int64_t lf = descend(A.T, B.bx + dx * A.c.res0, B.by + dy * A.c.res0, B.bz + dz * A.c.res0, 0);
+    const int64_t lf = descend(A.T, gx + dx * A.c.res0, gy + dy * A.c.res0, gz + dz * A.c.res0, 0);
     nb[t] = lf < 0 ? -1 : A.map[lf];
   }
   return true;
@@ -265,7 +299,7 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int use_bins) {
               atomicAdd(&cvs[3 * w + 2], (unsigned long long)fv[2]);
             }
           } else {
-            const int64_t g = direct_subcell(A, x, y, z);
+            const int64_t g = direct_subcell(A, x, y, z, B.ox, B.oy, B.oz);
             if (g < 0) continue;
             atomicAdd(&A.ocnt[g], 1);
             if (vel) add_vel_global(A.vsum, g, fv);
@@ -353,7 +387,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A) {
             if (base < 0) continue;
             k = (int64_t)base + atomicAdd(&lcnt[w], 1);
           } else {
-            const int64_t g = direct_subcell(A, x, y, z);
+            const int64_t g = direct_subcell(A, x, y, z, B.ox, B.oy, B.oz);
             if (g < 0) continue;
             k = (int64_t)A.off[g] + A.cnt_in[g] + atomicAdd(&A.ocur[g], 1);
           }
@@ -485,6 +519,7 @@ __global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __rest
     if (p.z == kNoHit) continue;
     const float q[3] = {p.x + A.P.tx, p.y + A.P.ty, p.z + A.P.tz};
     if (q[0] < 0 || q[1] < 0 || q[2] < 0) continue;
+    const int3 o = cloud_origin(A.P, i);
     float3 v = make_float3(0.f, 0.f, 0.f);
     long long fv[3] = {0, 0, 0};
     if (A.P.vel && (vel || A.list_vel)) {
@@ -498,7 +533,7 @@ __global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __rest
     for (int x = lo[0]; x <= hi[0]; x += c.sc)
       for (int y = lo[1]; y <= hi[1]; y += c.sc)
         for (int z = lo[2]; z <= hi[2]; z += c.sc) {
-          const int64_t g = direct_subcell(A, x, y, z);
+          const int64_t g = direct_subcell(A, x, y, z, o.x, o.y, o.z);
           if (g < 0) continue;
           if (!kInsert) {
             atomicAdd(&A.ocnt[g], 1);
@@ -574,7 +609,7 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int64_t nsc
   for (int s = threadIdx.x; s <= D3; s += blockDim.x) soff[s] = (k0 + s < nsc) ? off[k0 + s] : (int)len;
   if (threadIdx.x < 27) nb[threadIdx.x] = (kWithApron && slot >= 0) ? nbr[(int64_t)slot * 27 + threadIdx.x] : -1;
   __syncthreads();
-  const int bx = nodes0[leaf].pos[0], by = nodes0[leaf].pos[1], bz = nodes0[leaf].pos[2];
+  const int bx = nodes0[leaf].pos[0] & c.cmask, by = nodes0[leaf].pos[1] & c.cmask, bz = nodes0[leaf].pos[2] & c.cmask;
   // Voxels v with |p - v| < r lie in the integer cube [floor(p) - R + 1, floor(p) + R], R = ceil(r);
   // voxels at distance >= r cannot raise max(0, r - |p - v|), so the cube is exact.
   const int R = (int)ceilf(c.r);

@@ -9,6 +9,8 @@ constexpr int kBBoxParts = 1024;
 // two-pass bounding box: per-block partials, then one block writes meta[0..5] (ordered ints)
 void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s);
 
+void batch_ids(const PointsView& P, const int64_t* off, int ncloud, float limit, uint32_t* cloud, int* flag,
+               hipStream_t s);
 void home_keys(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, uint32_t* vals,
                hipStream_t s);
 void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStream_t s);

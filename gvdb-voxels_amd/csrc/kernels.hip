@@ -39,7 +39,8 @@ __global__ void __launch_bounds__(256) k_brick_keys(DevTree T, PointsView P, Key
     for (int l = 0; l < d; ++l) out[l] = pack_key<K>(f, d, 0, 0, 0);
     return;
   }
-  const int iq[3] = {(int)q[0], (int)q[1], (int)q[2]};
+  const int3 o = cloud_origin(P, i);
+  const int iq[3] = {(int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z};
   for (int l = 0; l < d; ++l) {
     const int sh = T.lgR[l];
     out[l] = pack_key<K>(f, l, (iq[0] >> sh) - (T.root[0] >> sh), (iq[1] >> sh) - (T.root[1] >> sh),
@@ -81,11 +82,13 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
   if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
     int lo[3], hi[3];
     const int R0 = 1 << T.lgR[0];
+    const int3 o = i < P.n ? cloud_origin(P, i) : make_int3(0, 0, 0);
+    const int org[3] = {o.x, o.y, o.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const float tq = (float)(int)q[a];
-      lo[a] = (int)(tq - r) / R0 * R0;
-      hi[a] = (int)(tq + hi_mul * r) / R0 * R0;
+      lo[a] = (int)(tq - r) / R0 * R0 + org[a];
+      hi[a] = (int)(tq + hi_mul * r) / R0 * R0 + org[a];
     }
     const int d = T.depth;
     for (int x = lo[0]; x <= hi[0]; x += R0)
@@ -396,8 +399,8 @@ __global__ void __launch_bounds__(512) k_gather_field(SubcellCfg c, const int* _
   const int nt = c.sc * c.sc * c.sc;
   const int t = threadIdx.x;
   const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
-  const float vx = (float)(sc_pos[3 * k + 0] + tx), vy = (float)(sc_pos[3 * k + 1] + ty),
-              vz = (float)(sc_pos[3 * k + 2] + tz);
+  const float vx = (float)((sc_pos[3 * k + 0] & c.cmask) + tx), vy = (float)((sc_pos[3 * k + 1] & c.cmask) + ty),
+              vz = (float)((sc_pos[3 * k + 2] & c.cmask) + tz);
   const int n = cnt[k], o = off[k];
   float md2 = INFINITY;
   for (int base = 0; base < n; base += nt) {
@@ -454,7 +457,7 @@ __global__ void __launch_bounds__(256) k_levelset_sc4(SubcellCfg c, int64_t nsc,
   const int64_t k = (int64_t)blockIdx.x * 4 + wave;
   if (k >= nsc) return;
   const int lane = threadIdx.x & 63;
-  const int bx = sc_pos[3 * k], by = sc_pos[3 * k + 1], bz = sc_pos[3 * k + 2];
+  const int bx = sc_pos[3 * k] & c.cmask, by = sc_pos[3 * k + 1] & c.cmask, bz = sc_pos[3 * k + 2] & c.cmask;
   const float vx = (float)(bx + (lane & 3)), vy = (float)(by + ((lane >> 2) & 3)), vz = (float)(bz + (lane >> 4));
   const int n = cnt[k];
   const float* __restrict__ lp = list_pos + 3 * (int64_t)off[k];

@@ -17,7 +17,20 @@ struct PointsView {
   const char* vel; int vel_off, vel_stride;
   int64_t n;
   float tx, ty, tz;
+  // Batched clouds: point i belongs to cloud cloud[i], whose cell origin is its grid cell
+  // (cloud % grid_x, cloud / grid_x % grid_y, cloud / (grid_x * grid_y)) << cell_lg.  Positions
+  // stay in the cloud's own frame; only integer brick / voxel coordinates carry the origin.
+  // cloud == nullptr: one cloud at the origin.
+  const uint32_t* cloud;
+  int cell_lg, grid_x, grid_y;
 };
+
+__device__ __forceinline__ int3 cloud_origin(const PointsView& P, int64_t i) {
+  if (!P.cloud) return make_int3(0, 0, 0);
+  const uint32_t k = P.cloud[i];
+  const uint32_t gx = (uint32_t)P.grid_x, gy = (uint32_t)P.grid_y;
+  return make_int3((int)(k % gx) << P.cell_lg, (int)((k / gx) % gy) << P.cell_lg, (int)(k / (gx * gy)) << P.cell_lg);
+}
 
 // ---- keys
 template <typename K> void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, K* keys, hipStream_t s);
@@ -58,6 +71,7 @@ void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t c
 struct SubcellCfg {
   int sc, lgsc, D, res0;
   float r;
+  int cmask;  // position bits inside a batched cloud's cell (all bits for a single cloud)
 };
 void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map, int* sc_pos, int* sc_nid,
                   int* map_leaf, hipStream_t s);

@@ -39,6 +39,8 @@ SIGNATURES = {
     "gvdbx_add_channel": [_P, _I, _I, _I],
     "gvdbx_fill_channel": [_P, _I, _F],
     "gvdbx_set_points": [_P, _I64, _P, _I, _I, _P, _I, _I, _P, _I, _I],
+    "gvdbx_batch_set_points": [_P, _I, _P, _P, _I, _I, _P, _I, _I, _I, _F],
+    "gvdbx_batch_get_layout": [_P, C.POINTER(_I), C.POINTER(_I), _P],
     "gvdbx_rebuild_topology": [_P, _I64, _F, _PF],
     "gvdbx_accumulate_topology": [_P, _I64, _F, _PF, _I],
     "gvdbx_request_full_rebuild": [_P, _I],

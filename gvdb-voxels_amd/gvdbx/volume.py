@@ -92,6 +92,31 @@ class VolumeGVDB:
         self._chk(self.L.gvdbx_set_points(self.h, n, ptr(pos), pos_off, pos_stride, ptr(vel), vel_off, vel_stride,
                                           ptr(clr), clr_off, clr_stride), "set_points")
 
+    # ------------------------------------------------------------ batched clouds
+    def BatchSetPoints(self, pos, counts, vel=None, cell_log2=12, max_radius=1.0, pos_off=0, pos_stride=12,
+                       vel_off=0, vel_stride=12):
+        """Bind several clouds stored back to back in `pos` (counts[k] points each) as one batch:
+        cloud k runs in its own grid cell (see batch_origin) through the usual calls."""
+        ptr = lambda a: None if a is None else (a.ptr if isinstance(a, DeviceArray) else int(a))
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        self._points = (pos, vel, None)
+        self.npts = int(cnt.sum())
+        self._chk(self.L.gvdbx_batch_set_points(self.h, len(cnt), cnt.ctypes.data, ptr(pos), pos_off, pos_stride,
+                                                ptr(vel), vel_off, vel_stride, cell_log2, float(max_radius)),
+                  "batch_set_points")
+
+    def batch_layout(self):
+        ncloud, cell = C.c_int(0), C.c_int(0)
+        grid = np.zeros(3, np.int32)
+        self._chk(self.L.gvdbx_batch_get_layout(self.h, C.byref(ncloud), C.byref(cell), grid.ctypes.data),
+                  "batch_get_layout")
+        return ncloud.value, cell.value, grid
+
+    def batch_origin(self, k):
+        """Voxel origin of cloud k's cell."""
+        _, cell, g = self.batch_layout()
+        return np.array([k % g[0], (k // g[0]) % g[1], k // (g[0] * g[1])], np.int64) << cell
+
     # ------------------------------------------------------------ topology
     def RebuildTopology(self, n, radius, orig=(0, 0, 0)):
         self._chk(self.L.gvdbx_rebuild_topology(self.h, n, radius, fptr(orig)), "RebuildTopology")

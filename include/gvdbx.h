@@ -84,6 +84,24 @@ int gvdbx_set_points(gvdbx_ctx* ctx, int64_t n,
                      const void* vel, int vel_off, int vel_stride,
                      const void* clr, int clr_off, int clr_stride);
 
+/* ---- batched clouds (no reference counterpart): ncloud independent clouds, each in its own
+ *      frame, run through one RebuildTopology / UpdateAtlas / InsertPointsSubcell / Gather /
+ *      UpdateApron sequence.  pos holds the clouds back to back (counts[k] points each, host
+ *      array); cloud k is placed in grid cell k of 2^cell_log2 voxels, cell origin
+ *      (k % g0, k / g0 % g1, k / (g0 g1)) << cell_log2 (gvdbx_batch_get_layout), where its
+ *      bricks, subcells and channel values are those of the cloud run alone, shifted by the
+ *      origin (positions stay in the cloud's frame, so every value is bit-identical).
+ *      cell_log2 must be the range of an inner level; every valid point must stay below
+ *      2^cell_log2 - res0 - 2 max_radius - 3 in each coordinate (GVDBX_ERR_CAPACITY otherwise),
+ *      and later radii must not exceed max_radius (at most 1, which keeps every extra-pass box
+ *      inside the cloud's own root).  The batch keeps one tree depth for all clouds: a cloud
+ *      whose own depth would change between frames is not restarted.  gvdbx_set_points ends
+ *      batched mode.  Leaf bricks must be 8^3 or larger (GVDBX_ERR_ARG otherwise). */
+int gvdbx_batch_set_points(gvdbx_ctx* ctx, int ncloud, const int64_t* counts, const void* pos, int pos_off,
+                           int pos_stride, const void* vel, int vel_off, int vel_stride, int cell_log2,
+                           float max_radius);
+int gvdbx_batch_get_layout(const gvdbx_ctx* ctx, int* ncloud, int* cell_log2, int grid[3]);
+
 /* ---- topology: RebuildTopology (V.cpp:1181-1218), AccumulateTopology (V.cpp:1158-1178),
  *      RequestFullRebuild (V.h:432), FinishTopology (V.cpp:1579-1593), UpdateAtlas (V.cpp:2630-2704) */
 int gvdbx_rebuild_topology(gvdbx_ctx* ctx, int64_t n, float radius, const float orig[3]);

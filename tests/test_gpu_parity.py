@@ -219,3 +219,26 @@ def test_apron_state_transitions(oracle_mod):
     v.UpdateApron(0, 3.0)
     t.update_apron(0, 3.0)
     U.compare_channel(v, t, 0)
+
+
+@pytest.mark.parametrize("sc,levelset", [(2, False), (4, False), (2, True), (1, False)])
+def test_small_bricks_4cubed(oracle_mod, sc, levelset):
+    """4^3 leaf bricks (log2 dims 2,3,3,3,5): the generic brick / subcell kernels."""
+    U.log(f"case 4^3 bricks sc={sc} levelset={levelset}")
+    ld = (2, 3, 3, 3, 5)
+    p = rand_points(4000, 40 + sc, 20, 90)
+    v, t = U.new_volume(ld), U.new_tree(oracle_mod, ld)
+    U.run_pipeline(v, t, p, 1.0, sc, density=not levelset, levelset=levelset, apron_bound=3.0 if levelset else 0.0)
+
+
+def test_bricks_4cubed_incremental_with_velocity(oracle_mod):
+    from gvdbx import synth
+    U.log("case 4^3 bricks incremental velocity")
+    ld = (2, 3, 3, 3, 5)
+    p, vel = synth.sph_block(n=32_768, seed=3, box=48.0)
+
+    def move(cur, step):
+        return (cur + vel * np.float32(0.5)).astype(np.float32)
+
+    v, t = U.new_volume(ld, chans=(0, 1, 2, 3)), U.new_tree(oracle_mod, ld, chans=(0, 1, 2, 3))
+    U.run_pipeline(v, t, p, 1.0, 2, vel=vel, chans_vel=(1, 2, 3), steps=4, move=move)
