@@ -125,12 +125,16 @@ int fail(gvdbx_ctx* c, int code, const char* fmt, ...) {
     if (s_ != GVDBX_OK) return s_; \
   } while (0)
 
-// Grow-only buffer; contents are not kept.
+// Grow-only buffer; contents are not kept.  Work already enqueued may still read the old
+// buffer, so the stream is drained before it is freed (growth is rare in steady state).
 int ensure(gvdbx_ctx* c, Buf& b, size_t bytes) {
   if (bytes <= b.cap) return GVDBX_OK;
   size_t nb = b.cap ? b.cap : 256;
   while (nb < bytes) nb += nb / 2 + 256;
-  if (b.p) HIPCHK(c, hipFree(b.p));
+  if (b.p) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(b.p));
+  }
   b.p = nullptr;
   b.cap = 0;
   hipError_t e = hipMalloc(&b.p, nb);

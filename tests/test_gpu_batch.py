@@ -132,6 +132,11 @@ def test_batch_rejects_cloud_beyond_its_cell():
         v.RebuildTopology(2 * len(a), 2.0)  # above the guard radius
     assert e.value.code == 1
     run(v, 2 * len(a))
+    # two copies of one cloud: the bricks of each cell (in pool order) carry the same values
+    _, pos = live_leaves(v)
     ch = v.get_channel(0)
-    assert ch[: len(ch) // 2].tobytes() == ch[len(ch) // 2:].tobytes()  # two copies of one cloud
+    c0 = np.all(pos < (1 << CELL), axis=1)
+    c1 = ~c0
+    assert np.array_equal(pos[c0] + np.array([1 << CELL, 0, 0]), pos[c1])
+    assert ch[c0].tobytes() == ch[c1].tobytes()
     v.close()
