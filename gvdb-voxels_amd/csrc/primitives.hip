@@ -135,11 +135,64 @@ __global__ void __launch_bounds__(256) k_scan_down(const int* __restrict__ in, i
   }
 }
 
+// Small arrays (n <= kScanSmall): one 1024-thread workgroup walks 4096-int tiles with a running
+// carry -- one launch instead of three.
+constexpr int64_t kScanSmall = 65536;
+
+__global__ void __launch_bounds__(1024) k_scan_small(const int* __restrict__ in, int* __restrict__ out, int64_t n,
+                                                     int* __restrict__ total) {
+  __shared__ int ws[16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int carry = 0;
+  for (int64_t base = 0; base < n; base += 4096) {
+    const int64_t i = base + (int64_t)tid * 4;
+    int v[4];
+    if (i + 3 < n) {
+      const int4 q = *(const int4*)(in + i);
+      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+      for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? in[i + k] : 0;
+    }
+    const int t = v[0] + v[1] + v[2] + v[3];
+    int x = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[wave] = x;
+    __syncthreads();
+    int pre = carry, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      const int sw = ws[w];
+      pre += (w < wave) ? sw : 0;
+      tot += sw;
+    }
+    __syncthreads();
+    pre += x - t;
+    int o[4];
+    o[0] = pre; o[1] = pre + v[0]; o[2] = o[1] + v[1]; o[3] = o[2] + v[2];
+    if (i + 3 < n) {
+      *(int4*)(out + i) = make_int4(o[0], o[1], o[2], o[3]);
+    } else {
+      for (int k = 0; k < 4; ++k)
+        if (i + k < n) out[i + k] = o[k];
+    }
+    carry += tot;
+  }
+  if (tid == 0 && total) *total = carry;
+}
+
 int64_t scan_tile_count(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 
 void exclusive_scan(const int* in, int* out, int64_t n, int* tile_buf, int* total, hipStream_t s) {
   if (n <= 0) {
     if (total) fill_u32((uint32_t*)total, 1, 0u, s);
+    return;
+  }
+  if (n <= kScanSmall) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, in, out, n, total);
     return;
   }
   const int64_t nt = scan_tile_count(n);
