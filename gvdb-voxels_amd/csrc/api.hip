@@ -25,7 +25,7 @@ struct Buf {
   size_t cap = 0;
 };
 
-enum MetaSlot { M_BBOX = 0, M_UNIQUE = 8, M_COUNT = 9, M_USED = 10, M_TOTAL = 11, M_LEV = 16, M_INTS = 64 };
+enum MetaSlot { M_BBOX = 0, M_UNIQUE = 8, M_COUNT = 9, M_USED = 10, M_TOTAL = 11, M_LEV = 16, M_WIDE = 32, M_INTS = 64 };
 
 }  // namespace
 
@@ -125,6 +125,15 @@ int fail(gvdbx_ctx* c, int code, const char* fmt, ...) {
     if (s_ != GVDBX_OK) return s_; \
   } while (0)
 
+// A kernel launch the runtime refuses (a grid or block it cannot dispatch) leaves its work
+// undone without stopping the stream; this turns it into an error before anything reads the
+// results back or relies on them.
+int launch_status(gvdbx_ctx* c, const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, GVDBX_ERR_HIP, "%s: kernel launch failed: %s", what, hipGetErrorString(e));
+  return GVDBX_OK;
+}
+
 // Grow-only buffer; contents are not kept.  Work already enqueued may still read the old
 // buffer, so the stream is drained before it is freed (growth is rare in steady state).
 int ensure(gvdbx_ctx* c, Buf& b, size_t bytes) {
@@ -213,6 +222,7 @@ PointsView points_of(gvdbx_ctx* c, int64_t n, const float* tr) {
 }
 
 int read_meta(gvdbx_ctx* c) {
+  CHK(launch_status(c, "read-back"));
   HIPCHK(c, hipMemcpyAsync(c->h_meta, c->meta.p, M_INTS * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return GVDBX_OK;
@@ -796,7 +806,7 @@ int gvdbx_fill_channel(gvdbx_ctx* c, int chan, float v) {  // FillChannel / Clea
   CHK(chan_ok(c, chan));
   if (c->chan_slot_cap > 0) fill_f32(P<float>(c->chan[chan]), c->chan_slot_cap * brick_floats(c), v, c->stream);
   c->nbr_apron_ok[chan] = true;
-  return GVDBX_OK;
+  return launch_status(c, "FillChannel");
 }
 
 int gvdbx_set_points(gvdbx_ctx* c, int64_t n, const void* pos, int pos_off, int pos_stride, const void* vel,
@@ -833,9 +843,6 @@ int gvdbx_batch_set_points(gvdbx_ctx* c, int ncloud, const int64_t* counts, cons
   // [trunc(q) - r, trunc(q) + 2r] lies within the bounding box margins -1 / +2), so a batched
   // cloud gets exactly the bricks it gets alone
   if (!(max_radius >= 0.f) || max_radius > 1.f) return fail(c, GVDBX_ERR_ARG, "batch radius must lie in [0, 1]");
-  // batches of leaf bricks below 8^3 are switched off: such a batch faulted the GPU once
-  // and the cause is not identified yet
-  if (res_of(c, 0) < 8) return fail(c, GVDBX_ERR_ARG, "batched clouds need leaf bricks of 8^3 or more");
   int cell_lev = -1;
   for (int l = 1; l < c->levs - 1; ++l)
     if (c->lgR[l] == cell_log2) cell_lev = l;
@@ -902,12 +909,14 @@ int gvdbx_batch_get_layout(const gvdbx_ctx* c, int* ncloud, int* cell_log2, int 
 
 int gvdbx_rebuild_topology(gvdbx_ctx* c, int64_t n, float r, const float orig[3]) {
   if (!c) return GVDBX_ERR_ARG;
-  return rebuild_impl(c, n, r, orig, false, 0);
+  CHK(rebuild_impl(c, n, r, orig, false, 0));
+  return launch_status(c, "RebuildTopology");
 }
 
 int gvdbx_accumulate_topology(gvdbx_ctx* c, int64_t n, float r, const float orig[3], int min_depth) {
   if (!c) return GVDBX_ERR_ARG;
-  return rebuild_impl(c, n, r, orig, true, min_depth);
+  CHK(rebuild_impl(c, n, r, orig, true, min_depth));
+  return launch_status(c, "AccumulateTopology");
 }
 
 int gvdbx_request_full_rebuild(gvdbx_ctx* c, int flag) {
@@ -968,7 +977,7 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   neighbor_table(T, c->nslots, P<int4>(c->slot_pos), P<int>(c->leaf_slot), res_of(c, 0), P<int>(c->nbr), c->stream);
   CHK(reserve_channels(c, c->nslots));
   c->atlas_version = c->topo_version;
-  return GVDBX_OK;
+  return launch_status(c, "UpdateAtlas");
 }
 
 int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t* len) {
@@ -1087,18 +1096,30 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   c->nsc = nsc;
   c->lists_have_vel = with_vel && c->store_vel_lists;
   c->have_vsum = with_vel;
-  // Lists sized for the largest possible length when that bound is modest, so the insert is
-  // enqueued before the single length read-back; otherwise read the length first.
+  // Lists sized for the largest possible length when that bound is modest (and below 2^31, so
+  // the 32-bit offsets cannot wrap), so the insert is enqueued before the single length
+  // read-back; otherwise the length is read and checked before anything is inserted.  From a
+  // bound of 2^32 on, the 32-bit scan total can wrap, so the length comes from a 64-bit sum.
   int64_t per_axis = (int64_t)((2.0f * r + 1.0f) / (float)sc) + 2;
   const int64_t bound = per_axis * per_axis * per_axis * n;
-  bool early = (size_t)(bound + 1) * 12 <= (c->list_pos.cap > ((size_t)1 << 30) ? c->list_pos.cap : ((size_t)1 << 30));
+  const size_t room = c->list_pos.cap > ((size_t)1 << 30) ? c->list_pos.cap : ((size_t)1 << 30);
+  const bool early = bound <= INT32_MAX && (size_t)(bound + 1) * 12 <= room;
   int64_t ln = -1;
   if (early) {
     CHK(ensure(c, c->list_pos, (size_t)(bound + 1) * 12));
     if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(bound + 1) * 12));
   } else {
+    const bool wide = bound >= ((int64_t)1 << 32);
+    if (wide) sum_wide(P<int>(c->sc_cnt), nsc, (unsigned long long*)(P<int>(c->meta) + M_WIDE), c->stream);
     CHK(read_meta(c));
-    ln = (uint32_t)c->h_meta[M_TOTAL];
+    if (wide) {
+      unsigned long long w;
+      memcpy(&w, c->h_meta + M_WIDE, sizeof(w));
+      ln = (int64_t)w;
+    } else {
+      ln = (uint32_t)c->h_meta[M_TOTAL];
+    }
+    if (ln > INT32_MAX || ln > bound) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);
     CHK(ensure(c, c->list_pos, (size_t)(ln + 1) * 12));
     if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(ln + 1) * 12));
   }
@@ -1117,7 +1138,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   if (ln > INT32_MAX || ln > bound) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);
   c->sclen = ln;
   if (len) *len = ln;
-  return GVDBX_OK;
+  return launch_status(c, "InsertPointsSubcell");
 }
 
 int gvdbx_set_subcell_options(gvdbx_ctx* c, int store_velocity_lists) {
@@ -1148,7 +1169,7 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
                    c->stream);
     c->nbr_apron_ok[chan] = false;
   }
-  return GVDBX_OK;
+  return launch_status(c, "Gather");
 }
 
 int gvdbx_gather_density(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t len, int chan,
@@ -1188,7 +1209,7 @@ int gvdbx_gather_velocity(gvdbx_ctx* c, int cx, int cy, int cz, int accum) {
                      P<float>(c->chan[cy]), P<float>(c->chan[cz]), accum, c->stream);
   }
   c->nbr_apron_ok[cx] = c->nbr_apron_ok[cy] = c->nbr_apron_ok[cz] = apron_done;
-  return GVDBX_OK;
+  return launch_status(c, "GatherVelocity");
 }
 
 int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {
@@ -1201,7 +1222,7 @@ int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {
   levelset_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
                  P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), 0, init, c->stream);
   c->nbr_apron_ok[chan] = false;
-  return GVDBX_OK;
+  return launch_status(c, "ScatterReduceLevelSet");
 }
 
 int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* boundvals, int nchan) {
@@ -1231,7 +1252,7 @@ int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* bou
     }
   }
   for (int i = 0; i < nchan; ++i) c->nbr_apron_ok[chans[i]] = true;
-  return GVDBX_OK;
+  return launch_status(c, "UpdateApron");
 }
 
 int gvdbx_update_apron(gvdbx_ctx* c, int chan, float boundval) {
@@ -1242,13 +1263,14 @@ int gvdbx_sample_trilinear(gvdbx_ctx* c, int chan, const void* q, int64_t n, voi
   CHK(chan_ok(c, chan));
   if (c->batch) return fail(c, GVDBX_ERR_STATE, "sampling takes world points: not for batched clouds");
   if (!q || !out || n < 0) return fail(c, GVDBX_ERR_ARG, "bad sample buffers");
+  if (n > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "more than 2^31 queries");
   if (c->apron < 1) return fail(c, GVDBX_ERR_STATE, "trilinear sampling needs an apron >= 1");
   CHK(atlas_current(c));
   if (c->depth < 0) return fail(c, GVDBX_ERR_STATE, "no topology");
   TimedRegion tr(c, "sample");
   sample_trilinear(tree_of(c), P<int>(c->leaf_slot), res_of(c, 0), c->apron, P<float>(c->chan[chan]),
                    (const float*)q, n, (float*)out, c->stream);
-  return GVDBX_OK;
+  return launch_status(c, "SampleTrilinear");
 }
 
 namespace {
@@ -1295,7 +1317,7 @@ int splat_impl(gvdbx_ctx* c, int chan, const float* q, const float* mass, int64_
   c->sp_order_n = n;
   c->sp_order = n > 0 ? P<uint32_t>(c->sp_perm) : nullptr;
   c->nbr_apron_ok[chan] = false;
-  return GVDBX_OK;
+  return launch_status(c, "SplatTrilinear");
 }
 
 }  // namespace
@@ -1315,7 +1337,7 @@ int gvdbx_splat_backward(gvdbx_ctx* c, int chan_grad, const void* q, const void*
   const uint32_t* order = (c->sp_order && c->sp_order_q == q && c->sp_order_n == n) ? c->sp_order : nullptr;
   splat_backward(tree_of(c), P<int>(c->leaf_slot), P<int>(c->nbr), c->ld[0], c->apron, P<float>(c->chan[chan_grad]),
                  (const float*)q, (const float*)mass, n, order, (float*)dq, (float*)dmass, c->stream);
-  return GVDBX_OK;
+  return launch_status(c, "SplatBackward");
 }
 
 int gvdbx_sample_trilinear_backward(gvdbx_ctx* c, int chan, const void* q, const void* dout, int64_t n, void* dq,
@@ -1329,7 +1351,7 @@ int gvdbx_sample_trilinear_backward(gvdbx_ctx* c, int chan, const void* q, const
     sample_backward(tree_of(c), P<int>(c->leaf_slot), res_of(c, 0), c->apron, P<float>(c->chan[chan]),
                     (const float*)q, (const float*)dout, n, (float*)dq, c->stream);
   if (chan_grad >= 0) CHK(splat_impl(c, chan_grad, (const float*)q, (const float*)dout, n, accum_grad));
-  return GVDBX_OK;
+  return launch_status(c, "SampleTrilinearBackward");
 }
 
 int gvdbx_get_depth(const gvdbx_ctx* c, int* depth, int32_t rootpos[3], float posmin[3], float posmax[3]) {

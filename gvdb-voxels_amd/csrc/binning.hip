@@ -232,11 +232,13 @@ __device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, Br
 }
 
 template <int kD>
-__global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int use_bins) {
+__global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_bins) {
   extern __shared__ unsigned long long smem64[];
   __shared__ int nb[27];
   BrickCtx B;
-  const int m = blockIdx.x;
+  const int64_t mi = item_index();
+  if (mi >= nmap) return;
+  const int m = (int)mi;
   const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W, D3 = D * D * D;
   if (!brick_setup(A, m, nb, B, D, W)) return;
   const int NB = use_bins ? D3 * 64 : 0;
@@ -353,11 +355,13 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int use_bins) {
 }
 
 template <int kD>
-__global__ void __launch_bounds__(256) k_bin_insert(BinArgs A) {
+__global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
   extern __shared__ unsigned long long smem64[];
   __shared__ int nb[27];
   BrickCtx B;
-  const int m = blockIdx.x;
+  const int64_t mi = item_index();
+  if (mi >= nmap) return;
+  const int m = (int)mi;
   const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W;
   if (!brick_setup(A, m, nb, B, D, W)) return;
   int* lbase = (int*)smem64;
@@ -414,9 +418,9 @@ size_t bin_count_smem(int W, int D) {
 void bin_count(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
   if (A.c.D == 2)
-    hipLaunchKernelGGL(k_bin_count<2>, dim3(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A, 1);
+    hipLaunchKernelGGL(k_bin_count<2>, item_grid(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A, nmap, 1);
   else
-    hipLaunchKernelGGL(k_bin_count<0>, dim3(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A,
+    hipLaunchKernelGGL(k_bin_count<0>, item_grid(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A, nmap,
                        bin_uses_class_bins(A.c.D));
 }
 
@@ -424,9 +428,9 @@ void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
   const size_t sm = (size_t)2 * A.W * A.W * A.W * 4 + 8;
   if (A.c.D == 2)
-    hipLaunchKernelGGL(k_bin_insert<2>, dim3(nmap), dim3(256), sm, s, A);
+    hipLaunchKernelGGL(k_bin_insert<2>, item_grid(nmap), dim3(256), sm, s, A, nmap);
   else
-    hipLaunchKernelGGL(k_bin_insert<0>, dim3(nmap), dim3(256), sm, s, A);
+    hipLaunchKernelGGL(k_bin_insert<0>, item_grid(nmap), dim3(256), sm, s, A, nmap);
 }
 
 __global__ void k_sum_counts(const int* __restrict__ a, const int* __restrict__ b, int* __restrict__ out, int64_t n) {
@@ -581,7 +585,7 @@ __device__ __forceinline__ int64_t atlas_index(int slot, int br, int lx, int ly,
 // brick computes for g.  Apron voxels whose neighbour brick exists are written with it; the
 // others keep waiting for UpdateApron's boundary value.
 template <int kRes, int kApron, int kD, bool kWithApron>
-__global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int64_t nsc, int64_t len,
+__global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len,
                                                       const int* __restrict__ map_leaf, const int* __restrict__ off,
                                                       const float* __restrict__ list_pos,
                                                       const int* __restrict__ leaf_slot,
@@ -599,7 +603,8 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int64_t nsc
   const int br = res0 + 2 * apron;        // channel brick layout
   const int bw = res0 + 2 * a;            // computed region
   const int D3 = D * D * D;
-  const int m = blockIdx.x;
+  const int64_t m = item_index();
+  if (m >= nmap) return;
   const int leaf = map_leaf[m];
   if (leaf < 0) return;
   const int slot = leaf_slot[leaf];
@@ -672,16 +677,16 @@ void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* 
   if (nmap <= 0) return;
   const bool fixed = c.res0 == 8 && c.D == 2 && apron == 1;
   if (fixed && nbr)
-    hipLaunchKernelGGL((k_density_brick<8, 1, 2, true>), dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off,
+    hipLaunchKernelGGL((k_density_brick<8, 1, 2, true>), item_grid(nmap), dim3(256), 0, s, c, nmap, nsc, len, map_leaf, off,
                        list_pos, leaf_slot, nodes0, apron, nbr, chan);
   else if (fixed)
-    hipLaunchKernelGGL((k_density_brick<8, 1, 2, false>), dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off,
+    hipLaunchKernelGGL((k_density_brick<8, 1, 2, false>), item_grid(nmap), dim3(256), 0, s, c, nmap, nsc, len, map_leaf, off,
                        list_pos, leaf_slot, nodes0, apron, nbr, chan);
   else if (nbr)
-    hipLaunchKernelGGL((k_density_brick<0, 0, 0, true>), dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off,
+    hipLaunchKernelGGL((k_density_brick<0, 0, 0, true>), item_grid(nmap), dim3(256), 0, s, c, nmap, nsc, len, map_leaf, off,
                        list_pos, leaf_slot, nodes0, apron, nbr, chan);
   else
-    hipLaunchKernelGGL((k_density_brick<0, 0, 0, false>), dim3(nmap), dim3(256), 0, s, c, nsc, len, map_leaf, off,
+    hipLaunchKernelGGL((k_density_brick<0, 0, 0, false>), item_grid(nmap), dim3(256), 0, s, c, nmap, nsc, len, map_leaf, off,
                        list_pos, leaf_slot, nodes0, apron, nbr, chan);
 }
 
@@ -698,7 +703,7 @@ __device__ __forceinline__ float mean_of(const int* cnt, const unsigned long lon
 }
 
 template <int kRes, int kApron, int kD>
-__global__ void __launch_bounds__(256) k_velocity_brick(SubcellCfg c, const int* __restrict__ map_leaf,
+__global__ void __launch_bounds__(256) k_velocity_brick(SubcellCfg c, int nmap, const int* __restrict__ map_leaf,
                                                        const int* __restrict__ cnt,
                                                        const unsigned long long* __restrict__ vsum,
                                                        const int* __restrict__ leaf_slot, int apron_rt,
@@ -710,7 +715,8 @@ __global__ void __launch_bounds__(256) k_velocity_brick(SubcellCfg c, const int*
   const int apron = kRes ? kApron : apron_rt;
   const int D = kRes ? kD : c.D;
   const int lgsc = kRes ? (kD == 2 ? 2 : c.lgsc) : c.lgsc;
-  const int m = blockIdx.x;
+  const int64_t m = item_index();
+  if (m >= nmap) return;
   const int leaf = map_leaf[m];
   if (leaf < 0) return;
   const int slot = leaf_slot[leaf];
@@ -762,10 +768,10 @@ void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt,
                     hipStream_t s) {
   if (nmap <= 0) return;
   if (c.res0 == 8 && c.D == 2 && apron == 1)
-    hipLaunchKernelGGL((k_velocity_brick<8, 1, 2>), dim3(nmap), dim3(256), 0, s, c, map_leaf, cnt, vsum, leaf_slot,
+    hipLaunchKernelGGL((k_velocity_brick<8, 1, 2>), item_grid(nmap), dim3(256), 0, s, c, nmap, map_leaf, cnt, vsum, leaf_slot,
                        apron, nbr, cx, cy, cz, accum);
   else
-    hipLaunchKernelGGL((k_velocity_brick<0, 0, 0>), dim3(nmap), dim3(256), 0, s, c, map_leaf, cnt, vsum, leaf_slot,
+    hipLaunchKernelGGL((k_velocity_brick<0, 0, 0>), item_grid(nmap), dim3(256), 0, s, c, nmap, map_leaf, cnt, vsum, leaf_slot,
                        apron, nbr, cx, cy, cz, accum);
 }
 
@@ -775,7 +781,8 @@ __global__ void __launch_bounds__(256) k_apron_bound(int64_t nslots, const int* 
                                                     ApronChannels ch) {
   __shared__ int nb[27];
   __shared__ int any;
-  const int64_t s = blockIdx.x;
+  const int64_t s = item_index();
+  if (s >= nslots) return;
   if (threadIdx.x == 0) any = 0;
   __syncthreads();
   if (threadIdx.x < 27) {
@@ -798,15 +805,16 @@ __global__ void __launch_bounds__(256) k_apron_bound(int64_t nslots, const int* 
 
 void apron_bound(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s) {
   if (nslots <= 0 || apron <= 0 || ch.n <= 0) return;
-  hipLaunchKernelGGL(k_apron_bound, dim3((unsigned)nslots), dim3(256), 0, s, nslots, nbr, res0, apron, ch);
+  hipLaunchKernelGGL(k_apron_bound, item_grid(nslots), dim3(256), 0, s, nslots, nbr, res0, apron, ch);
 }
 
 // Per-subcell form of the velocity means, for bricks with more than 512 subcells.
-__global__ void k_velocity_subcell(SubcellCfg c, const int* __restrict__ sc_pos, const int* __restrict__ sc_nid,
+__global__ void k_velocity_subcell(SubcellCfg c, int64_t nsc, const int* __restrict__ sc_pos, const int* __restrict__ sc_nid,
                                    const int* __restrict__ cnt, const unsigned long long* __restrict__ vsum,
                                    const int* __restrict__ leaf_slot, int apron, float* __restrict__ cx,
                                    float* __restrict__ cy, float* __restrict__ cz, int accum) {
-  const int64_t k = blockIdx.x;
+  const int64_t k = item_index();
+  if (k >= nsc) return;
   const int slot = leaf_slot[sc_nid[k]];
   if (slot < 0) return;
   const double den = (double)((long long)cnt[k] + (accum ? 1 : 0));
@@ -832,7 +840,7 @@ void velocity_subcell(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* s
                       float* cz, int accum, hipStream_t s) {
   if (nsc <= 0) return;
   const int nv = c.sc * c.sc * c.sc;
-  hipLaunchKernelGGL(k_velocity_subcell, dim3((unsigned)nsc), dim3(nv < 64 ? 64 : nv), 0, s, c, sc_pos, sc_nid, cnt,
+  hipLaunchKernelGGL(k_velocity_subcell, item_grid(nsc), dim3(nv < 64 ? 64 : nv), 0, s, c, nsc, sc_pos, sc_nid, cnt,
                      vsum, leaf_slot, apron, cx, cy, cz, accum);
 }
 
@@ -863,7 +871,8 @@ void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, cons
 __global__ void __launch_bounds__(256) k_apron_multi(int64_t nslots, const int* __restrict__ nbr, int res0, int apron,
                                                     ApronChannels ch) {
   __shared__ int nb[27];
-  const int64_t s = blockIdx.x;
+  const int64_t s = item_index();
+  if (s >= nslots) return;
   if (threadIdx.x < 27) nb[threadIdx.x] = nbr[s * 27 + threadIdx.x];
   __syncthreads();
   const int br = res0 + 2 * apron;
@@ -889,7 +898,7 @@ __global__ void __launch_bounds__(256) k_apron_multi(int64_t nslots, const int* 
 
 void apron_multi(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s) {
   if (nslots <= 0 || apron <= 0 || ch.n <= 0) return;
-  hipLaunchKernelGGL(k_apron_multi, dim3((unsigned)nslots), dim3(256), 0, s, nslots, nbr, res0, apron, ch);
+  hipLaunchKernelGGL(k_apron_multi, item_grid(nslots), dim3(256), 0, s, nslots, nbr, res0, apron, ch);
 }
 
 }  // namespace gvdbx

@@ -30,6 +30,19 @@ struct DevTree {
   uint64_t* clist[kMaxLev];
 };
 
+// A dispatch carries at most 2^32 - 1 work-items per grid dimension (the grid sizes of
+// hsa_kernel_dispatch_packet_t are 32-bit), so a kernel that gives one workgroup to each brick,
+// slot or subcell would overflow it from 2^32 / blockDim items on (16.7M bricks at 256 threads).
+// Such kernels lay their workgroups out on a 2D grid of at most kGridCols columns, recover the
+// linear item with item_index() and return when it is past the item count.
+constexpr int64_t kGridCols = 65536;
+inline dim3 item_grid(int64_t items) {
+  if (items < 1) items = 1;
+  const int64_t cols = items < kGridCols ? items : kGridCols;
+  return dim3((unsigned)cols, (unsigned)((items + cols - 1) / cols));
+}
+__device__ __forceinline__ int64_t item_index() { return (int64_t)blockIdx.y * gridDim.x + blockIdx.x; }
+
 __device__ __forceinline__ uint64_t elem0(int lev, uint64_t ndx) { return ((uint64_t)lev << 8) | (ndx << 16); }
 
 __device__ __forceinline__ bool in_root(const DevTree& T, int x, int y, int z) {

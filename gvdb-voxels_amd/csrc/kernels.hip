@@ -273,7 +273,7 @@ void delink_leaves(const DevTree& T, int64_t count, hipStream_t s) {
 
 // gvdbDelinkBricks (N.cuh:431-467): one wave per inner node scans its child list.
 __global__ void __launch_bounds__(256) k_delink_level(DevTree T, int lev, int64_t count) {
-  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t i = item_index() * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= count) return;
   gvdbx_node* nd = T.nodes[lev] + i;
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(256) k_delink_level(DevTree T, int lev, int64_
 }
 void delink_level(const DevTree& T, int lev, int64_t count, hipStream_t s) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_delink_level, dim3(blocks_for(count, 4)), dim3(256), 0, s, T, lev, count);
+  hipLaunchKernelGGL(k_delink_level, item_grid((count + 3) / 4), dim3(256), 0, s, T, lev, count);
 }
 
 // by_parent = 0: mFlags (UpdateAtlas V.cpp:2655-2661); 1: mParent != ID_UNDEF64
@@ -389,13 +389,14 @@ __device__ __forceinline__ int64_t atlas_voxel(int slot, int br, int lx, int ly,
   return (((int64_t)slot * br + lz) * br + ly) * br + lx;
 }
 
-__global__ void __launch_bounds__(512) k_gather_field(SubcellCfg c, const int* __restrict__ sc_pos,
+__global__ void __launch_bounds__(512) k_gather_field(SubcellCfg c, int64_t nsc, const int* __restrict__ sc_pos,
                                                       const int* __restrict__ sc_nid, const int* __restrict__ cnt,
                                                       const int* __restrict__ off, const float* __restrict__ list_pos,
                                                       const int* __restrict__ leaf_slot, int apron,
                                                       float* __restrict__ chan, int accum, int mode, float init) {
   __shared__ float sx[512], sy[512], sz[512];
-  const int k = blockIdx.x;
+  const int64_t k = item_index();
+  if (k >= nsc) return;
   const int nt = c.sc * c.sc * c.sc;
   const int t = threadIdx.x;
   const int tx = t % c.sc, ty = (t / c.sc) % c.sc, tz = t / (c.sc * c.sc);
@@ -438,7 +439,7 @@ void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_ni
                   const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode, float init,
                   hipStream_t s) {
   if (nsc <= 0) return;
-  hipLaunchKernelGGL(k_gather_field, dim3((unsigned)nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, sc_pos, sc_nid, cnt, off,
+  hipLaunchKernelGGL(k_gather_field, item_grid(nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, nsc, sc_pos, sc_nid, cnt, off,
                      list_pos, leaf_slot, apron, chan, accum, mode, init);
 }
 
@@ -454,7 +455,7 @@ __global__ void __launch_bounds__(256) k_levelset_sc4(SubcellCfg c, int64_t nsc,
                                                      const int* __restrict__ leaf_slot, int apron,
                                                      float* __restrict__ chan, int accum, float init) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t k = (int64_t)blockIdx.x * 4 + wave;
+  const int64_t k = item_index() * 4 + wave;
   if (k >= nsc) return;
   const int lane = threadIdx.x & 63;
   const int bx = sc_pos[3 * k] & c.cmask, by = sc_pos[3 * k + 1] & c.cmask, bz = sc_pos[3 * k + 2] & c.cmask;
@@ -489,7 +490,7 @@ void levelset_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_
     gather_field(c, nsc, sc_pos, sc_nid, cnt, off, list_pos, leaf_slot, apron, chan, accum, 1, init, s);
     return;
   }
-  hipLaunchKernelGGL(k_levelset_sc4, dim3(blocks_for(nsc, 4, 1ll << 31)), dim3(256), 0, s, c, nsc, sc_pos, sc_nid,
+  hipLaunchKernelGGL(k_levelset_sc4, item_grid((nsc + 3) / 4), dim3(256), 0, s, c, nsc, sc_pos, sc_nid,
                      cnt, off, list_pos, leaf_slot, apron, chan, accum, init);
 }
 

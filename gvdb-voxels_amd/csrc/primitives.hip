@@ -35,6 +35,25 @@ void fill_f32(float* p, int64_t n, float v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
 }
 
+// ------------------------------------------------------------------ wide sum
+
+__global__ void __launch_bounds__(256) k_sum_wide(const int* __restrict__ in, int64_t n,
+                                                 unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long ws[4];
+  unsigned long long acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (unsigned int)in[i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(total, ws[0] + ws[1] + ws[2] + ws[3]);
+}
+
+void sum_wide(const int* in, int64_t n, unsigned long long* total, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sum_wide, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, in, n, total);
+}
+
 // ------------------------------------------------------------------ scan
 
 // Exclusive scan of a 256-thread block's values; returns the block total in *total.

@@ -16,6 +16,10 @@ void fill_f32(float* p, int64_t n, float v, hipStream_t s);
 int64_t scan_tile_count(int64_t n);
 void exclusive_scan(const int* in, int* out, int64_t n, int* tile_buf, int* total, hipStream_t s);
 
+// *total (device, zeroed by the caller) += sum of n non-negative int32 values, in 64 bits: the
+// scan's 32-bit total wraps once the sum passes 2^32.
+void sum_wide(const int* in, int64_t n, unsigned long long* total, hipStream_t s);
+
 // Stable LSD radix sort over bits [begin_bit, end_bit) in 8-bit digits, ping-ponging between
 // keys and keys_alt (vals/vals_alt optional).  Returns 0 if the result is in keys, 1 if in
 // keys_alt.  counts holds radix_count_ints(n) ints, tile_buf scan_tile_count of that.

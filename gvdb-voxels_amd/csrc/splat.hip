@@ -97,14 +97,17 @@ __device__ __forceinline__ void axis_combo(int k, int& d, int& r) {
 }
 
 template <int kLg>
-__global__ void __launch_bounds__(64) k_splat_brick(const int* __restrict__ start, const float4* __restrict__ payload,
+__global__ void __launch_bounds__(64) k_splat_brick(int64_t nslots, const int* __restrict__ start,
+                                                   const float4* __restrict__ payload,
                                                    const int* __restrict__ nbr, const int4* __restrict__ slot_pos,
                                                    int apron, float* __restrict__ chan, int accum) {
   constexpr int kRes = 1 << kLg, kVox = kRes * kRes * kRes, kRanges = 125;
   __shared__ long long acc[kVox];
   __shared__ int rbeg[kRanges];      // non-empty (direction, region) ranges: first sorted position
   __shared__ int rpre[kRanges + 1];  // their starts in the brick's virtual point list
-  const int s = blockIdx.x, lane = threadIdx.x;  // one wave per brick
+  const int64_t s = item_index();  // one wave per brick
+  if (s >= nslots) return;
+  const int lane = threadIdx.x;
   for (int v = lane; v < kVox; v += 64) acc[v] = 0;
   const int nbv = lane < 27 ? nbr[(int64_t)s * 27 + lane] : -1;
   // lane l owns ranges l and l + 64; empty ones are dropped by ballot compaction
@@ -314,12 +317,12 @@ bool splat_brick_supported(int lgres) { return lgres >= 1 && lgres <= 4; }
 void splat_brick(int64_t nslots, const int* start, const float4* payload, const int* nbr, const int4* slot_pos,
                  int lgres, int apron, float* chan, int accum, hipStream_t s) {
   if (nslots <= 0) return;
-  const dim3 g((unsigned)nslots), b(64);
+  const dim3 g = item_grid(nslots), b(64);
   switch (lgres) {
-    case 1: hipLaunchKernelGGL(k_splat_brick<1>, g, b, 0, s, start, payload, nbr, slot_pos, apron, chan, accum); break;
-    case 2: hipLaunchKernelGGL(k_splat_brick<2>, g, b, 0, s, start, payload, nbr, slot_pos, apron, chan, accum); break;
-    case 3: hipLaunchKernelGGL(k_splat_brick<3>, g, b, 0, s, start, payload, nbr, slot_pos, apron, chan, accum); break;
-    default: hipLaunchKernelGGL(k_splat_brick<4>, g, b, 0, s, start, payload, nbr, slot_pos, apron, chan, accum); break;
+    case 1: hipLaunchKernelGGL(k_splat_brick<1>, g, b, 0, s, nslots, start, payload, nbr, slot_pos, apron, chan, accum); break;
+    case 2: hipLaunchKernelGGL(k_splat_brick<2>, g, b, 0, s, nslots, start, payload, nbr, slot_pos, apron, chan, accum); break;
+    case 3: hipLaunchKernelGGL(k_splat_brick<3>, g, b, 0, s, nslots, start, payload, nbr, slot_pos, apron, chan, accum); break;
+    default: hipLaunchKernelGGL(k_splat_brick<4>, g, b, 0, s, nslots, start, payload, nbr, slot_pos, apron, chan, accum); break;
   }
 }
 

@@ -11,9 +11,8 @@ import time
 from .lib import GvdbxError
 from .volume import VolumeGVDB
 
-# (leaf log2, subcell) candidates; inner levels stay at 3 (8^3 children) and the root at 5.
-# Batches of 4^3 bricks are refused by the library for now, so only 8^3 bricks are tried.
-CANDIDATES = ((3, 4), (3, 2))
+# (leaf log2, subcell) candidates; inner levels stay at 3 (8^3 children) and the root at 5
+CANDIDATES = ((3, 4), (3, 2), (2, 2), (2, 4))
 
 
 def levels_for(leaf_log2):

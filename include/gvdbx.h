@@ -96,7 +96,7 @@ int gvdbx_set_points(gvdbx_ctx* ctx, int64_t n,
  *      and later radii must not exceed max_radius (at most 1, which keeps every extra-pass box
  *      inside the cloud's own root).  The batch keeps one tree depth for all clouds: a cloud
  *      whose own depth would change between frames is not restarted.  gvdbx_set_points ends
- *      batched mode.  Leaf bricks must be 8^3 or larger (GVDBX_ERR_ARG otherwise). */
+ *      batched mode. */
 int gvdbx_batch_set_points(gvdbx_ctx* ctx, int ncloud, const int64_t* counts, const void* pos, int pos_off,
                            int pos_stride, const void* vel, int vel_off, int vel_stride, int cell_log2,
                            float max_radius);
