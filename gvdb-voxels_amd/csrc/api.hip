@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "binning.hpp"
@@ -386,10 +387,14 @@ void link_all(gvdbx_ctx* c, int d) {  // per-level top-down gvdbLinkBricks launc
 
 template <typename K>
 int full_pass(gvdbx_ctx* c, int64_t n, const float* tr, int d) {  // ActivateBricksGPU (V.cpp:1229-1413)
+  // The sorted unique keys of every level (level in the top bits) are the unique leaf keys
+  // followed by their ancestors, so only the n leaf keys are sorted at full size; each unique
+  // leaf then expands into its d keys, and that short list goes through the usual sort /
+  // unique / allocation.  The node set and order are those of sorting all n * d keys.
   KeyFmt f = key_fmt(c, d);
-  const int64_t m = n * d;
-  CHK(ensure(c, c->keys, (size_t)m * sizeof(K)));
-  CHK(ensure(c, c->keys_alt, (size_t)m * sizeof(K)));
+  if (n > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "point count exceeds 2^31");
+  CHK(ensure(c, c->keys, (size_t)n * sizeof(K)));
+  CHK(ensure(c, c->keys_alt, (size_t)n * sizeof(K)));
   CHK(reserve_level(c, d, 1));
   c->cnt[d] = 1;
   DevTree T = tree_of(c);
@@ -397,8 +402,33 @@ int full_pass(gvdbx_ctx* c, int64_t n, const float* tr, int d) {  // ActivateBri
   fill_u64(P<uint64_t>(c->clist[d]), clist_per(c, d), kUndef64, c->stream);
   {
     TimedRegion trg(c, "keys");
-    brick_keys<K>(T, points_of(c, n, tr), f, P<K>(c->keys), c->stream);
+    brick_keys<K>(T, points_of(c, n, tr), f, 1, P<K>(c->keys), c->stream);
   }
+  const int64_t ncnt = radix_count_ints(n);
+  CHK(ensure(c, c->radix_counts, (size_t)ncnt * 4));
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(ncnt > n ? ncnt : n) + 16) * 4));
+  CHK(ensure(c, c->flags, (size_t)n * 4));
+  CHK(ensure(c, c->rank, (size_t)n * 4));
+  int par;
+  {
+    TimedRegion trs(c, "sort");
+    if (sizeof(K) == 4)
+      par = radix_sort_u32(P<uint32_t>(c->keys), P<uint32_t>(c->keys_alt), nullptr, nullptr, n, 0, f.nbits,
+                           P<int>(c->radix_counts), P<int>(c->tile_buf), c->stream);
+    else
+      par = radix_sort_u64(P<uint64_t>(c->keys), P<uint64_t>(c->keys_alt), nullptr, nullptr, n, 0, f.nbits,
+                           P<int>(c->radix_counts), P<int>(c->tile_buf), c->stream);
+  }
+  if (par) std::swap(c->keys, c->keys_alt);  // sorted leaf keys now in c->keys
+  CHK(zero_meta(c));
+  unique_flags<K>(P<K>(c->keys), n, f, d, P<int>(c->flags), c->stream);
+  exclusive_scan(P<int>(c->flags), P<int>(c->rank), n, P<int>(c->tile_buf), P<int>(c->meta) + M_UNIQUE, c->stream);
+  CHK(read_meta(c));
+  const int64_t m = (int64_t)c->h_meta[M_UNIQUE] * d;
+  CHK(ensure(c, c->keys_alt, (size_t)(m > 0 ? m : 1) * sizeof(K)));
+  expand_ancestors<K>(T, P<K>(c->keys), P<int>(c->flags), P<int>(c->rank), n, f, P<K>(c->keys_alt), c->stream);
+  std::swap(c->keys, c->keys_alt);  // expanded keys in c->keys for the sort / unique / allocation
+  CHK(ensure(c, c->keys_alt, (size_t)(m > 0 ? m : 1) * sizeof(K)));
   CHK(sort_unique_alloc<K>(c, m, f, d));
   link_all(c, d);
   return GVDBX_OK;

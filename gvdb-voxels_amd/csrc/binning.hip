@@ -206,8 +206,10 @@ __device__ __forceinline__ void add_vel_global(unsigned long long* vsum, int64_t
   atomicAdd(&vsum[3 * g + 2], (unsigned long long)fv[2]);
 }
 
-// Loads the 27 neighbour subcell-map indices; returns false for an unused map entry.
-__device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, BrickCtx& B, int D, int W) {
+// Loads the 27 neighbour subcell-map indices (by tree descent, or from nbm where the count
+// kernel of the same insert stored them); returns false for an unused map entry.
+__device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, BrickCtx& B, int D, int W,
+                                            bool stored = false) {
   const int leaf = A.map_leaf[m];
   if (leaf < 0) return false;
   const gvdbx_node* nd = A.T.nodes[0] + leaf;
@@ -222,7 +224,9 @@ __device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, Br
   B.D = D;
   B.D3 = D * D * D;
   B.nb = nb;
-  if (threadIdx.x < 27) {
+  if (threadIdx.x < 27 && stored) {
+    nb[threadIdx.x] = A.nbm[(int64_t)m * 27 + threadIdx.x];
+  } else if (threadIdx.x < 27) {
     const int t = threadIdx.x;
     const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
     const int64_t lf = descend(A.T, gx + dx * A.c.res0, gy + dy * A.c.res0, gz + dz * A.c.res0, 0);
@@ -244,17 +248,18 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_
   const int NB = use_bins ? D3 * 64 : 0;
   const bool vel = A.vsum != nullptr && A.P.vel != nullptr;
   // layout: [cvs 3*W3 u64][bvs 3*NB u64][ccnt W3 int][bcnt NB int]
+  // without velocities only the counts are allocated (bin_count_smem)
   unsigned long long* cvs = smem64;
   unsigned long long* bvs = smem64 + 3 * W3;
-  int* ccnt = (int*)(smem64 + 3 * W3 + 3 * NB);
+  int* ccnt = (int*)(vel ? smem64 + 3 * W3 + 3 * NB : smem64);
   int* bcnt = ccnt + W3;
   for (int t = threadIdx.x; t < W3; t += blockDim.x) {
     ccnt[t] = 0;
-    cvs[3 * t] = cvs[3 * t + 1] = cvs[3 * t + 2] = 0ull;
+    if (vel) cvs[3 * t] = cvs[3 * t + 1] = cvs[3 * t + 2] = 0ull;
   }
   for (int t = threadIdx.x; t < NB; t += blockDim.x) {
     bcnt[t] = 0;
-    bvs[3 * t] = bvs[3 * t + 1] = bvs[3 * t + 2] = 0ull;
+    if (vel) bvs[3 * t] = bvs[3 * t + 1] = bvs[3 * t + 2] = 0ull;
   }
   __syncthreads();
   const SubcellCfg c = A.c;
@@ -363,7 +368,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
   if (mi >= nmap) return;
   const int m = (int)mi;
   const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W;
-  if (!brick_setup(A, m, nb, B, D, W)) return;
+  if (!brick_setup(A, m, nb, B, D, W, true)) return;
   int* lbase = (int*)smem64;
   int* lcnt = lbase + W3;
   __syncthreads();
@@ -409,28 +414,39 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
 
 int bin_uses_class_bins(int D) { return D <= 2 ? 1 : 0; }
 
-size_t bin_count_smem(int W, int D) {
+size_t bin_count_smem(int W, int D, bool vel) {
   const int W3 = W * W * W;
   const int NB = bin_uses_class_bins(D) ? D * D * D * 64 : 0;
-  return (size_t)(W3 + NB) * 24 + (size_t)(W3 + NB) * 4 + 8;
+  return (vel ? (size_t)(W3 + NB) * 24 : 0) + (size_t)(W3 + NB) * 4 + 8;
+}
+
+// Threads per brick workgroup.  The per-brick kernels stride every loop by blockDim, so any
+// multiple of 64 is valid: one wave when bricks hold few items (the time then goes to each
+// brick's chain of dependent loads, and one-wave workgroups keep four times as many bricks in
+// flight), four waves otherwise.
+static inline int brick_threads(int64_t items, int64_t bricks, int64_t per_brick) {
+  return items < per_brick * bricks ? 64 : 256;
 }
 
 void bin_count(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
+  const int bs = brick_threads(A.P.n, nmap, 64);
+  const size_t sm = bin_count_smem(A.W, A.c.D, A.vsum != nullptr && A.P.vel != nullptr);
   if (A.c.D == 2)
-    hipLaunchKernelGGL(k_bin_count<2>, item_grid(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A, nmap, 1);
+    hipLaunchKernelGGL(k_bin_count<2>, item_grid(nmap), dim3(bs), sm, s, A, nmap, 1);
   else
-    hipLaunchKernelGGL(k_bin_count<0>, item_grid(nmap), dim3(256), bin_count_smem(A.W, A.c.D), s, A, nmap,
+    hipLaunchKernelGGL(k_bin_count<0>, item_grid(nmap), dim3(bs), sm, s, A, nmap,
                        bin_uses_class_bins(A.c.D));
 }
 
 void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
   const size_t sm = (size_t)2 * A.W * A.W * A.W * 4 + 8;
+  const int bs = brick_threads(A.P.n, nmap, 64);
   if (A.c.D == 2)
-    hipLaunchKernelGGL(k_bin_insert<2>, item_grid(nmap), dim3(256), sm, s, A, nmap);
+    hipLaunchKernelGGL(k_bin_insert<2>, item_grid(nmap), dim3(bs), sm, s, A, nmap);
   else
-    hipLaunchKernelGGL(k_bin_insert<0>, item_grid(nmap), dim3(256), sm, s, A, nmap);
+    hipLaunchKernelGGL(k_bin_insert<0>, item_grid(nmap), dim3(bs), sm, s, A, nmap);
 }
 
 __global__ void k_sum_counts(const int* __restrict__ a, const int* __restrict__ b, int* __restrict__ out, int64_t n) {
@@ -591,8 +607,11 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
                                                       const int* __restrict__ leaf_slot,
                                                       const gvdbx_node* __restrict__ nodes0, int apron_rt,
                                                       const int* __restrict__ nbr, float* __restrict__ chan) {
-  __shared__ unsigned int d2[kMaxPaddedVox];
-  __shared__ int soff[513];
+  // LDS sized for the compile-time geometry when there is one (more bricks fit on a CU)
+  constexpr int kVox = kRes ? (kRes + 2 * kApron) * (kRes + 2 * kApron) * (kRes + 2 * kApron) : kMaxPaddedVox;
+  constexpr int kOff = kRes ? kD * kD * kD + 1 : 513;
+  __shared__ unsigned int d2[kVox];
+  __shared__ int soff[kOff];
   __shared__ int nb[27];
   // geometry: compile-time for the default 8^3 bricks / apron 1 / 2^3 subcells, else runtime
   const int res0 = kRes ? kRes : c.res0;
@@ -675,18 +694,19 @@ void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* 
                    const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, const int* nbr,
                    float* chan, hipStream_t s) {
   if (nmap <= 0) return;
+  const int bs = brick_threads(len, nmap, 256);
   const bool fixed = c.res0 == 8 && c.D == 2 && apron == 1;
   if (fixed && nbr)
-    hipLaunchKernelGGL((k_density_brick<8, 1, 2, true>), item_grid(nmap), dim3(256), 0, s, c, nmap, nsc, len, map_leaf, off,
+    hipLaunchKernelGGL((k_density_brick<8, 1, 2, true>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, off,
                        list_pos, leaf_slot, nodes0, apron, nbr, chan);
   else if (fixed)
-    hipLaunchKernelGGL((k_density_brick<8, 1, 2, false>), item_grid(nmap), dim3(256), 0, s, c, nmap, nsc, len, map_leaf, off,
+    hipLaunchKernelGGL((k_density_brick<8, 1, 2, false>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, off,
                        list_pos, leaf_slot, nodes0, apron, nbr, chan);
   else if (nbr)
-    hipLaunchKernelGGL((k_density_brick<0, 0, 0, true>), item_grid(nmap), dim3(256), 0, s, c, nmap, nsc, len, map_leaf, off,
+    hipLaunchKernelGGL((k_density_brick<0, 0, 0, true>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, off,
                        list_pos, leaf_slot, nodes0, apron, nbr, chan);
   else
-    hipLaunchKernelGGL((k_density_brick<0, 0, 0, false>), item_grid(nmap), dim3(256), 0, s, c, nmap, nsc, len, map_leaf, off,
+    hipLaunchKernelGGL((k_density_brick<0, 0, 0, false>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, off,
                        list_pos, leaf_slot, nodes0, apron, nbr, chan);
 }
 
@@ -805,7 +825,8 @@ __global__ void __launch_bounds__(256) k_apron_bound(int64_t nslots, const int* 
 
 void apron_bound(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s) {
   if (nslots <= 0 || apron <= 0 || ch.n <= 0) return;
-  hipLaunchKernelGGL(k_apron_bound, item_grid(nslots), dim3(256), 0, s, nslots, nbr, res0, apron, ch);
+  // one wave per brick once there are enough bricks to fill the chip with one-wave workgroups
+  hipLaunchKernelGGL(k_apron_bound, item_grid(nslots), dim3(nslots >= (1 << 16) ? 64 : 256), 0, s, nslots, nbr, res0, apron, ch);
 }
 
 // Per-subcell form of the velocity means, for bricks with more than 512 subcells.

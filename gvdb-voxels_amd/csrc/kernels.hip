@@ -24,24 +24,26 @@ __device__ __forceinline__ K pack_key(KeyFmt f, int lev, int xr, int yr, int zr)
   return ((K)lev << (3 * f.b)) | ((K)xr << (2 * f.b)) | ((K)yr << f.b) | (K)zr;
 }
 
-// gvdbCalcBrickId (P.cuh:352-382): per point and level < depth the covering node of q.
-// For q >= 0, floor(q / R) (the ceilf-and-adjust form of GetCoveringNode, P.cuh:88-100,
-// with R a power of two) equals trunc(q) >> log2(R).
+// gvdbCalcBrickId (P.cuh:352-382): per point and level < nlev the covering node of q (nlev =
+// depth gives every level; the full pass takes the leaf level only and derives the rest from
+// the unique leaves, k_expand_ancestors).  For q >= 0, floor(q / R) (the ceilf-and-adjust form
+// of GetCoveringNode, P.cuh:88-100, with R a power of two) equals trunc(q) >> log2(R).
 template <typename K>
-__global__ void __launch_bounds__(256) k_brick_keys(DevTree T, PointsView P, KeyFmt f, K* __restrict__ keys) {
+__global__ void __launch_bounds__(256) k_brick_keys(DevTree T, PointsView P, KeyFmt f, int nlev,
+                                                   K* __restrict__ keys) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const int d = T.depth;
   float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
   const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
-  K* out = keys + i * d;
+  K* out = keys + i * nlev;
   if (q[0] < 0 || q[1] < 0 || q[2] < 0) {
-    for (int l = 0; l < d; ++l) out[l] = pack_key<K>(f, d, 0, 0, 0);
+    for (int l = 0; l < nlev; ++l) out[l] = pack_key<K>(f, d, 0, 0, 0);
     return;
   }
   const int3 o = cloud_origin(P, i);
   const int iq[3] = {(int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z};
-  for (int l = 0; l < d; ++l) {
+  for (int l = 0; l < nlev; ++l) {
     const int sh = T.lgR[l];
     out[l] = pack_key<K>(f, l, (iq[0] >> sh) - (T.root[0] >> sh), (iq[1] >> sh) - (T.root[1] >> sh),
                          (iq[2] >> sh) - (T.root[2] >> sh));
@@ -49,8 +51,36 @@ __global__ void __launch_bounds__(256) k_brick_keys(DevTree T, PointsView P, Key
 }
 
 template <typename K>
-void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, K* keys, hipStream_t s) {
-  hipLaunchKernelGGL(k_brick_keys<K>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, keys);
+void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, int nlev, K* keys, hipStream_t s) {
+  hipLaunchKernelGGL(k_brick_keys<K>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, nlev, keys);
+}
+
+// Every unique leaf key (flags / rank of the sorted leaf keys) expands into its own key and the
+// keys of its ancestors at levels 1..depth-1, at out[rank * depth + level].  Root-relative
+// coordinates shift down level by level, since the root corner is aligned to its range:
+// (a >> s) - (root >> s) == ((a >> t) - (root >> t)) >> (s - t) for t <= s.
+template <typename K>
+__global__ void k_expand_ancestors(DevTree T, const K* __restrict__ sorted, const int* __restrict__ flags,
+                                   const int* __restrict__ rank, int64_t n, KeyFmt f, K* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flags[i]) return;
+  const int d = T.depth;
+  const K v = sorted[i];
+  const uint64_t mask = (1ull << f.b) - 1ull;
+  const int xr = (int)((v >> (2 * f.b)) & mask), yr = (int)((v >> f.b) & mask), zr = (int)(v & mask);
+  K* o = out + (int64_t)rank[i] * d;
+  for (int l = 0; l < d; ++l) {
+    const int sh = T.lgR[l] - T.lgR[0];
+    o[l] = pack_key<K>(f, l, xr >> sh, yr >> sh, zr >> sh);
+  }
+}
+
+template <typename K>
+void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const int* rank, int64_t n, KeyFmt f,
+                      K* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_expand_ancestors<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, sorted, flags, rank, n, f,
+                     out);
 }
 
 // Box enumeration of gvdbCalcExtraBrickId (P.cuh:467-512, hi = trunc(q) + 2r) and
@@ -533,8 +563,12 @@ void sample_trilinear(const DevTree& T, const int* leaf_slot, int res0, int apro
 }
 
 // explicit instantiations
-template void brick_keys<uint32_t>(const DevTree&, const PointsView&, KeyFmt, uint32_t*, hipStream_t);
-template void brick_keys<uint64_t>(const DevTree&, const PointsView&, KeyFmt, uint64_t*, hipStream_t);
+template void brick_keys<uint32_t>(const DevTree&, const PointsView&, KeyFmt, int, uint32_t*, hipStream_t);
+template void brick_keys<uint64_t>(const DevTree&, const PointsView&, KeyFmt, int, uint64_t*, hipStream_t);
+template void expand_ancestors<uint32_t>(const DevTree&, const uint32_t*, const int*, const int*, int64_t, KeyFmt,
+                                         uint32_t*, hipStream_t);
+template void expand_ancestors<uint64_t>(const DevTree&, const uint64_t*, const int*, const int*, int64_t, KeyFmt,
+                                         uint64_t*, hipStream_t);
 template void box_keys<uint32_t>(const DevTree&, const PointsView&, KeyFmt, float, float, uint32_t*, int64_t, int*,
                                  int*, hipStream_t);
 template void box_keys<uint64_t>(const DevTree&, const PointsView&, KeyFmt, float, float, uint64_t*, int64_t, int*,

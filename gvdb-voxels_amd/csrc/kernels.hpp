@@ -33,7 +33,10 @@ __device__ __forceinline__ int3 cloud_origin(const PointsView& P, int64_t i) {
 }
 
 // ---- keys
-template <typename K> void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, K* keys, hipStream_t s);
+template <typename K> void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, int nlev, K* keys, hipStream_t s);
+template <typename K>
+void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const int* rank, int64_t n, KeyFmt f, K* out,
+                      hipStream_t s);
 // Box-node keys of the extra / incremental passes: block b of 256 points writes its keys to
 // region[b * cap_b ...] and their number to blockcnt[b]; compact_regions packs the slices at
 // blockoff (exclusive scan of blockcnt).  cap_b must hold 256 * the per-point bound.
