@@ -314,9 +314,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-levelset", action="store_true")
-    ap.add_argument("--batched", action="store_true",
-                    help="also time the 256-cloud batch (off by default: a batched run faulted the GPU once and "
-                         "the cause is not identified yet)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-cloud batch against per-cloud runs")
     ap.add_argument("--kernel", default="gather", help="kernel family timed for the roofline")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the counter-calibration copy")
     args = ap.parse_args()
@@ -404,7 +402,7 @@ def main():
     if not args.no_levelset:
         out["levelset"] = levelset_bench(3)
         progress("level set done")
-    if args.batched:
+    if not args.no_batched:
         out["batched"] = batched_bench(2)
         progress("batched done")
     if not args.no_cpu:

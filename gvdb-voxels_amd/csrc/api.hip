@@ -55,10 +55,17 @@ struct gvdbx_ctx {
   // apron voxels that have a neighbour brick hold current copies of its interior (set by the
   // gathers that write the apron and by full apron passes; cleared by every other writer)
   bool nbr_apron_ok[32] = {};
+  // apron voxels that have no neighbour brick hold bound_val (set by writers that fill them:
+  // FillChannel, the density gather with its apron (0), UpdateApron; cleared by the others)
+  bool bound_ok[32] = {};
+  float bound_val[32] = {};
   Buf chan[32];
   int64_t chan_slot_cap = 0;
   int64_t nslots = 0;
   Buf leaf_slot, slot_leaf, slot_pos, nbr;
+  Buf apron_tab;                     // apron voxel table of the boundary fill
+  std::vector<uint32_t> h_apron_tab;
+  int apron_tab_res = -1, apron_tab_apron = -1;
 
   const char* pos = nullptr;
   int pos_off = 0, pos_stride = 12;
@@ -385,14 +392,34 @@ void link_all(gvdbx_ctx* c, int d) {  // per-level top-down gvdbLinkBricks launc
   for (int l = d - 1; l >= 0; --l) link_level(T, l, c->cnt[l], c->stream);
 }
 
+// full rebuilds of up to this many keys (points x levels) sort them all at once
+constexpr int64_t kOneSortKeys = (int64_t)1 << 22;
+
 template <typename K>
 int full_pass(gvdbx_ctx* c, int64_t n, const float* tr, int d) {  // ActivateBricksGPU (V.cpp:1229-1413)
   // The sorted unique keys of every level (level in the top bits) are the unique leaf keys
   // followed by their ancestors, so only the n leaf keys are sorted at full size; each unique
   // leaf then expands into its d keys, and that short list goes through the usual sort /
-  // unique / allocation.  The node set and order are those of sorting all n * d keys.
+  // unique / allocation.  The node set and order are those of sorting all n * d keys, which
+  // small inputs still do in one sort (one host read-back fewer).
   KeyFmt f = key_fmt(c, d);
   if (n > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "point count exceeds 2^31");
+  if (n * d <= kOneSortKeys) {
+    CHK(ensure(c, c->keys, (size_t)(n * d) * sizeof(K)));
+    CHK(ensure(c, c->keys_alt, (size_t)(n * d) * sizeof(K)));
+    CHK(reserve_level(c, d, 1));
+    c->cnt[d] = 1;
+    DevTree T = tree_of(c);
+    setup_root(T, c->stream);
+    fill_u64(P<uint64_t>(c->clist[d]), clist_per(c, d), kUndef64, c->stream);
+    {
+      TimedRegion trg(c, "keys");
+      brick_keys<K>(T, points_of(c, n, tr), f, d, P<K>(c->keys), c->stream);
+    }
+    CHK(sort_unique_alloc<K>(c, n * d, f, d));
+    link_all(c, d);
+    return GVDBX_OK;
+  }
   CHK(ensure(c, c->keys, (size_t)n * sizeof(K)));
   CHK(ensure(c, c->keys_alt, (size_t)n * sizeof(K)));
   CHK(reserve_level(c, d, 1));
@@ -742,7 +769,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
   if (!c) return GVDBX_ERR_ARG;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  Buf* bufs[] = {&c->leaf_slot, &c->slot_leaf, &c->slot_pos, &c->nbr, &c->sc_map, &c->sc_cnt, &c->sc_off, &c->sc_pos,
+  Buf* bufs[] = {&c->leaf_slot, &c->slot_leaf, &c->slot_pos, &c->nbr, &c->apron_tab, &c->sc_map, &c->sc_cnt, &c->sc_off, &c->sc_pos,
                  &c->sc_nid, &c->list_pos, &c->list_vel, &c->map_leaf, &c->vsum, &c->hkeys, &c->hkeys_alt,
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
                  &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta, &c->box_region, &c->box_cnt,
@@ -836,6 +863,8 @@ int gvdbx_fill_channel(gvdbx_ctx* c, int chan, float v) {  // FillChannel / Clea
   CHK(chan_ok(c, chan));
   if (c->chan_slot_cap > 0) fill_f32(P<float>(c->chan[chan]), c->chan_slot_cap * brick_floats(c), v, c->stream);
   c->nbr_apron_ok[chan] = true;
+  c->bound_ok[chan] = true;
+  c->bound_val[chan] = v;
   return launch_status(c, "FillChannel");
 }
 
@@ -1002,7 +1031,7 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   assign_slots(T, P<int>(c->flags), P<int>(c->rank), total, c->apron, br, c->axis_cx, c->axis_cy, P<int>(c->leaf_slot),
                P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
   c->nslots = c->used0;
-  for (int ch = 0; ch < 32; ++ch) c->nbr_apron_ok[ch] = false;
+  for (int ch = 0; ch < 32; ++ch) c->nbr_apron_ok[ch] = c->bound_ok[ch] = false;
   CHK(ensure(c, c->nbr, (size_t)(c->nslots * 27 + 1) * 4));
   neighbor_table(T, c->nslots, P<int4>(c->slot_pos), P<int>(c->leaf_slot), res_of(c, 0), P<int>(c->nbr), c->stream);
   CHK(reserve_channels(c, c->nslots));
@@ -1187,7 +1216,8 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
     density_brick(cfg, (int)c->used0, c->nsc, c->sclen, P<int>(c->map_leaf), P<int>(c->sc_off),
                   P<float>(c->list_pos), P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]), c->apron,
                   c->apron > 0 ? P<int>(c->nbr) : nullptr, P<float>(c->chan[chan]), c->stream);
-    c->nbr_apron_ok[chan] = c->apron > 0;
+    c->nbr_apron_ok[chan] = c->bound_ok[chan] = c->apron > 0;
+    c->bound_val[chan] = 0.0f;
   } else {
     if (mode == 1)
       levelset_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
@@ -1197,7 +1227,7 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
       gather_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
                    P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), accum, mode, 0.0f,
                    c->stream);
-    c->nbr_apron_ok[chan] = false;
+    c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;
   }
   return launch_status(c, "Gather");
 }
@@ -1239,6 +1269,7 @@ int gvdbx_gather_velocity(gvdbx_ctx* c, int cx, int cy, int cz, int accum) {
                      P<float>(c->chan[cy]), P<float>(c->chan[cz]), accum, c->stream);
   }
   c->nbr_apron_ok[cx] = c->nbr_apron_ok[cy] = c->nbr_apron_ok[cz] = apron_done;
+  c->bound_ok[cx] = c->bound_ok[cy] = c->bound_ok[cz] = false;
   return launch_status(c, "GatherVelocity");
 }
 
@@ -1251,8 +1282,22 @@ int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {
   // segment of its subcell, starting from init
   levelset_field(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
                  P<float>(c->list_pos), P<int>(c->leaf_slot), c->apron, P<float>(c->chan[chan]), 0, init, c->stream);
-  c->nbr_apron_ok[chan] = false;
+  c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;
   return launch_status(c, "ScatterReduceLevelSet");
+}
+
+// The boundary fill's apron voxel table for the context's brick geometry, uploaded once.
+static int apron_table_current(gvdbx_ctx* c) {
+  const int res0 = res_of(c, 0);
+  if (c->apron_tab_res == res0 && c->apron_tab_apron == c->apron) return GVDBX_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // an earlier upload may still read the host table
+  apron_table(res0, c->apron, c->h_apron_tab);
+  CHK(ensure(c, c->apron_tab, c->h_apron_tab.size() * 4 + 4));
+  HIPCHK(c, hipMemcpyAsync(c->apron_tab.p, c->h_apron_tab.data(), c->h_apron_tab.size() * 4, hipMemcpyHostToDevice,
+                           c->stream));
+  c->apron_tab_res = res0;
+  c->apron_tab_apron = c->apron;
+  return GVDBX_OK;
 }
 
 int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* boundvals, int nchan) {
@@ -1260,6 +1305,9 @@ int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* bou
   for (int i = 0; i < nchan; ++i) CHK(chan_ok(c, chans[i]));
   CHK(atlas_current(c));
   if (c->apron == 0 || c->nslots == 0) return GVDBX_OK;
+  CHK(apron_table_current(c));
+  const uint32_t* tab = P<uint32_t>(c->apron_tab);
+  const int ntab = (int)c->h_apron_tab.size();
   TimedRegion tr(c, "apron");
   // channels whose neighbour copies are current only need the boundary value; the others
   // take the full neighbour pull (UpdateApron<T>, O.cuh:72-126)
@@ -1268,20 +1316,26 @@ int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* bou
     ac.n = 0;
     for (int i = 0; i < nchan; ++i) {
       if (c->nbr_apron_ok[chans[i]] != (pass == 0)) continue;
+      const float bv = boundvals ? boundvals[i] : 0.0f;
+      if (pass == 0 && c->bound_ok[chans[i]] && __builtin_memcmp(&bv, &c->bound_val[chans[i]], 4) == 0)
+        continue;  // neighbour copies and boundary value both current
       ac.ptr[ac.n] = P<float>(c->chan[chans[i]]);
       ac.bound[ac.n] = boundvals ? boundvals[i] : 0.0f;
       if (++ac.n == kMaxApronChannels || i == nchan - 1) {
-        if (pass == 0) apron_bound(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
+        if (pass == 0) apron_bound(c->nslots, P<int>(c->nbr), tab, ntab, res_of(c, 0), c->apron, ac, c->stream);
         else apron_multi(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
         ac.n = 0;
       }
     }
     if (ac.n) {
-      if (pass == 0) apron_bound(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
+      if (pass == 0) apron_bound(c->nslots, P<int>(c->nbr), tab, ntab, res_of(c, 0), c->apron, ac, c->stream);
       else apron_multi(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
     }
   }
-  for (int i = 0; i < nchan; ++i) c->nbr_apron_ok[chans[i]] = true;
+  for (int i = 0; i < nchan; ++i) {
+    c->nbr_apron_ok[chans[i]] = c->bound_ok[chans[i]] = true;
+    c->bound_val[chans[i]] = boundvals ? boundvals[i] : 0.0f;
+  }
   return launch_status(c, "UpdateApron");
 }
 
@@ -1346,7 +1400,7 @@ int splat_impl(gvdbx_ctx* c, int chan, const float* q, const float* mass, int64_
   c->sp_order_q = n > 0 ? q : nullptr;
   c->sp_order_n = n;
   c->sp_order = n > 0 ? P<uint32_t>(c->sp_perm) : nullptr;
-  c->nbr_apron_ok[chan] = false;
+  c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;
   return launch_status(c, "SplatTrilinear");
 }
 
@@ -1439,7 +1493,7 @@ int gvdbx_copy_channel(gvdbx_ctx* c, int chan, float* out, int64_t max_floats) {
 
 int gvdbx_get_channel_ptr(gvdbx_ctx* c, int chan, void** p, int64_t* floats) {
   CHK(chan_ok(c, chan));
-  c->nbr_apron_ok[chan] = false;  // the caller may write through the pointer
+  c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;  // the caller may write through the pointer
   if (p) *p = c->chan[chan].p;
   if (floats) *floats = c->nslots * brick_floats(c);
   return GVDBX_OK;

@@ -414,9 +414,10 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
 
 int bin_uses_class_bins(int D) { return D <= 2 ? 1 : 0; }
 
+// D = 0: no class bins
 size_t bin_count_smem(int W, int D, bool vel) {
   const int W3 = W * W * W;
-  const int NB = bin_uses_class_bins(D) ? D * D * D * 64 : 0;
+  const int NB = D > 0 && bin_uses_class_bins(D) ? D * D * D * 64 : 0;
   return (vel ? (size_t)(W3 + NB) * 24 : 0) + (size_t)(W3 + NB) * 4 + 8;
 }
 
@@ -431,12 +432,13 @@ static inline int brick_threads(int64_t items, int64_t bricks, int64_t per_brick
 void bin_count(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
   const int bs = brick_threads(A.P.n, nmap, 64);
-  const size_t sm = bin_count_smem(A.W, A.c.D, A.vsum != nullptr && A.P.vel != nullptr);
+  // class bins pay off when many points share a brick; sparse bricks count pairs directly
+  const int bins = bs == 256 ? bin_uses_class_bins(A.c.D) : 0;
+  const size_t sm = bin_count_smem(A.W, bins ? A.c.D : 0, A.vsum != nullptr && A.P.vel != nullptr);
   if (A.c.D == 2)
-    hipLaunchKernelGGL(k_bin_count<2>, item_grid(nmap), dim3(bs), sm, s, A, nmap, 1);
+    hipLaunchKernelGGL(k_bin_count<2>, item_grid(nmap), dim3(bs), sm, s, A, nmap, bins);
   else
-    hipLaunchKernelGGL(k_bin_count<0>, item_grid(nmap), dim3(bs), sm, s, A, nmap,
-                       bin_uses_class_bins(A.c.D));
+    hipLaunchKernelGGL(k_bin_count<0>, item_grid(nmap), dim3(bs), sm, s, A, nmap, bins);
 }
 
 void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
@@ -637,15 +639,24 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
   // Voxels v with |p - v| < r lie in the integer cube [floor(p) - R + 1, floor(p) + R], R = ceil(r);
   // voxels at distance >= r cannot raise max(0, r - |p - v|), so the cube is exact.
   const int R = (int)ceilf(c.r);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  for (int s = wave; s < D3; s += nw) {  // one wave per subcell: its list entries over the lanes
+  // The brick's subcell lists are contiguous, [soff[0], soff[D3]): the lanes take its entries in
+  // order and each finds its subcell by binary search over the offsets (sparse bricks fill the
+  // lanes across subcells instead of running one short list per wave).
+  for (int j = soff[0] + (int)threadIdx.x; j < soff[D3]; j += blockDim.x) {
+    int s = 0;
+    {
+      int hi = D3;  // soff[s] <= j < soff[hi]
+      while (hi - s > 1) {
+        const int mid = (s + hi) >> 1;
+        if (soff[mid] <= j) s = mid; else hi = mid;
+      }
+    }
     const int ix = s % D, iy = (s / D) % D, iz = s / (D * D);
     // voxel range of the subcell, extended into the apron on the brick's boundary
     const int lx0 = ix * sc - (ix == 0 ? a : 0), lx1 = ix * sc + sc - 1 + (ix == D - 1 ? a : 0);
     const int ly0 = iy * sc - (iy == 0 ? a : 0), ly1 = iy * sc + sc - 1 + (iy == D - 1 ? a : 0);
     const int lz0 = iz * sc - (iz == 0 ? a : 0), lz1 = iz * sc + sc - 1 + (iz == D - 1 ? a : 0);
-    const int je = soff[s + 1];
-    for (int j = soff[s] + lane; j < je; j += 64) {
+    {
       const float gx = list_pos[3 * (int64_t)j], gy = list_pos[3 * (int64_t)j + 1], gz = list_pos[3 * (int64_t)j + 2];
       const int fx = (int)gx - bx, fy = (int)gy - by, fz = (int)gz - bz;
       if (R == 1) {
@@ -679,13 +690,14 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
   for (int t = threadIdx.x; t < nv; t += blockDim.x) {
     const int zy = t / bw;
     const int x = t - zy * bw - a, z = zy / bw - a, y = zy - (zy / bw) * bw - a;
+    bool missing = false;  // apron voxel without a neighbour brick: boundary value 0
     if (kWithApron) {
       const int ox = x < 0 ? -1 : (x >= res0 ? 1 : 0), oy = y < 0 ? -1 : (y >= res0 ? 1 : 0),
                 oz = z < 0 ? -1 : (z >= res0 ? 1 : 0);
-      if ((ox | oy | oz) != 0 && nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] < 0) continue;
+      missing = (ox | oy | oz) != 0 && nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] < 0;
     }
     const unsigned int u = d2[t];
-    const float val = u == 0x7F800000u ? 0.0f : fmaxf(0.0f, c.r - sqrtf(__uint_as_float(u)));
+    const float val = (missing || u == 0x7F800000u) ? 0.0f : fmaxf(0.0f, c.r - sqrtf(__uint_as_float(u)));
     bptr[((z + apron) * br + (y + apron)) * br + (x + apron)] = val;
   }
 }
@@ -796,37 +808,47 @@ void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt,
 }
 
 // Boundary value into the apron voxels that have no neighbour brick (the part of
-// UpdateApron left once the neighbour copies are current).
-__global__ void __launch_bounds__(256) k_apron_bound(int64_t nslots, const int* __restrict__ nbr, int res0, int apron,
-                                                    ApronChannels ch) {
+// UpdateApron left once the neighbour copies are current).  One wave per brick walks only the
+// apron voxels, from a table of (voxel index | neighbour direction << 27) entries.
+__global__ void __launch_bounds__(64) k_apron_bound(int64_t nslots, const int* __restrict__ nbr,
+                                                   const uint32_t* __restrict__ tab, int ntab, int bv,
+                                                   ApronChannels ch) {
   __shared__ int nb[27];
-  __shared__ int any;
   const int64_t s = item_index();
   if (s >= nslots) return;
-  if (threadIdx.x == 0) any = 0;
+  const int lane = threadIdx.x;
+  const int v = lane < 27 ? nbr[s * 27 + lane] : 0;
+  if (lane < 27) nb[lane] = v;
+  if (__ballot(lane < 27 && v < 0) == 0) return;  // every neighbour exists (wave-uniform)
   __syncthreads();
-  if (threadIdx.x < 27) {
-    nb[threadIdx.x] = nbr[s * 27 + threadIdx.x];
-    if (nb[threadIdx.x] < 0) atomicOr(&any, 1);
-  }
-  __syncthreads();
-  if (!any) return;
-  const int br = res0 + 2 * apron;
-  const int nvox = br * br * br;
-  const int64_t bv = (int64_t)nvox;
-  for (int t = threadIdx.x; t < nvox; t += blockDim.x) {
-    const int x = t % br - apron, y = (t / br) % br - apron, z = t / (br * br) - apron;
-    const int ox = x < 0 ? -1 : (x >= res0 ? 1 : 0), oy = y < 0 ? -1 : (y >= res0 ? 1 : 0),
-              oz = z < 0 ? -1 : (z >= res0 ? 1 : 0);
-    if ((ox | oy | oz) == 0 || nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] >= 0) continue;
-    for (int c = 0; c < ch.n; ++c) ch.ptr[c][s * bv + t] = ch.bound[c];
+  for (int u = lane; u < ntab; u += 64) {
+    const uint32_t e = tab[u];
+    if (nb[e >> 27] >= 0) continue;
+    const int64_t dst = s * (int64_t)bv + (int64_t)(e & 0x7FFFFFFu);
+    for (int c = 0; c < ch.n; ++c) ch.ptr[c][dst] = ch.bound[c];
   }
 }
 
-void apron_bound(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s) {
+void apron_table(int res0, int apron, std::vector<uint32_t>& tab) {
+  const int br = res0 + 2 * apron;
+  tab.clear();
+  for (int z = 0; z < br; ++z)
+    for (int y = 0; y < br; ++y)
+      for (int x = 0; x < br; ++x) {
+        const int gx = x - apron, gy = y - apron, gz = z - apron;
+        const int ox = gx < 0 ? -1 : (gx >= res0 ? 1 : 0), oy = gy < 0 ? -1 : (gy >= res0 ? 1 : 0),
+                  oz = gz < 0 ? -1 : (gz >= res0 ? 1 : 0);
+        if ((ox | oy | oz) == 0) continue;
+        const uint32_t k = (uint32_t)((oz + 1) * 9 + (oy + 1) * 3 + (ox + 1));
+        tab.push_back((uint32_t)((z * br + y) * br + x) | (k << 27));
+      }
+}
+
+void apron_bound(int64_t nslots, const int* nbr, const uint32_t* tab, int ntab, int res0, int apron,
+                 const ApronChannels& ch, hipStream_t s) {
   if (nslots <= 0 || apron <= 0 || ch.n <= 0) return;
-  // one wave per brick once there are enough bricks to fill the chip with one-wave workgroups
-  hipLaunchKernelGGL(k_apron_bound, item_grid(nslots), dim3(nslots >= (1 << 16) ? 64 : 256), 0, s, nslots, nbr, res0, apron, ch);
+  const int br = res0 + 2 * apron;
+  hipLaunchKernelGGL(k_apron_bound, item_grid(nslots), dim3(64), 0, s, nslots, nbr, tab, ntab, br * br * br, ch);
 }
 
 // Per-subcell form of the velocity means, for bricks with more than 512 subcells.

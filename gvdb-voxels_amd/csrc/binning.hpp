@@ -1,5 +1,6 @@
 // binning.hpp -- launchers of the home-brick subcell binning and its gathers (binning.hip).
 #pragma once
+#include <vector>
 #include "kernels.hpp"
 
 namespace gvdbx {
@@ -71,6 +72,10 @@ struct ApronChannels {
 void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int* nbr,
                     hipStream_t s);
 void apron_multi(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s);
-void apron_bound(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s);
+// Apron voxels of a (res0 + 2 apron)^3 brick as (voxel index | neighbour direction << 27); the
+// boundary fill reads it from device memory.
+void apron_table(int res0, int apron, std::vector<uint32_t>& tab);
+void apron_bound(int64_t nslots, const int* nbr, const uint32_t* tab, int ntab, int res0, int apron,
+                 const ApronChannels& ch, hipStream_t s);
 
 }  // namespace gvdbx
