@@ -219,7 +219,7 @@ def levelset_bench(reps):
             "queries_Mper_s": round(len(qs) / (best["sample"] * 1e-3) / 1e6, 1)}
 
 
-def batched_bench(reps):
+def batched_bench(reps, tune=True, per_cloud=True):
     """The batched workload: 256 Gaussian-blob clouds with log-uniform sizes in
     [4,096, 1,048,576] (8 blobs each, sigma U(4, 32), centres U(96, 416)^3, seed 5), each with
     its own grid.  Brick and subcell size come from the autotuner; then one step (full
@@ -232,7 +232,10 @@ def batched_bench(reps):
     allp = np.concatenate(cs)
     n = len(allp)
     mc = autotune.max_valid_coord(cs)
-    tuned = autotune.tune(lambda v: v.to_device(allp), counts, mc, radius=RADIUS, reps=2, log=progress)
+    if tune:
+        tuned = autotune.tune(lambda v: v.to_device(allp), counts, mc, radius=RADIUS, reps=2, log=progress)
+    else:  # the configuration the autotuner picks for this batch
+        tuned = {"leaf_log2": 3, "subcell": 4, "cell_log2": autotune.cell_for(3, mc, RADIUS), "fixed": True}
     leaf, sc = tuned["leaf_log2"], tuned["subcell"]
 
     v = autotune.new_batch_volume(leaf)
@@ -249,6 +252,9 @@ def batched_bench(reps):
     bricks = v.getNumUsedNodes(0)
     v.close()
     del d
+    if not per_cloud:
+        return {"workload": "batched_256", "points": n, "bricks": int(bricks), "autotune": tuned,
+                "batched_ms": round(best_b, 3), "batched_Mpoints_s": round(n / (best_b * 1e-3) / 1e6, 1)}
 
     vols, devs = [], []
     for c in cs:
@@ -315,11 +321,17 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-levelset", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-cloud batch against per-cloud runs")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only time the 256-cloud batch (8^3 bricks, subcell 4; no autotune, no per-cloud runs), "
+                         "for kernel profiles")
     ap.add_argument("--kernel", default="gather", help="kernel family timed for the roofline")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the counter-calibration copy")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.batched_only:
+        print(json.dumps({"batched": batched_bench(args.steps, tune=False, per_cloud=False)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "

@@ -99,6 +99,7 @@ struct gvdbx_ctx {
   Buf keys, keys_alt, flags, rank, radix_counts, tile_buf, markers, meta;
   Buf box_region, box_cnt, box_off;  // per-workgroup slices of the box-pass keys
   int* h_meta = nullptr;
+  hipEvent_t meta_ev = nullptr;  // marks an early meta read-back (read_meta_begin / _end)
 
   int64_t topo_version = 0, atlas_version = -1;
 
@@ -236,6 +237,20 @@ int read_meta(gvdbx_ctx* c) {
   return GVDBX_OK;
 }
 
+// Read-back split in two: the copy is enqueued where the values are final, more work can be
+// enqueued behind it, and the host waits only for the copy.
+int read_meta_begin(gvdbx_ctx* c) {
+  CHK(launch_status(c, "read-back"));
+  HIPCHK(c, hipMemcpyAsync(c->h_meta, c->meta.p, M_INTS * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipEventRecord(c->meta_ev, c->stream));
+  return GVDBX_OK;
+}
+
+int read_meta_end(gvdbx_ctx* c) {
+  HIPCHK(c, hipEventSynchronize(c->meta_ev));
+  return GVDBX_OK;
+}
+
 int zero_meta(gvdbx_ctx* c) {
   fill_u32(P<uint32_t>(c->meta), M_INTS, 0u, c->stream);
   return GVDBX_OK;
@@ -349,21 +364,30 @@ int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d) {
   CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(ncnt > m ? ncnt : m) + 16) * 4));
   CHK(ensure(c, c->flags, (size_t)m * 4));
   CHK(ensure(c, c->rank, (size_t)m * 4));
-  int par;
-  {
+  const K* sorted;
+  if (m <= kSmallSortKeys) {  // one workgroup: sort + unique + ranks + level counts
     TimedRegion tr(c, "sort");
-    if (sizeof(K) == 4)
-      par = radix_sort_u32(P<uint32_t>(c->keys), P<uint32_t>(c->keys_alt), nullptr, nullptr, m, 0, f.nbits,
-                           P<int>(c->radix_counts), P<int>(c->tile_buf), c->stream);
-    else
-      par = radix_sort_u64(P<uint64_t>(c->keys), P<uint64_t>(c->keys_alt), nullptr, nullptr, m, 0, f.nbits,
-                           P<int>(c->radix_counts), P<int>(c->tile_buf), c->stream);
+    sort_unique_small<K>(P<K>(c->keys), m, f, d, P<K>(c->keys_alt), P<int>(c->flags), P<int>(c->rank),
+                         P<int>(c->meta) + M_UNIQUE, P<int>(c->meta) + M_LEV, c->stream);
+    sorted = P<K>(c->keys_alt);
+  } else {
+    int par;
+    {
+      TimedRegion tr(c, "sort");
+      if (sizeof(K) == 4)
+        par = radix_sort_u32(P<uint32_t>(c->keys), P<uint32_t>(c->keys_alt), nullptr, nullptr, m, 0, f.nbits,
+                             P<int>(c->radix_counts), P<int>(c->tile_buf), c->stream);
+      else
+        par = radix_sort_u64(P<uint64_t>(c->keys), P<uint64_t>(c->keys_alt), nullptr, nullptr, m, 0, f.nbits,
+                             P<int>(c->radix_counts), P<int>(c->tile_buf), c->stream);
+    }
+    sorted = par ? P<K>(c->keys_alt) : P<K>(c->keys);
+    CHK(zero_meta(c));
+    unique_flags<K>(sorted, m, f, d, P<int>(c->flags), c->stream);
+    exclusive_scan(P<int>(c->flags), P<int>(c->rank), m, P<int>(c->tile_buf), P<int>(c->meta) + M_UNIQUE,
+                   c->stream);
+    level_hist<K>(sorted, P<int>(c->flags), m, f, P<int>(c->meta) + M_LEV, c->stream);
   }
-  const K* sorted = par ? P<K>(c->keys_alt) : P<K>(c->keys);
-  CHK(zero_meta(c));
-  unique_flags<K>(sorted, m, f, d, P<int>(c->flags), c->stream);
-  exclusive_scan(P<int>(c->flags), P<int>(c->rank), m, P<int>(c->tile_buf), P<int>(c->meta) + M_UNIQUE, c->stream);
-  level_hist<K>(sorted, P<int>(c->flags), m, f, P<int>(c->meta) + M_LEV, c->stream);
   CHK(read_meta(c));
   AllocPlan plan;
   memset(&plan, 0, sizeof(plan));
@@ -757,7 +781,8 @@ int gvdbx_ctx_create(int device, void* hip_stream, gvdbx_ctx** out) {
     c->own_stream = true;
   }
   if (ensure(c, c->meta, M_INTS * sizeof(int)) != GVDBX_OK ||
-      hipHostMalloc((void**)&c->h_meta, M_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_meta, M_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->meta_ev, hipEventDisableTiming) != hipSuccess) {
     gvdbx_ctx_destroy(c);
     return GVDBX_ERR_HIP;
   }
@@ -784,6 +809,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
   for (int ch = 0; ch < 32; ++ch)
     if (c->chan[ch].p) (void)hipFree(c->chan[ch].p);
   if (c->h_meta) (void)hipHostFree(c->h_meta);
+  if (c->meta_ev) (void)hipEventDestroy(c->meta_ev);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1184,6 +1210,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   }
   A.list_pos = P<float>(c->list_pos);
   A.list_vel = c->lists_have_vel ? P<float>(c->list_vel) : nullptr;
+  if (early) CHK(read_meta_begin(c));  // the length is final after the scan
   fill_u32(P<uint32_t>(c->ocur), nsc, 0u, c->stream);
   {
     TimedRegion tr(c, "insert");
@@ -1191,7 +1218,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
     bin_points_insert(A, tail, c->stream);
   }
   if (early) {
-    CHK(read_meta(c));
+    CHK(read_meta_end(c));  // returns while the insert kernels still run
     ln = (uint32_t)c->h_meta[M_TOTAL];
   }
   if (ln > INT32_MAX || ln > bound) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);

@@ -189,6 +189,88 @@ void unique_flags(const K* k, int64_t m, KeyFmt f, int depth, int* flags, hipStr
   hipLaunchKernelGGL(k_unique_flags<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, k, m, f, depth, flags);
 }
 
+// Sort, unique flags, ranks and per-level counts of up to kSmallSortKeys keys in one
+// workgroup (bitonic sort in LDS): the same outputs as radix sort + k_unique_flags + scan +
+// k_level_hist, in one launch instead of about a dozen for the few keys an incremental step
+// adds.  Writes meta_unique = number of unique keys and meta_lev[l] = unique keys of level l.
+template <typename K>
+__global__ void __launch_bounds__(1024) k_sort_unique_small(const K* __restrict__ in, int m, KeyFmt f, int depth,
+                                                            K* __restrict__ out, int* __restrict__ flags,
+                                                            int* __restrict__ rank, int* __restrict__ meta_unique,
+                                                            int* __restrict__ meta_lev) {
+  __shared__ K s[kSmallSortKeys];
+  __shared__ int lev_h[kMaxLev];
+  __shared__ int ws[16];
+  const int t = threadIdx.x;
+  int np = 2;
+  while (np < m) np <<= 1;
+  for (int i = t; i < np; i += 1024) s[i] = i < m ? in[i] : ~(K)0;
+  if (t < kMaxLev) lev_h[t] = 0;
+  __syncthreads();
+  for (int k = 2; k <= np; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < np; i += 1024) {
+        const int p = i ^ j;
+        if (p > i) {
+          const K a = s[i], b = s[p];
+          if ((a > b) == ((i & k) == 0)) {
+            s[i] = b;
+            s[p] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  // thread t owns keys [4t, 4t + 4)
+  int fl[4], sum = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = 4 * t + q;
+    fl[q] = 0;
+    if (i < m) {
+      const K v = s[i];
+      const int lev = (int)(v >> (3 * f.b));
+      fl[q] = ((i == 0 || s[i - 1] != v) && lev < depth) ? 1 : 0;
+      if (fl[q]) atomicAdd(&lev_h[lev], 1);
+      out[i] = v;
+      flags[i] = fl[q];
+    }
+    sum += fl[q];
+  }
+  // exclusive scan of the per-thread sums over 16 waves
+  const int lane = t & 63, wave = t >> 6;
+  int x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[wave] = x;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 16; ++w) {
+    pre += w < wave ? ws[w] : 0;
+    tot += ws[w];
+  }
+  pre += x - sum;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = 4 * t + q;
+    if (i < m) rank[i] = pre;
+    pre += fl[q];
+  }
+  if (t == 0) *meta_unique = tot;
+  if (t < kMaxLev) meta_lev[t] = lev_h[t];
+}
+
+template <typename K>
+void sort_unique_small(const K* in, int64_t m, KeyFmt f, int depth, K* out, int* flags, int* rank, int* meta_unique,
+                       int* meta_lev, hipStream_t s) {
+  hipLaunchKernelGGL(k_sort_unique_small<K>, dim3(1), dim3(1024), 0, s, in, (int)m, f, depth, out, flags, rank,
+                     meta_unique, meta_lev);
+}
+
 // Per-level unique counts (the levCnt atomics of gvdbFindUnique), LDS histogram per block.
 template <typename K>
 __global__ void __launch_bounds__(256) k_level_hist(const K* __restrict__ k, const int* __restrict__ flags, int64_t m,
@@ -577,6 +659,10 @@ template void compact_regions<uint32_t>(const uint32_t*, int64_t, const int*, co
                                         hipStream_t);
 template void compact_regions<uint64_t>(const uint64_t*, int64_t, const int*, const int*, int64_t, uint64_t*,
                                         hipStream_t);
+template void sort_unique_small<uint32_t>(const uint32_t*, int64_t, KeyFmt, int, uint32_t*, int*, int*, int*, int*,
+                                          hipStream_t);
+template void sort_unique_small<uint64_t>(const uint64_t*, int64_t, KeyFmt, int, uint64_t*, int*, int*, int*, int*,
+                                          hipStream_t);
 template void unique_flags<uint32_t>(const uint32_t*, int64_t, KeyFmt, int, int*, hipStream_t);
 template void unique_flags<uint64_t>(const uint64_t*, int64_t, KeyFmt, int, int*, hipStream_t);
 template void level_hist<uint32_t>(const uint32_t*, const int*, int64_t, KeyFmt, int*, hipStream_t);

@@ -34,6 +34,11 @@ __device__ __forceinline__ int3 cloud_origin(const PointsView& P, int64_t i) {
 
 // ---- keys
 template <typename K> void brick_keys(const DevTree& T, const PointsView& P, KeyFmt f, int nlev, K* keys, hipStream_t s);
+// up to this many topology keys are sorted and uniqued by one workgroup
+constexpr int kSmallSortKeys = 4096;
+template <typename K>
+void sort_unique_small(const K* in, int64_t m, KeyFmt f, int depth, K* out, int* flags, int* rank, int* meta_unique,
+                       int* meta_lev, hipStream_t s);
 template <typename K>
 void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const int* rank, int64_t n, KeyFmt f, K* out,
                       hipStream_t s);
