@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cmath>
 #include <string>
 #include <utility>
 #include <vector>
@@ -90,6 +91,7 @@ struct gvdbx_ctx {
   Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
   Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb, nbm, spos, svel;
   Buf sp_cnt, sp_start, sp_keyrank, sp_payload, sp_perm;  // splat: points counting-sorted by (brick, region)
+  Buf scatter_acc;  // ScatterDensity: 64-bit fixed-point sums per padded voxel
   // brick order of the last splat (a permutation of its n points), reused by the splat
   // backward on the same point buffer for locality
   const void* sp_order_q = nullptr;
@@ -798,7 +800,8 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
                  &c->sc_nid, &c->list_pos, &c->list_vel, &c->map_leaf, &c->vsum, &c->hkeys, &c->hkeys_alt,
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
                  &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta, &c->box_region, &c->box_cnt,
-                 &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->cloud_id,
+                 &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->scatter_acc,
+                 &c->cloud_id,
                  &c->cloud_off_dev};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
@@ -1463,6 +1466,38 @@ int gvdbx_sample_trilinear_backward(gvdbx_ctx* c, int chan, const void* q, const
                     (const float*)q, (const float*)dout, n, (float*)dq, c->stream);
   if (chan_grad >= 0) CHK(splat_impl(c, chan_grad, (const float*)q, (const float*)dout, n, accum_grad));
   return launch_status(c, "SampleTrilinearBackward");
+}
+
+int gvdbx_scatter_density(gvdbx_ctx* c, int64_t n, float radius, float amp, const float trans[3], int expand,
+                          int chan) {
+  (void)amp;  // unused by the reference kernel as well (P.cuh:525-573)
+  CHK(chan_ok(c, chan));  // ScatterDensity (V.cpp:5625-5659)
+  if (c->batch) return fail(c, GVDBX_ERR_STATE, "ScatterDensity takes world points: not for batched clouds");
+  CHK(check_points(c, n));
+  if (!(radius > 0.0f) || !(radius < 1.0e30f)) return fail(c, GVDBX_ERR_ARG, "radius %g out of range", (double)radius);
+  CHK(atlas_current(c));
+  if (c->depth < 0) return fail(c, GVDBX_ERR_STATE, "no topology");
+  if (c->nslots == 0 || n == 0) return GVDBX_OK;
+  // Exact sums need every voxel's total within 2^62 at the scale 2^shift: a voxel takes at most
+  // one contribution per point, each |w| <= max(1, (cmax - 1)^3) with cmax = (expand ? 6 : 3)/r^2
+  // (|v - p'|^2 < 3 for the point's voxel, < 6 for a face neighbour).  A bound above 2^40 would
+  // leave less than 2^-22 of resolution: refused.
+  const double cmax = (expand ? 6.0 : 3.0) / ((double)radius * (double)radius);
+  const double wmax = cmax > 2.0 ? (cmax - 1.0) * (cmax - 1.0) * (cmax - 1.0) : 1.0;
+  const double bound = (double)n * wmax * 1.01 + 1.0;
+  if (bound > std::ldexp(1.0, 40))
+    return fail(c, GVDBX_ERR_CAPACITY, "ScatterDensity: %lld points at radius %g exceed the exact-sum range",
+                (long long)n, (double)radius);
+  int shift = 62 - (int)std::ceil(std::log2(bound));
+  if (shift > 48) shift = 48;
+  const int br = res_of(c, 0) + 2 * c->apron;
+  const int64_t nvox = c->nslots * (int64_t)br * br * br;
+  CHK(ensure(c, c->scatter_acc, (size_t)nvox * 8));
+  TimedRegion tr(c, "scatter_density");
+  scatter_density(tree_of(c), points_of(c, n, trans), P<int>(c->leaf_slot), c->ld[0], c->apron, radius, expand,
+                  shift, c->nslots, P<unsigned long long>(c->scatter_acc), P<float>(c->chan[chan]), c->stream);
+  c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;
+  return launch_status(c, "ScatterDensity");
 }
 
 int gvdbx_get_depth(const gvdbx_ctx* c, int* depth, int32_t rootpos[3], float posmin[3], float posmax[3]) {

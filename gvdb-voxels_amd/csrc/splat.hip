@@ -26,6 +26,8 @@
 // sampler); dL/dfield = splat of dout (the forward above).
 #include "splat.hpp"
 
+#include "primitives.hpp"
+
 namespace gvdbx {
 
 namespace {
@@ -339,6 +341,81 @@ void sample_backward(const DevTree& T, const int* leaf_slot, int res0, int apron
   if (n <= 0) return;
   hipLaunchKernelGGL(k_sample_backward, dim3(nblocks(n, 256)), dim3(256), 0, s, T, leaf_slot, res0, apron, chan, q,
                      dout, n, dq);
+}
+
+// ------------------------------------------------------------------ ScatterDensity
+// gvdbScatterPointDensity (P.cuh:516-573) with the leaf of q = p + trans found as
+// gvdbInsertPoints does (P.cuh:19-36): for a valid point in a leaf with corner b, p' = q - b,
+// pi = trunc(p'); the voxel pi, and with expand its 6 face neighbours (unclamped: they may lie
+// in the brick's apron; dropped without an apron), gains (1 - c)^3 = 1 + c(-3 + c(3 - c)),
+// c = |v - p'|^2 / r^2 (P.cuh:514-521).  The reference's read-modify-write races; here every
+// contribution is summed, as a 64-bit fixed-point integer (scale 2^shift) per padded voxel, so
+// the sum does not depend on the order of the atomics.
+__device__ __forceinline__ float scatter_weight(float px, float py, float pz, float vx, float vy, float vz, float r) {
+  const float bx = vx - px, by = vy - py, bz = vz - pz;
+  const float c = (bx * bx + by * by + bz * bz) / (r * r);
+  return 1.0f + c * (-3.0f + c * (3.0f - c));
+}
+
+__global__ void __launch_bounds__(256) k_scatter_density(DevTree T, PointsView P, const int* __restrict__ leaf_slot,
+                                                         int lgres, int apron, float r, int expand, double scale,
+                                                         unsigned long long* __restrict__ acc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+  const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
+  // invalid (negative), and far beyond any root (NOHIT = 1e10): no leaf
+  if (!(q[0] >= 0.f && q[1] >= 0.f && q[2] >= 0.f && q[0] < 1073741824.f && q[1] < 1073741824.f &&
+        q[2] < 1073741824.f))
+    return;
+  const int iq[3] = {(int)q[0], (int)q[1], (int)q[2]};
+  const int64_t lf = descend(T, iq[0], iq[1], iq[2], 0);
+  if (lf < 0) return;
+  const int slot = leaf_slot[lf];
+  if (slot < 0) return;
+  const int res = 1 << lgres, br = res + 2 * apron;
+  float pl[3];
+  int pi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    pl[a] = q[a] - (float)(iq[a] & ~(res - 1));  // brick corner: nodes are aligned to their range
+    pi[a] = (int)pl[a];
+  }
+  unsigned long long* dst = acc + (int64_t)slot * br * br * br;
+  const int nk = expand ? 7 : 1;
+  for (int k = 0; k < nk; ++k) {
+    const int ax = k == 0 ? -1 : (k - 1) >> 1, d = (k & 1) ? -1 : 1;
+    int v[3] = {pi[0], pi[1], pi[2]};
+    if (ax >= 0) v[ax] += d;
+    if (v[0] < -apron || v[1] < -apron || v[2] < -apron || v[0] >= res + apron || v[1] >= res + apron ||
+        v[2] >= res + apron)
+      continue;
+    const float w = scatter_weight(pl[0], pl[1], pl[2], (float)v[0], (float)v[1], (float)v[2], r);
+    const long long fx = __double2ll_rn((double)w * scale);
+    atomicAdd(&dst[((v[2] + apron) * br + (v[1] + apron)) * br + (v[0] + apron)], (unsigned long long)fx);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_scatter_apply(const unsigned long long* __restrict__ acc, int64_t n,
+                                                       double inv_scale, float* __restrict__ chan) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const long long v = (long long)acc[i];
+    if (v != 0) chan[i] += (float)((double)v * inv_scale);
+  }
+}
+
+void scatter_density(const DevTree& T, const PointsView& P, const int* leaf_slot, int lgres, int apron, float r,
+                     int expand, int shift, int64_t nslots, unsigned long long* acc, float* chan, hipStream_t s) {
+  const int br = (1 << lgres) + 2 * apron;
+  const int64_t nvox = nslots * br * br * br;
+  if (nvox <= 0) return;
+  fill_u64((uint64_t*)acc, nvox, 0ull, s);
+  if (P.n > 0)
+    hipLaunchKernelGGL(k_scatter_density, dim3(nblocks(P.n, 256)), dim3(256), 0, s, T, P, leaf_slot, lgres, apron, r,
+                       expand, ldexp(1.0, shift), acc);
+  const int64_t g = (nvox + 255) / 256;
+  hipLaunchKernelGGL(k_scatter_apply, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(256), 0, s, acc, nvox,
+                     ldexp(1.0, -shift), chan);
 }
 
 }  // namespace gvdbx

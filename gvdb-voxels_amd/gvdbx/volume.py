@@ -184,6 +184,14 @@ class VolumeGVDB:
     def _ptr(d):
         return None if d is None else d.ptr
 
+    def ScatterDensity(self, num_pnts, radius, amp, trans=(0, 0, 0), expand=True, avgColor=False, chan=0):
+        """ScatterDensity (V.cpp:5625-5659): (1 - c)^3 of every bound point into its voxel (and
+        its 6 face neighbours with expand) of chan; exact sums instead of the reference's race."""
+        if avgColor:
+            raise GvdbxError(1, "ScatterDensity", "colour channels are not part of this library")
+        self._chk(self.L.gvdbx_scatter_density(self.h, num_pnts, radius, amp, fptr(trans), int(bool(expand)), chan),
+                  "ScatterDensity")
+
     def SplatTrilinear(self, chan, q_dev, n, mass_dev=None, accum=False):
         """Trilinear mass splat (transpose of SampleTrilinear) into chan's interior."""
         self._chk(self.L.gvdbx_splat_trilinear(self.h, chan, q_dev.ptr, self._ptr(mass_dev), n, int(accum)),

@@ -127,6 +127,17 @@ class VolumeGVDB {
                                 bAccum ? 1 : 0),
           "GatherLevelSet", true);
   }
+  // ScatterDensity (V.cpp:5625-5659) into channel 0 as the reference's kernel writes it;
+  // avgColor is not supported (no colour channels): reported like a runtime error.
+  void ScatterDensity(int num_pnts, float radius, float amp, Vector3DF trans, bool expand = true,
+                      bool avgColor = false) {
+    if (avgColor) {
+      fprintf(stderr, "VolumeGVDB::ScatterDensity: colour channels are not part of this library\n");
+      exit(-1);
+    }
+    const float t[3] = {trans.x, trans.y, trans.z};
+    check(gvdbx_scatter_density(ctx_, num_pnts, radius, amp, t, expand ? 1 : 0, 0), "ScatterDensity", true);
+  }
   // Additions of this library (no reference counterpart): velocity channel means and the
   // scatter-reduce level set.
   void GatherVelocity(int chanX, int chanY, int chanZ, bool bAccum = false) {

@@ -139,6 +139,17 @@ int gvdbx_update_apron_channels(gvdbx_ctx* ctx, const int* chans, const float* b
  *      0 outside active bricks.  q: device float3[n]; out: device float[n]. */
 int gvdbx_sample_trilinear(gvdbx_ctx* ctx, int chan, const void* q, int64_t n, void* out);
 
+/* ---- ScatterDensity (V.h:555, V.cpp:5625-5659; kernel gvdbScatterPointDensity P.cuh:516-573):
+ *      for every bound point with q = p + trans in a leaf with corner b, p' = q - b,
+ *      pi = trunc(p'): voxel pi (and, with expand, its 6 face neighbours, which may lie in the
+ *      brick's apron) of chan gains (1 - c)^3, c = |v - p'|^2 / radius^2.  amp is unused, as in
+ *      the reference.  The reference's racy read-modify-write is replaced by exact 64-bit
+ *      fixed-point sums (deterministic); inputs whose worst-case voxel sum would exceed 2^40
+ *      (tiny radii at many points) return GVDBX_ERR_CAPACITY.  The colour average (avgColor)
+ *      is not part of this library (no colour channels).  Not for batched clouds. */
+int gvdbx_scatter_density(gvdbx_ctx* ctx, int64_t n, float radius, float amp, const float trans[3], int expand,
+                          int chan);
+
 /* ---- trilinear splat and adjoints (no reference counterpart: the transpose of the sampler
  *      above, R.cuh:104-126).  splat: chan interior (+)= sum_i m_i w_g(q_i) over the 8 voxels
  *      g of floor(q - 0.5) that lie in active bricks, points in no brick skipped; mass NULL =

@@ -841,6 +841,51 @@ int orc_sample_backward(const orc_tree* t, int chan, const float* q, const doubl
   return ORC_OK;
 }
 
+/* ScatterDensity (gvdbScatterPointDensity, P.cuh:516-573, with the leaf of q = p + trans found
+ * as gvdbInsertPoints does, P.cuh:19-36): for a valid point in a leaf with corner b, p' = q - b
+ * (FP32), pi = trunc(p'); the voxel pi, and with expand its 6 face neighbours (unclamped, so
+ * they may lie in the brick's apron; dropped when there is no apron), gains
+ * distFunc = 1 + c(-3 + c(3 - c)) = (1 - c)^3 with c = |v - p'|^2 / r^2 evaluated in FP32 as the
+ * reference does (P.cuh:514-521).  The reference's read-modify-write is racy; here every
+ * contribution is summed (FP64). */
+static float scatter_weight(const float* p, float vx, float vy, float vz, float r) {
+  float bx = vx - p[0], by = vy - p[1], bz = vz - p[2];
+  float c = (bx * bx + by * by + bz * bz) / (r * r);
+  return (float)(1.0 + (double)(c * (-3.0f + c * (3.0f - c))));
+}
+
+int orc_scatter_density(orc_tree* t, int chan, const float* pos, int64_t n, int64_t stride, float r,
+                        const float* trans, int expand) {
+  if (!t->chan[chan]) return ORC_ERR_STATE;
+  if (!(r > 0.0f)) return ORC_ERR_ARG;
+  const int a = t->apron, res = res_of(t, 0);
+  double* ch = t->chan[chan];
+  for (int64_t i = 0; i < n; i++) {
+    const float* pp = (const float*)((const char*)pos + i * stride);
+    float q[3];
+    for (int k = 0; k < 3; k++) q[k] = pp[k] + (trans ? trans[k] : 0.0f);
+    if (!(q[0] < 1073741824.0f && q[1] < 1073741824.0f && q[2] < 1073741824.0f)) continue;  /* NOHIT etc. */
+    int64_t f = leaf_at(t, q);
+    if (f < 0 || t->leaf_slot[f] < 0) continue;
+    const orc_node* nd = &t->nodes[0][f];
+    float pl[3];
+    int pi[3];
+    for (int k = 0; k < 3; k++) {
+      pl[k] = q[k] - (float)nd->pos[k];
+      pi[k] = (int)pl[k];
+    }
+    if (pi[0] < 0 || pi[1] < 0 || pi[2] < 0 || pi[0] >= res || pi[1] >= res || pi[2] >= res) continue;
+    static const int off[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    for (int k = 0; k < (expand ? 7 : 1); k++) {
+      const int v[3] = {pi[0] + off[k][0], pi[1] + off[k][1], pi[2] + off[k][2]};
+      if (v[0] < -a || v[1] < -a || v[2] < -a || v[0] >= res + a || v[1] >= res + a || v[2] >= res + a) continue;
+      ch[atlas_index(t, t->leaf_slot[f], v[0] + a, v[1] + a, v[2] + a)] +=
+          (double)scatter_weight(pl, (float)v[0], (float)v[1], (float)v[2], r);
+    }
+  }
+  return ORC_OK;
+}
+
 /* Trilinear splat rho(g) += m * w_g(q) into interior voxels of the leaves holding the
  * corners (the transpose of sampling over the global voxel lattice). */
 int orc_splat_trilinear(orc_tree* t, int chan, const float* q, const float* mass, int64_t n) {

@@ -109,6 +109,8 @@ int orc_scatter_reduce_levelset(orc_tree* t, float radius, int chan, double init
 int orc_sample_trilinear(const orc_tree* t, int chan, const float* q, int64_t n, double* out);
 /* Trilinear mass splat (transpose of sampling) and its adjoint. */
 int orc_splat_trilinear(orc_tree* t, int chan, const float* q, const float* mass, int64_t n);
+int orc_scatter_density(orc_tree* t, int chan, const float* pos, int64_t n, int64_t stride, float r,
+                        const float* trans, int expand);
 int orc_splat_backward(const orc_tree* t, int chan_grad, const float* q, const float* mass, int64_t n,
                        double* dq, double* dmass);
 /* Sampling backward with respect to the points: dq = dout * grad sample(q). */

@@ -73,6 +73,7 @@ def lib():
         L.orc_scatter_reduce_levelset.argtypes = [P, f32, C.c_int, C.c_double]
         L.orc_sample_trilinear.argtypes = [P, C.c_int, P, i64, P]
         L.orc_splat_trilinear.argtypes = [P, C.c_int, P, P, i64]
+        L.orc_scatter_density.argtypes = [P, C.c_int, P, i64, i64, C.c_float, P, C.c_int]
         L.orc_splat_backward.argtypes = [P, C.c_int, P, P, i64, P, P]
         L.orc_sample_backward.argtypes = [P, C.c_int, P, P, i64, P]
         L.orc_find_leaf.restype = i64
@@ -263,6 +264,12 @@ class Tree:
         q = np.ascontiguousarray(q, np.float32)
         m = None if mass is None else np.ascontiguousarray(mass, np.float32)
         _chk(lib().orc_splat_trilinear(self.h, chan, _ptr(q), _ptr(m), len(q)), "splat_trilinear")
+
+    def scatter_density(self, chan, pos, radius, trans=(0, 0, 0), expand=True):
+        pos = np.ascontiguousarray(pos, np.float32)
+        tr = np.ascontiguousarray(trans, np.float32)
+        _chk(lib().orc_scatter_density(self.h, chan, _ptr(pos), len(pos), 12, radius, _ptr(tr), int(expand)),
+             "scatter_density")
 
     def splat_backward(self, chan_grad, q, mass=None):
         q = np.ascontiguousarray(q, np.float32)

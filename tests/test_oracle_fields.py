@@ -256,3 +256,72 @@ def test_sample_backward_matches_finite_differences(oracle_mod):
     assert (np.abs(dq) > 0).sum() > 500
     outside = np.array([[1.0, 1.0, 1.0], [500.0, 3.0, 3.0]], np.float32)  # no leaf: zero gradient
     assert not t.sample_backward(0, outside, np.ones(2)).any()
+
+
+def _scatter_tree(oracle_mod, topo, apron=1):
+    t = oracle_mod.Tree((3, 3, 3, 3, 5), 16, 16, apron)
+    t.rebuild_topology(topo, 1.0)
+    t.add_channel(0)
+    t.update_atlas()
+    t.fill_channel(0, 0.0)
+    return t
+
+
+def _brick_of(t, corner):
+    """(slot, padded brick array) of the leaf whose corner is `corner`."""
+    nodes = t.nodes(0)
+    live = np.nonzero(nodes["flags"] == 1)[0]
+    k = np.nonzero((nodes["pos"][live] == np.asarray(corner)).all(1))[0]
+    assert len(k) == 1
+    slot = t.leaf_slot()[live[k[0]]]
+    return slot, t.channel(0)[slot]
+
+
+def test_scatter_density_closed_forms(oracle_mod):
+    """ScatterDensity (P.cuh:516-573) at hand-computed points of the brick at (40, 40, 40):
+    a point at voxel centre (42.5, 43.5, 44.5) with r = 1 puts (1 - 0.75)^3 = 0.015625 on voxel
+    (2, 3, 4); its -x face neighbour is 1.5, 0.5, 0.5 away: c = 2.75, (1 - 2.75)^3 = -5.359375;
+    its +x neighbour 0.015625.  A point exactly on voxel corner (45, 41, 42) gives 1 there and 0
+    on the 6 neighbours (c = 1).  r = 2 scales c by 1/4: (1 - 0.1875)^3 = 0.536376953125."""
+    topo = np.array([[44.0, 44.0, 44.0], [150.0, 150.0, 150.0]], np.float32)
+    t = _scatter_tree(oracle_mod, topo)
+    t.scatter_density(0, np.array([[42.5, 43.5, 44.5]], np.float32), 1.0)
+    _, b = _brick_of(t, (40, 40, 40))
+    a = 1  # apron: padded index = local + 1, array order [z, y, x]
+    assert b[4 + a, 3 + a, 2 + a] == 0.015625
+    assert b[4 + a, 3 + a, 1 + a] == -5.359375
+    assert b[4 + a, 3 + a, 3 + a] == 0.015625
+    assert b[4 + a, 2 + a, 2 + a] == -5.359375 and b[4 + a, 4 + a, 2 + a] == 0.015625
+    assert b[3 + a, 3 + a, 2 + a] == -5.359375 and b[5 + a, 3 + a, 2 + a] == 0.015625
+    assert np.count_nonzero(b) == 7
+    t = _scatter_tree(oracle_mod, topo)
+    t.scatter_density(0, np.array([[45.0, 41.0, 42.0]], np.float32), 1.0)
+    _, b = _brick_of(t, (40, 40, 40))
+    assert b[2 + a, 1 + a, 5 + a] == 1.0 and np.count_nonzero(b) == 1
+    t = _scatter_tree(oracle_mod, topo)
+    t.scatter_density(0, np.array([[42.5, 43.5, 44.5]], np.float32), 2.0, expand=False)
+    _, b = _brick_of(t, (40, 40, 40))
+    assert b[4 + a, 3 + a, 2 + a] == 0.536376953125 and np.count_nonzero(b) == 1
+
+
+def test_scatter_density_apron_skips_and_sums(oracle_mod):
+    """The -x neighbour of a point in local voxel 0 lands in the brick's own apron (no clamp);
+    a point with a negative coordinate, one outside every brick and a NOHIT point add nothing;
+    two calls add up (the contributions sum, so the reference's race is defined away); a
+    translation shifts the point before the leaf lookup."""
+    topo = np.array([[44.0, 44.0, 44.0], [150.0, 150.0, 150.0]], np.float32)
+    t = _scatter_tree(oracle_mod, topo)
+    pts = np.array([[40.5, 43.5, 44.5], [-1.0, 43.0, 43.0], [200.0, 200.0, 200.0], [43.0, 43.0, 1e10]], np.float32)
+    t.scatter_density(0, pts, 1.0)
+    _, b = _brick_of(t, (40, 40, 40))
+    a = 1
+    assert b[4 + a, 3 + a, 0] == -5.359375  # padded x = -1 + apron
+    assert b[4 + a, 3 + a, 0 + a] == 0.015625
+    assert np.count_nonzero(b) == 7
+    t.scatter_density(0, pts, 1.0)
+    _, b2 = _brick_of(t, (40, 40, 40))
+    assert np.array_equal(b2, 2 * b)
+    t = _scatter_tree(oracle_mod, topo)
+    t.scatter_density(0, np.array([[40.0, 41.0, 42.0]], np.float32), 1.0, trans=(2.5, 2.5, 2.5), expand=False)
+    _, b = _brick_of(t, (40, 40, 40))
+    assert b[4 + a, 3 + a, 2 + a] == 0.015625 and np.count_nonzero(b) == 1
