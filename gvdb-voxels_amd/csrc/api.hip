@@ -89,9 +89,11 @@ struct gvdbx_ctx {
   bool store_vel_lists = false;
   bool have_vsum = false;
   Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
+  bool sc_map_current = false;  // sc_map holds the scan map (the insert may have used the slots)
   Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb, nbm, spos, svel;
   Buf sp_cnt, sp_start, sp_keyrank, sp_payload, sp_perm;  // splat: points counting-sorted by (brick, region)
   Buf scatter_acc;  // ScatterDensity: 64-bit fixed-point sums per padded voxel
+  Buf wvs;          // insert: per-brick window velocity sums
   // brick order of the last splat (a permutation of its n points), reused by the splat
   // backward on the same point buffer for locality
   const void* sp_order_q = nullptr;
@@ -800,7 +802,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
                  &c->sc_nid, &c->list_pos, &c->list_vel, &c->map_leaf, &c->vsum, &c->hkeys, &c->hkeys_alt,
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
                  &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta, &c->box_region, &c->box_cnt,
-                 &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->scatter_acc,
+                 &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->scatter_acc, &c->wvs,
                  &c->cloud_id,
                  &c->cloud_off_dev};
   for (Buf* b : bufs)
@@ -1113,19 +1115,26 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   DevTree T = tree_of(c);
   PointsView pv = points_of(c, n, trans);
 
-  // subcell map (gvdbSetFlagSubcell + PrefixSum, P.cuh:109-118) and corners (P.cuh:677-694)
-  leaf_flags(T, total, 1, P<int>(c->flags), c->stream);
-  exclusive_scan(P<int>(c->flags), P<int>(c->sc_map), total, P<int>(c->tile_buf), nullptr, c->stream);
-  fill_u32(P<uint32_t>(c->map_leaf), total + 1, 0xFFFFFFFFu, c->stream);
-  sc_positions(T, total, cfg, P<int>(c->sc_map), P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->map_leaf),
-               c->stream);
+  // subcell map (gvdbSetFlagSubcell + PrefixSum, P.cuh:109-118) and corners (P.cuh:677-694).
+  // Leaves are flagged exactly when they are linked (linking takes every flagged node, delinking
+  // every unflagged leaf), so on a current atlas the map of a linked leaf is its slot: the
+  // kernels read leaf_slot, and the full map array (dead leaves included) is made on request.
+  const bool slot_map = c->atlas_version == c->topo_version && c->nslots == c->used0 && c->leaf_slot.p != nullptr;
+  const int* mapp = slot_map ? P<int>(c->leaf_slot) : P<int>(c->sc_map);
+  if (!slot_map) {
+    leaf_flags(T, total, 1, P<int>(c->flags), c->stream);
+    exclusive_scan(P<int>(c->flags), P<int>(c->sc_map), total, P<int>(c->tile_buf), nullptr, c->stream);
+    fill_u32(P<uint32_t>(c->map_leaf), total + 1, 0xFFFFFFFFu, c->stream);
+  }
+  c->sc_map_current = !slot_map;
+  sc_positions(T, total, cfg, mapp, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->map_leaf), c->stream);
 
   // sort points by home brick
   uint32_t* skeys;
   uint32_t* svals;
   {
     TimedRegion tr(c, "home");
-    home_keys(T, pv, P<int>(c->sc_map), nmap, P<uint32_t>(c->hkeys), P<uint32_t>(c->hvals), c->stream);
+    home_keys(T, pv, mapp, nmap, P<uint32_t>(c->hkeys), P<uint32_t>(c->hvals), c->stream);
     int bits = 1;
     while (((int64_t)1 << bits) <= nmap) ++bits;
     const int par = radix_sort_u32(P<uint32_t>(c->hkeys), P<uint32_t>(c->hkeys_alt), P<uint32_t>(c->hvals),
@@ -1151,7 +1160,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   A.P = pv;
   A.c = cfg;
   A.W = W;
-  A.map = P<int>(c->sc_map);
+  A.map = mapp;
   A.map_leaf = P<int>(c->map_leaf);
   A.svals = svals;
   A.start = P<int>(c->ranges);
@@ -1166,6 +1175,8 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   A.list_pos = nullptr;
   A.list_vel = nullptr;
   A.vsum = with_vel ? P<unsigned long long>(c->vsum) : nullptr;
+  if (with_vel) CHK(ensure(c, c->wvs, (size_t)((int64_t)nmap * W * W * W * 3 + 1) * 8));
+  A.wvs = with_vel ? P<unsigned long long>(c->wvs) : nullptr;
 
   fill_u32(P<uint32_t>(c->ocnt), nsc, 0u, c->stream);
   if (with_vel) fill_u64(P<uint64_t>(c->vsum), nsc * 3, 0ull, c->stream);
@@ -1299,7 +1310,8 @@ int gvdbx_gather_velocity(gvdbx_ctx* c, int cx, int cy, int cz, int accum) {
                      P<float>(c->chan[cy]), P<float>(c->chan[cz]), accum, c->stream);
   }
   c->nbr_apron_ok[cx] = c->nbr_apron_ok[cy] = c->nbr_apron_ok[cz] = apron_done;
-  c->bound_ok[cx] = c->bound_ok[cy] = c->bound_ok[cz] = false;
+  c->bound_ok[cx] = c->bound_ok[cy] = c->bound_ok[cz] = apron_done;  // velocity_brick writes 0 there
+  c->bound_val[cx] = c->bound_val[cy] = c->bound_val[cz] = 0.0f;
   return launch_status(c, "GatherVelocity");
 }
 
@@ -1574,7 +1586,18 @@ int gvdbx_copy_subcell_array(gvdbx_ctx* c, int which, void* out, int64_t max_byt
   const void* src = nullptr;
   int64_t bytes = 0;
   switch (which) {
-    case 0: src = c->sc_map.p; bytes = c->cnt[0] * 4; break;
+    case 0:
+      if (!c->sc_map_current && c->cnt[0] > 0) {  // the insert used the slots: make the full map
+        CHK(ensure(c, c->flags, (size_t)(c->cnt[0] + 1) * 4));
+        CHK(ensure(c, c->sc_map, (size_t)(c->cnt[0] + 1) * 4));
+        CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(c->cnt[0] + 1) + 16) * 4));
+        leaf_flags(tree_of(c), c->cnt[0], 1, P<int>(c->flags), c->stream);
+        exclusive_scan(P<int>(c->flags), P<int>(c->sc_map), c->cnt[0], P<int>(c->tile_buf), nullptr, c->stream);
+        c->sc_map_current = true;
+      }
+      src = c->sc_map.p;
+      bytes = c->cnt[0] * 4;
+      break;
     case 1: src = c->sc_cnt.p; bytes = c->nsc * 4; break;
     case 2: src = c->sc_off.p; bytes = c->nsc * 4; break;
     case 3: src = c->sc_pos.p; bytes = c->nsc * 12; break;

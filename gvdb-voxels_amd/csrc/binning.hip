@@ -348,10 +348,11 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_
       const int64_t g = B.global_of(w);
       if (g >= 0) {
         keep = n;
-        if (vel) {
-          atomicAdd(&A.vsum[3 * g + 0], cvs[3 * w]);
-          atomicAdd(&A.vsum[3 * g + 1], cvs[3 * w + 1]);
-          atomicAdd(&A.vsum[3 * g + 2], cvs[3 * w + 2]);
+        if (vel) {  // pulled into vsum by the cell walk (no device atomics)
+          unsigned long long* o = A.wvs + ((int64_t)m * W3 + w) * 3;
+          o[0] = cvs[3 * w];
+          o[1] = cvs[3 * w + 1];
+          o[2] = cvs[3 * w + 2];
         }
       }
     }
@@ -493,6 +494,8 @@ __global__ void k_cell_walk(BinArgs A, int64_t nsc, int* __restrict__ cnt) {
   int p = 0;
   int base = 0;
   if (kBases) base = A.off[g];
+  const bool vel = !kBases && A.vsum != nullptr && A.P.vel != nullptr;
+  unsigned long long vs[3] = {0ull, 0ull, 0ull};
   for (int dz = (lz == 0 ? -1 : 0); dz <= (lz == D - 1 ? 1 : 0); ++dz)
     for (int dy = (ly == 0 ? -1 : 0); dy <= (ly == D - 1 ? 1 : 0); ++dy)
       for (int dx = (lx == 0 ? -1 : 0); dx <= (lx == D - 1 ? 1 : 0); ++dx) {
@@ -503,10 +506,21 @@ __global__ void k_cell_walk(BinArgs A, int64_t nsc, int* __restrict__ cnt) {
         const int c = *cell;
         if (kBases) *cell = base + p;
         p += c;
+        if (vel && c > 0) {
+          const unsigned long long* v = A.wvs + ((int64_t)b * W3 + w) * 3;
+          vs[0] += v[0];
+          vs[1] += v[1];
+          vs[2] += v[2];
+        }
       }
   if (!kBases) {
     A.cnt_in[g] = p;
     cnt[g] = p + A.ocnt[g];
+    if (vel && p > 0) {  // the pairs placed one by one added theirs with atomics before
+      A.vsum[3 * g + 0] += vs[0];
+      A.vsum[3 * g + 1] += vs[1];
+      A.vsum[3 * g + 2] += vs[2];
+    }
   }
 }
 
@@ -642,16 +656,27 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
   // The brick's subcell lists are contiguous, [soff[0], soff[D3]): the lanes take its entries in
   // order and each finds its subcell by binary search over the offsets (sparse bricks fill the
   // lanes across subcells instead of running one short list per wave).
+  // fixed geometry: the D3 + 1 offsets in scalar registers, the subcell by counting starts <= j
+  constexpr int kFixedD3 = kRes ? kD * kD * kD : 1;
+  int so[kFixedD3 + 1];
+  if (kRes) {
+#pragma unroll
+    for (int k = 0; k <= kFixedD3; ++k) so[k] = __builtin_amdgcn_readfirstlane(soff[k]);
+  }
   for (int j = soff[0] + (int)threadIdx.x; j < soff[D3]; j += blockDim.x) {
-    int s = 0;
-    {
-      int hi = D3;  // soff[s] <= j < soff[hi]
+    unsigned s = 0;
+    if (kRes) {
+#pragma unroll
+      for (int k = 1; k < kFixedD3; ++k) s += j >= so[k] ? 1u : 0u;
+    } else {
+      unsigned hi = D3;  // soff[s] <= j < soff[hi]
       while (hi - s > 1) {
-        const int mid = (s + hi) >> 1;
+        const unsigned mid = (s + hi) >> 1;
         if (soff[mid] <= j) s = mid; else hi = mid;
       }
     }
-    const int ix = s % D, iy = (s / D) % D, iz = s / (D * D);
+    const int ix = (int)(s % (unsigned)D), iy = (int)((s / (unsigned)D) % (unsigned)D),
+              iz = (int)(s / (unsigned)(D * D));
     // voxel range of the subcell, extended into the apron on the brick's boundary
     const int lx0 = ix * sc - (ix == 0 ? a : 0), lx1 = ix * sc + sc - 1 + (ix == D - 1 ? a : 0);
     const int ly0 = iy * sc - (iy == 0 ? a : 0), ly1 = iy * sc + sc - 1 + (iy == D - 1 ? a : 0);
@@ -660,16 +685,25 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
       const float gx = list_pos[3 * (int64_t)j], gy = list_pos[3 * (int64_t)j + 1], gz = list_pos[3 * (int64_t)j + 2];
       const int fx = (int)gx - bx, fy = (int)gy - by, fz = (int)gz - bz;
       if (R == 1) {
+        // corners fx + {0, 1} per axis: offsets and x squares once per entry, then each corner's
+        // |v - p|^2 as dist2 forms it (fmaf(z, z, fmaf(y, y, x * x))), bit for bit
+        const float ex[2] = {gx - (float)(bx + fx), gx - (float)(bx + fx + 1)};
+        const float ey[2] = {gy - (float)(by + fy), gy - (float)(by + fy + 1)};
+        const float ez[2] = {gz - (float)(bz + fz), gz - (float)(bz + fz + 1)};
+        const float px[2] = {ex[0] * ex[0], ex[1] * ex[1]};
+        const bool vx[2] = {fx >= lx0 && fx <= lx1, fx + 1 >= lx0 && fx + 1 <= lx1};
+        const bool vy[2] = {fy >= ly0 && fy <= ly1, fy + 1 >= ly0 && fy + 1 <= ly1};
+        const bool vz[2] = {fz >= lz0 && fz <= lz1, fz + 1 >= lz0 && fz + 1 <= lz1};
+        const int i0 = ((fz + a) * bw + (fy + a)) * bw + (fx + a);
 #pragma unroll
         for (int dz = 0; dz < 2; ++dz)
 #pragma unroll
           for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
-              const int x = fx + dx, y = fy + dy, z = fz + dz;
-              if (x < lx0 || x > lx1 || y < ly0 || y > ly1 || z < lz0 || z > lz1) continue;
-              const float d = dist2(gx - (float)(bx + x), gy - (float)(by + y), gz - (float)(bz + z));
-              atomicMin(&d2[((z + a) * bw + (y + a)) * bw + (x + a)], __float_as_uint(d));
+              if (!(vx[dx] && vy[dy] && vz[dz])) continue;
+              const float d = fmaf(ez[dz], ez[dz], fmaf(ey[dy], ey[dy], px[dx]));
+              atomicMin(&d2[i0 + (dz * bw + dy) * bw + dx], __float_as_uint(d));
             }
       } else {
         const int x0 = max(lx0, fx - R + 1), x1 = min(lx1, fx + R);
@@ -698,7 +732,8 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
     }
     const unsigned int u = d2[t];
     const float val = (missing || u == 0x7F800000u) ? 0.0f : fmaxf(0.0f, c.r - sqrtf(__uint_as_float(u)));
-    bptr[((z + apron) * br + (y + apron)) * br + (x + apron)] = val;
+    // the computed region is the whole padded brick when the apron is computed here
+    bptr[a == apron ? t : ((z + apron) * br + (y + apron)) * br + (x + apron)] = val;
   }
 }
 
@@ -781,12 +816,14 @@ __global__ void __launch_bounds__(256) k_velocity_brick(SubcellCfg c, int nmap, 
     const int x = t - zy * bw - a, z = zy / bw - a, y = zy - (zy / bw) * bw - a;
     const int ox = x < 0 ? -1 : (x >= res0 ? 1 : 0), oy = y < 0 ? -1 : (y >= res0 ? 1 : 0),
               oz = z < 0 ? -1 : (z >= res0 ? 1 : 0);
-    if ((ox | oy | oz) != 0 && nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] < 0) continue;
+    // apron voxel without a neighbour brick: the boundary value 0 (without accumulation)
+    const bool missing = (ox | oy | oz) != 0 && nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] < 0;
+    if (missing && accum) continue;
     const int64_t vi = base + ((z + apron) * br + (y + apron)) * br + (x + apron);
     if (!accum) {
       const int w = (((z >> lgsc) + 1) * W + ((y >> lgsc) + 1)) * W + ((x >> lgsc) + 1);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) ch[q][vi] = mean[w][q];
+      for (int q = 0; q < 3; ++q) ch[q][vi] = missing ? 0.0f : mean[w][q];
     } else {
       const int64_t k = (int64_t)m * D3 + ((z >> lgsc) * D + (y >> lgsc)) * D + (x >> lgsc);
 #pragma unroll

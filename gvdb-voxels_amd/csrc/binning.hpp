@@ -36,6 +36,7 @@ struct BinArgs {
   float* list_pos;
   float* list_vel;
   unsigned long long* vsum;
+  unsigned long long* wvs;  // [nmap][W^3][3]: the brick's window velocity sums (where its count > 0)
 };
 
 void bin_count(const BinArgs& A, int nmap, hipStream_t s);
