@@ -238,21 +238,35 @@ __global__ void __launch_bounds__(256) k_radix_up(const K* __restrict__ keys, in
   counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
+// Scatter of one tile: keys are ranked stably by digit into LDS first (local digit starts from
+// the scanned counts: a digit's tile count is the next offset minus this one, the offsets being
+// a digit-major exclusive scan), then written out in local order, so consecutive threads store
+// consecutive addresses within each digit's run instead of scattering single keys.
 template <typename K, bool kHasVal>
 __global__ void __launch_bounds__(256) k_radix_down(const K* __restrict__ kin, K* __restrict__ kout,
                                                    const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                    int64_t n, int shift, const int* __restrict__ offsets,
                                                    int ntiles) {
-  __shared__ int base_of[256];
+  __shared__ K sk[kRadixTile];
+  __shared__ uint32_t sv[kHasVal ? kRadixTile : 1];
+  __shared__ int base_of[256], lstart[256], run[256];
   __shared__ int whist[4][256];
   const int tid = threadIdx.x, wave = tid >> 6;
-  base_of[tid] = offsets[(int64_t)tid * ntiles + blockIdx.x];
+  const int64_t tile = blockIdx.x;
+  const int64_t tbase = tile * kRadixTile;
+  const int mine = offsets[(int64_t)tid * ntiles + tile];
+  const int next = tile + 1 < ntiles ? offsets[(int64_t)tid * ntiles + tile + 1]
+                                     : (tid < 255 ? offsets[(int64_t)(tid + 1) * ntiles] : (int)n);
+  int total;
+  const int ls = block_exclusive_scan_256(next - mine, &total);
+  base_of[tid] = mine;
+  lstart[tid] = ls;
+  run[tid] = 0;
   const uint64_t lt = lanemask_lt();
-  const int64_t tbase = (int64_t)blockIdx.x * kRadixTile;
   for (int r = 0; r < kRadixTile / 256; ++r) {
     const int64_t i = tbase + r * 256 + tid;
     const bool valid = i < n;
-    K k = valid ? kin[i] : (K)0;
+    const K k = valid ? kin[i] : (K)0;
     uint32_t v = 0;
     if (kHasVal && valid) v = vin[i];
     const int d = (int)((k >> shift) & 255);
@@ -270,14 +284,22 @@ __global__ void __launch_bounds__(256) k_radix_down(const K* __restrict__ kin, K
     if (valid && rank == 0) whist[wave][d] = __popcll(peers);
     __syncthreads();
     if (valid) {
-      int off = base_of[d] + rank;
-      for (int w = 0; w < wave; ++w) off += whist[w][d];
-      kout[off] = k;
-      if (kHasVal) vout[off] = v;
+      int pos = lstart[d] + run[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += whist[w][d];
+      sk[pos] = k;
+      if (kHasVal) sv[pos] = v;
     }
     __syncthreads();
-    base_of[tid] += whist[0][tid] + whist[1][tid] + whist[2][tid] + whist[3][tid];
-    __syncthreads();
+    run[tid] += whist[0][tid] + whist[1][tid] + whist[2][tid] + whist[3][tid];
+  }
+  __syncthreads();
+  const int tn = (int)(n - tbase < kRadixTile ? n - tbase : kRadixTile);
+  for (int j = tid; j < tn; j += 256) {
+    const K k = sk[j];
+    const int d = (int)((k >> shift) & 255);
+    const int64_t o = (int64_t)base_of[d] + (j - lstart[d]);
+    kout[o] = k;
+    if (kHasVal) vout[o] = sv[j];
   }
 }
 
