@@ -1115,6 +1115,15 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   DevTree T = tree_of(c);
   PointsView pv = points_of(c, n, trans);
 
+  const int W = cfg.D + 2;
+  CHK(ensure(c, c->cnt_in, (size_t)(nsc_cap + 1) * 4));
+  CHK(ensure(c, c->ocnt, (size_t)(nsc_cap + 1) * 4));
+  CHK(ensure(c, c->ocur, (size_t)(nsc_cap + 1) * 4));
+  CHK(ensure(c, c->wgb, (size_t)((int64_t)nmap * W * W * W + 1) * 4));
+  CHK(ensure(c, c->nbm, (size_t)((int64_t)nmap * 27 + 1) * 4));
+  CHK(ensure(c, c->spos, (size_t)(n + 1) * 16));
+  if (with_vel) CHK(ensure(c, c->svel, (size_t)(n + 1) * 16));
+
   // subcell map (gvdbSetFlagSubcell + PrefixSum, P.cuh:109-118) and corners (P.cuh:677-694).
   // Leaves are flagged exactly when they are linked (linking takes every flagged node, delinking
   // every unflagged leaf), so on a current atlas the map of a linked leaf is its slot: the
@@ -1127,7 +1136,9 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
     fill_u32(P<uint32_t>(c->map_leaf), total + 1, 0xFFFFFFFFu, c->stream);
   }
   c->sc_map_current = !slot_map;
-  sc_positions(T, total, cfg, mapp, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->map_leaf), c->stream);
+  // the corner kernel also clears the one-by-one counts and the velocity sums of every subcell
+  sc_positions(T, total, cfg, mapp, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->map_leaf), P<int>(c->ocnt),
+               with_vel ? P<unsigned long long>(c->vsum) : nullptr, c->stream);
 
   // sort points by home brick
   uint32_t* skeys;
@@ -1144,17 +1155,8 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
     svals = par ? P<uint32_t>(c->hvals_alt) : P<uint32_t>(c->hvals);
     key_ranges(skeys, n, nmap, P<int>(c->ranges), c->stream);
   }
-  const int W = cfg.D + 2;
   const int* tail = P<int>(c->ranges) + nmap;  // points whose own leaf is missing, and invalid ones
-  CHK(ensure(c, c->cnt_in, (size_t)(nsc_cap + 1) * 4));
-  CHK(ensure(c, c->ocnt, (size_t)(nsc_cap + 1) * 4));
-  CHK(ensure(c, c->ocur, (size_t)(nsc_cap + 1) * 4));
-  CHK(ensure(c, c->wgb, (size_t)((int64_t)nmap * W * W * W + 1) * 4));
-  CHK(ensure(c, c->nbm, (size_t)((int64_t)nmap * 27 + 1) * 4));
-  CHK(ensure(c, c->spos, (size_t)(n + 1) * 16));
-  if (with_vel) CHK(ensure(c, c->svel, (size_t)(n + 1) * 16));
   gather_sorted(pv, svals, n, P<float4>(c->spos), with_vel ? P<float4>(c->svel) : nullptr, c->stream);
-  fill_u32(P<uint32_t>(c->nbm), (int64_t)nmap * 27, 0xFFFFFFFFu, c->stream);
   BinArgs A;
   A.T = T;
   A.P = pv;
@@ -1178,8 +1180,6 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   if (with_vel) CHK(ensure(c, c->wvs, (size_t)((int64_t)nmap * W * W * W * 3 + 1) * 8));
   A.wvs = with_vel ? P<unsigned long long>(c->wvs) : nullptr;
 
-  fill_u32(P<uint32_t>(c->ocnt), nsc, 0u, c->stream);
-  if (with_vel) fill_u64(P<uint64_t>(c->vsum), nsc * 3, 0ull, c->stream);
   {
     TimedRegion tr(c, "count");
     bin_count(A, nmap, c->stream);
@@ -1225,7 +1225,6 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   A.list_pos = P<float>(c->list_pos);
   A.list_vel = c->lists_have_vel ? P<float>(c->list_vel) : nullptr;
   if (early) CHK(read_meta_begin(c));  // the length is final after the scan
-  fill_u32(P<uint32_t>(c->ocur), nsc, 0u, c->stream);
   {
     TimedRegion tr(c, "insert");
     bin_insert(A, nmap, c->stream);

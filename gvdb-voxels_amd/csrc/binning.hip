@@ -244,7 +244,10 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_
   if (mi >= nmap) return;
   const int m = (int)mi;
   const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W, D3 = D * D * D;
-  if (!brick_setup(A, m, nb, B, D, W)) return;
+  if (!brick_setup(A, m, nb, B, D, W)) {  // no leaf: nothing reaches the cell walk through it
+    if (threadIdx.x < 27) A.nbm[(int64_t)m * 27 + threadIdx.x] = -1;
+    return;
+  }
   const int NB = use_bins ? D3 * 64 : 0;
   const bool vel = A.vsum != nullptr && A.P.vel != nullptr;
   // layout: [cvs 3*W3 u64][bvs 3*NB u64][ccnt W3 int][bcnt NB int]
@@ -493,7 +496,10 @@ __global__ void k_cell_walk(BinArgs A, int64_t nsc, int* __restrict__ cnt) {
   const int lx = l % D, ly = (l / D) % D, lz = l / (D * D);
   int p = 0;
   int base = 0;
-  if (kBases) base = A.off[g];
+  if (kBases) {
+    base = A.off[g];
+    A.ocur[g] = 0;  // the insert's one-by-one cursor starts at zero
+  }
   const bool vel = !kBases && A.vsum != nullptr && A.P.vel != nullptr;
   unsigned long long vs[3] = {0ull, 0ull, 0ull};
   for (int dz = (lz == 0 ? -1 : 0); dz <= (lz == D - 1 ? 1 : 0); ++dz)

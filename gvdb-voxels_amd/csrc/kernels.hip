@@ -469,7 +469,8 @@ void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t c
 
 // gvdbCalcSubcellPos (P.cuh:677-694).
 __global__ void k_sc_positions(DevTree T, int64_t leaves, SubcellCfg c, const int* __restrict__ map,
-                               int* __restrict__ sc_pos, int* __restrict__ sc_nid, int* __restrict__ map_leaf) {
+                               int* __restrict__ sc_pos, int* __restrict__ sc_nid, int* __restrict__ map_leaf,
+                               int* __restrict__ ocnt, unsigned long long* __restrict__ vsum) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= leaves) return;
   const gvdbx_node* nd = T.nodes[0] + i;
@@ -482,13 +483,15 @@ __global__ void k_sc_positions(DevTree T, int64_t leaves, SubcellCfg c, const in
     sc_pos[3 * k + 1] = nd->pos[1] + ((s / c.D) % c.D) * c.sc;
     sc_pos[3 * k + 2] = nd->pos[2] + ((s / c.D / c.D) % c.D) * c.sc;
     sc_nid[k] = (int)i;
+    if (ocnt) ocnt[k] = 0;
+    if (vsum) vsum[3 * k] = vsum[3 * k + 1] = vsum[3 * k + 2] = 0ull;
   }
 }
 void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map, int* sc_pos, int* sc_nid,
-                  int* map_leaf, hipStream_t s) {
+                  int* map_leaf, int* ocnt, unsigned long long* vsum, hipStream_t s) {
   if (leaves <= 0) return;
   hipLaunchKernelGGL(k_sc_positions, dim3(blocks_for(leaves, 256)), dim3(256), 0, s, T, leaves, c, map, sc_pos,
-                     sc_nid, map_leaf);
+                     sc_nid, map_leaf, ocnt, vsum);
 }
 
 // ================================================================ gathers

@@ -81,8 +81,9 @@ struct SubcellCfg {
   float r;
   int cmask;  // position bits inside a batched cloud's cell (all bits for a single cloud)
 };
+// subcell corners and leaves of every linked leaf; also zeroes ocnt and vsum (when given) there
 void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map, int* sc_pos, int* sc_nid,
-                  int* map_leaf, hipStream_t s);
+                  int* map_leaf, int* ocnt, unsigned long long* vsum, hipStream_t s);
 
 // ---- gathers (mode 0 density, 1 level set starting from `init`)
 void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
