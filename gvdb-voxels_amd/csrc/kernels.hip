@@ -93,7 +93,7 @@ void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const
 // Emission needs no device-scope atomic: each workgroup ranks its keys with an LDS counter into
 // its own slice of `region` (cap_b = 256 * the per-point bound), writes its count, and a scan
 // plus k_compact_regions pack the slices.
-template <typename K>
+template <typename K, bool kTwo>
 __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFmt f, float r, float hi_mul,
                                                   K* __restrict__ region, int64_t cap_b, int* __restrict__ blockcnt,
                                                   int* __restrict__ markers) {
@@ -121,28 +121,65 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
       hi[a] = (int)(tq + hi_mul * r) / R0 * R0 + org[a];
     }
     const int d = T.depth;
-    for (int x = lo[0]; x <= hi[0]; x += R0)
-      for (int y = lo[1]; y <= hi[1]; y += R0)
-        for (int z = lo[2]; z <= hi[2]; z += R0) {
-          if (!in_root(T, x, y, z)) continue;
-          int64_t cur = 0;
-          int miss = -1;  // highest missing level along the chain
-          for (int l = d; l > 0; --l) {
-            const uint64_t e = T.clist[l][((uint64_t)cur << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, x, y, z)];
-            if (e == kUndef64) { miss = l - 1; break; }
-            cur = (int64_t)(e >> 16);
-          }
-          if (miss < 0) {
-            if (markers && markers[cur] == 0) markers[cur] = 1;
-            continue;
-          }
-          const int k = atomicAdd(&bcount, miss + 1);
-          for (int l = miss; l >= 0; --l) {
-            const int sh = T.lgR[l];
-            out[k + miss - l] = pack_key<K>(f, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
-                                            (z >> sh) - (T.root[2] >> sh));
-          }
+    auto emit = [&](int x, int y, int z, int64_t cur, int miss) {
+      if (miss < 0) {
+        if (markers && markers[cur] == 0) markers[cur] = 1;
+        return;
+      }
+      const int k = atomicAdd(&bcount, miss + 1);
+      for (int l = miss; l >= 0; --l) {
+        const int sh = T.lgR[l];
+        out[k + miss - l] = pack_key<K>(f, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
+                                        (z >> sh) - (T.root[2] >> sh));
+      }
+    };
+    if (kTwo) {
+      // at most two leaf nodes per axis: the 8 candidates descend level by level together, so the
+      // child-list loads of one level are independent and in flight at once
+      int cx[8], cy[8], cz[8], miss[8];
+      int64_t cur[8];
+      bool act[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        cx[k] = lo[0] + (k & 1) * R0;
+        cy[k] = lo[1] + ((k >> 1) & 1) * R0;
+        cz[k] = lo[2] + (k >> 2) * R0;
+        act[k] = cx[k] <= hi[0] && cy[k] <= hi[1] && cz[k] <= hi[2] && in_root(T, cx[k], cy[k], cz[k]);
+        cur[k] = 0;
+        miss[k] = -1;
+      }
+      for (int l = d; l > 0; --l) {
+        uint64_t e[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          e[k] = (act[k] && miss[k] < 0)
+                     ? T.clist[l][((uint64_t)cur[k] << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, cx[k], cy[k], cz[k])]
+                     : 0ull;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (!(act[k] && miss[k] < 0)) continue;
+          if (e[k] == kUndef64) miss[k] = l - 1;
+          else cur[k] = (int64_t)(e[k] >> 16);
         }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (act[k]) emit(cx[k], cy[k], cz[k], cur[k], miss[k]);
+    } else {
+      for (int x = lo[0]; x <= hi[0]; x += R0)
+        for (int y = lo[1]; y <= hi[1]; y += R0)
+          for (int z = lo[2]; z <= hi[2]; z += R0) {
+            if (!in_root(T, x, y, z)) continue;
+            int64_t cur = 0;
+            int miss = -1;  // highest missing level along the chain
+            for (int l = d; l > 0; --l) {
+              const uint64_t e = T.clist[l][((uint64_t)cur << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, x, y, z)];
+              if (e == kUndef64) { miss = l - 1; break; }
+              cur = (int64_t)(e >> 16);
+            }
+            emit(x, y, z, cur, miss);
+          }
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) blockcnt[blockIdx.x] = bcount;
@@ -152,8 +189,15 @@ template <typename K>
 void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi_mul, K* region, int64_t cap_b,
               int* blockcnt, int* markers, hipStream_t s) {
   if (P.n <= 0) return;
-  hipLaunchKernelGGL(k_box_keys<K>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, r, hi_mul, region, cap_b,
-                     blockcnt, markers);
+  // a box [trunc(q) - r, trunc(q) + hi_mul r] spans at most 2 leaf nodes per axis when it is
+  // shorter than one leaf node
+  const bool two = (1.0 + (double)hi_mul) * (double)r + 1.0 < (double)(1 << T.lgR[0]);
+  if (two)
+    hipLaunchKernelGGL((k_box_keys<K, true>), dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, r, hi_mul, region,
+                       cap_b, blockcnt, markers);
+  else
+    hipLaunchKernelGGL((k_box_keys<K, false>), dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, r, hi_mul,
+                       region, cap_b, blockcnt, markers);
 }
 
 template <typename K>
