@@ -49,8 +49,8 @@ def run_step(v, dpos, dvel, n):
     v.RebuildTopology(n, RADIUS)
     v.UpdateAtlas()
     ln = v.InsertPointsSubcell(SUBCELL, n, RADIUS)
-    v.GatherDensity(SUBCELL, n, RADIUS, (0, 0, 0), ln, 0)
-    v.GatherVelocity(1, 2, 3)
+    # GatherDensity into channel 0 with the velocity means (channels 1-3) in the same pass
+    v.GatherDensityVelocity(SUBCELL, n, RADIUS, (0, 0, 0), ln, 0, (1, 2, 3))
     v.UpdateApron()  # all four channels, boundary value 0
     return ln
 
@@ -63,10 +63,14 @@ def algorithmic_bytes(n, s_len, nsc, bricks, chans=4, vel=True):
             + 8 * chans * bricks * 488)
 
 
-def gather_bytes(s_len, nsc, bricks):
-    """Algorithmic bytes of one GatherDensity launch: list positions read, per-subcell
-    count/offset/corner/leaf read, interior voxels written."""
-    return s_len * 12 + nsc * (4 + 4 + 12 + 4) + bricks * 512 * 4
+def gather_bytes(s_len, nsc, bricks, vel=True):
+    """Algorithmic bytes of one gather launch: list positions read, per-subcell
+    count/offset/corner/leaf read, interior voxels written; with the velocity means in the same
+    pass, per-subcell count and 64-bit sums read and 3 more channels' interior written."""
+    b = s_len * 12 + nsc * (4 + 4 + 12 + 4) + bricks * 512 * 4
+    if vel:
+        b += nsc * (4 + 24) + 3 * bricks * 512 * 4
+    return b
 
 
 def cpu_baseline(p, vel, sample):

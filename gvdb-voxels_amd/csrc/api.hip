@@ -1255,7 +1255,7 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
   if (mode == 0 && !accum && cfg.res0 <= 8 && br * br * br <= kMaxPaddedVox) {
     density_brick(cfg, (int)c->used0, c->nsc, c->sclen, P<int>(c->map_leaf), P<int>(c->sc_off),
                   P<float>(c->list_pos), P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]), c->apron,
-                  c->apron > 0 ? P<int>(c->nbr) : nullptr, P<float>(c->chan[chan]), c->stream);
+                  c->apron > 0 ? P<int>(c->nbr) : nullptr, P<float>(c->chan[chan]), nullptr, c->stream);
     c->nbr_apron_ok[chan] = c->bound_ok[chan] = c->apron > 0;
     c->bound_val[chan] = 0.0f;
   } else {
@@ -1312,6 +1312,39 @@ int gvdbx_gather_velocity(gvdbx_ctx* c, int cx, int cy, int cz, int accum) {
   c->bound_ok[cx] = c->bound_ok[cy] = c->bound_ok[cz] = apron_done;  // velocity_brick writes 0 there
   c->bound_val[cx] = c->bound_val[cy] = c->bound_val[cz] = 0.0f;
   return launch_status(c, "GatherVelocity");
+}
+
+int gvdbx_gather_density_velocity(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t len,
+                                  int chan, int cx, int cy, int cz) {
+  (void)n; (void)trans; (void)len;
+  if (!c) return GVDBX_ERR_ARG;
+  if (sc != c->sc) return fail(c, GVDBX_ERR_ARG, "subcell size %d differs from the insert (%d)", sc, c->sc);
+  CHK(gather_checks(c, chan));
+  CHK(chan_ok(c, cx));
+  CHK(chan_ok(c, cy));
+  CHK(chan_ok(c, cz));
+  if (chan == cx || chan == cy || chan == cz || cx == cy || cx == cz || cy == cz)
+    return fail(c, GVDBX_ERR_ARG, "density and velocity channels must be four different channels");
+  if (!c->have_vsum) return fail(c, GVDBX_ERR_STATE, "no velocities were bound at InsertPointsSubcell");
+  if (c->nsc == 0) return GVDBX_OK;
+  const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
+  const int br = cfg.res0 + 2 * c->apron;
+  const bool one_pass = c->apron > 0 && cfg.res0 <= 8 && br * br * br <= kMaxPaddedVox && cfg.D <= 8;
+  if (!one_pass) {  // the two gathers one after the other give the same channels
+    CHK(gather_impl(c, r, chan, 0, 0));
+    return gvdbx_gather_velocity(c, cx, cy, cz, 0);
+  }
+  TimedRegion tr(c, "gather");
+  const VelOut vo{P<int>(c->sc_cnt), P<unsigned long long>(c->vsum),
+                  {P<float>(c->chan[cx]), P<float>(c->chan[cy]), P<float>(c->chan[cz])}};
+  density_brick(cfg, (int)c->used0, c->nsc, c->sclen, P<int>(c->map_leaf), P<int>(c->sc_off), P<float>(c->list_pos),
+                P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]), c->apron, P<int>(c->nbr), P<float>(c->chan[chan]),
+                &vo, c->stream);
+  for (int ch : {chan, cx, cy, cz}) {
+    c->nbr_apron_ok[ch] = c->bound_ok[ch] = true;
+    c->bound_val[ch] = 0.0f;
+  }
+  return launch_status(c, "GatherDensityVelocity");
 }
 
 int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {

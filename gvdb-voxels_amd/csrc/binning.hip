@@ -611,6 +611,14 @@ __device__ __forceinline__ int64_t atlas_index(int slot, int br, int lx, int ly,
   return (((int64_t)slot * br + lz) * br + ly) * br + lx;
 }
 
+__device__ __forceinline__ float mean_of(const int* cnt, const unsigned long long* vsum, int64_t k, int a, int acc,
+                                         float old) {
+  const double den = (double)((long long)cnt[k] + acc);
+  double v = (double)(long long)vsum[3 * k + a] * (1.0 / kVelScale);
+  if (acc) v += (double)old;
+  return den > 0 ? (float)(v / den) : 0.0f;
+}
+
 // GatherDensity without accumulation (P.cuh:696-733): val = max(0, max_p (r - |p - v|)) over
 // the points listed for the voxel's subcell.  Only points within r of a voxel can raise it
 // above 0, so each list entry visits the voxels of its own subcell inside its r-ball (a
@@ -622,19 +630,26 @@ __device__ __forceinline__ int64_t atlas_index(int slot, int br, int lx, int ly,
 // ball push of boundary subcells by the apron width yields exactly the value the neighbour
 // brick computes for g.  Apron voxels whose neighbour brick exists are written with it; the
 // others keep waiting for UpdateApron's boundary value.
-template <int kRes, int kApron, int kD, bool kWithApron>
+//
+// kVel: the velocity means of GatherVelocity (non-accumulating, k_velocity_brick's rule) are
+// written in the same pass, as the reference computes its colour channel inside the density
+// gather (P.cuh:717-731): window-cell means in LDS, three more stores per voxel.
+template <int kRes, int kApron, int kD, bool kWithApron, bool kVel>
 __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len,
                                                       const int* __restrict__ map_leaf, const int* __restrict__ off,
                                                       const float* __restrict__ list_pos,
                                                       const int* __restrict__ leaf_slot,
                                                       const gvdbx_node* __restrict__ nodes0, int apron_rt,
-                                                      const int* __restrict__ nbr, float* __restrict__ chan) {
+                                                      const int* __restrict__ nbr, float* __restrict__ chan,
+                                                      VelOut vo) {
   // LDS sized for the compile-time geometry when there is one (more bricks fit on a CU)
   constexpr int kVox = kRes ? (kRes + 2 * kApron) * (kRes + 2 * kApron) * (kRes + 2 * kApron) : kMaxPaddedVox;
   constexpr int kOff = kRes ? kD * kD * kD + 1 : 513;
   __shared__ unsigned int d2[kVox];
   __shared__ int soff[kOff];
   __shared__ int nb[27];
+  constexpr int kMeanCells = kVel ? (kRes ? (kD + 2) * (kD + 2) * (kD + 2) : kMaxWindowCells) : 1;
+  __shared__ float vmean[kMeanCells][3];
   // geometry: compile-time for the default 8^3 bricks / apron 1 / 2^3 subcells, else runtime
   const int res0 = kRes ? kRes : c.res0;
   const int apron = kRes ? kApron : apron_rt;
@@ -656,6 +671,19 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
   if (threadIdx.x < 27) nb[threadIdx.x] = (kWithApron && slot >= 0) ? nbr[(int64_t)slot * 27 + threadIdx.x] : -1;
   __syncthreads();
   const int bx = nodes0[leaf].pos[0] & c.cmask, by = nodes0[leaf].pos[1] & c.cmask, bz = nodes0[leaf].pos[2] & c.cmask;
+  if (kVel) {  // means of the window subcells (own D^3 and the one-subcell shell)
+    const int W = D + 2, W3 = W * W * W;
+    for (int w = threadIdx.x; w < W3; w += blockDim.x) {
+      const int sx = w % W - 1, sy = (w / W) % W - 1, sz = w / (W * W) - 1;
+      const int ox = sx < 0 ? -1 : (sx >= D ? 1 : 0), oy = sy < 0 ? -1 : (sy >= D ? 1 : 0),
+                oz = sz < 0 ? -1 : (sz >= D ? 1 : 0);
+      const int b = (ox | oy | oz) == 0 ? (int)m : nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)];
+      if (b < 0) continue;  // slot order equals subcell-map order
+      const int64_t k = (int64_t)b * D3 + ((sz - oz * D) * D + (sy - oy * D)) * D + (sx - ox * D);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) vmean[w][q] = mean_of(vo.cnt, vo.vsum, k, q, 0, 0.0f);
+    }
+  }
   // Voxels v with |p - v| < r lie in the integer cube [floor(p) - R + 1, floor(p) + R], R = ceil(r);
   // voxels at distance >= r cannot raise max(0, r - |p - v|), so the cube is exact.
   const int R = (int)ceilf(c.r);
@@ -739,41 +767,48 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
     const unsigned int u = d2[t];
     const float val = (missing || u == 0x7F800000u) ? 0.0f : fmaxf(0.0f, c.r - sqrtf(__uint_as_float(u)));
     // the computed region is the whole padded brick when the apron is computed here
-    bptr[a == apron ? t : ((z + apron) * br + (y + apron)) * br + (x + apron)] = val;
+    const int64_t vi = a == apron ? t : ((z + apron) * br + (y + apron)) * br + (x + apron);
+    bptr[vi] = val;
+    if (kVel) {
+      const int lgsc = kRes ? 31 - __builtin_clz(kRes / kD) : c.lgsc;
+      const int W = D + 2;
+      const int w = (((z >> lgsc) + 1) * W + ((y >> lgsc) + 1)) * W + ((x >> lgsc) + 1);
+      const int64_t gi = (int64_t)slot * br * br * br + vi;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) vo.ch[q][gi] = missing ? 0.0f : vmean[w][q];
+    }
   }
 }
 
 void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
                    const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, const int* nbr,
-                   float* chan, hipStream_t s) {
+                   float* chan, const VelOut* vel, hipStream_t s) {
   if (nmap <= 0) return;
   const int bs = brick_threads(len, nmap, 256);
   const bool fixed = c.res0 == 8 && c.D == 2 && apron == 1;
-  if (fixed && nbr)
-    hipLaunchKernelGGL((k_density_brick<8, 1, 2, true>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, off,
-                       list_pos, leaf_slot, nodes0, apron, nbr, chan);
-  else if (fixed)
-    hipLaunchKernelGGL((k_density_brick<8, 1, 2, false>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, off,
-                       list_pos, leaf_slot, nodes0, apron, nbr, chan);
-  else if (nbr)
-    hipLaunchKernelGGL((k_density_brick<0, 0, 0, true>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, off,
-                       list_pos, leaf_slot, nodes0, apron, nbr, chan);
-  else
-    hipLaunchKernelGGL((k_density_brick<0, 0, 0, false>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, off,
-                       list_pos, leaf_slot, nodes0, apron, nbr, chan);
+  const VelOut vo = vel ? *vel : VelOut{nullptr, nullptr, {nullptr, nullptr, nullptr}};
+#define GVDBX_DENSITY(R, A, D, AP, V)                                                                             \
+  hipLaunchKernelGGL((k_density_brick<R, A, D, AP, V>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, \
+                     off, list_pos, leaf_slot, nodes0, apron, nbr, chan, vo)
+  if (vel) {  // velocity means need the apron pass (a neighbour table)
+    if (fixed) GVDBX_DENSITY(8, 1, 2, true, true);
+    else GVDBX_DENSITY(0, 0, 0, true, true);
+  } else if (fixed && nbr) {
+    GVDBX_DENSITY(8, 1, 2, true, false);
+  } else if (fixed) {
+    GVDBX_DENSITY(8, 1, 2, false, false);
+  } else if (nbr) {
+    GVDBX_DENSITY(0, 0, 0, true, false);
+  } else {
+    GVDBX_DENSITY(0, 0, 0, false, false);
+  }
+#undef GVDBX_DENSITY
 }
 
 // Velocity means from the fixed-point sums: (accum ? old : 0) + sum, over (cnt + accum); the
 // colour rule of P.cuh:717-731 for float3; an empty subcell without accumulation gives 0.
 // One workgroup per brick.  Without accumulation the apron voxels whose neighbour exists get
 // the mean of the neighbour subcell holding them (slot order equals subcell-map order).
-__device__ __forceinline__ float mean_of(const int* cnt, const unsigned long long* vsum, int64_t k, int a, int acc,
-                                         float old) {
-  const double den = (double)((long long)cnt[k] + acc);
-  double v = (double)(long long)vsum[3 * k + a] * (1.0 / kVelScale);
-  if (acc) v += (double)old;
-  return den > 0 ? (float)(v / den) : 0.0f;
-}
 
 template <int kRes, int kApron, int kD>
 __global__ void __launch_bounds__(256) k_velocity_brick(SubcellCfg c, int nmap, const int* __restrict__ map_leaf,

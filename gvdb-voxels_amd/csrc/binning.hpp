@@ -53,9 +53,16 @@ void cell_bases(const BinArgs& A, int64_t nsc, hipStream_t s);
 constexpr int kMaxWindowCells = 1000;  // (D + 2)^3 for D <= 8
 constexpr int kMaxPaddedVox = 1728;  // (8 + 2*2)^3: brick-window kernels handle res0 <= 8, apron <= 2
 // nbr == nullptr: interior only; else apron voxels with a neighbour brick are written as well
+// velocity means written by the density pass (GatherDensity with velocity channels bound)
+struct VelOut {
+  const int* cnt;
+  const unsigned long long* vsum;
+  float* ch[3];
+};
+// vel: null, or the velocity means' inputs and channels (needs nbr: the apron is computed)
 void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
                    const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, const int* nbr,
-                   float* chan, hipStream_t s);
+                   float* chan, const VelOut* vel, hipStream_t s);
 void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* vsum,
                     const int* leaf_slot, int apron, const int* nbr, float* cx, float* cy, float* cz, int accum,
                     hipStream_t s);

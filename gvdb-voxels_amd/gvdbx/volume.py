@@ -184,6 +184,13 @@ class VolumeGVDB:
     def _ptr(d):
         return None if d is None else d.ptr
 
+    def GatherDensityVelocity(self, subcell, n, radius, trans, sc_len, chan, chans_vel):
+        """GatherDensity into chan and the velocity means (GatherVelocity, no accumulation) into
+        chans_vel in one pass over the bricks."""
+        cx, cy, cz = chans_vel
+        self._chk(self.L.gvdbx_gather_density_velocity(self.h, subcell, n, radius, fptr(trans), sc_len, chan, cx, cy, cz),
+                  "GatherDensityVelocity")
+
     def ScatterDensity(self, num_pnts, radius, amp, trans=(0, 0, 0), expand=True, avgColor=False, chan=0):
         """ScatterDensity (V.cpp:5625-5659): (1 - c)^3 of every bound point into its voxel (and
         its 6 face neighbours with expand) of chan; exact sums instead of the reference's race."""

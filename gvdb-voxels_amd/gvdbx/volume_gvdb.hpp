@@ -143,6 +143,15 @@ class VolumeGVDB {
   void GatherVelocity(int chanX, int chanY, int chanZ, bool bAccum = false) {
     check(gvdbx_gather_velocity(ctx_, chanX, chanY, chanZ, bAccum ? 1 : 0), "GatherVelocity", true);
   }
+  // GatherDensity with the velocity means in the same pass (channels must differ)
+  void GatherDensityVelocity(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength,
+                             int chanDensity, int chanX, int chanY, int chanZ) {
+    if (pSCPntsLength == 0) return;
+    const float t[3] = {trans.x, trans.y, trans.z};
+    check(gvdbx_gather_density_velocity(ctx_, subcell_size, num_pnts, radius, t, pSCPntsLength, chanDensity, chanX,
+                                        chanY, chanZ),
+          "GatherDensityVelocity", true);
+  }
   void ScatterReduceLevelSet(float radius, int chan, float init = 3.0f) {
     check(gvdbx_scatter_reduce_levelset(ctx_, radius, chan, init), "ScatterReduceLevelSet", true);
   }

@@ -139,6 +139,14 @@ int gvdbx_update_apron_channels(gvdbx_ctx* ctx, const int* chans, const float* b
  *      0 outside active bricks.  q: device float3[n]; out: device float[n]. */
 int gvdbx_sample_trilinear(gvdbx_ctx* ctx, int chan, const void* q, int64_t n, void* out);
 
+/* ---- GatherDensity with velocity channels bound (no reference counterpart as such: the
+ *      float3 analogue of GatherDensity's colour channel, which the reference computes in the
+ *      same kernel, P.cuh:717-731): chan as gvdbx_gather_density, (cx, cy, cz) as
+ *      gvdbx_gather_velocity without accumulation, in one pass over the bricks.  The four
+ *      channels must differ. */
+int gvdbx_gather_density_velocity(gvdbx_ctx* ctx, int sc, int64_t n, float r, const float trans[3], int64_t len,
+                                  int chan, int cx, int cy, int cz);
+
 /* ---- ScatterDensity (V.h:555, V.cpp:5625-5659; kernel gvdbScatterPointDensity P.cuh:516-573):
  *      for every bound point with q = p + trans in a leaf with corner b, p' = q - b,
  *      pi = trunc(p'): voxel pi (and, with expand, its 6 face neighbours, which may lie in the

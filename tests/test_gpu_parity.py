@@ -242,3 +242,46 @@ def test_bricks_4cubed_incremental_with_velocity(oracle_mod):
 
     v, t = U.new_volume(ld, chans=(0, 1, 2, 3)), U.new_tree(oracle_mod, ld, chans=(0, 1, 2, 3))
     U.run_pipeline(v, t, p, 1.0, 2, vel=vel, chans_vel=(1, 2, 3), steps=4, move=move)
+
+
+@pytest.mark.parametrize("ld", [(3, 3, 3, 3, 5), (2, 3, 3, 3, 5)])
+def test_density_velocity_one_pass(oracle_mod, ld):
+    """GatherDensityVelocity equals GatherDensity + GatherVelocity bit for bit (all four padded
+    channels, before and after UpdateApron) and the oracle within the tolerance, over
+    incremental SPH steps."""
+    from gvdbx import synth
+    U.log(f"case density+velocity one pass ld={ld}")
+    p, vel = synth.sph_block(n=65_536, seed=4, box=80.0)
+    t = U.new_tree(oracle_mod, ld, chans=(0, 1, 2, 3))
+    vs = [U.new_volume(ld, chans=(0, 1, 2, 3)) for _ in range(2)]
+    dvs = [(v.to_device(p), v.to_device(vel)) for v in vs]
+    cur = p
+    for step in range(4):
+        outs = []
+        for k, v in enumerate(vs):
+            dp, dv = dvs[k]
+            dp.upload(cur)
+            v.SetPoints(dp, dv, n=len(cur))
+            v.RebuildTopology(len(cur), 1.0)
+            v.UpdateAtlas()
+            ln = v.InsertPointsSubcell(4, len(cur), 1.0)
+            if k == 0:
+                v.GatherDensity(4, len(cur), 1.0, (0, 0, 0), ln, 0)
+                v.GatherVelocity(1, 2, 3)
+            else:
+                v.GatherDensityVelocity(4, len(cur), 1.0, (0, 0, 0), ln, 0, (1, 2, 3))
+            before = [v.get_channel(c).tobytes() for c in range(4)]
+            v.UpdateApron()
+            outs.append((before, [v.get_channel(c).tobytes() for c in range(4)]))
+        assert outs[0] == outs[1], f"step {step}: one pass differs from the two gathers"
+        t.rebuild_topology(cur, 1.0)
+        t.update_atlas()
+        t.insert_points_subcell(4, cur, 1.0)
+        t.gather_density(1.0, 0)
+        t.gather_velocity(vel, (1, 2, 3))
+        for c in range(4):
+            t.update_apron(c, 0.0)
+            U.compare_channel(vs[1], t, c)
+        cur = (cur + vel * np.float32(0.5)).astype(np.float32)
+    for v in vs:
+        v.close()
