@@ -360,7 +360,9 @@ KeyFmt key_fmt(const gvdbx_ctx* c, int d) {
 // Sort m compact keys in c->keys, unique them, and append the nodes per level in sorted
 // order (FindUniqueBrick V.cpp:843-873 + host allocation V.cpp:1308-1331 / 975-997).
 template <typename K>
-int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d) {
+// count_leaves >= 0: also count the flagged leaves among the first count_leaves into the used
+// count, read back with the level counts (the incremental pass's used count, V.cpp:1144-1147).
+int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, int64_t count_leaves = -1) {
   if (m <= 0) return GVDBX_OK;
   if (m > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "%lld topology keys exceed the 32-bit sort index", (long long)m);
   const int64_t ncnt = radix_count_ints(m);
@@ -372,7 +374,7 @@ int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d) {
   if (m <= kSmallSortKeys) {  // one workgroup: sort + unique + ranks + level counts
     TimedRegion tr(c, "sort");
     sort_unique_small<K>(P<K>(c->keys), m, f, d, P<K>(c->keys_alt), P<int>(c->flags), P<int>(c->rank),
-                         P<int>(c->meta) + M_UNIQUE, P<int>(c->meta) + M_LEV, c->stream);
+                         P<int>(c->meta) + M_UNIQUE, P<int>(c->meta) + M_LEV, P<int>(c->meta) + M_USED, c->stream);
     sorted = P<K>(c->keys_alt);
   } else {
     int par;
@@ -392,7 +394,9 @@ int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d) {
                    c->stream);
     level_hist<K>(sorted, P<int>(c->flags), m, f, P<int>(c->meta) + M_LEV, c->stream);
   }
+  if (count_leaves >= 0) count_leaf_flags(tree_of(c), count_leaves, P<int>(c->meta) + M_USED, c->stream);
   CHK(read_meta(c));
+  if (count_leaves >= 0) c->used0 = c->h_meta[M_USED];
   AllocPlan plan;
   memset(&plan, 0, sizeof(plan));
   int64_t acc = 0;
@@ -597,18 +601,13 @@ int incre_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool ac
   CHK(box_pass<K>(c, n, tr, r, 1.0f, d, P<int>(c->markers), &m, true));
   if (m == 0) return GVDBX_OK;  // no new node: topology unchanged (V.cpp:1066)
   apply_markers(tree_of(c), P<int>(c->markers), total0, c->stream);
-  CHK(sort_unique_alloc<K>(c, m, key_fmt(c, d), d));
+  // used count (V.cpp:1144-1147): the flagged leaves are the marked old ones (read back with the
+  // level counts) plus the new ones (sort_unique_alloc adds those); delinking clears no flag
+  CHK(sort_unique_alloc<K>(c, m, key_fmt(c, d), d, total0));
   link_all(c, d);
   DevTree T = tree_of(c);
   delink_leaves(T, c->cnt[0], c->stream);
   for (int l = 1; l < d; ++l) delink_level(T, l, c->cnt[l], c->stream);
-  // used count (V.cpp:1144-1147)
-  CHK(ensure(c, c->flags, (size_t)c->cnt[0] * 4));
-  leaf_flags(T, c->cnt[0], 0, P<int>(c->flags), c->stream);
-  CHK(zero_meta(c));
-  count_flags(P<int>(c->flags), c->cnt[0], P<int>(c->meta) + M_USED, c->stream);
-  CHK(read_meta(c));
-  c->used0 = c->h_meta[M_USED];
   return GVDBX_OK;
 }
 

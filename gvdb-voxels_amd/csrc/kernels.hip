@@ -241,7 +241,7 @@ template <typename K>
 __global__ void __launch_bounds__(1024) k_sort_unique_small(const K* __restrict__ in, int m, KeyFmt f, int depth,
                                                             K* __restrict__ out, int* __restrict__ flags,
                                                             int* __restrict__ rank, int* __restrict__ meta_unique,
-                                                            int* __restrict__ meta_lev) {
+                                                            int* __restrict__ meta_lev, int* __restrict__ meta_zero) {
   __shared__ K s[kSmallSortKeys];
   __shared__ int lev_h[kMaxLev];
   __shared__ int ws[16];
@@ -304,15 +304,18 @@ __global__ void __launch_bounds__(1024) k_sort_unique_small(const K* __restrict_
     if (i < m) rank[i] = pre;
     pre += fl[q];
   }
-  if (t == 0) *meta_unique = tot;
+  if (t == 0) {
+    *meta_unique = tot;
+    *meta_zero = 0;  // a counter the caller accumulates next
+  }
   if (t < kMaxLev) meta_lev[t] = lev_h[t];
 }
 
 template <typename K>
 void sort_unique_small(const K* in, int64_t m, KeyFmt f, int depth, K* out, int* flags, int* rank, int* meta_unique,
-                       int* meta_lev, hipStream_t s) {
+                       int* meta_lev, int* meta_zero, hipStream_t s) {
   hipLaunchKernelGGL(k_sort_unique_small<K>, dim3(1), dim3(1024), 0, s, in, (int)m, f, depth, out, flags, rank,
-                     meta_unique, meta_lev);
+                     meta_unique, meta_lev, meta_zero);
 }
 
 // Per-level unique counts (the levCnt atomics of gvdbFindUnique), LDS histogram per block.
@@ -479,6 +482,19 @@ __global__ void __launch_bounds__(256) k_count_flags(const int* __restrict__ f, 
 void count_flags(const int* flags, int64_t n, int* out, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_count_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, flags, n, out);
+}
+
+// *out (zeroed by the caller) += flagged leaves among the first n
+__global__ void __launch_bounds__(256) k_count_leaf_flags(DevTree T, int64_t n, int* out) {
+  int c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    c += T.nodes[0][i].flags != 0;
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+void count_leaf_flags(const DevTree& T, int64_t n, int* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_count_leaf_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, T, n, out);
 }
 
 // ================================================================ atlas
@@ -707,9 +723,9 @@ template void compact_regions<uint32_t>(const uint32_t*, int64_t, const int*, co
 template void compact_regions<uint64_t>(const uint64_t*, int64_t, const int*, const int*, int64_t, uint64_t*,
                                         hipStream_t);
 template void sort_unique_small<uint32_t>(const uint32_t*, int64_t, KeyFmt, int, uint32_t*, int*, int*, int*, int*,
-                                          hipStream_t);
+                                          int*, hipStream_t);
 template void sort_unique_small<uint64_t>(const uint64_t*, int64_t, KeyFmt, int, uint64_t*, int*, int*, int*, int*,
-                                          hipStream_t);
+                                          int*, hipStream_t);
 template void unique_flags<uint32_t>(const uint32_t*, int64_t, KeyFmt, int, int*, hipStream_t);
 template void unique_flags<uint64_t>(const uint64_t*, int64_t, KeyFmt, int, int*, hipStream_t);
 template void level_hist<uint32_t>(const uint32_t*, const int*, int64_t, KeyFmt, int*, hipStream_t);

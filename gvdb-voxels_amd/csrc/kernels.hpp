@@ -38,7 +38,9 @@ template <typename K> void brick_keys(const DevTree& T, const PointsView& P, Key
 constexpr int kSmallSortKeys = 4096;
 template <typename K>
 void sort_unique_small(const K* in, int64_t m, KeyFmt f, int depth, K* out, int* flags, int* rank, int* meta_unique,
-                       int* meta_lev, hipStream_t s);
+                       int* meta_lev, int* meta_zero, hipStream_t s);
+// *out (zeroed by the caller) += flagged leaves among the first n
+void count_leaf_flags(const DevTree& T, int64_t n, int* out, hipStream_t s);
 template <typename K>
 void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const int* rank, int64_t n, KeyFmt f, K* out,
                       hipStream_t s);
