@@ -362,7 +362,8 @@ KeyFmt key_fmt(const gvdbx_ctx* c, int d) {
 template <typename K>
 // count_leaves >= 0: also count the flagged leaves among the first count_leaves into the used
 // count, read back with the level counts (the incremental pass's used count, V.cpp:1144-1147).
-int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, int64_t count_leaves = -1) {
+int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, int64_t count_leaves = -1,
+                      const int* markers = nullptr) {
   if (m <= 0) return GVDBX_OK;
   if (m > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "%lld topology keys exceed the 32-bit sort index", (long long)m);
   const int64_t ncnt = radix_count_ints(m);
@@ -394,7 +395,7 @@ int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, int64_t count_le
                    c->stream);
     level_hist<K>(sorted, P<int>(c->flags), m, f, P<int>(c->meta) + M_LEV, c->stream);
   }
-  if (count_leaves >= 0) count_leaf_flags(tree_of(c), count_leaves, P<int>(c->meta) + M_USED, c->stream);
+  if (count_leaves >= 0) count_leaf_flags(tree_of(c), count_leaves, markers, P<int>(c->meta) + M_USED, c->stream);
   CHK(read_meta(c));
   if (count_leaves >= 0) c->used0 = c->h_meta[M_USED];
   AllocPlan plan;
@@ -600,10 +601,10 @@ int incre_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool ac
   int64_t m = 0;
   CHK(box_pass<K>(c, n, tr, r, 1.0f, d, P<int>(c->markers), &m, true));
   if (m == 0) return GVDBX_OK;  // no new node: topology unchanged (V.cpp:1066)
-  apply_markers(tree_of(c), P<int>(c->markers), total0, c->stream);
-  // used count (V.cpp:1144-1147): the flagged leaves are the marked old ones (read back with the
-  // level counts) plus the new ones (sort_unique_alloc adds those); delinking clears no flag
-  CHK(sort_unique_alloc<K>(c, m, key_fmt(c, d), d, total0));
+  // flags = markers, then the used count (V.cpp:1144-1147): the flagged leaves are the marked old
+  // ones (counted with the markers' application, read back with the level counts) plus the new
+  // ones (sort_unique_alloc adds those); delinking clears no flag
+  CHK(sort_unique_alloc<K>(c, m, key_fmt(c, d), d, total0, P<int>(c->markers)));
   link_all(c, d);
   DevTree T = tree_of(c);
   delink_leaves(T, c->cnt[0], c->stream);
@@ -1055,11 +1056,16 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   CHK(ensure(c, c->slot_leaf, (size_t)(total + 1) * 4));
   CHK(ensure(c, c->slot_pos, (size_t)(total + 1) * 16));
   DevTree T = tree_of(c);
-  leaf_flags(T, total, 0, P<int>(c->flags), c->stream);
-  exclusive_scan(P<int>(c->flags), P<int>(c->rank), total, P<int>(c->tile_buf), nullptr, c->stream);
   const int br = res_of(c, 0) + 2 * c->apron;
-  assign_slots(T, P<int>(c->flags), P<int>(c->rank), total, c->apron, br, c->axis_cx, c->axis_cy, P<int>(c->leaf_slot),
-               P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
+  if (total <= kAtlasSmallLeaves) {
+    assign_slots_small(T, total, c->apron, br, c->axis_cx, c->axis_cy, P<int>(c->leaf_slot), P<int>(c->slot_leaf),
+                       P<int4>(c->slot_pos), c->stream);
+  } else {
+    leaf_flags(T, total, 0, P<int>(c->flags), c->stream);
+    exclusive_scan(P<int>(c->flags), P<int>(c->rank), total, P<int>(c->tile_buf), nullptr, c->stream);
+    assign_slots(T, P<int>(c->flags), P<int>(c->rank), total, c->apron, br, c->axis_cx, c->axis_cy,
+                 P<int>(c->leaf_slot), P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
+  }
   c->nslots = c->used0;
   for (int ch = 0; ch < 32; ++ch) c->nbr_apron_ok[ch] = c->bound_ok[ch] = false;
   CHK(ensure(c, c->nbr, (size_t)(c->nslots * 27 + 1) * 4));

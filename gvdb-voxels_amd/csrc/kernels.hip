@@ -484,29 +484,34 @@ void count_flags(const int* flags, int64_t n, int* out, hipStream_t s) {
   hipLaunchKernelGGL(k_count_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, flags, n, out);
 }
 
-// *out (zeroed by the caller) += flagged leaves among the first n
-__global__ void __launch_bounds__(256) k_count_leaf_flags(DevTree T, int64_t n, int* out) {
+// *out (zeroed by the caller) += flagged leaves among the first n; with markers, each leaf's
+// flag is first set to its marker (k_apply_markers folded in)
+__global__ void __launch_bounds__(256) k_count_leaf_flags(DevTree T, int64_t n, const int* __restrict__ markers,
+                                                         int* out) {
   int c = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    c += T.nodes[0][i].flags != 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int f;
+    if (markers) {
+      f = markers[i] ? 1 : 0;
+      T.nodes[0][i].flags = (uint8_t)f;
+    } else {
+      f = T.nodes[0][i].flags != 0;
+    }
+    c += f;
+  }
   c = wave_sum(c);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
 }
-void count_leaf_flags(const DevTree& T, int64_t n, int* out, hipStream_t s) {
+void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, int* out, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_count_leaf_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, T, n, out);
+  hipLaunchKernelGGL(k_count_leaf_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, T, n, markers, out);
 }
 
 // ================================================================ atlas
 // UpdateAtlas (V.cpp:2630-2704): slot = rank of flagged leaves; mValue = reference atlas
 // corner of the slot (getAtlasPos A.cpp:705-715).  Unflagged leaves keep their stale mValue.
-__global__ void k_assign_slots(DevTree T, const int* __restrict__ flags, const int* __restrict__ rank, int64_t count,
-                               int apron, int br, int cx, int cy, int* __restrict__ leaf_slot,
-                               int* __restrict__ slot_leaf, int4* __restrict__ slot_pos) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  if (!flags[i]) { leaf_slot[i] = -1; return; }
-  const int s = rank[i];
+__device__ __forceinline__ void assign_slot(const DevTree& T, int64_t i, int s, int apron, int br, int cx, int cy,
+                                            int* leaf_slot, int* slot_leaf, int4* slot_pos) {
   gvdbx_node* nd = T.nodes[0] + i;
   const int a2 = cx * cy;
   const int z = s / a2, rem = s - z * a2, y = rem / cx, x = rem - y * cx;
@@ -517,6 +522,68 @@ __global__ void k_assign_slots(DevTree T, const int* __restrict__ flags, const i
   slot_leaf[s] = (int)i;
   slot_pos[s] = make_int4(nd->pos[0], nd->pos[1], nd->pos[2], (int)i);
 }
+
+__global__ void k_assign_slots(DevTree T, const int* __restrict__ flags, const int* __restrict__ rank, int64_t count,
+                               int apron, int br, int cx, int cy, int* __restrict__ leaf_slot,
+                               int* __restrict__ slot_leaf, int4* __restrict__ slot_pos) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  if (!flags[i]) { leaf_slot[i] = -1; return; }
+  assign_slot(T, i, rank[i], apron, br, cx, cy, leaf_slot, slot_leaf, slot_pos);
+}
+
+// The same for up to kAtlasSmallLeaves leaves in one workgroup: the leaf flags, their exclusive
+// scan and the slot writes in one launch (4,096 leaves per round, the carry across rounds).
+__global__ void __launch_bounds__(1024) k_assign_slots_small(DevTree T, int64_t count, int apron, int br, int cx,
+                                                            int cy, int* __restrict__ leaf_slot,
+                                                            int* __restrict__ slot_leaf,
+                                                            int4* __restrict__ slot_pos) {
+  __shared__ int ws[16];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  int carry = 0;
+  for (int64_t base = 0; base < count; base += 4096) {
+    int fl[4], sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t i = base + 4 * t + q;
+      fl[q] = (i < count && T.nodes[0][i].flags) ? 1 : 0;
+      sum += fl[q];
+    }
+    int x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[wave] = x;
+    __syncthreads();
+    int pre = carry, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      pre += w < wave ? ws[w] : 0;
+      tot += ws[w];
+    }
+    pre += x - sum;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t i = base + 4 * t + q;
+      if (i < count) {
+        if (fl[q]) assign_slot(T, i, pre, apron, br, cx, cy, leaf_slot, slot_leaf, slot_pos);
+        else leaf_slot[i] = -1;
+      }
+      pre += fl[q];
+    }
+    carry += tot;
+    __syncthreads();  // ws is rewritten next round
+  }
+}
+void assign_slots_small(const DevTree& T, int64_t count, int apron, int br, int cx, int cy, int* leaf_slot,
+                        int* slot_leaf, int4* slot_pos, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_assign_slots_small, dim3(1), dim3(1024), 0, s, T, count, apron, br, cx, cy, leaf_slot, slot_leaf,
+                     slot_pos);
+}
+
 void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t count, int apron, int br, int cx, int cy,
                   int* leaf_slot, int* slot_leaf, int4* slot_pos, hipStream_t s) {
   if (count <= 0) return;

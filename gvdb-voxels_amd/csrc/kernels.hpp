@@ -39,8 +39,13 @@ constexpr int kSmallSortKeys = 4096;
 template <typename K>
 void sort_unique_small(const K* in, int64_t m, KeyFmt f, int depth, K* out, int* flags, int* rank, int* meta_unique,
                        int* meta_lev, int* meta_zero, hipStream_t s);
-// *out (zeroed by the caller) += flagged leaves among the first n
-void count_leaf_flags(const DevTree& T, int64_t n, int* out, hipStream_t s);
+// *out (zeroed by the caller) += flagged leaves among the first n (markers: set each flag to
+// its marker first)
+void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, int* out, hipStream_t s);
+// UpdateAtlas's flags + scan + slot writes in one workgroup, for up to kAtlasSmallLeaves leaves
+constexpr int64_t kAtlasSmallLeaves = 65536;
+void assign_slots_small(const DevTree& T, int64_t count, int apron, int br, int cx, int cy, int* leaf_slot,
+                        int* slot_leaf, int4* slot_pos, hipStream_t s);
 template <typename K>
 void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const int* rank, int64_t n, KeyFmt f, K* out,
                       hipStream_t s);
