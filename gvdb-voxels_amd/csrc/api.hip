@@ -90,6 +90,7 @@ struct gvdbx_ctx {
   bool have_vsum = false;
   Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
   bool sc_map_current = false;  // sc_map holds the scan map (the insert may have used the slots)
+  bool spec_sorted = false;     // incre_launch also sorted the box keys (see there)
   Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb, nbm, spos, svel;
   Buf sp_cnt, sp_start, sp_keyrank, sp_payload, sp_perm;  // splat: points counting-sorted by (brick, region)
   Buf scatter_acc;  // ScatterDensity: 64-bit fixed-point sums per padded voxel
@@ -357,11 +358,14 @@ KeyFmt key_fmt(const gvdbx_ctx* c, int d) {
   return f;
 }
 
+template <typename K>
+int alloc_unique(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, const K* sorted, bool with_used);
+
 // Sort m compact keys in c->keys, unique them, and append the nodes per level in sorted
 // order (FindUniqueBrick V.cpp:843-873 + host allocation V.cpp:1308-1331 / 975-997).
-template <typename K>
 // count_leaves >= 0: also count the flagged leaves among the first count_leaves into the used
 // count, read back with the level counts (the incremental pass's used count, V.cpp:1144-1147).
+template <typename K>
 int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, int64_t count_leaves = -1,
                       const int* markers = nullptr) {
   if (m <= 0) return GVDBX_OK;
@@ -375,7 +379,8 @@ int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, int64_t count_le
   if (m <= kSmallSortKeys) {  // one workgroup: sort + unique + ranks + level counts
     TimedRegion tr(c, "sort");
     sort_unique_small<K>(P<K>(c->keys), m, f, d, P<K>(c->keys_alt), P<int>(c->flags), P<int>(c->rank),
-                         P<int>(c->meta) + M_UNIQUE, P<int>(c->meta) + M_LEV, P<int>(c->meta) + M_USED, c->stream);
+                         P<int>(c->meta) + M_UNIQUE, P<int>(c->meta) + M_LEV, P<int>(c->meta) + M_USED, nullptr,
+                         c->stream);
     sorted = P<K>(c->keys_alt);
   } else {
     int par;
@@ -395,9 +400,17 @@ int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, int64_t count_le
                    c->stream);
     level_hist<K>(sorted, P<int>(c->flags), m, f, P<int>(c->meta) + M_LEV, c->stream);
   }
-  if (count_leaves >= 0) count_leaf_flags(tree_of(c), count_leaves, markers, P<int>(c->meta) + M_USED, c->stream);
+  if (count_leaves >= 0)
+    count_leaf_flags(tree_of(c), count_leaves, markers, nullptr, P<int>(c->meta) + M_USED, c->stream);
   CHK(read_meta(c));
-  if (count_leaves >= 0) c->used0 = c->h_meta[M_USED];
+  return alloc_unique<K>(c, m, f, d, sorted, count_leaves >= 0);
+}
+
+// The allocation half: per-level counts from the read-back meta, nodes appended in sorted order
+// (with_used: the used count was read back too).
+template <typename K>
+int alloc_unique(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, const K* sorted, bool with_used) {
+  if (with_used) c->used0 = c->h_meta[M_USED];
   AllocPlan plan;
   memset(&plan, 0, sizeof(plan));
   int64_t acc = 0;
@@ -552,7 +565,7 @@ int box_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, float hi_mul, in
     if (total + m > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "more than 2^31 box-pass keys");
     CHK(ensure_keep(c, c->keys, (size_t)(total + m + 1) * sizeof(K), (size_t)total * sizeof(K)));
     compact_regions<K>(P<K>(c->box_region), 256 * g.per_point, P<int>(c->box_cnt), P<int>(c->box_off),
-                       (cnt + 255) / 256, P<K>(c->keys) + total, c->stream);
+                       (cnt + 255) / 256, P<K>(c->keys) + total, INT64_MAX, c->stream);
     total += m;
   }
   CHK(ensure(c, c->keys_alt, (size_t)(total > 0 ? total : 1) * sizeof(K)));
@@ -584,7 +597,26 @@ int incre_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool 
     fill_u32(P<uint32_t>(c->markers), total0, 0u, c->stream);
   BoxGeom g;
   CHK(box_geom(c, n, r, 1.0f, d, sizeof(K), &g));
+  c->spec_sorted = false;
   if (n > 0) CHK(box_launch<K>(c, 0, g.chunk, tr, r, 1.0f, d, P<int>(c->markers), g));
+  if (n > 0 && g.chunk >= n) {
+    // One box launch covers the points: compact its keys, sort and unique them in one workgroup
+    // and apply the markers with the used count, all reading the key count on the device, so
+    // the read-back that returns it also returns the level and used counts.  A count of 0 or
+    // above the one-workgroup limit leaves this to the host path (markers applied only if > 0).
+    CHK(ensure(c, c->keys, (size_t)kSmallSortKeys * sizeof(K)));
+    CHK(ensure(c, c->keys_alt, (size_t)kSmallSortKeys * sizeof(K)));
+    CHK(ensure(c, c->flags, (size_t)kSmallSortKeys * 4));
+    CHK(ensure(c, c->rank, (size_t)kSmallSortKeys * 4));
+    const int* mdev = P<int>(c->meta) + M_COUNT;
+    compact_regions<K>(P<K>(c->box_region), 256 * g.per_point, P<int>(c->box_cnt), P<int>(c->box_off),
+                       (n + 255) / 256, P<K>(c->keys), kSmallSortKeys, c->stream);
+    sort_unique_small<K>(P<K>(c->keys), 0, key_fmt(c, d), d, P<K>(c->keys_alt), P<int>(c->flags), P<int>(c->rank),
+                         P<int>(c->meta) + M_UNIQUE, P<int>(c->meta) + M_LEV, P<int>(c->meta) + M_USED, mdev,
+                         c->stream);
+    count_leaf_flags(T, total0, P<int>(c->markers), mdev, P<int>(c->meta) + M_USED, c->stream);
+    c->spec_sorted = true;
+  }
   return GVDBX_OK;
 }
 
@@ -599,6 +631,19 @@ int incre_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool ac
     CHK(read_meta(c));
   }
   int64_t m = 0;
+  if (c->spec_sorted) {
+    c->spec_sorted = false;
+    m = (uint32_t)c->h_meta[M_COUNT];
+    if (m == 0) return GVDBX_OK;  // no new node: topology unchanged (V.cpp:1066)
+    if (c->h_meta[M_UNIQUE] >= 0) {  // sorted, applied and counted on the device already
+      CHK(alloc_unique<K>(c, m, key_fmt(c, d), d, P<K>(c->keys_alt), true));
+      link_all(c, d);
+      DevTree T = tree_of(c);
+      delink_leaves(T, c->cnt[0], c->stream);
+      for (int l = 1; l < d; ++l) delink_level(T, l, c->cnt[l], c->stream);
+      return GVDBX_OK;
+    }
+  }
   CHK(box_pass<K>(c, n, tr, r, 1.0f, d, P<int>(c->markers), &m, true));
   if (m == 0) return GVDBX_OK;  // no new node: topology unchanged (V.cpp:1066)
   // flags = markers, then the used count (V.cpp:1144-1147): the flagged leaves are the marked old
@@ -1057,15 +1102,10 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   CHK(ensure(c, c->slot_pos, (size_t)(total + 1) * 16));
   DevTree T = tree_of(c);
   const int br = res_of(c, 0) + 2 * c->apron;
-  if (total <= kAtlasSmallLeaves) {
-    assign_slots_small(T, total, c->apron, br, c->axis_cx, c->axis_cy, P<int>(c->leaf_slot), P<int>(c->slot_leaf),
-                       P<int4>(c->slot_pos), c->stream);
-  } else {
-    leaf_flags(T, total, 0, P<int>(c->flags), c->stream);
-    exclusive_scan(P<int>(c->flags), P<int>(c->rank), total, P<int>(c->tile_buf), nullptr, c->stream);
-    assign_slots(T, P<int>(c->flags), P<int>(c->rank), total, c->apron, br, c->axis_cx, c->axis_cy,
-                 P<int>(c->leaf_slot), P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
-  }
+  leaf_flags(T, total, 0, P<int>(c->flags), c->stream);
+  exclusive_scan(P<int>(c->flags), P<int>(c->rank), total, P<int>(c->tile_buf), nullptr, c->stream);
+  assign_slots(T, P<int>(c->flags), P<int>(c->rank), total, c->apron, br, c->axis_cx, c->axis_cy,
+               P<int>(c->leaf_slot), P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
   c->nslots = c->used0;
   for (int ch = 0; ch < 32; ++ch) c->nbr_apron_ok[ch] = c->bound_ok[ch] = false;
   CHK(ensure(c, c->nbr, (size_t)(c->nslots * 27 + 1) * 4));

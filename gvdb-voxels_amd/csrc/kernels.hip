@@ -203,19 +203,21 @@ void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi
 template <typename K>
 __global__ void __launch_bounds__(256) k_compact_regions(const K* __restrict__ region, int64_t cap_b,
                                                          const int* __restrict__ blockcnt,
-                                                         const int* __restrict__ blockoff, K* __restrict__ out) {
+                                                         const int* __restrict__ blockoff, K* __restrict__ out,
+                                                         int64_t out_cap) {
   const int64_t b = blockIdx.x;
   const int n = blockcnt[b];
   const int64_t o = blockoff[b];
-  for (int j = threadIdx.x; j < n; j += blockDim.x) out[o + j] = region[b * cap_b + j];
+  for (int j = threadIdx.x; j < n; j += blockDim.x)
+    if (o + j < out_cap) out[o + j] = region[b * cap_b + j];
 }
 
 template <typename K>
 void compact_regions(const K* region, int64_t cap_b, const int* blockcnt, const int* blockoff, int64_t nblocks, K* out,
-                     hipStream_t s) {
+                     int64_t out_cap, hipStream_t s) {
   if (nblocks <= 0) return;
   hipLaunchKernelGGL(k_compact_regions<K>, dim3((unsigned)nblocks), dim3(256), 0, s, region, cap_b, blockcnt,
-                     blockoff, out);
+                     blockoff, out, out_cap);
 }
 
 // gvdbFindUnique (P.cuh:336-350): run starts of the sorted keys, invalid level excluded.
@@ -241,11 +243,20 @@ template <typename K>
 __global__ void __launch_bounds__(1024) k_sort_unique_small(const K* __restrict__ in, int m, KeyFmt f, int depth,
                                                             K* __restrict__ out, int* __restrict__ flags,
                                                             int* __restrict__ rank, int* __restrict__ meta_unique,
-                                                            int* __restrict__ meta_lev, int* __restrict__ meta_zero) {
+                                                            int* __restrict__ meta_lev, int* __restrict__ meta_zero,
+                                                            const int* __restrict__ mdev) {
   __shared__ K s[kSmallSortKeys];
   __shared__ int lev_h[kMaxLev];
   __shared__ int ws[16];
   const int t = threadIdx.x;
+  if (mdev) m = *mdev;  // the key count a kernel before wrote (uniform over the workgroup)
+  if (m <= 0 || m > kSmallSortKeys) {  // nothing sorted: the caller sees meta_unique = -1
+    if (t == 0) {
+      *meta_unique = -1;
+      *meta_zero = 0;
+    }
+    return;
+  }
   int np = 2;
   while (np < m) np <<= 1;
   for (int i = t; i < np; i += 1024) s[i] = i < m ? in[i] : ~(K)0;
@@ -313,9 +324,9 @@ __global__ void __launch_bounds__(1024) k_sort_unique_small(const K* __restrict_
 
 template <typename K>
 void sort_unique_small(const K* in, int64_t m, KeyFmt f, int depth, K* out, int* flags, int* rank, int* meta_unique,
-                       int* meta_lev, int* meta_zero, hipStream_t s) {
+                       int* meta_lev, int* meta_zero, const int* mdev, hipStream_t s) {
   hipLaunchKernelGGL(k_sort_unique_small<K>, dim3(1), dim3(1024), 0, s, in, (int)m, f, depth, out, flags, rank,
-                     meta_unique, meta_lev, meta_zero);
+                     meta_unique, meta_lev, meta_zero, mdev);
 }
 
 // Per-level unique counts (the levCnt atomics of gvdbFindUnique), LDS histogram per block.
@@ -487,11 +498,14 @@ void count_flags(const int* flags, int64_t n, int* out, hipStream_t s) {
 // *out (zeroed by the caller) += flagged leaves among the first n; with markers, each leaf's
 // flag is first set to its marker (k_apply_markers folded in)
 __global__ void __launch_bounds__(256) k_count_leaf_flags(DevTree T, int64_t n, const int* __restrict__ markers,
-                                                         int* out) {
+                                                         const int* __restrict__ mcount, int* out) {
+  // mcount: apply the markers only when the box pass found new keys (without any the step
+  // leaves the topology, flags included, as it was: V.cpp:1066)
+  const bool apply = markers != nullptr && (mcount == nullptr || *mcount > 0);
   int c = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     int f;
-    if (markers) {
+    if (apply) {
       f = markers[i] ? 1 : 0;
       T.nodes[0][i].flags = (uint8_t)f;
     } else {
@@ -502,9 +516,9 @@ __global__ void __launch_bounds__(256) k_count_leaf_flags(DevTree T, int64_t n, 
   c = wave_sum(c);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
 }
-void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, int* out, hipStream_t s) {
+void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, const int* mcount, int* out, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_count_leaf_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, T, n, markers, out);
+  hipLaunchKernelGGL(k_count_leaf_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, T, n, markers, mcount, out);
 }
 
 // ================================================================ atlas
@@ -530,58 +544,6 @@ __global__ void k_assign_slots(DevTree T, const int* __restrict__ flags, const i
   if (i >= count) return;
   if (!flags[i]) { leaf_slot[i] = -1; return; }
   assign_slot(T, i, rank[i], apron, br, cx, cy, leaf_slot, slot_leaf, slot_pos);
-}
-
-// The same for up to kAtlasSmallLeaves leaves in one workgroup: the leaf flags, their exclusive
-// scan and the slot writes in one launch (4,096 leaves per round, the carry across rounds).
-__global__ void __launch_bounds__(1024) k_assign_slots_small(DevTree T, int64_t count, int apron, int br, int cx,
-                                                            int cy, int* __restrict__ leaf_slot,
-                                                            int* __restrict__ slot_leaf,
-                                                            int4* __restrict__ slot_pos) {
-  __shared__ int ws[16];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  int carry = 0;
-  for (int64_t base = 0; base < count; base += 4096) {
-    int fl[4], sum = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int64_t i = base + 4 * t + q;
-      fl[q] = (i < count && T.nodes[0][i].flags) ? 1 : 0;
-      sum += fl[q];
-    }
-    int x = sum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) ws[wave] = x;
-    __syncthreads();
-    int pre = carry, tot = 0;
-#pragma unroll
-    for (int w = 0; w < 16; ++w) {
-      pre += w < wave ? ws[w] : 0;
-      tot += ws[w];
-    }
-    pre += x - sum;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int64_t i = base + 4 * t + q;
-      if (i < count) {
-        if (fl[q]) assign_slot(T, i, pre, apron, br, cx, cy, leaf_slot, slot_leaf, slot_pos);
-        else leaf_slot[i] = -1;
-      }
-      pre += fl[q];
-    }
-    carry += tot;
-    __syncthreads();  // ws is rewritten next round
-  }
-}
-void assign_slots_small(const DevTree& T, int64_t count, int apron, int br, int cx, int cy, int* leaf_slot,
-                        int* slot_leaf, int4* slot_pos, hipStream_t s) {
-  if (count <= 0) return;
-  hipLaunchKernelGGL(k_assign_slots_small, dim3(1), dim3(1024), 0, s, T, count, apron, br, cx, cy, leaf_slot, slot_leaf,
-                     slot_pos);
 }
 
 void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t count, int apron, int br, int cx, int cy,
@@ -786,13 +748,13 @@ template void box_keys<uint32_t>(const DevTree&, const PointsView&, KeyFmt, floa
 template void box_keys<uint64_t>(const DevTree&, const PointsView&, KeyFmt, float, float, uint64_t*, int64_t, int*,
                                  int*, hipStream_t);
 template void compact_regions<uint32_t>(const uint32_t*, int64_t, const int*, const int*, int64_t, uint32_t*,
-                                        hipStream_t);
+                                        int64_t, hipStream_t);
 template void compact_regions<uint64_t>(const uint64_t*, int64_t, const int*, const int*, int64_t, uint64_t*,
-                                        hipStream_t);
+                                        int64_t, hipStream_t);
 template void sort_unique_small<uint32_t>(const uint32_t*, int64_t, KeyFmt, int, uint32_t*, int*, int*, int*, int*,
-                                          int*, hipStream_t);
+                                          int*, const int*, hipStream_t);
 template void sort_unique_small<uint64_t>(const uint64_t*, int64_t, KeyFmt, int, uint64_t*, int*, int*, int*, int*,
-                                          int*, hipStream_t);
+                                          int*, const int*, hipStream_t);
 template void unique_flags<uint32_t>(const uint32_t*, int64_t, KeyFmt, int, int*, hipStream_t);
 template void unique_flags<uint64_t>(const uint64_t*, int64_t, KeyFmt, int, int*, hipStream_t);
 template void level_hist<uint32_t>(const uint32_t*, const int*, int64_t, KeyFmt, int*, hipStream_t);

@@ -37,15 +37,13 @@ template <typename K> void brick_keys(const DevTree& T, const PointsView& P, Key
 // up to this many topology keys are sorted and uniqued by one workgroup
 constexpr int kSmallSortKeys = 4096;
 template <typename K>
+// mdev: read m from device memory instead (meta_unique = -1 when it is 0 or above the limit)
 void sort_unique_small(const K* in, int64_t m, KeyFmt f, int depth, K* out, int* flags, int* rank, int* meta_unique,
-                       int* meta_lev, int* meta_zero, hipStream_t s);
+                       int* meta_lev, int* meta_zero, const int* mdev, hipStream_t s);
 // *out (zeroed by the caller) += flagged leaves among the first n (markers: set each flag to
 // its marker first)
-void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, int* out, hipStream_t s);
-// UpdateAtlas's flags + scan + slot writes in one workgroup, for up to kAtlasSmallLeaves leaves
-constexpr int64_t kAtlasSmallLeaves = 65536;
-void assign_slots_small(const DevTree& T, int64_t count, int apron, int br, int cx, int cy, int* leaf_slot,
-                        int* slot_leaf, int4* slot_pos, hipStream_t s);
+// (mcount: apply the markers only when *mcount > 0)
+void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, const int* mcount, int* out, hipStream_t s);
 template <typename K>
 void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const int* rank, int64_t n, KeyFmt f, K* out,
                       hipStream_t s);
@@ -57,7 +55,7 @@ void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi
               int* blockcnt, int* markers, hipStream_t s);
 template <typename K>
 void compact_regions(const K* region, int64_t cap_b, const int* blockcnt, const int* blockoff, int64_t nblocks, K* out,
-                     hipStream_t s);
+                     int64_t out_cap, hipStream_t s);
 template <typename K> void unique_flags(const K* k, int64_t m, KeyFmt f, int depth, int* flags, hipStream_t s);
 template <typename K>
 void level_hist(const K* k, const int* flags, int64_t m, KeyFmt f, int* lev_cnt, hipStream_t s);
