@@ -53,6 +53,7 @@ struct gvdbx_ctx {
   int axis_cx = 16, axis_cy = 16, axis_cz = 1;
   int apron = -1;
   bool chan_on[32] = {};
+  int chan_type[32] = {};  // GVDBX_T_FLOAT or GVDBX_T_UCHAR4 (32-bit words either way)
   // apron voxels that have a neighbour brick hold current copies of its interior (set by the
   // gathers that write the apron and by full apron passes; cleared by every other writer)
   bool nbr_apron_ok[32] = {};
@@ -86,6 +87,7 @@ struct gvdbx_ctx {
   int sc = 0, scdim = 0;
   int64_t nsc = 0, sclen = 0;
   bool lists_have_vel = false;
+  bool lists_have_clr = false;  // the insert stored the points' colours (AUX_SUBCELL_PNT_CLR)
   bool store_vel_lists = false;
   bool have_vsum = false;
   Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
@@ -95,6 +97,7 @@ struct gvdbx_ctx {
   Buf sp_cnt, sp_start, sp_keyrank, sp_payload, sp_perm;  // splat: points counting-sorted by (brick, region)
   Buf scatter_acc;  // ScatterDensity: 64-bit fixed-point sums per padded voxel
   Buf wvs;          // insert: per-brick window velocity sums
+  Buf list_clr, csum, cmean;  // subcell list colours; per subcell colour sums and means
   // brick order of the last splat (a permutation of its n points), reused by the splat
   // backward on the same point buffer for locality
   const void* sp_order_q = nullptr;
@@ -224,6 +227,9 @@ PointsView points_of(gvdbx_ctx* c, int64_t n, const float* tr) {
   v.vel = c->vel;
   v.vel_off = c->vel_off;
   v.vel_stride = c->vel_stride;
+  v.clr = c->clr;
+  v.clr_off = c->clr_off;
+  v.clr_stride = c->clr_stride;
   v.n = n;
   v.tx = tr ? tr[0] : 0.f;
   v.ty = tr ? tr[1] : 0.f;
@@ -735,10 +741,39 @@ int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accum
                                 : incre_pass<uint32_t>(c, n, orig, r, c->depth, false, spec);
 }
 
-int chan_ok(gvdbx_ctx* c, int chan) {
+int chan_any(gvdbx_ctx* c, int chan) {
   if (!c) return GVDBX_ERR_ARG;
   if (chan < 0 || chan >= 32 || !c->chan_on[chan]) return fail(c, GVDBX_ERR_ARG, "channel %d does not exist", chan);
   return GVDBX_OK;
+}
+
+// a float channel (every field operation but the colour means)
+int chan_ok(gvdbx_ctx* c, int chan) {
+  CHK(chan_any(c, chan));
+  if (c->chan_type[chan] != GVDBX_T_FLOAT)
+    return fail(c, GVDBX_ERR_ARG, "channel %d holds uchar4 colours, not floats", chan);
+  return GVDBX_OK;
+}
+
+int chan_clr_ok(gvdbx_ctx* c, int chan) {
+  CHK(chan_any(c, chan));
+  if (c->chan_type[chan] != GVDBX_T_UCHAR4)
+    return fail(c, GVDBX_ERR_ARG, "colour channel %d is not a uchar4 channel", chan);
+  return GVDBX_OK;
+}
+
+// float -> uchar as the colour macros convert (uchar(x)): truncation, saturated to [0, 255]
+// (NaN -> 0)
+uint32_t sat_u8(float x) { return x >= 255.0f ? 255u : (x >= 0.0f ? (uint32_t)x : 0u); }
+
+// The 32-bit word a boundary or fill value stands for in the channel: the float itself, or for
+// uchar4 channels make_uchar4(v, v, v, v) (gvdbUpdateApronC4, O.cuh:143-146).
+float bound_word(const gvdbx_ctx* c, int chan, float v) {
+  if (c->chan_type[chan] != GVDBX_T_UCHAR4) return v;
+  const uint32_t u = sat_u8(v) * 0x01010101u;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
 }
 
 int64_t brick_floats(const gvdbx_ctx* c) {
@@ -848,6 +883,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
                  &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta, &c->box_region, &c->box_cnt,
                  &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->scatter_acc, &c->wvs,
+                 &c->list_clr, &c->csum, &c->cmean,
                  &c->cloud_id,
                  &c->cloud_off_dev};
   for (Buf* b : bufs)
@@ -919,13 +955,17 @@ int gvdbx_set_channel_default(gvdbx_ctx* c, int cx, int cy, int cz) {  // SetCha
 
 int gvdbx_add_channel(gvdbx_ctx* c, int chan, int dtype, int apron) {  // AddChannel (V.cpp:2437-2453)
   if (!c || !c->configured) return fail(c, GVDBX_ERR_ARG, "context not configured");
-  if (chan < 0 || chan >= 32 || dtype != GVDBX_T_FLOAT || apron < 0 || apron > res_of(c, 0))
+  if (chan < 0 || chan >= 32 || (dtype != GVDBX_T_FLOAT && dtype != GVDBX_T_UCHAR4) || apron < 0 ||
+      apron > res_of(c, 0))
     return fail(c, GVDBX_ERR_ARG, "unsupported channel (chan %d dtype %d apron %d)", chan, dtype, apron);
+  if (c->chan_on[chan] && c->chan_type[chan] != dtype)
+    return fail(c, GVDBX_ERR_ARG, "channel %d exists with another type", chan);
   if (c->apron >= 0 && apron != c->apron)
     return fail(c, GVDBX_ERR_ARG, "all channels share one apron width (%d)", c->apron);
   c->apron = apron;
   if (!c->chan_on[chan]) {
     c->chan_on[chan] = true;
+    c->chan_type[chan] = dtype;
     if (c->chan_slot_cap > 0) {
       const size_t nb = (size_t)(c->chan_slot_cap * brick_floats(c)) * 4;
       CHK(ensure(c, c->chan[chan], nb));
@@ -935,13 +975,29 @@ int gvdbx_add_channel(gvdbx_ctx* c, int chan, int dtype, int apron) {  // AddCha
   return GVDBX_OK;
 }
 
-int gvdbx_fill_channel(gvdbx_ctx* c, int chan, float v) {  // FillChannel / ClearChannel
-  CHK(chan_ok(c, chan));
-  if (c->chan_slot_cap > 0) fill_f32(P<float>(c->chan[chan]), c->chan_slot_cap * brick_floats(c), v, c->stream);
+// FillChannel (V.cpp:2456-2471): all-zero values clear the channel; a float channel takes
+// val[0]; a uchar4 channel takes CLR2INT(val[0], val[1], val[2], 1) (gvdbOpFillC4, O.cuh:368-373).
+int gvdbx_fill_channel4(gvdbx_ctx* c, int chan, const float val[4]) {
+  CHK(chan_any(c, chan));
+  if (!val) return fail(c, GVDBX_ERR_ARG, "fill value is null");
+  float w = val[0];
+  if (c->chan_type[chan] == GVDBX_T_UCHAR4) {
+    const bool zero = val[0] == 0.0f && val[1] == 0.0f && val[2] == 0.0f && val[3] == 0.0f;
+    const uint32_t u = zero ? 0u
+                            : sat_u8(val[0] * 255.0f) | sat_u8(val[1] * 255.0f) << 8 |
+                                  sat_u8(val[2] * 255.0f) << 16 | 255u << 24;
+    memcpy(&w, &u, 4);
+  }
+  if (c->chan_slot_cap > 0) fill_f32(P<float>(c->chan[chan]), c->chan_slot_cap * brick_floats(c), w, c->stream);
   c->nbr_apron_ok[chan] = true;
   c->bound_ok[chan] = true;
-  c->bound_val[chan] = v;
+  c->bound_val[chan] = w;
   return launch_status(c, "FillChannel");
+}
+
+int gvdbx_fill_channel(gvdbx_ctx* c, int chan, float v) {  // FillChannel / ClearChannel
+  const float val[4] = {v, 0.0f, 0.0f, 0.0f};
+  return gvdbx_fill_channel4(c, chan, val);
 }
 
 int gvdbx_set_points(gvdbx_ctx* c, int64_t n, const void* pos, int pos_off, int pos_stride, const void* vel,
@@ -950,6 +1006,8 @@ int gvdbx_set_points(gvdbx_ctx* c, int64_t n, const void* pos, int pos_off, int 
     return fail(c, GVDBX_ERR_ARG, "bad position buffer");
   if (vel && (vel_stride < 12 || vel_off < 0 || (vel_off & 3) || (vel_stride & 3)))
     return fail(c, GVDBX_ERR_ARG, "bad velocity buffer");
+  if (clr && (clr_stride < 4 || clr_off < 0 || (clr_off & 3) || (clr_stride & 3)))
+    return fail(c, GVDBX_ERR_ARG, "bad colour buffer");
   if (c->batch) {  // leaving the batched layout: the next rebuild starts over
     c->batch = false;
     c->ncloud = 0;
@@ -1221,6 +1279,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   A.nbm = P<int>(c->nbm);
   A.list_pos = nullptr;
   A.list_vel = nullptr;
+  A.list_clr = nullptr;
   A.vsum = with_vel ? P<unsigned long long>(c->vsum) : nullptr;
   if (with_vel) CHK(ensure(c, c->wvs, (size_t)((int64_t)nmap * W * W * W * 3 + 1) * 8));
   A.wvs = with_vel ? P<unsigned long long>(c->wvs) : nullptr;
@@ -1239,6 +1298,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   c->scdim = cfg.D;
   c->nsc = nsc;
   c->lists_have_vel = with_vel && c->store_vel_lists;
+  c->lists_have_clr = c->clr != nullptr;  // gvdbInsertSubcell stores colours when bound (P.cuh:268)
   c->have_vsum = with_vel;
   // Lists sized for the largest possible length when that bound is modest (and below 2^31, so
   // the 32-bit offsets cannot wrap), so the insert is enqueued before the single length
@@ -1252,6 +1312,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   if (early) {
     CHK(ensure(c, c->list_pos, (size_t)(bound + 1) * 12));
     if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(bound + 1) * 12));
+    if (c->lists_have_clr) CHK(ensure(c, c->list_clr, (size_t)(bound + 1) * 4));
   } else {
     const bool wide = bound >= ((int64_t)1 << 32);
     if (wide) sum_wide(P<int>(c->sc_cnt), nsc, (unsigned long long*)(P<int>(c->meta) + M_WIDE), c->stream);
@@ -1266,9 +1327,11 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
     if (ln > INT32_MAX || ln > bound) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);
     CHK(ensure(c, c->list_pos, (size_t)(ln + 1) * 12));
     if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(ln + 1) * 12));
+    if (c->lists_have_clr) CHK(ensure(c, c->list_clr, (size_t)(ln + 1) * 4));
   }
   A.list_pos = P<float>(c->list_pos);
   A.list_vel = c->lists_have_vel ? P<float>(c->list_vel) : nullptr;
+  A.list_clr = c->lists_have_clr ? P<uint32_t>(c->list_clr) : nullptr;
   if (early) CHK(read_meta_begin(c));  // the length is final after the scan
   {
     TimedRegion tr(c, "insert");
@@ -1317,20 +1380,46 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
   return launch_status(c, "Gather");
 }
 
-int gvdbx_gather_density(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t len, int chan,
-                         int chan_clr, int accum) {
-  (void)n; (void)trans; (void)len; (void)chan_clr;
+// The colour part of GatherDensity / GatherLevelSet (P.cuh:713-731): runs when the insert stored
+// colours (a colour attribute was bound) and a colour channel is given.
+static int gather_colour(gvdbx_ctx* c, int chan_clr, int accum) {
+  if (chan_clr < 0 || !c->lists_have_clr || c->nsc == 0) return GVDBX_OK;
+  const SubcellCfg cfg = subcell_cfg(c, c->sc, 0.f);
+  TimedRegion tr(c, "colour");
+  CHK(ensure(c, c->csum, (size_t)(c->nsc * 4 + 1) * 8));
+  CHK(ensure(c, c->cmean, (size_t)(c->nsc + 1) * 4));
+  colour_sums(c->nsc, P<int>(c->sc_off), P<int>(c->sc_cnt), P<uint32_t>(c->list_clr),
+              P<unsigned long long>(c->csum), P<uint32_t>(c->cmean), c->stream);
+  const bool with_apron = c->apron > 0 && !accum;
+  colour_brick(cfg, (int)c->used0, P<int>(c->map_leaf), P<int>(c->sc_cnt), P<unsigned long long>(c->csum),
+               P<uint32_t>(c->cmean), P<int>(c->leaf_slot), c->apron, with_apron ? P<int>(c->nbr) : nullptr,
+               P<uint32_t>(c->chan[chan_clr]), accum, c->stream);
+  c->nbr_apron_ok[chan_clr] = c->bound_ok[chan_clr] = with_apron;  // 0 where no neighbour exists
+  c->bound_val[chan_clr] = 0.0f;
+  return launch_status(c, "Gather colour");
+}
+
+static int gather_with_colour(gvdbx_ctx* c, int sc, float r, int chan, int chan_clr, int accum, int mode) {
   if (!c) return GVDBX_ERR_ARG;
   if (sc != c->sc) return fail(c, GVDBX_ERR_ARG, "subcell size %d differs from the insert (%d)", sc, c->sc);
-  return gather_impl(c, r, chan, accum, 0);
+  if (chan_clr >= 0) {
+    CHK(chan_clr_ok(c, chan_clr));
+    if (chan_clr == chan) return fail(c, GVDBX_ERR_ARG, "field and colour channels must differ");
+  }
+  CHK(gather_impl(c, r, chan, accum, mode));
+  return gather_colour(c, chan_clr, accum);
+}
+
+int gvdbx_gather_density(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t len, int chan,
+                         int chan_clr, int accum) {
+  (void)n; (void)trans; (void)len;
+  return gather_with_colour(c, sc, r, chan, chan_clr, accum, 0);
 }
 
 int gvdbx_gather_levelset(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t len, int chan,
                           int chan_clr, int accum) {
-  (void)n; (void)trans; (void)len; (void)chan_clr;
-  if (!c) return GVDBX_ERR_ARG;
-  if (sc != c->sc) return fail(c, GVDBX_ERR_ARG, "subcell size %d differs from the insert (%d)", sc, c->sc);
-  return gather_impl(c, r, chan, accum, 1);
+  (void)n; (void)trans; (void)len;
+  return gather_with_colour(c, sc, r, chan, chan_clr, accum, 1);
 }
 
 int gvdbx_gather_velocity(gvdbx_ctx* c, int cx, int cy, int cz, int accum) {
@@ -1421,7 +1510,7 @@ static int apron_table_current(gvdbx_ctx* c) {
 
 int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* boundvals, int nchan) {
   if (!c || !chans || nchan < 0) return GVDBX_ERR_ARG;  // UpdateApron (V.cpp:4411-4457)
-  for (int i = 0; i < nchan; ++i) CHK(chan_ok(c, chans[i]));
+  for (int i = 0; i < nchan; ++i) CHK(chan_any(c, chans[i]));
   CHK(atlas_current(c));
   if (c->apron == 0 || c->nslots == 0) return GVDBX_OK;
   CHK(apron_table_current(c));
@@ -1435,11 +1524,11 @@ int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* bou
     ac.n = 0;
     for (int i = 0; i < nchan; ++i) {
       if (c->nbr_apron_ok[chans[i]] != (pass == 0)) continue;
-      const float bv = boundvals ? boundvals[i] : 0.0f;
+      const float bv = bound_word(c, chans[i], boundvals ? boundvals[i] : 0.0f);
       if (pass == 0 && c->bound_ok[chans[i]] && __builtin_memcmp(&bv, &c->bound_val[chans[i]], 4) == 0)
         continue;  // neighbour copies and boundary value both current
       ac.ptr[ac.n] = P<float>(c->chan[chans[i]]);
-      ac.bound[ac.n] = boundvals ? boundvals[i] : 0.0f;
+      ac.bound[ac.n] = bv;
       if (++ac.n == kMaxApronChannels || i == nchan - 1) {
         if (pass == 0) apron_bound(c->nslots, P<int>(c->nbr), tab, ntab, res_of(c, 0), c->apron, ac, c->stream);
         else apron_multi(c->nslots, P<int>(c->nbr), res_of(c, 0), c->apron, ac, c->stream);
@@ -1453,7 +1542,7 @@ int gvdbx_update_apron_channels(gvdbx_ctx* c, const int* chans, const float* bou
   }
   for (int i = 0; i < nchan; ++i) {
     c->nbr_apron_ok[chans[i]] = c->bound_ok[chans[i]] = true;
-    c->bound_val[chans[i]] = boundvals ? boundvals[i] : 0.0f;
+    c->bound_val[chans[i]] = bound_word(c, chans[i], boundvals ? boundvals[i] : 0.0f);
   }
   return launch_status(c, "UpdateApron");
 }
@@ -1635,7 +1724,7 @@ int gvdbx_get_num_slots(const gvdbx_ctx* c, int64_t* slots, int* brick_res) {
 }
 
 int gvdbx_copy_channel(gvdbx_ctx* c, int chan, float* out, int64_t max_floats) {
-  CHK(chan_ok(c, chan));
+  CHK(chan_any(c, chan));
   if (!out) return GVDBX_ERR_ARG;
   int64_t n = c->nslots * brick_floats(c);
   if (n > max_floats) n = max_floats;
@@ -1643,7 +1732,7 @@ int gvdbx_copy_channel(gvdbx_ctx* c, int chan, float* out, int64_t max_floats) {
 }
 
 int gvdbx_get_channel_ptr(gvdbx_ctx* c, int chan, void** p, int64_t* floats) {
-  CHK(chan_ok(c, chan));
+  CHK(chan_any(c, chan));
   c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;  // the caller may write through the pointer
   if (p) *p = c->chan[chan].p;
   if (floats) *floats = c->nslots * brick_floats(c);
@@ -1681,6 +1770,7 @@ int gvdbx_copy_subcell_array(gvdbx_ctx* c, int which, void* out, int64_t max_byt
     case 4: src = c->sc_nid.p; bytes = c->nsc * 4; break;
     case 5: src = c->list_pos.p; bytes = c->sclen * 12; break;
     case 6: src = c->lists_have_vel ? c->list_vel.p : nullptr; bytes = c->lists_have_vel ? c->sclen * 12 : 0; break;
+    case 7: src = c->lists_have_clr ? c->list_clr.p : nullptr; bytes = c->lists_have_clr ? c->sclen * 4 : 0; break;
     default: return fail(c, GVDBX_ERR_ARG, "unknown subcell array %d", which);
   }
   if (bytes > max_bytes) bytes = max_bytes;

@@ -388,6 +388,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
     const float q[3] = {p.x, p.y, p.z};
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (A.list_vel) v = A.svel[j];
+    const uint32_t clr = A.list_clr ? load_u32(A.P.clr, __float_as_uint(p.w), A.P.clr_off, A.P.clr_stride) : 0u;
     int lo[3], hi[3];
     subcell_range(q, c.r, c.sc, lo, hi);
     for (int x = lo[0]; x <= hi[0]; x += c.sc)
@@ -412,6 +413,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
             A.list_vel[3 * k + 1] = v.y;
             A.list_vel[3 * k + 2] = v.z;
           }
+          if (A.list_clr) A.list_clr[k] = clr;
         }
   }
 }
@@ -570,6 +572,7 @@ __global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __rest
       fv[1] = vel_fixed(v.y);
       fv[2] = vel_fixed(v.z);
     }
+    const uint32_t clr = (kInsert && A.list_clr) ? load_u32(A.P.clr, i, A.P.clr_off, A.P.clr_stride) : 0u;
     int lo[3], hi[3];
     subcell_range(q, c.r, c.sc, lo, hi);
     for (int x = lo[0]; x <= hi[0]; x += c.sc)
@@ -591,6 +594,7 @@ __global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __rest
             A.list_vel[3 * k + 1] = v.y;
             A.list_vel[3 * k + 2] = v.z;
           }
+          if (A.list_clr) A.list_clr[k] = clr;
         }
   }
 }
@@ -883,6 +887,103 @@ void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt,
   else
     hipLaunchKernelGGL((k_velocity_brick<0, 0, 0>), item_grid(nmap), dim3(256), 0, s, c, nmap, map_leaf, cnt, vsum, leaf_slot,
                        apron, nbr, cx, cy, cz, accum);
+}
+
+// ------------------------------------------------------------------ colour means
+// Colour rule of GatherDensity / GatherLevelSet (P.cuh:713-731, 752-770), per component q of
+// the uchar4 words: (start_q + sum of the listed colours' q) / (cnt + accum), start = the old
+// voxel with accumulation, else (0, 0, 0, 255) -- the reference's (0, 0, 0, 1) in units of
+// 1/255.  Sums are exact integers and the quotient is floored and saturated at 255 (the
+// reference's uchar(c * 255) truncation of the same ratio); an empty subcell without
+// accumulation gives 0.
+
+__device__ __forceinline__ uint32_t colour_mean(const unsigned long long* s, unsigned long long den, uint32_t start) {
+  uint32_t out = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const unsigned long long num = s[q] + ((start >> (8 * q)) & 255u);
+    const unsigned long long v = den ? num / den : 0ull;
+    out |= (uint32_t)(v < 255ull ? v : 255ull) << (8 * q);
+  }
+  return out;
+}
+
+// one thread per subcell: its list segment [off, off + cnt) summed component by component
+__global__ void __launch_bounds__(256) k_colour_sums(int64_t nsc, const int* __restrict__ off,
+                                                    const int* __restrict__ cnt,
+                                                    const uint32_t* __restrict__ list_clr,
+                                                    unsigned long long* __restrict__ csum,
+                                                    uint32_t* __restrict__ cmean) {
+  const int64_t g = item_index() * blockDim.x + threadIdx.x;
+  if (g >= nsc) return;
+  const int b = off[g], n = cnt[g];
+  unsigned long long sum[4] = {0ull, 0ull, 0ull, 0ull};
+  for (int j = b; j < b + n; ++j) {
+    const uint32_t w = list_clr[j];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sum[q] += (w >> (8 * q)) & 255u;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) csum[4 * g + q] = sum[q];
+  cmean[g] = colour_mean(sum, (unsigned long long)n, 0xFF000000u);
+}
+
+void colour_sums(int64_t nsc, const int* off, const int* cnt, const uint32_t* list_clr, unsigned long long* csum,
+                 uint32_t* cmean, hipStream_t s) {
+  if (nsc <= 0) return;
+  hipLaunchKernelGGL(k_colour_sums, item_grid(nblocks(nsc, 256)), dim3(256), 0, s, nsc, off, cnt, list_clr, csum,
+                     cmean);
+}
+
+// One workgroup per brick (subcell-map index m; linked leaves have slot == m, so neighbour
+// slots index the subcell arrays as well).
+__global__ void __launch_bounds__(256) k_colour_brick(SubcellCfg c, int nmap, const int* __restrict__ map_leaf,
+                                                     const int* __restrict__ cnt,
+                                                     const unsigned long long* __restrict__ csum,
+                                                     const uint32_t* __restrict__ cmean,
+                                                     const int* __restrict__ leaf_slot, int apron,
+                                                     const int* __restrict__ nbr, uint32_t* __restrict__ chan,
+                                                     int accum) {
+  __shared__ int nb[27];
+  const int64_t m = item_index();
+  if (m >= nmap) return;
+  const int leaf = map_leaf[m];
+  if (leaf < 0) return;
+  const int slot = leaf_slot[leaf];
+  if (slot < 0) return;
+  const int res0 = c.res0, D = c.D, D3 = D * D * D, lgsc = c.lgsc;
+  const int a = (nbr && !accum) ? apron : 0;
+  if (threadIdx.x < 27) nb[threadIdx.x] = a ? nbr[(int64_t)slot * 27 + threadIdx.x] : -1;
+  __syncthreads();
+  const int br = res0 + 2 * apron, bw = res0 + 2 * a, nv = bw * bw * bw;
+  const int64_t base = (int64_t)slot * br * br * br;
+  for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+    const int zy = t / bw;
+    const int x = t - zy * bw - a, z = zy / bw - a, y = zy - (zy / bw) * bw - a;
+    const int ox = x < 0 ? -1 : (x >= res0 ? 1 : 0), oy = y < 0 ? -1 : (y >= res0 ? 1 : 0),
+              oz = z < 0 ? -1 : (z >= res0 ? 1 : 0);
+    const int b = (ox | oy | oz) == 0 ? (int)m : nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)];
+    const int64_t vi = base + ((int64_t)(z + apron) * br + (y + apron)) * br + (x + apron);
+    if (b < 0) {  // apron voxel without a neighbour brick: the boundary value 0
+      chan[vi] = 0u;
+      continue;
+    }
+    const int lx = x - ox * res0, ly = y - oy * res0, lz = z - oz * res0;
+    const int64_t k = (int64_t)b * D3 + ((lz >> lgsc) * D + (ly >> lgsc)) * D + (lx >> lgsc);
+    if (!accum) {
+      chan[vi] = cmean[k];
+    } else {
+      chan[vi] = colour_mean(csum + 4 * k, (unsigned long long)cnt[k] + 1ull, chan[vi]);
+    }
+  }
+}
+
+void colour_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* csum,
+                  const uint32_t* cmean, const int* leaf_slot, int apron, const int* nbr, uint32_t* chan, int accum,
+                  hipStream_t s) {
+  if (nmap <= 0) return;
+  hipLaunchKernelGGL(k_colour_brick, item_grid(nmap), dim3(256), 0, s, c, nmap, map_leaf, cnt, csum, cmean, leaf_slot,
+                     apron, nbr, chan, accum);
 }
 
 // Boundary value into the apron voxels that have no neighbour brick (the part of

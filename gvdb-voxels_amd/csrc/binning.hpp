@@ -35,6 +35,7 @@ struct BinArgs {
   int* nbm;             // [nmap][27] neighbour subcell-map indices, written by the count kernel
   float* list_pos;
   float* list_vel;
+  uint32_t* list_clr;   // the listed points' colours, or null
   unsigned long long* vsum;
   unsigned long long* wvs;  // [nmap][W^3][3]: the brick's window velocity sums (where its count > 0)
 };
@@ -66,6 +67,17 @@ void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* 
 void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* vsum,
                     const int* leaf_slot, int apron, const int* nbr, float* cx, float* cy, float* cz, int accum,
                     hipStream_t s);
+
+// Colour means (the GatherDensity / GatherLevelSet colour rule, P.cuh:713-731): per subcell the
+// exact component sums of its listed colours (csum, 4 per subcell) and the packed mean without
+// accumulation (cmean); then every voxel of chan (a uchar4 channel) takes its subcell's mean, or
+// with accum (old + sum) / (cnt + 1) from its own old value.  nbr != null (and !accum): the
+// apron is written from the neighbour subcells, 0 where no neighbour brick exists.
+void colour_sums(int64_t nsc, const int* off, const int* cnt, const uint32_t* list_clr, unsigned long long* csum,
+                 uint32_t* cmean, hipStream_t s);
+void colour_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* csum,
+                  const uint32_t* cmean, const int* leaf_slot, int apron, const int* nbr, uint32_t* chan, int accum,
+                  hipStream_t s);
 
 void velocity_subcell(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
                       const unsigned long long* vsum, const int* leaf_slot, int apron, float* cx, float* cy,

@@ -103,4 +103,8 @@ __device__ __forceinline__ float3 load_f3(const char* base, int64_t i, int off, 
   return make_float3(p[0], p[1], p[2]);
 }
 
+__device__ __forceinline__ uint32_t load_u32(const char* base, int64_t i, int off, int stride) {
+  return *(const uint32_t*)(base + i * (int64_t)stride + off);
+}
+
 }  // namespace gvdbx

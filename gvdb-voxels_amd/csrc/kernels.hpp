@@ -15,6 +15,7 @@ struct KeyFmt {
 struct PointsView {
   const char* pos; int pos_off, pos_stride;
   const char* vel; int vel_off, vel_stride;
+  const char* clr; int clr_off, clr_stride;  // uchar4 colours (32-bit words), or null
   int64_t n;
   float tx, ty, tz;
   // Batched clouds: point i belongs to cloud cloud[i], whose cell origin is its grid cell

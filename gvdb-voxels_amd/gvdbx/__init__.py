@@ -5,4 +5,4 @@ Python mirror of the reference VolumeGVDB point API over the C ABI in include/gv
 fallback, and loading fails loudly when libgvdbx.so is missing.
 """
 from .lib import GvdbxError, load_library, LIB_PATH, NODE_DTYPE, DeviceArray  # noqa: F401
-from .volume import VolumeGVDB, T_FLOAT  # noqa: F401
+from .volume import VolumeGVDB, T_FLOAT, T_UCHAR4  # noqa: F401

@@ -11,6 +11,7 @@ import numpy as np
 
 from .lib import NODE_DTYPE, DeviceArray, GvdbxError, fptr, load_library
 
+T_UCHAR4 = 2  # src/gvdb_types.h:75-76
 T_FLOAT = 3
 
 
@@ -26,6 +27,7 @@ class VolumeGVDB:
         self.apron = None
         self._points = None
         self._channels = []
+        self._types = {}
 
     # ------------------------------------------------------------ plumbing
     def _chk(self, code, what):
@@ -73,9 +75,14 @@ class VolumeGVDB:
         self.apron = apron
         if chan not in self._channels:
             self._channels.append(chan)
+        self._types[chan] = dtype
 
     def FillChannel(self, chan, value):
-        self._chk(self.L.gvdbx_fill_channel(self.h, chan, float(value)), "fill_channel")
+        """FillChannel(chan, Vector4DF): value is a float (x) or an (x, y, z, w) sequence; a uchar4
+        channel takes CLR2INT(x, y, z, 1), all zeros clear it."""
+        v = [float(x) for x in value] if hasattr(value, "__len__") else [float(value), 0.0, 0.0, 0.0]
+        v += [0.0] * (4 - len(v))
+        self._chk(self.L.gvdbx_fill_channel4(self.h, chan, fptr(v[:4])), "fill_channel")
 
     def ClearChannel(self, chan):
         self.FillChannel(chan, 0.0)
@@ -267,11 +274,14 @@ class VolumeGVDB:
         return s.value, br.value
 
     def get_channel(self, chan):
-        """Brick-linear channel as (slots, br, br, br) indexed [slot, z, y, x]."""
+        """Brick-linear channel as (slots, br, br, br) indexed [slot, z, y, x]: float32, or for a
+        uchar4 channel uint32 words (r | g << 8 | b << 16 | a << 24)."""
         s, br = self.num_slots()
         out = np.zeros(s * br ** 3, np.float32)
         if out.size:
             self._chk(self.L.gvdbx_copy_channel(self.h, chan, out.ctypes.data, out.size), "copy_channel")
+        if self._types.get(chan) == T_UCHAR4:
+            out = out.view(np.uint32)
         return out.reshape(s, br, br, br)
 
     def channel_ptr(self, chan):
@@ -301,6 +311,7 @@ class VolumeGVDB:
             map=grab(0, total, np.int32), cnt=grab(1, nsc, np.int32), off=grab(2, nsc, np.int32),
             pos=grab(3, nsc * 3, np.int32, (nsc, 3)), nid=grab(4, nsc, np.int32),
             list_pos=grab(5, ln * 3, np.float32, (ln, 3)), list_vel=grab(6, ln * 3, np.float32, (ln, 3)),
+            list_clr=grab(7, ln, np.uint32),
         )
 
     # ------------------------------------------------------------ timing

@@ -26,6 +26,12 @@ struct Vector3DF {
   Vector3DF(float a, float b, float c) : x(a), y(b), z(c) {}
 };
 
+struct Vector4DF {
+  float x = 0, y = 0, z = 0, w = 0;
+  Vector4DF() {}
+  Vector4DF(float a, float b, float c, float d) : x(a), y(b), z(c), w(d) {}
+};
+
 // DataPtr subset used by the point path (gvdb_allocator.h:35-55): a caller-owned device
 // buffer with element count, byte offset and byte stride.
 struct DataPtr {
@@ -35,7 +41,7 @@ struct DataPtr {
   int stride = 0;
 };
 
-enum { T_FLOAT = GVDBX_T_FLOAT };
+enum { T_UCHAR4 = GVDBX_T_UCHAR4, T_FLOAT = GVDBX_T_FLOAT };
 
 class VolumeGVDB {
  public:
@@ -73,8 +79,12 @@ class VolumeGVDB {
     check(gvdbx_add_channel(ctx_, chan, dt, apron), "AddChannel", true);
     channels_.push_back(chan);
   }
-  void FillChannel(uchar chan, float val) { check(gvdbx_fill_channel(ctx_, chan, val), "FillChannel", true); }
-  void ClearChannel(uchar chan) { FillChannel(chan, 0.0f); }
+  void FillChannel(uchar chan, Vector4DF val) {
+    const float v[4] = {val.x, val.y, val.z, val.w};
+    check(gvdbx_fill_channel4(ctx_, chan, v), "FillChannel", true);
+  }
+  void FillChannel(uchar chan, float val) { FillChannel(chan, Vector4DF(val, 0.0f, 0.0f, 0.0f)); }
+  void ClearChannel(uchar chan) { FillChannel(chan, Vector4DF(0.0f, 0.0f, 0.0f, 0.0f)); }
 
   // SetDataGPU (V.cpp:4872-4880) / SetPoints (V.cpp:4900-4908)
   void SetDataGPU(DataPtr& dat, int cnt, void* gptr, int offs, int stride) {

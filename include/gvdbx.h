@@ -48,7 +48,8 @@ enum gvdbx_status {
   GVDBX_ERR_DEGENERATE = 8    /* the whole topology is one brick at level 0 */
 };
 
-enum gvdbx_dtype { GVDBX_T_FLOAT = 3 };   /* T_FLOAT (src/gvdb_types.h) */
+/* channel types: T_UCHAR4 and T_FLOAT (src/gvdb_types.h:75-76); both store one 32-bit word per voxel */
+enum gvdbx_dtype { GVDBX_T_UCHAR4 = 2, GVDBX_T_FLOAT = 3 };
 
 /* Node, same field order and size as the reference Node (src/gvdb_node.h:27-40). */
 typedef struct gvdbx_node {
@@ -76,7 +77,11 @@ int gvdbx_get_stream(const gvdbx_ctx* ctx, void** hip_stream);
 int gvdbx_configure(gvdbx_ctx* ctx, int levs, const int* log2dims);
 int gvdbx_set_channel_default(gvdbx_ctx* ctx, int cx, int cy, int cz);
 int gvdbx_add_channel(gvdbx_ctx* ctx, int chan, int dtype, int apron);
-int gvdbx_fill_channel(gvdbx_ctx* ctx, int chan, float value);
+int gvdbx_fill_channel(gvdbx_ctx* ctx, int chan, float value);   /* = gvdbx_fill_channel4 with (value, 0, 0, 0) */
+/* FillChannel(chan, Vector4DF) (V.cpp:2456-2471): all-zero values clear the channel; a float
+ * channel takes val[0]; a uchar4 channel takes CLR2INT(val[0], val[1], val[2], 1)
+ * (gvdbOpFillC4, O.cuh:368-373), each component uchar(x * 255) saturated to [0, 255]. */
+int gvdbx_fill_channel4(gvdbx_ctx* ctx, int chan, const float val[4]);
 
 /* ---- points: SetPoints + SetDataGPU (V.cpp:4900-4908, 4872-4880); byte offsets and strides */
 int gvdbx_set_points(gvdbx_ctx* ctx, int64_t n,
@@ -114,8 +119,16 @@ int gvdbx_update_atlas(gvdbx_ctx* ctx);
  *      GatherDensity (V.cpp:5318-5344), GatherLevelSet (V.cpp:5348-5373) */
 int gvdbx_insert_points_subcell(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, const float trans[3],
                                 int64_t* sc_pnts_len);
+/* Colour (P.cuh:713-731): when the points had colours bound (gvdbx_set_points clr) at the
+ * insert and chan_clr >= 0, the uchar4 channel chan_clr receives per voxel, component by
+ * component, (start + sum of the listed points' colours) / (cnt + accum) over the voxel's
+ * subcell, start = the old value with accum, else (0, 0, 0, 255) (the reference's initial
+ * colour (0, 0, 0, 1)).  Computed exactly: floor of the integer quotient, saturated at 255; an
+ * empty subcell without accumulation gives 0 (the reference divides by zero there).  Without
+ * accumulation the apron is written from the neighbour bricks (missing ones: 0). */
 int gvdbx_gather_density(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, const float trans[3],
                          int64_t sc_pnts_len, int chan, int chan_clr, int accum);
+/* colour: as gvdbx_gather_density (the same rule, P.cuh:752-770) */
 int gvdbx_gather_levelset(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, const float trans[3],
                           int64_t sc_pnts_len, int chan, int chan_clr, int accum);
 /* Velocity channels: the reference colour-mean rule (P.cuh:717-731) applied to the binned
@@ -184,7 +197,8 @@ int gvdbx_get_num_slots(const gvdbx_ctx* ctx, int64_t* slots, int* brick_res);
 int gvdbx_copy_channel(gvdbx_ctx* ctx, int chan, float* host_out, int64_t max_floats);
 int gvdbx_get_channel_ptr(gvdbx_ctx* ctx, int chan, void** dev_ptr, int64_t* floats);
 int gvdbx_get_subcell_info(const gvdbx_ctx* ctx, int64_t* num_subcells, int64_t* list_len, int* subcell);
-/* which: 0 map(int32 per leaf) 1 cnt 2 off 3 pos(int32x3) 4 nid 5 list pos(float x3) 6 list vel(float x3) */
+/* which: 0 map(int32 per leaf) 1 cnt 2 off 3 pos(int32x3) 4 nid 5 list pos(float x3) 6 list vel(float x3)
+ *        7 list colour (uint32 = uchar4) */
 int gvdbx_copy_subcell_array(gvdbx_ctx* ctx, int which, void* host_out, int64_t max_bytes);
 
 /* ---- device memory helpers: AllocData/CommitData/RetrieveData (V.cpp:4822-4860, A.cpp:263-298) */

@@ -686,6 +686,54 @@ int orc_gather_velocity(orc_tree* t, const float* vel, int64_t st, int cx, int c
   return ORC_OK;
 }
 
+/* Colour part of gvdbGatherDensity / gvdbGatherLevelSet (P.cuh:713-731, 752-770): per voxel of
+ * subcell k and component q of the uchar4 words,
+ *   clr = (accum ? CHAR2CLR(old) : (0, 0, 0, 1)) + sum_j CHAR2CLR(clr_j);  clr /= cnt + accum;
+ *   out = CLR2CHAR(clr) = uchar(clr * 255).
+ * CHAR2CLR divides by 255 and CLR2CHAR multiplies by 255, so in units of 1/255 this is
+ *   out_q = uchar((start_q + sum_j c_jq) / (cnt + accum)),  start = old or (0, 0, 0, 255).
+ * Restated exactly: integer sums, the quotient floored (uchar truncates), saturated at 255
+ * (the alpha sum exceeds cnt * 255 by the initial 255), and 0 for an empty subcell without
+ * accumulation (the reference divides by zero there).  Channels hold the packed word
+ * r | g << 8 | b << 16 | a << 24 as a double.  clr: uint32 words, stride in words. */
+static uint32_t colour_mean(const uint64_t sum[4], uint64_t den, uint32_t start) {
+  uint32_t out = 0;
+  for (int q = 0; q < 4; q++) {
+    uint64_t num = sum[q] + ((start >> (8 * q)) & 255u);
+    uint64_t v = den ? num / den : 0;
+    out |= (uint32_t)(v < 255 ? v : 255) << (8 * q);
+  }
+  return out;
+}
+
+int orc_gather_colour(orc_tree* t, const uint32_t* clr, int64_t st, int chan, int accum) {
+  if (!t->chan[chan]) return ORC_ERR_STATE;
+  if (!t->sc_cnt) return ORC_ERR_STATE;
+  int sc = t->sc, a = t->apron;
+  double* ch = t->chan[chan];
+  for (int64_t k = 0; k < t->nsc; k++) {
+    int64_t leaf = t->sc_nid[k];
+    const orc_node* nd = &t->nodes[0][leaf];
+    int64_t slot = t->leaf_slot[leaf];
+    if (slot < 0) continue;
+    uint64_t sum[4] = {0, 0, 0, 0};
+    for (int64_t j = t->sc_off[k]; j < (int64_t)t->sc_off[k] + t->sc_cnt[k]; j++) {
+      uint32_t w = clr[t->sc_list[j] * st];
+      for (int q = 0; q < 4; q++) sum[q] += (w >> (8 * q)) & 255u;
+    }
+    uint64_t den = (uint64_t)t->sc_cnt[k] + (accum ? 1 : 0);
+    for (int tz = 0; tz < sc; tz++)
+      for (int ty = 0; ty < sc; ty++)
+        for (int tx = 0; tx < sc; tx++) {
+          int32_t v[3] = {t->sc_pos[3 * k] + tx, t->sc_pos[3 * k + 1] + ty, t->sc_pos[3 * k + 2] + tz};
+          int64_t ai = atlas_index(t, slot, v[0] - nd->pos[0] + a, v[1] - nd->pos[1] + a, v[2] - nd->pos[2] + a);
+          uint32_t start = accum ? (uint32_t)ch[ai] : 0xFF000000u;
+          ch[ai] = (double)colour_mean(sum, den, start);
+        }
+  }
+  return ORC_OK;
+}
+
 /* UpdateApron<T> (O.cuh:72-126): every apron voxel of every active brick takes the value
  * of the interior voxel at the same world position, or boundval if no leaf holds it. */
 int orc_update_apron(orc_tree* t, int chan, double boundval) {

@@ -100,6 +100,8 @@ int orc_gather_density(orc_tree* t, float radius, int chan, int accum);      /* 
 int orc_gather_levelset(orc_tree* t, float radius, int chan, int accum);     /* P.cuh:736-773 */
 int orc_gather_velocity(orc_tree* t, const float* vel, int64_t stride, int chan_x, int chan_y, int chan_z,
                         int accum);                                             /* colour rule P.cuh:717-731 */
+/* colour means of the gathers (P.cuh:713-731): packed uchar4 words stored as doubles */
+int orc_gather_colour(orc_tree* t, const uint32_t* clr, int64_t stride_words, int chan, int accum);
 int orc_update_apron(orc_tree* t, int chan, double boundval);                 /* O.cuh:72-126 */
 
 /* ScatterReduceLevelSet: same field as GatherLevelSet (a21). */

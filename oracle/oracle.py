@@ -69,6 +69,7 @@ def lib():
         L.orc_gather_density.argtypes = [P, f32, C.c_int, C.c_int]
         L.orc_gather_levelset.argtypes = [P, f32, C.c_int, C.c_int]
         L.orc_gather_velocity.argtypes = [P, P, i64, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_gather_colour.argtypes = [P, P, i64, C.c_int, C.c_int]
         L.orc_update_apron.argtypes = [P, C.c_int, C.c_double]
         L.orc_scatter_reduce_levelset.argtypes = [P, f32, C.c_int, C.c_double]
         L.orc_sample_trilinear.argtypes = [P, C.c_int, P, i64, P]
@@ -247,6 +248,12 @@ class Tree:
         vel, st = _pts(vel)
         _chk(lib().orc_gather_velocity(self.h, _ptr(vel), st, chans[0], chans[1], chans[2], int(accum)),
              "gather_velocity")
+
+    def gather_colour(self, clr, chan, accum=False):
+        """Colour means of the gathers; clr: uint32 words (r | g << 8 | b << 16 | a << 24) per
+        point, as listed at the insert.  The channel holds the words as doubles."""
+        clr = np.ascontiguousarray(clr, dtype=np.uint32)
+        _chk(lib().orc_gather_colour(self.h, _ptr(clr), 1, chan, int(accum)), "gather_colour")
 
     def update_apron(self, chan, boundval):
         _chk(lib().orc_update_apron(self.h, chan, boundval), "update_apron")

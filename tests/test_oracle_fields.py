@@ -325,3 +325,49 @@ def test_scatter_density_apron_skips_and_sums(oracle_mod):
     t.scatter_density(0, np.array([[40.0, 41.0, 42.0]], np.float32), 1.0, trans=(2.5, 2.5, 2.5), expand=False)
     _, b = _brick_of(t, (40, 40, 40))
     assert b[4 + a, 3 + a, 2 + a] == 0.015625 and np.count_nonzero(b) == 1
+
+
+def _word(r, g, b, a):
+    return r | g << 8 | b << 16 | a << 24
+
+
+def test_colour_means_hand_computed(oracle_mod):
+    """GatherDensity's colour rule (P.cuh:713-731) at hand-computed subcells (8^3 bricks, sc 4,
+    r = 1).  Points at 9.x reach only the subcell at (8, 8, 8) (int(p - 1) and int(p + 2) both
+    floor to 8), so its list holds exactly the first two points:
+      colours (10, 20, 30, 40) and (11, 0, 255, 255), start (0, 0, 0, 255) without accumulation:
+      (21 / 2, 20 / 2, 285 / 2, 550 / 2) -> (10, 10, 142, 255 saturated);
+      the subcell at (12, 12, 12) of the same brick is empty -> 0;
+      a lone point (200, 200, 200, 200) at 40.5 -> (200, 200, 200, 455 -> 255);
+    with accumulation over the old word (100, 50, 0, 255):
+      (121 / 3, 70 / 3, 285 / 3, 550 / 3) -> (40, 23, 95, 183); the empty subcell keeps the old
+      word (old / 1)."""
+    p = np.array([[9.2, 9.3, 9.4], [9.6, 9.1, 9.9], [40.5, 40.5, 40.5], [150.0, 150.0, 150.0]], np.float32)
+    clr = np.array([_word(10, 20, 30, 40), _word(11, 0, 255, 255), _word(200, 200, 200, 200),
+                    _word(1, 2, 3, 4)], np.uint32)
+    t = oracle_mod.Tree((3, 3, 3, 3, 5), 16, 16, 1)
+    t.rebuild_topology(p, 1.0)
+    t.add_channel(1)
+    t.update_atlas()
+    t.insert_points_subcell(4, p, 1.0)
+    t.fill_channel(1, 12345.0)
+    t.gather_colour(clr, 1)
+
+    def brick(corner):
+        nodes = t.nodes(0)
+        live = np.nonzero(nodes["flags"] == 1)[0]
+        k = np.nonzero((nodes["pos"][live] == np.asarray(corner)).all(1))[0]
+        assert len(k) == 1
+        return t.channel(1)[t.leaf_slot()[live[k[0]]]]
+
+    a = 1
+    b = brick((8, 8, 8))
+    assert b[1 + a, 1 + a, 1 + a] == _word(10, 10, 142, 255)   # voxel (9, 9, 9)
+    assert b[3 + a, 0 + a, 2 + a] == _word(10, 10, 142, 255)   # voxel (10, 8, 11): same subcell
+    assert b[5 + a, 6 + a, 4 + a] == 0.0                       # subcell (12, 12, 12): empty
+    assert brick((40, 40, 40))[1 + a, 1 + a, 1 + a] == _word(200, 200, 200, 255)
+    t.fill_channel(1, float(_word(100, 50, 0, 255)))
+    t.gather_colour(clr, 1, accum=True)
+    b = brick((8, 8, 8))
+    assert b[1 + a, 1 + a, 1 + a] == _word(40, 23, 95, 183)
+    assert b[5 + a, 6 + a, 4 + a] == _word(100, 50, 0, 255)
