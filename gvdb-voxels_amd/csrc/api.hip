@@ -96,6 +96,7 @@ struct gvdbx_ctx {
   Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb, nbm, spos, svel;
   Buf sp_cnt, sp_start, sp_keyrank, sp_payload, sp_perm;  // splat: points counting-sorted by (brick, region)
   Buf scatter_acc;  // ScatterDensity: 64-bit fixed-point sums per padded voxel
+  Buf scatter_clr;  // ScatterDensity colour: per interior voxel sums (avgColor) or point index
   Buf wvs;          // insert: per-brick window velocity sums
   Buf list_clr, csum, cmean;  // subcell list colours; per subcell colour sums and means
   // brick order of the last splat (a permutation of its n points), reused by the splat
@@ -883,7 +884,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
                  &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta, &c->box_region, &c->box_cnt,
                  &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->scatter_acc, &c->wvs,
-                 &c->list_clr, &c->csum, &c->cmean,
+                 &c->list_clr, &c->csum, &c->cmean, &c->scatter_clr,
                  &c->cloud_id,
                  &c->cloud_off_dev};
   for (Buf* b : bufs)
@@ -1647,9 +1648,14 @@ int gvdbx_sample_trilinear_backward(gvdbx_ctx* c, int chan, const void* q, const
 }
 
 int gvdbx_scatter_density(gvdbx_ctx* c, int64_t n, float radius, float amp, const float trans[3], int expand,
-                          int chan) {
+                          int chan, int avg_colour, int chan_clr) {
   (void)amp;  // unused by the reference kernel as well (P.cuh:525-573)
   CHK(chan_ok(c, chan));  // ScatterDensity (V.cpp:5625-5659)
+  const int cmode = (c->clr && chan_clr >= 0) ? (avg_colour ? 1 : 2) : 0;
+  if (cmode) {
+    CHK(chan_clr_ok(c, chan_clr));
+    if (chan_clr == chan) return fail(c, GVDBX_ERR_ARG, "density and colour channels must differ");
+  }
   if (c->batch) return fail(c, GVDBX_ERR_STATE, "ScatterDensity takes world points: not for batched clouds");
   CHK(check_points(c, n));
   if (!(radius > 0.0f) || !(radius < 1.0e30f)) return fail(c, GVDBX_ERR_ARG, "radius %g out of range", (double)radius);
@@ -1671,10 +1677,14 @@ int gvdbx_scatter_density(gvdbx_ctx* c, int64_t n, float radius, float amp, cons
   const int br = res_of(c, 0) + 2 * c->apron;
   const int64_t nvox = c->nslots * (int64_t)br * br * br;
   CHK(ensure(c, c->scatter_acc, (size_t)nvox * 8));
+  const int64_t nin = c->nslots << (3 * c->ld[0]);
+  if (cmode) CHK(ensure(c, c->scatter_clr, (size_t)nin * (cmode == 1 ? 32 : 4) + 8));
   TimedRegion tr(c, "scatter_density");
   scatter_density(tree_of(c), points_of(c, n, trans), P<int>(c->leaf_slot), c->ld[0], c->apron, radius, expand,
-                  shift, c->nslots, P<unsigned long long>(c->scatter_acc), P<float>(c->chan[chan]), c->stream);
+                  shift, c->nslots, P<unsigned long long>(c->scatter_acc), P<float>(c->chan[chan]), cmode,
+                  cmode ? c->scatter_clr.p : nullptr, cmode ? P<uint32_t>(c->chan[chan_clr]) : nullptr, c->stream);
   c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;
+  if (cmode) c->nbr_apron_ok[chan_clr] = c->bound_ok[chan_clr] = false;
   return launch_status(c, "ScatterDensity");
 }
 

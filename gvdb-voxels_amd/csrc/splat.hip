@@ -357,9 +357,13 @@ __device__ __forceinline__ float scatter_weight(float px, float py, float pz, fl
   return 1.0f + c * (-3.0f + c * (3.0f - c));
 }
 
+// cmode: colour part of the same kernel (P.cuh:553-572) -- 0 none; 1 avgColor: count and r, g, b
+// sums per interior voxel (cbuf: 4 x u64 per voxel); 2 the point colour itself: the largest
+// index + 1 of the points in the voxel (cbuf: u32 per voxel).
 __global__ void __launch_bounds__(256) k_scatter_density(DevTree T, PointsView P, const int* __restrict__ leaf_slot,
                                                          int lgres, int apron, float r, int expand, double scale,
-                                                         unsigned long long* __restrict__ acc) {
+                                                         unsigned long long* __restrict__ acc, int cmode,
+                                                         void* __restrict__ cbuf) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
@@ -380,6 +384,19 @@ __global__ void __launch_bounds__(256) k_scatter_density(DevTree T, PointsView P
   for (int a = 0; a < 3; ++a) {
     pl[a] = q[a] - (float)(iq[a] & ~(res - 1));  // brick corner: nodes are aligned to their range
     pi[a] = (int)pl[a];
+  }
+  if (cmode && (unsigned)pi[0] < (unsigned)res && (unsigned)pi[1] < (unsigned)res && (unsigned)pi[2] < (unsigned)res) {
+    const int64_t vx = ((int64_t)slot << (3 * lgres)) + (((pi[2] << lgres) + pi[1]) << lgres) + pi[0];
+    if (cmode == 1) {
+      const uint32_t w = load_u32(P.clr, i, P.clr_off, P.clr_stride);
+      unsigned long long* cs = (unsigned long long*)cbuf + 4 * vx;
+      atomicAdd(cs, 1ull);
+      atomicAdd(cs + 1, (unsigned long long)(w & 255u));
+      atomicAdd(cs + 2, (unsigned long long)((w >> 8) & 255u));
+      atomicAdd(cs + 3, (unsigned long long)((w >> 16) & 255u));
+    } else {
+      atomicMax((uint32_t*)cbuf + vx, (uint32_t)(i + 1));
+    }
   }
   unsigned long long* dst = acc + (int64_t)slot * br * br * br;
   const int nk = expand ? 7 : 1;
@@ -404,18 +421,53 @@ __global__ void __launch_bounds__(256) k_scatter_apply(const unsigned long long*
   }
 }
 
+// Colour words of the voxels the points reached (gvdbScatterPointAvgCol, P.cuh:610-641, for
+// cmode 1: (r, g, b sums / count, 255); cmode 2: the colour of the recorded point); voxels no
+// point reached keep their word.
+__global__ void __launch_bounds__(256) k_scatter_colour_apply(PointsView P, int64_t nin, int lgres, int apron, int cmode,
+                                                              const void* __restrict__ cbuf,
+                                                              uint32_t* __restrict__ chan) {
+  const int res = 1 << lgres, br = res + 2 * apron;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nin; v += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t word;
+    if (cmode == 1) {
+      const unsigned long long* cs = (const unsigned long long*)cbuf + 4 * v;
+      const unsigned long long n = cs[0];
+      if (n == 0) continue;
+      word = (uint32_t)(cs[1] / n) | (uint32_t)(cs[2] / n) << 8 | (uint32_t)(cs[3] / n) << 16 | 255u << 24;
+    } else {
+      const uint32_t k = ((const uint32_t*)cbuf)[v];
+      if (k == 0) continue;
+      word = load_u32(P.clr, (int64_t)k - 1, P.clr_off, P.clr_stride);
+    }
+    const int64_t slot = v >> (3 * lgres);
+    const int l = (int)(v & ((1 << (3 * lgres)) - 1));
+    const int x = l & (res - 1), y = (l >> lgres) & (res - 1), z = l >> (2 * lgres);
+    chan[((slot * br + z + apron) * br + y + apron) * br + x + apron] = word;
+  }
+}
+
 void scatter_density(const DevTree& T, const PointsView& P, const int* leaf_slot, int lgres, int apron, float r,
-                     int expand, int shift, int64_t nslots, unsigned long long* acc, float* chan, hipStream_t s) {
+                     int expand, int shift, int64_t nslots, unsigned long long* acc, float* chan, int cmode,
+                     void* cbuf, uint32_t* cchan, hipStream_t s) {
   const int br = (1 << lgres) + 2 * apron;
   const int64_t nvox = nslots * br * br * br;
   if (nvox <= 0) return;
+  const int64_t nin = nslots << (3 * lgres);
   fill_u64((uint64_t*)acc, nvox, 0ull, s);
+  if (cmode == 1) fill_u64((uint64_t*)cbuf, 4 * nin, 0ull, s);
+  if (cmode == 2) fill_u32((uint32_t*)cbuf, nin, 0u, s);
   if (P.n > 0)
     hipLaunchKernelGGL(k_scatter_density, dim3(nblocks(P.n, 256)), dim3(256), 0, s, T, P, leaf_slot, lgres, apron, r,
-                       expand, ldexp(1.0, shift), acc);
+                       expand, ldexp(1.0, shift), acc, cmode, cbuf);
   const int64_t g = (nvox + 255) / 256;
   hipLaunchKernelGGL(k_scatter_apply, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(256), 0, s, acc, nvox,
                      ldexp(1.0, -shift), chan);
+  if (cmode) {
+    const int64_t gc = (nin + 255) / 256;
+    hipLaunchKernelGGL(k_scatter_colour_apply, dim3((unsigned)(gc < 65536 ? gc : 65536)), dim3(256), 0, s, P, nin,
+                       lgres, apron, cmode, (const void*)cbuf, cchan);
+  }
 }
 
 }  // namespace gvdbx

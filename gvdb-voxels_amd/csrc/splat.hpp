@@ -22,8 +22,12 @@ void splat_backward(const DevTree& T, const int* leaf_slot, const int* nbr, int 
 void sample_backward(const DevTree& T, const int* leaf_slot, int res0, int apron, const float* chan, const float* q,
                      const float* dout, int64_t n, float* dq, hipStream_t s);
 // ScatterDensity: chan (padded bricks of nslots slots) += the points' (1 - c)^3 contributions,
-// summed exactly as 64-bit integers of scale 2^shift in acc (nslots * br^3 entries, cleared here)
+// summed exactly as 64-bit integers of scale 2^shift in acc (nslots * br^3 entries, cleared here).
+// Colour (cmode 1: average per voxel, cbuf = 4 x u64 per interior voxel; 2: the colour of the
+// voxel's highest-index point, cbuf = u32 per interior voxel; cleared here) into the uchar4
+// channel cchan, for the interior voxels the points reached.
 void scatter_density(const DevTree& T, const PointsView& P, const int* leaf_slot, int lgres, int apron, float r,
-                     int expand, int shift, int64_t nslots, unsigned long long* acc, float* chan, hipStream_t s);
+                     int expand, int shift, int64_t nslots, unsigned long long* acc, float* chan, int cmode,
+                     void* cbuf, uint32_t* cchan, hipStream_t s);
 
 }  // namespace gvdbx

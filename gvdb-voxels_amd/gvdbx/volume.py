@@ -198,13 +198,17 @@ class VolumeGVDB:
         self._chk(self.L.gvdbx_gather_density_velocity(self.h, subcell, n, radius, fptr(trans), sc_len, chan, cx, cy, cz),
                   "GatherDensityVelocity")
 
-    def ScatterDensity(self, num_pnts, radius, amp, trans=(0, 0, 0), expand=True, avgColor=False, chan=0):
+    def ScatterDensity(self, num_pnts, radius, amp, trans=(0, 0, 0), expand=True, avgColor=False, chan=0,
+                       chan_clr=None):
         """ScatterDensity (V.cpp:5625-5659): (1 - c)^3 of every bound point into its voxel (and
-        its 6 face neighbours with expand) of chan; exact sums instead of the reference's race."""
-        if avgColor:
-            raise GvdbxError(1, "ScatterDensity", "colour channels are not part of this library")
-        self._chk(self.L.gvdbx_scatter_density(self.h, num_pnts, radius, amp, fptr(trans), int(bool(expand)), chan),
-                  "ScatterDensity")
+        its 6 face neighbours with expand) of chan; exact sums instead of the reference's race.
+        With colours bound, the uchar4 channel chan_clr (default: the reference's channel, 1 with
+        avgColor, else 3) takes the voxels' average colours (avgColor) or the colour of each
+        voxel's highest-index point."""
+        if chan_clr is None:
+            chan_clr = (1 if avgColor else 3) if (self._points and self._points[2] is not None) else -1
+        self._chk(self.L.gvdbx_scatter_density(self.h, num_pnts, radius, amp, fptr(trans), int(bool(expand)), chan,
+                                               int(bool(avgColor)), chan_clr), "ScatterDensity")
 
     def SplatTrilinear(self, chan, q_dev, n, mass_dev=None, accum=False):
         """Trilinear mass splat (transpose of SampleTrilinear) into chan's interior."""

@@ -97,6 +97,7 @@ class VolumeGVDB {
     check(gvdbx_set_points(ctx_, pntpos.lastEle, pntpos.gpu, pntpos.offs, pntpos.stride, pntvel.gpu, pntvel.offs,
                            pntvel.stride, pntclr.gpu, pntclr.offs, pntclr.stride),
           "SetPoints", true);
+    clr_bound_ = pntclr.gpu != nullptr;
   }
 
   // Topology (V.cpp:1158-1218, 1579-1593, 2630-2704)
@@ -137,16 +138,14 @@ class VolumeGVDB {
                                 bAccum ? 1 : 0),
           "GatherLevelSet", true);
   }
-  // ScatterDensity (V.cpp:5625-5659) into channel 0 as the reference's kernel writes it;
-  // avgColor is not supported (no colour channels): reported like a runtime error.
+  // ScatterDensity (V.cpp:5625-5659) into the reference kernel's channels: density 0, colours
+  // (when bound) 1 with avgColor, else 3.
   void ScatterDensity(int num_pnts, float radius, float amp, Vector3DF trans, bool expand = true,
                       bool avgColor = false) {
-    if (avgColor) {
-      fprintf(stderr, "VolumeGVDB::ScatterDensity: colour channels are not part of this library\n");
-      exit(-1);
-    }
     const float t[3] = {trans.x, trans.y, trans.z};
-    check(gvdbx_scatter_density(ctx_, num_pnts, radius, amp, t, expand ? 1 : 0, 0), "ScatterDensity", true);
+    const int chan_clr = clr_bound_ ? (avgColor ? 1 : 3) : -1;
+    check(gvdbx_scatter_density(ctx_, num_pnts, radius, amp, t, expand ? 1 : 0, 0, avgColor ? 1 : 0, chan_clr),
+          "ScatterDensity", true);
   }
   // Additions of this library (no reference counterpart): velocity channel means and the
   // scatter-reduce level set.
@@ -210,6 +209,7 @@ class VolumeGVDB {
   }
 
   gvdbx_ctx* ctx_ = nullptr;
+  bool clr_bound_ = false;
   std::vector<int> channels_;
   float bmin_[3] = {0, 0, 0}, bmax_[3] = {0, 0, 0};
 };

@@ -166,10 +166,15 @@ int gvdbx_gather_density_velocity(gvdbx_ctx* ctx, int sc, int64_t n, float r, co
  *      brick's apron) of chan gains (1 - c)^3, c = |v - p'|^2 / radius^2.  amp is unused, as in
  *      the reference.  The reference's racy read-modify-write is replaced by exact 64-bit
  *      fixed-point sums (deterministic); inputs whose worst-case voxel sum would exceed 2^40
- *      (tiny radii at many points) return GVDBX_ERR_CAPACITY.  The colour average (avgColor)
- *      is not part of this library (no colour channels).  Not for batched clouds. */
+ *      (tiny radii at many points) return GVDBX_ERR_CAPACITY.  Not for batched clouds.
+ *      Colour (P.cuh:553-572), when the points have colours bound and chan_clr >= 0 (a uchar4
+ *      channel; the reference uses channel 1 with avgColor, else 3): avg_colour != 0 writes
+ *      each voxel pi reached by points (r, g, b sums / count, 255) (gvdbScatterPointAvgCol,
+ *      P.cuh:610-641); avg_colour == 0 writes the colour of one point of the voxel -- the
+ *      reference's unordered last write, defined here as the point with the largest index.
+ *      Voxels no point reached keep their word. */
 int gvdbx_scatter_density(gvdbx_ctx* ctx, int64_t n, float radius, float amp, const float trans[3], int expand,
-                          int chan);
+                          int chan, int avg_colour, int chan_clr);
 
 /* ---- trilinear splat and adjoints (no reference counterpart: the transpose of the sampler
  *      above, R.cuh:104-126).  splat: chan interior (+)= sum_i m_i w_g(q_i) over the 8 voxels

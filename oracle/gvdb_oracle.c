@@ -934,6 +934,58 @@ int orc_scatter_density(orc_tree* t, int chan, const float* pos, int64_t n, int6
   return ORC_OK;
 }
 
+/* Colour part of gvdbScatterPointDensity (P.cuh:553-572) and gvdbScatterPointAvgCol
+ * (P.cuh:610-641): for every point that reaches the density part with its voxel pi inside the
+ * brick, avg != 0 counts it and sums its r, g, b at pi, then every voxel with a count takes
+ * (r / count, g / count, b / count, 255) (unsigned integer division, as the reference);
+ * avg == 0 gives the voxel the colour of one of its points -- the reference's unordered last
+ * write, defined as the point with the largest index.  Voxels without points keep their value.
+ * Channel values are the packed words as doubles; clr: one uint32 word per point. */
+int orc_scatter_colour(orc_tree* t, int chan, const float* pos, int64_t n, int64_t stride, const float* trans,
+                       const uint32_t* clr, int avg) {
+  if (!t->chan[chan]) return ORC_ERR_STATE;
+  const int a = t->apron, res = res_of(t, 0);
+  const int64_t nslots = t->nslots, nin = nslots * res * res * res;
+  uint64_t* sums = (uint64_t*)calloc((size_t)(nin > 0 ? nin : 1) * 4, sizeof(uint64_t));
+  int64_t* last = (int64_t*)malloc((size_t)(nin > 0 ? nin : 1) * sizeof(int64_t));
+  if (!sums || !last) { free(sums); free(last); return ORC_ERR_NOMEM; }
+  for (int64_t v = 0; v < nin; v++) last[v] = -1;
+  for (int64_t i = 0; i < n; i++) {
+    const float* pp = (const float*)((const char*)pos + i * stride);
+    float q[3];
+    for (int k = 0; k < 3; k++) q[k] = pp[k] + (trans ? trans[k] : 0.0f);
+    if (!(q[0] < 1073741824.0f && q[1] < 1073741824.0f && q[2] < 1073741824.0f)) continue;
+    int64_t f = leaf_at(t, q);
+    if (f < 0 || t->leaf_slot[f] < 0) continue;
+    const orc_node* nd = &t->nodes[0][f];
+    int pi[3];
+    for (int k = 0; k < 3; k++) pi[k] = (int)(q[k] - (float)nd->pos[k]);
+    if (pi[0] < 0 || pi[1] < 0 || pi[2] < 0 || pi[0] >= res || pi[1] >= res || pi[2] >= res) continue;
+    const int64_t v = ((t->leaf_slot[f] * res + pi[2]) * res + pi[1]) * res + pi[0];
+    const uint32_t w = clr[i];
+    sums[4 * v] += 1;
+    for (int c = 0; c < 3; c++) sums[4 * v + 1 + c] += (w >> (8 * c)) & 255u;
+    last[v] = i;  /* increasing i: the largest index stays */
+  }
+  double* ch = t->chan[chan];
+  for (int64_t v = 0; v < nin; v++) {
+    const uint64_t cnt = sums[4 * v];
+    if (!cnt) continue;
+    uint32_t word;
+    if (avg)
+      word = (uint32_t)(sums[4 * v + 1] / cnt) | (uint32_t)(sums[4 * v + 2] / cnt) << 8 |
+             (uint32_t)(sums[4 * v + 3] / cnt) << 16 | 255u << 24;
+    else
+      word = clr[last[v]];
+    const int64_t slot = v / ((int64_t)res * res * res);
+    const int l = (int)(v % ((int64_t)res * res * res));
+    ch[atlas_index(t, slot, l % res + a, (l / res) % res + a, l / (res * res) + a)] = (double)word;
+  }
+  free(sums);
+  free(last);
+  return ORC_OK;
+}
+
 /* Trilinear splat rho(g) += m * w_g(q) into interior voxels of the leaves holding the
  * corners (the transpose of sampling over the global voxel lattice). */
 int orc_splat_trilinear(orc_tree* t, int chan, const float* q, const float* mass, int64_t n) {

@@ -52,7 +52,7 @@ typedef struct orc_node {
 typedef struct orc_tree orc_tree;
 
 /* status codes mirror include/gvdbx.h */
-enum { ORC_OK = 0, ORC_ERR_ARG = 1, ORC_ERR_DEPTH0 = 2, ORC_ERR_DEPTH_MAX = 3, ORC_ERR_STATE = 6,
+enum { ORC_OK = 0, ORC_ERR_ARG = 1, ORC_ERR_DEPTH0 = 2, ORC_ERR_DEPTH_MAX = 3, ORC_ERR_STATE = 6, ORC_ERR_NOMEM = 7,
        ORC_ERR_DEGENERATE = 8 };
 
 orc_tree* orc_create(int levs, const int* log2dims, int atlas_cx, int atlas_cy, int apron);
@@ -103,6 +103,9 @@ int orc_gather_velocity(orc_tree* t, const float* vel, int64_t stride, int chan_
 /* colour means of the gathers (P.cuh:713-731): packed uchar4 words stored as doubles */
 int orc_gather_colour(orc_tree* t, const uint32_t* clr, int64_t stride_words, int chan, int accum);
 int orc_update_apron(orc_tree* t, int chan, double boundval);                 /* O.cuh:72-126 */
+/* colour part of ScatterDensity (P.cuh:553-572, 610-641); clr: one uint32 word per point */
+int orc_scatter_colour(orc_tree* t, int chan, const float* pos, int64_t n, int64_t stride, const float* trans,
+                       const uint32_t* clr, int avg);
 
 /* ScatterReduceLevelSet: same field as GatherLevelSet (a21). */
 int orc_scatter_reduce_levelset(orc_tree* t, float radius, int chan, double init);

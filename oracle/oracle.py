@@ -70,6 +70,7 @@ def lib():
         L.orc_gather_levelset.argtypes = [P, f32, C.c_int, C.c_int]
         L.orc_gather_velocity.argtypes = [P, P, i64, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_gather_colour.argtypes = [P, P, i64, C.c_int, C.c_int]
+        L.orc_scatter_colour.argtypes = [P, C.c_int, P, i64, i64, P, P, C.c_int]
         L.orc_update_apron.argtypes = [P, C.c_int, C.c_double]
         L.orc_scatter_reduce_levelset.argtypes = [P, f32, C.c_int, C.c_double]
         L.orc_sample_trilinear.argtypes = [P, C.c_int, P, i64, P]
@@ -254,6 +255,15 @@ class Tree:
         point, as listed at the insert.  The channel holds the words as doubles."""
         clr = np.ascontiguousarray(clr, dtype=np.uint32)
         _chk(lib().orc_gather_colour(self.h, _ptr(clr), 1, chan, int(accum)), "gather_colour")
+
+    def scatter_colour(self, chan, pos, clr, avg=True, trans=(0, 0, 0)):
+        """Colour part of ScatterDensity: averages (avg) or the largest-index point's colour per
+        reached voxel; clr: uint32 words per point."""
+        pos, st = _pts(pos)
+        clr = np.ascontiguousarray(clr, dtype=np.uint32)
+        tr = np.asarray(trans, np.float32)
+        _chk(lib().orc_scatter_colour(self.h, chan, _ptr(pos), len(pos), st * 4, _ptr(tr), _ptr(clr), int(avg)),
+             "scatter_colour")
 
     def update_apron(self, chan, boundval):
         _chk(lib().orc_update_apron(self.h, chan, boundval), "update_apron")

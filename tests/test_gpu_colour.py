@@ -132,3 +132,44 @@ def test_colour_channel_rules(oracle_mod):
     v.GatherDensity(4, n, 1.0, (0, 0, 0), ln, 0, chan_clr=1)
     assert (v.get_channel(1) == 0xFFFFFFFF).all()
     v.close()
+
+
+@pytest.mark.parametrize("avg", [True, False])
+def test_scatter_density_colour_matches_oracle(oracle_mod, avg):
+    """ScatterDensity with colours bound (P.cuh:553-572, 610-641): the uchar4 channel's words
+    equal the oracle's bit for bit -- averages per reached voxel, or the colour of the voxel's
+    highest-index point -- whatever order the points come in, while the density part still
+    matches; voxels no point reached keep the fill word."""
+    from gvdbx import T_UCHAR4
+    U.log(f"case scatter colour avg={avg}")
+    g = np.random.default_rng(11)
+    p = g.uniform(20, 100, (40000, 3)).astype(np.float32)
+    p[:20] -= 150.0
+    clr = g.integers(0, 1 << 32, len(p), dtype=np.uint64).astype(np.uint32)
+    v, t = U.new_volume(chans=(0,)), U.new_tree(oracle_mod, chans=(0, 1))
+    v.AddChannel(1, T_UCHAR4, apron=1)
+    dp = v.to_device(p)
+    v.SetPoints(dp, n=len(p))
+    v.RebuildTopology(len(p), 1.0)
+    v.UpdateAtlas()
+    t.rebuild_topology(p, 1.0)
+    t.update_atlas()
+    for perm in (np.arange(len(p)), g.permutation(len(p))):
+        pp, cc = p[perm], clr[perm]
+        dpp, dcc = v.to_device(pp), v.to_device(cc)
+        v.SetPoints(dpp, clr=dcc, n=len(p))
+        v.FillChannel(0, 0.0)
+        v.FillChannel(1, (0.2, 0.4, 0.6, 1.0))
+        v.ScatterDensity(len(p), 1.0, 1.0, (0, 0, 0), True, avgColor=avg, chan=0, chan_clr=1)
+        t.fill_channel(0, 0.0)
+        fill = word(51, 102, 153, 255)
+        t.fill_channel(1, float(fill))
+        t.scatter_density(0, pp, 1.0)
+        t.scatter_colour(1, pp, cc, avg=avg)
+        got = v.get_channel(1)
+        want = t.channel(1)[: len(got)]
+        same_words(got, want)
+        reached = np.count_nonzero(want != fill)
+        assert reached > 10000, reached
+        U.compare_channel(v, t, 0)
+    v.close()

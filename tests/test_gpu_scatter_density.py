@@ -62,6 +62,8 @@ def test_scatter_density_order_independent_and_limits(oracle_mod):
     with pytest.raises(GvdbxError) as e:
         v.ScatterDensity(len(q), 0.01, 1.0)  # (6 / 0.01^2)^3 per point: beyond exact sums
     assert e.value.code == 4
-    with pytest.raises(GvdbxError):
-        v.ScatterDensity(len(q), 1.0, 1.0, avgColor=True)
+    # avgColor without colours bound: the density part alone (V.cpp:5637, 5649)
+    v.FillChannel(0, 0.0)
+    v.ScatterDensity(len(q), 1.0, 1.0, avgColor=True)
+    assert v.get_channel(0).tobytes() == outs[1]
     v.close()

@@ -371,3 +371,25 @@ def test_colour_means_hand_computed(oracle_mod):
     b = brick((8, 8, 8))
     assert b[1 + a, 1 + a, 1 + a] == _word(40, 23, 95, 183)
     assert b[5 + a, 6 + a, 4 + a] == _word(100, 50, 0, 255)
+
+
+def test_scatter_colour_hand_computed(oracle_mod):
+    """ScatterDensity's colour part (P.cuh:553-572, 610-641): points 0 and 1 share voxel
+    (42, 43, 44) of the brick at (40, 40, 40), point 2 is alone in (45, 45, 45).
+    avgColor: (21 / 2, 20 / 2, 285 / 2, 255) = (10, 10, 142, 255) and (200, 100, 50, 255);
+    without it the voxel takes one point's word as it is -- the largest index, point 1 -- and
+    the lone point's word keeps its alpha 0.  Voxels without points keep the fill word."""
+    topo = np.array([[44.0, 44.0, 44.0], [150.0, 150.0, 150.0]], np.float32)
+    pts = np.array([[42.5, 43.5, 44.5], [42.2, 43.7, 44.1], [45.5, 45.5, 45.5], [-1.0, 43.0, 43.0]], np.float32)
+    clr = np.array([_word(10, 20, 30, 40), _word(11, 0, 255, 7), _word(200, 100, 50, 0), _word(9, 9, 9, 9)],
+                   np.uint32)
+    a = 1
+    for avg, want0, want1 in ((True, _word(10, 10, 142, 255), _word(200, 100, 50, 255)),
+                              (False, _word(11, 0, 255, 7), _word(200, 100, 50, 0))):
+        t = _scatter_tree(oracle_mod, topo)
+        t.fill_channel(0, 77.0)
+        t.scatter_colour(0, pts, clr, avg=avg)
+        _, b = _brick_of(t, (40, 40, 40))
+        assert b[4 + a, 3 + a, 2 + a] == want0
+        assert b[5 + a, 5 + a, 5 + a] == want1
+        assert np.count_nonzero(b != 77.0) == 2
