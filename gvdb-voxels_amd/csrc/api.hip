@@ -111,6 +111,11 @@ struct gvdbx_ctx {
   hipEvent_t meta_ev = nullptr;  // marks an early meta read-back (read_meta_begin / _end)
 
   int64_t topo_version = 0, atlas_version = -1;
+  // leaf lookup table (DevTree::bmap), valid from UpdateAtlas until the tree next changes
+  Buf bmap;
+  bool bm_valid = false;
+  int bm_lo[3] = {0, 0, 0}, bm_n[3] = {0, 0, 0};
+  float last_r = 0.f;  // radius of the last rebuild
 
   // named-kernel timer: event pairs recorded on the stream, summed when read
   std::string timer_name;
@@ -217,6 +222,11 @@ DevTree tree_of(gvdbx_ctx* c) {
   T.root[0] = c->root[0];
   T.root[1] = c->root[1];
   T.root[2] = c->root[2];
+  T.bmap = c->bm_valid ? P<int>(c->bmap) : nullptr;
+  for (int a = 0; a < 3; ++a) {
+    T.bm_lo[a] = c->bm_lo[a];
+    T.bm_n[a] = c->bm_n[a];
+  }
   return T;
 }
 
@@ -299,6 +309,7 @@ struct TimedRegion {
 // ---------------------------------------------------------------- topology
 
 void clear_tree(gvdbx_ctx* c) {  // Clear (V.cpp:2487-2503)
+  c->bm_valid = false;
   for (int l = 0; l < c->levs; ++l) c->cnt[l] = 0;
   c->used0 = 0;
   c->rebuild = 1;
@@ -417,6 +428,7 @@ int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, int64_t count_le
 // (with_used: the used count was read back too).
 template <typename K>
 int alloc_unique(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, const K* sorted, bool with_used) {
+  c->bm_valid = false;  // the tree changes from here on
   if (with_used) c->used0 = c->h_meta[M_USED];
   AllocPlan plan;
   memset(&plan, 0, sizeof(plan));
@@ -456,6 +468,7 @@ int full_pass(gvdbx_ctx* c, int64_t n, const float* tr, int d) {  // ActivateBri
   // unique / allocation.  The node set and order are those of sorting all n * d keys, which
   // small inputs still do in one sort (one host read-back fewer).
   KeyFmt f = key_fmt(c, d);
+  c->bm_valid = false;
   if (n > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "point count exceeds 2^31");
   if (n * d <= kOneSortKeys) {
     CHK(ensure(c, c->keys, (size_t)(n * d) * sizeof(K)));
@@ -679,6 +692,7 @@ int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accum
   CHK(check_points(c, n));
   c->topo_version++;
   c->sc = 0;
+  c->last_r = r;
   // One read-back returns the bounding box and, when the tree is expected to stay at its
   // depth, the key count of the speculatively launched incremental box pass (it only writes
   // keys and markers, so it is simply dropped when a full rebuild follows).
@@ -884,7 +898,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
                  &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta, &c->box_region, &c->box_cnt,
                  &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->scatter_acc, &c->wvs,
-                 &c->list_clr, &c->csum, &c->cmean, &c->scatter_clr,
+                 &c->list_clr, &c->csum, &c->cmean, &c->scatter_clr, &c->bmap,
                  &c->cloud_id,
                  &c->cloud_off_dev};
   for (Buf* b : bufs)
@@ -1149,6 +1163,40 @@ int gvdbx_finish_topology(gvdbx_ctx* c, float bmin[3], float bmax[3]) {  // Comp
   return GVDBX_OK;
 }
 
+// The leaf lookup table over the brick cells around the last rebuild's points (their bounding
+// box widened by the reach of the box passes and two bricks), clipped to the root; leaves outside
+// it are still found by descent.  Not built for batched clouds (cells far apart) or when the box
+// holds more than kBrickMapMax cells.
+static int build_brick_map(gvdbx_ctx* c) {
+  c->bm_valid = false;
+  if (c->batch || c->depth < 1 || c->nslots == 0) return GVDBX_OK;
+  const int s = c->lgR[0];
+  const int64_t R0 = (int64_t)1 << s;
+  const double margin = 2.0 * (double)c->last_r + 2.0 * (double)R0 + 4.0;
+  int64_t lo[3], n[3], cells = 1;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t rlo = (int64_t)c->root[a] >> s, rn = (int64_t)1 << (c->lgR[c->depth] - s);
+    int64_t l = (int64_t)std::floor(((double)c->posmin[a] - margin) / (double)R0);
+    int64_t h = (int64_t)std::floor(((double)c->posmax[a] + margin) / (double)R0);
+    if (l < rlo) l = rlo;
+    if (h > rlo + rn - 1) h = rlo + rn - 1;
+    if (h < l) return GVDBX_OK;
+    lo[a] = l;
+    n[a] = h - l + 1;
+    cells *= n[a];
+    if (cells > kBrickMapMax) return GVDBX_OK;
+  }
+  CHK(ensure(c, c->bmap, (size_t)cells * 4));
+  for (int a = 0; a < 3; ++a) {
+    c->bm_lo[a] = (int)lo[a];
+    c->bm_n[a] = (int)n[a];
+  }
+  brick_map(c->nslots, P<int4>(c->slot_pos), P<int>(c->slot_leaf), s, c->bm_lo, c->bm_n, P<int>(c->bmap),
+            c->stream);
+  c->bm_valid = true;
+  return GVDBX_OK;
+}
+
 int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   if (!c || !c->configured) return GVDBX_ERR_ARG;
   if (c->apron < 0) return fail(c, GVDBX_ERR_STATE, "UpdateAtlas needs channel 0 (AddChannel)");
@@ -1167,6 +1215,8 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
                P<int>(c->leaf_slot), P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
   c->nslots = c->used0;
   for (int ch = 0; ch < 32; ++ch) c->nbr_apron_ok[ch] = c->bound_ok[ch] = false;
+  CHK(build_brick_map(c));
+  T = tree_of(c);
   CHK(ensure(c, c->nbr, (size_t)(c->nslots * 27 + 1) * 4));
   neighbor_table(T, c->nslots, P<int4>(c->slot_pos), P<int>(c->leaf_slot), res_of(c, 0), P<int>(c->nbr), c->stream);
   CHK(reserve_channels(c, c->nslots));
@@ -1278,6 +1328,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   A.spos = P<float4>(c->spos);
   A.svel = with_vel ? P<float4>(c->svel) : nullptr;
   A.nbm = P<int>(c->nbm);
+  A.nbr = slot_map ? P<int>(c->nbr) : nullptr;
   A.list_pos = nullptr;
   A.list_vel = nullptr;
   A.list_clr = nullptr;

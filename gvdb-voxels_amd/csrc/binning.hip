@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(256) k_home_keys(DevTree T, PointsView P, cons
     const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
     if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
       const int3 o = cloud_origin(P, i);
-      const int64_t lf = descend(T, (int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z, 0);
+      const int64_t lf = find_leaf(T, (int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z);
       if (lf >= 0) key = (uint32_t)map[lf];
     }
   }
@@ -193,7 +193,7 @@ __device__ __forceinline__ int64_t direct_subcell(const BinArgs& A, int x, int y
   x += ox;
   y += oy;
   z += oz;
-  const int64_t lf = descend(A.T, x, y, z, 0);
+  const int64_t lf = find_leaf(A.T, x, y, z);
   if (lf < 0) return -1;
   const int msk = A.c.res0 - 1, D = A.c.D;
   return (int64_t)A.map[lf] * (D * D * D) + ((((z & msk) >> A.c.lgsc) * D + ((y & msk) >> A.c.lgsc)) * D) +
@@ -226,10 +226,14 @@ __device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, Br
   B.nb = nb;
   if (threadIdx.x < 27 && stored) {
     nb[threadIdx.x] = A.nbm[(int64_t)m * 27 + threadIdx.x];
+  } else if (threadIdx.x < 27 && A.nbr) {
+    // slot map: m is the leaf's slot, and the atlas neighbour table holds the slots (= map
+    // indices) of the 27 bricks around it, in this order
+    nb[threadIdx.x] = A.nbr[(int64_t)m * 27 + threadIdx.x];
   } else if (threadIdx.x < 27) {
     const int t = threadIdx.x;
     const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-    const int64_t lf = descend(A.T, gx + dx * A.c.res0, gy + dy * A.c.res0, gz + dz * A.c.res0, 0);
+    const int64_t lf = find_leaf(A.T, gx + dx * A.c.res0, gy + dy * A.c.res0, gz + dz * A.c.res0);
     nb[t] = lf < 0 ? -1 : A.map[lf];
   }
   return true;
@@ -1079,8 +1083,28 @@ __global__ void k_neighbor_table(DevTree T, int64_t nslots, const int4* __restri
   const int k = (int)(i % 27);
   const int4 bp = slot_pos[s];
   const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
-  const int64_t lf = descend(T, bp.x + dx * res0, bp.y + dy * res0, bp.z + dz * res0, 0);
+  const int64_t lf = find_leaf(T, bp.x + dx * res0, bp.y + dy * res0, bp.z + dz * res0);
   nbr[i] = lf < 0 ? -1 : leaf_slot[lf];
+}
+
+// The leaf lookup table: every slot's leaf at its brick cell (cells were cleared to -1).
+__global__ void k_brick_map(int64_t nslots, const int4* __restrict__ slot_pos, const int* __restrict__ slot_leaf,
+                            int lg0, int lo0, int lo1, int lo2, int n0, int n1, int n2, int* __restrict__ bmap) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int4 p = slot_pos[s];
+  const unsigned bx = (unsigned)((p.x >> lg0) - lo0), by = (unsigned)((p.y >> lg0) - lo1),
+                 bz = (unsigned)((p.z >> lg0) - lo2);
+  if (bx < (unsigned)n0 && by < (unsigned)n1 && bz < (unsigned)n2)
+    bmap[((int64_t)bz * n1 + by) * n0 + bx] = slot_leaf[s];
+}
+
+void brick_map(int64_t nslots, const int4* slot_pos, const int* slot_leaf, int lg0, const int lo[3], const int n[3],
+               int* bmap, hipStream_t s) {
+  fill_u32((uint32_t*)bmap, (int64_t)n[0] * n[1] * n[2], 0xFFFFFFFFu, s);
+  if (nslots <= 0) return;
+  hipLaunchKernelGGL(k_brick_map, dim3(nblocks(nslots, 256)), dim3(256), 0, s, nslots, slot_pos, slot_leaf, lg0,
+                     lo[0], lo[1], lo[2], n[0], n[1], n[2], bmap);
 }
 
 void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int* nbr,

@@ -33,6 +33,7 @@ struct BinArgs {
   const float4* spos;   // sorted positions (q = p + trans; w = point index bits)
   const float4* svel;   // sorted velocities, or null
   int* nbm;             // [nmap][27] neighbour subcell-map indices, written by the count kernel
+  const int* nbr;       // the atlas neighbour table when the subcell map is the slot map, or null
   float* list_pos;
   float* list_vel;
   uint32_t* list_clr;   // the listed points' colours, or null
@@ -89,6 +90,11 @@ struct ApronChannels {
   float* ptr[kMaxApronChannels];
   float bound[kMaxApronChannels];
 };
+// Leaf lookup table over the box of brick cells [lo, lo + n) (DevTree::bmap): cleared to -1,
+// then every slot's leaf written at its cell.
+constexpr int64_t kBrickMapMax = (int64_t)1 << 24;
+void brick_map(int64_t nslots, const int4* slot_pos, const int* slot_leaf, int lg0, const int lo[3], const int n[3],
+               int* bmap, hipStream_t s);
 void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int* nbr,
                     hipStream_t s);
 void apron_multi(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s);

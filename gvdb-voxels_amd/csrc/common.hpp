@@ -28,6 +28,12 @@ struct DevTree {
   int root[3];               // root corner (aligned to its range)
   gvdbx_node* nodes[kMaxLev];
   uint64_t* clist[kMaxLev];
+  // Leaf lookup table over a box of leaf-brick cells inside the root, or null: cell
+  // (bx, by, bz) (brick coordinates, global) at bmap[((bz - lo) * n1 + by - lo) * n0 + bx - lo]
+  // holds the leaf whose brick it is, -1 where none is linked.  Built from the slots, which are
+  // exactly the linked leaves, so a lookup inside the box equals the descent.
+  const int* bmap;
+  int bm_lo[3], bm_n[3];
 };
 
 // A dispatch carries at most 2^32 - 1 work-items per grid dimension (the grid sizes of
@@ -69,6 +75,24 @@ __device__ __forceinline__ int64_t descend(const DevTree& T, int x, int y, int z
     cur = (int64_t)(e >> 16);
   }
   return cur;
+}
+
+// The lookup table's leaf for voxel (x, y, z): the leaf index, -1 when no leaf is linked there,
+// -2 when there is no table or the voxel's brick lies outside its box.
+__device__ __forceinline__ int64_t table_leaf(const DevTree& T, int x, int y, int z) {
+  if (!T.bmap) return -2;
+  const int s = T.lgR[0];
+  const unsigned bx = (unsigned)((x >> s) - T.bm_lo[0]), by = (unsigned)((y >> s) - T.bm_lo[1]),
+                 bz = (unsigned)((z >> s) - T.bm_lo[2]);
+  if (bx >= (unsigned)T.bm_n[0] || by >= (unsigned)T.bm_n[1] || bz >= (unsigned)T.bm_n[2]) return -2;
+  return T.bmap[((int64_t)bz * T.bm_n[1] + by) * T.bm_n[0] + bx];
+}
+
+// The leaf holding voxel (x, y, z), or -1: descend(T, x, y, z, 0), through the lookup table
+// when the voxel's brick lies in its box.
+__device__ __forceinline__ int64_t find_leaf(const DevTree& T, int x, int y, int z) {
+  const int64_t lf = table_leaf(T, x, y, z);
+  return lf != -2 ? lf : descend(T, x, y, z, 0);
 }
 
 // |d|^2 with a fixed operation order, so every kernel that reduces distances (gathers,

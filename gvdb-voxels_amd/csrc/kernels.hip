@@ -138,7 +138,8 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
       // child-list loads of one level are independent and in flight at once
       int cx[8], cy[8], cz[8], miss[8];
       int64_t cur[8];
-      bool act[8];
+      bool act[8], walk[8];
+      bool any_walk = false;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         cx[k] = lo[0] + (k & 1) * R0;
@@ -147,17 +148,23 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
         act[k] = cx[k] <= hi[0] && cy[k] <= hi[1] && cz[k] <= hi[2] && in_root(T, cx[k], cy[k], cz[k]);
         cur[k] = 0;
         miss[k] = -1;
+        // a leaf found in the lookup table needs no descent; a missing one still descends, to
+        // find the level where its chain breaks
+        const int64_t lf = act[k] ? table_leaf(T, cx[k], cy[k], cz[k]) : -2;
+        if (lf >= 0) cur[k] = lf;
+        walk[k] = act[k] && lf < 0;
+        any_walk = any_walk || walk[k];
       }
-      for (int l = d; l > 0; --l) {
+      for (int l = d; l > 0 && any_walk; --l) {
         uint64_t e[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-          e[k] = (act[k] && miss[k] < 0)
+          e[k] = (walk[k] && miss[k] < 0)
                      ? T.clist[l][((uint64_t)cur[k] << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, cx[k], cy[k], cz[k])]
                      : 0ull;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-          if (!(act[k] && miss[k] < 0)) continue;
+          if (!(walk[k] && miss[k] < 0)) continue;
           if (e[k] == kUndef64) miss[k] = l - 1;
           else cur[k] = (int64_t)(e[k] >> 16);
         }
@@ -170,6 +177,11 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
         for (int y = lo[1]; y <= hi[1]; y += R0)
           for (int z = lo[2]; z <= hi[2]; z += R0) {
             if (!in_root(T, x, y, z)) continue;
+            const int64_t lf = table_leaf(T, x, y, z);
+            if (lf >= 0) {  // found in the lookup table
+              emit(x, y, z, lf, -1);
+              continue;
+            }
             int64_t cur = 0;
             int miss = -1;  // highest missing level along the chain
             for (int l = d; l > 0; --l) {
@@ -709,7 +721,7 @@ __global__ void k_sample(DevTree T, const int* __restrict__ leaf_slot, int res0,
   const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
   if (!(px >= 0.f && py >= 0.f && pz >= 0.f)) { out[i] = 0.f; return; }
   const int ix = (int)px, iy = (int)py, iz = (int)pz;
-  const int64_t lf = descend(T, ix, iy, iz, 0);
+  const int64_t lf = find_leaf(T, ix, iy, iz);
   const int slot = lf < 0 ? -1 : leaf_slot[lf];
   if (slot < 0) { out[i] = 0.f; return; }
   const int m = ~(res0 - 1);
