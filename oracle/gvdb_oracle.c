@@ -426,7 +426,10 @@ int orc_accumulate_topology(orc_tree* t, const float* pos, int64_t n, int64_t st
   int s = check_depth(t, depth);
   if (s) return s;
   if (t->rebuild) return orc_rebuild_topology(t, pos, n, st, r, tr);
-  activate_incre(t, pos, n, st, tr, r, depth, 1);
+  /* The reference hands the new points' depth and root to the incremental pass over a tree of
+   * the old depth (V.cpp:1177), which is undefined when they differ; the pass keeps the tree's
+   * depth and root (points outside the root add nothing), as the library does. */
+  activate_incre(t, pos, n, st, tr, r, t->depth, 1);
   return ORC_OK;
 }
 
