@@ -367,7 +367,10 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_
   }
 }
 
-template <int kD>
+// kTwo (2r + 2 <= sc): a point's subcell box [trunc(q - r), trunc(q + 1 + r)] spans under one
+// subcell per axis, so it reaches at most 2 subcells per axis, all inside its brick's window; the
+// up to 8 entries then take their list slots with independent LDS atomics issued back to back.
+template <int kD, bool kTwo>
 __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
   extern __shared__ unsigned long long smem64[];
   __shared__ int nb[27];
@@ -395,6 +398,33 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
     const uint32_t clr = A.list_clr ? load_u32(A.P.clr, __float_as_uint(p.w), A.P.clr_off, A.P.clr_stride) : 0u;
     int lo[3], hi[3];
     subcell_range(q, c.r, c.sc, lo, hi);
+    if (kTwo) {
+      int slot[8];
+      bool act[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int x = lo[0] + (e & 1) * c.sc, y = lo[1] + ((e >> 1) & 1) * c.sc, z = lo[2] + (e >> 2) * c.sc;
+        const int w = (x <= hi[0] && y <= hi[1] && z <= hi[2]) ? B.window_of(x, y, z, c.lgsc) : -1;
+        const int base = w >= 0 ? lbase[w] : -1;
+        act[e] = base >= 0;
+        slot[e] = act[e] ? base + atomicAdd(&lcnt[w], 1) : 0;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        if (!act[e]) continue;
+        const int64_t k = slot[e];
+        A.list_pos[3 * k + 0] = q[0];
+        A.list_pos[3 * k + 1] = q[1];
+        A.list_pos[3 * k + 2] = q[2];
+        if (A.list_vel) {
+          A.list_vel[3 * k + 0] = v.x;
+          A.list_vel[3 * k + 1] = v.y;
+          A.list_vel[3 * k + 2] = v.z;
+        }
+        if (A.list_clr) A.list_clr[k] = clr;
+      }
+      continue;
+    }
     for (int x = lo[0]; x <= hi[0]; x += c.sc)
       for (int y = lo[1]; y <= hi[1]; y += c.sc)
         for (int z = lo[2]; z <= hi[2]; z += c.sc) {
@@ -455,10 +485,15 @@ void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
   const size_t sm = (size_t)2 * A.W * A.W * A.W * 4 + 8;
   const int bs = brick_threads(A.P.n, nmap, 64);
-  if (A.c.D == 2)
-    hipLaunchKernelGGL(k_bin_insert<2>, item_grid(nmap), dim3(bs), sm, s, A, nmap);
+  const bool two = 2.0 * (double)A.c.r + 2.0 <= (double)A.c.sc;
+  if (A.c.D == 2 && two)
+    hipLaunchKernelGGL((k_bin_insert<2, true>), item_grid(nmap), dim3(bs), sm, s, A, nmap);
+  else if (A.c.D == 2)
+    hipLaunchKernelGGL((k_bin_insert<2, false>), item_grid(nmap), dim3(bs), sm, s, A, nmap);
+  else if (two)
+    hipLaunchKernelGGL((k_bin_insert<0, true>), item_grid(nmap), dim3(bs), sm, s, A, nmap);
   else
-    hipLaunchKernelGGL(k_bin_insert<0>, item_grid(nmap), dim3(bs), sm, s, A, nmap);
+    hipLaunchKernelGGL((k_bin_insert<0, false>), item_grid(nmap), dim3(bs), sm, s, A, nmap);
 }
 
 __global__ void k_sum_counts(const int* __restrict__ a, const int* __restrict__ b, int* __restrict__ out, int64_t n) {
