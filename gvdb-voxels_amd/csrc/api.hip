@@ -111,6 +111,7 @@ struct gvdbx_ctx {
   hipEvent_t meta_ev = nullptr;  // marks an early meta read-back (read_meta_begin / _end)
 
   int64_t topo_version = 0, atlas_version = -1;
+  int64_t new_nodes[kMaxLev] = {};  // per level: nodes the last allocation appended
   // leaf lookup table (DevTree::bmap), valid from UpdateAtlas until the tree next changes
   Buf bmap;
   bool bm_valid = false;
@@ -447,14 +448,20 @@ int alloc_unique(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, const K* sorted, bool
     if (levcnt[l])
       fill_u64(P<uint64_t>(c->clist[l]) + c->cnt[l] * clist_per(c, l), levcnt[l] * clist_per(c, l), kUndef64,
                c->stream);
+  for (int l = 0; l < kMaxLev; ++l) c->new_nodes[l] = l < d ? levcnt[l] : 0;
   for (int l = 0; l < d; ++l) c->cnt[l] += levcnt[l];
   c->used0 += levcnt[0];
   return GVDBX_OK;
 }
 
-void link_all(gvdbx_ctx* c, int d) {  // per-level top-down gvdbLinkBricks launches
+// Per-level top-down gvdbLinkBricks launches over the nodes the last allocation appended: the
+// older nodes are linked already (live ones since they were appended; dead ones are unflagged,
+// and a delinked leaf is never found again, so it never comes back), and relinking them would
+// write the same entries.
+void link_all(gvdbx_ctx* c, int d) {
   DevTree T = tree_of(c);
-  for (int l = d - 1; l >= 0; --l) link_level(T, l, c->cnt[l], c->stream);
+  for (int l = d - 1; l >= 0; --l)
+    if (c->new_nodes[l] > 0) link_level(T, l, c->cnt[l] - c->new_nodes[l], c->new_nodes[l], c->stream);
 }
 
 // full rebuilds of up to this many keys (points x levels) sort them all at once
@@ -1167,9 +1174,10 @@ int gvdbx_finish_topology(gvdbx_ctx* c, float bmin[3], float bmax[3]) {  // Comp
 // box widened by the reach of the box passes and two bricks), clipped to the root; leaves outside
 // it are still found by descent.  Not built for batched clouds (cells far apart) or when the box
 // holds more than kBrickMapMax cells.
-static int build_brick_map(gvdbx_ctx* c) {
+static int plan_brick_map(gvdbx_ctx* c, BrickMapOut* out, bool* use) {
   c->bm_valid = false;
-  if (c->batch || c->depth < 1 || c->nslots == 0) return GVDBX_OK;
+  *use = false;
+  if (c->batch || c->depth < 1 || c->used0 == 0) return GVDBX_OK;
   const int s = c->lgR[0];
   const int64_t R0 = (int64_t)1 << s;
   const double margin = 2.0 * (double)c->last_r + 2.0 * (double)R0 + 4.0;
@@ -1187,13 +1195,13 @@ static int build_brick_map(gvdbx_ctx* c) {
     if (cells > kBrickMapMax) return GVDBX_OK;
   }
   CHK(ensure(c, c->bmap, (size_t)cells * 4));
+  out->bmap = P<int>(c->bmap);
+  out->lg0 = s;
   for (int a = 0; a < 3; ++a) {
-    c->bm_lo[a] = (int)lo[a];
-    c->bm_n[a] = (int)n[a];
+    out->lo[a] = (int)lo[a];
+    out->n[a] = (int)n[a];
   }
-  brick_map(c->nslots, P<int4>(c->slot_pos), P<int>(c->slot_leaf), s, c->bm_lo, c->bm_n, P<int>(c->bmap),
-            c->stream);
-  c->bm_valid = true;
+  *use = true;
   return GVDBX_OK;
 }
 
@@ -1207,15 +1215,26 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   CHK(ensure(c, c->leaf_slot, (size_t)(total + 1) * 4));
   CHK(ensure(c, c->slot_leaf, (size_t)(total + 1) * 4));
   CHK(ensure(c, c->slot_pos, (size_t)(total + 1) * 16));
+  // the lookup table is cleared by the flag launch and filled by the slot launch
+  BrickMapOut tab;
+  bool use_tab = false;
+  CHK(plan_brick_map(c, &tab, &use_tab));
   DevTree T = tree_of(c);
   const int br = res_of(c, 0) + 2 * c->apron;
-  leaf_flags(T, total, 0, P<int>(c->flags), c->stream);
+  leaf_flags(T, total, 0, P<int>(c->flags), c->stream, use_tab ? &tab : nullptr);
   exclusive_scan(P<int>(c->flags), P<int>(c->rank), total, P<int>(c->tile_buf), nullptr, c->stream);
   assign_slots(T, P<int>(c->flags), P<int>(c->rank), total, c->apron, br, c->axis_cx, c->axis_cy,
-               P<int>(c->leaf_slot), P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream);
+               P<int>(c->leaf_slot), P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream,
+               use_tab ? &tab : nullptr);
   c->nslots = c->used0;
   for (int ch = 0; ch < 32; ++ch) c->nbr_apron_ok[ch] = c->bound_ok[ch] = false;
-  CHK(build_brick_map(c));
+  if (use_tab) {
+    for (int a = 0; a < 3; ++a) {
+      c->bm_lo[a] = tab.lo[a];
+      c->bm_n[a] = tab.n[a];
+    }
+    c->bm_valid = true;
+  }
   T = tree_of(c);
   CHK(ensure(c, c->nbr, (size_t)(c->nslots * 27 + 1) * 4));
   neighbor_table(T, c->nslots, P<int4>(c->slot_pos), P<int>(c->leaf_slot), res_of(c, 0), P<int>(c->nbr), c->stream);

@@ -1122,26 +1122,6 @@ __global__ void k_neighbor_table(DevTree T, int64_t nslots, const int4* __restri
   nbr[i] = lf < 0 ? -1 : leaf_slot[lf];
 }
 
-// The leaf lookup table: every slot's leaf at its brick cell (cells were cleared to -1).
-__global__ void k_brick_map(int64_t nslots, const int4* __restrict__ slot_pos, const int* __restrict__ slot_leaf,
-                            int lg0, int lo0, int lo1, int lo2, int n0, int n1, int n2, int* __restrict__ bmap) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
-  const int4 p = slot_pos[s];
-  const unsigned bx = (unsigned)((p.x >> lg0) - lo0), by = (unsigned)((p.y >> lg0) - lo1),
-                 bz = (unsigned)((p.z >> lg0) - lo2);
-  if (bx < (unsigned)n0 && by < (unsigned)n1 && bz < (unsigned)n2)
-    bmap[((int64_t)bz * n1 + by) * n0 + bx] = slot_leaf[s];
-}
-
-void brick_map(int64_t nslots, const int4* slot_pos, const int* slot_leaf, int lg0, const int lo[3], const int n[3],
-               int* bmap, hipStream_t s) {
-  fill_u32((uint32_t*)bmap, (int64_t)n[0] * n[1] * n[2], 0xFFFFFFFFu, s);
-  if (nslots <= 0) return;
-  hipLaunchKernelGGL(k_brick_map, dim3(nblocks(nslots, 256)), dim3(256), 0, s, nslots, slot_pos, slot_leaf, lg0,
-                     lo[0], lo[1], lo[2], n[0], n[1], n[2], bmap);
-}
-
 void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int* nbr,
                     hipStream_t s) {
   if (nslots <= 0) return;

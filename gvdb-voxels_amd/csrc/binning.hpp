@@ -90,11 +90,8 @@ struct ApronChannels {
   float* ptr[kMaxApronChannels];
   float bound[kMaxApronChannels];
 };
-// Leaf lookup table over the box of brick cells [lo, lo + n) (DevTree::bmap): cleared to -1,
-// then every slot's leaf written at its cell.
+// largest leaf lookup table (DevTree::bmap), in cells
 constexpr int64_t kBrickMapMax = (int64_t)1 << 24;
-void brick_map(int64_t nslots, const int4* slot_pos, const int* slot_leaf, int lg0, const int lo[3], const int n[3],
-               int* bmap, hipStream_t s);
 void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int* nbr,
                     hipStream_t s);
 void apron_multi(int64_t nslots, const int* nbr, int res0, int apron, const ApronChannels& ch, hipStream_t s);

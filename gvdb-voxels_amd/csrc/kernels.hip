@@ -4,6 +4,7 @@
 // (P.cuh = kernels/cuda_gvdb_particles.cuh, N.cuh = kernels/cuda_gvdb_nodes.cuh,
 //  O.cuh = kernels/cuda_gvdb_operators.cuh, R.cuh = kernels/cuda_gvdb_raycast.cuh).
 #include <limits.h>
+#include <string.h>
 
 #include "kernels.hpp"
 #include "primitives.hpp"
@@ -399,9 +400,9 @@ __global__ void k_setup_root(DevTree T) {
 void setup_root(const DevTree& T, hipStream_t s) { hipLaunchKernelGGL(k_setup_root, dim3(1), dim3(1), 0, s, T); }
 
 // gvdbLinkBricks (N.cuh:470-505): flagged node -> parent by descent, parent's child slot.
-__global__ void k_link_level(DevTree T, int lev, int64_t count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+__global__ void k_link_level(DevTree T, int lev, int64_t first, int64_t count) {
+  const int64_t i = first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= first + count) return;
   gvdbx_node* nd = T.nodes[lev] + i;
   if (!nd->flags) return;
   const int x = nd->pos[0], y = nd->pos[1], z = nd->pos[2];
@@ -411,9 +412,9 @@ __global__ void k_link_level(DevTree T, int lev, int64_t count) {
   T.clist[lev + 1][((uint64_t)pi << (3 * T.ld[lev + 1])) + (uint64_t)child_bit(T, lev + 1, x, y, z)] =
       elem0(lev, (uint64_t)i);
 }
-void link_level(const DevTree& T, int lev, int64_t count, hipStream_t s) {
+void link_level(const DevTree& T, int lev, int64_t first, int64_t count, hipStream_t s) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_link_level, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, lev, count);
+  hipLaunchKernelGGL(k_link_level, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, lev, first, count);
 }
 
 // Leaf flags <- markers (V.cpp:1075-1081).
@@ -484,15 +485,20 @@ void delink_level(const DevTree& T, int lev, int64_t count, hipStream_t s) {
 
 // by_parent = 0: mFlags (UpdateAtlas V.cpp:2655-2661); 1: mParent != ID_UNDEF64
 // (gvdbSetFlagSubcell P.cuh:109-118).
-__global__ void k_leaf_flags(DevTree T, int64_t count, int by_parent, int* __restrict__ out) {
+__global__ void k_leaf_flags(DevTree T, int64_t count, int by_parent, int* __restrict__ out,
+                             int* __restrict__ clear, int64_t nclear) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nclear) clear[i] = -1;
   if (i >= count) return;
   const gvdbx_node* nd = T.nodes[0] + i;
   out[i] = by_parent ? (nd->parent != kUndef64 ? 1 : 0) : (nd->flags ? 1 : 0);
 }
-void leaf_flags(const DevTree& T, int64_t count, int by_parent, int* out, hipStream_t s) {
-  if (count <= 0) return;
-  hipLaunchKernelGGL(k_leaf_flags, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, count, by_parent, out);
+void leaf_flags(const DevTree& T, int64_t count, int by_parent, int* out, hipStream_t s, const BrickMapOut* clear) {
+  const int64_t nclear = clear ? (int64_t)clear->n[0] * clear->n[1] * clear->n[2] : 0;
+  const int64_t items = count > nclear ? count : nclear;
+  if (items <= 0) return;
+  hipLaunchKernelGGL(k_leaf_flags, dim3(blocks_for(items, 256)), dim3(256), 0, s, T, count, by_parent, out,
+                     clear ? clear->bmap : nullptr, nclear);
 }
 
 __global__ void __launch_bounds__(256) k_count_flags(const int* __restrict__ f, int64_t n, int* out) {
@@ -551,18 +557,29 @@ __device__ __forceinline__ void assign_slot(const DevTree& T, int64_t i, int s, 
 
 __global__ void k_assign_slots(DevTree T, const int* __restrict__ flags, const int* __restrict__ rank, int64_t count,
                                int apron, int br, int cx, int cy, int* __restrict__ leaf_slot,
-                               int* __restrict__ slot_leaf, int4* __restrict__ slot_pos) {
+                               int* __restrict__ slot_leaf, int4* __restrict__ slot_pos, BrickMapOut tab) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   if (!flags[i]) { leaf_slot[i] = -1; return; }
   assign_slot(T, i, rank[i], apron, br, cx, cy, leaf_slot, slot_leaf, slot_pos);
+  if (tab.bmap) {  // the leaf's cell of the lookup table
+    const gvdbx_node* nd = T.nodes[0] + i;
+    const unsigned bx = (unsigned)((nd->pos[0] >> tab.lg0) - tab.lo[0]),
+                   by = (unsigned)((nd->pos[1] >> tab.lg0) - tab.lo[1]),
+                   bz = (unsigned)((nd->pos[2] >> tab.lg0) - tab.lo[2]);
+    if (bx < (unsigned)tab.n[0] && by < (unsigned)tab.n[1] && bz < (unsigned)tab.n[2])
+      tab.bmap[((int64_t)bz * tab.n[1] + by) * tab.n[0] + bx] = (int)i;
+  }
 }
 
 void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t count, int apron, int br, int cx, int cy,
-                  int* leaf_slot, int* slot_leaf, int4* slot_pos, hipStream_t s) {
+                  int* leaf_slot, int* slot_leaf, int4* slot_pos, hipStream_t s, const BrickMapOut* table) {
   if (count <= 0) return;
+  BrickMapOut tab;
+  if (table) tab = *table;
+  else memset(&tab, 0, sizeof(tab));
   hipLaunchKernelGGL(k_assign_slots, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, flags, rank, count, apron, br,
-                     cx, cy, leaf_slot, slot_leaf, slot_pos);
+                     cx, cy, leaf_slot, slot_leaf, slot_pos, tab);
 }
 
 // ================================================================ subcells

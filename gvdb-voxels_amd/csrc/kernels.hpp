@@ -69,17 +69,28 @@ template <typename K>
 void setup_nodes(const DevTree& T, const K* k, const int* flags, const int* rank, int64_t m, KeyFmt f,
                  const AllocPlan& plan, hipStream_t s);
 void setup_root(const DevTree& T, hipStream_t s);
-void link_level(const DevTree& T, int lev, int64_t count, hipStream_t s);
+// gvdbLinkBricks over the level-lev nodes [first, first + count)
+void link_level(const DevTree& T, int lev, int64_t first, int64_t count, hipStream_t s);
 void apply_markers(const DevTree& T, const int* markers, int64_t count, hipStream_t s);
 void delink_leaves(const DevTree& T, int64_t count, hipStream_t s);
 void delink_level(const DevTree& T, int lev, int64_t count, hipStream_t s);
-void leaf_flags(const DevTree& T, int64_t count, int by_parent, int* out, hipStream_t s);
+// Leaf lookup table being built (DevTree::bmap): cells [lo, lo + n) of 2^lg0-voxel bricks, or none
+struct BrickMapOut {
+  int* bmap;
+  int lg0;
+  int lo[3], n[3];
+};
+// clear: when given, its table is also cleared to -1 by the same launch
+void leaf_flags(const DevTree& T, int64_t count, int by_parent, int* out, hipStream_t s,
+                const BrickMapOut* clear = nullptr);
 void count_flags(const int* flags, int64_t n, int* out, hipStream_t s);
 void copy_flags_to_markers(const DevTree& T, int64_t count, int* markers, hipStream_t s);
 
 // ---- atlas
+// table: when given, every flagged leaf is also written at its cell (cleared by leaf_flags)
 void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t count, int apron, int br, int cx,
-                  int cy, int* leaf_slot, int* slot_leaf, int4* slot_pos, hipStream_t s);
+                  int cy, int* leaf_slot, int* slot_leaf, int4* slot_pos, hipStream_t s,
+                  const BrickMapOut* table = nullptr);
 
 // ---- subcells
 struct SubcellCfg {
