@@ -740,7 +740,19 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
 #pragma unroll
     for (int k = 0; k <= kFixedD3; ++k) so[k] = __builtin_amdgcn_readfirstlane(soff[k]);
   }
-  for (int j = soff[0] + (int)threadIdx.x; j < soff[D3]; j += blockDim.x) {
+  // each entry's position is loaded one iteration ahead, so its latency overlaps the
+  // previous entry's corner updates
+  const int jend = soff[D3];
+  float3 gnext = make_float3(0.f, 0.f, 0.f);
+  if (soff[0] + (int)threadIdx.x < jend) {
+    const int64_t j0 = soff[0] + (int)threadIdx.x;
+    gnext = make_float3(list_pos[3 * j0], list_pos[3 * j0 + 1], list_pos[3 * j0 + 2]);
+  }
+  for (int j = soff[0] + (int)threadIdx.x; j < jend; j += blockDim.x) {
+    const float3 gcur = gnext;
+    const int jn = j + (int)blockDim.x;
+    if (jn < jend) gnext = make_float3(list_pos[3 * (int64_t)jn], list_pos[3 * (int64_t)jn + 1],
+                                       list_pos[3 * (int64_t)jn + 2]);
     unsigned s = 0;
     if (kRes) {
 #pragma unroll
@@ -759,7 +771,7 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
     const int ly0 = iy * sc - (iy == 0 ? a : 0), ly1 = iy * sc + sc - 1 + (iy == D - 1 ? a : 0);
     const int lz0 = iz * sc - (iz == 0 ? a : 0), lz1 = iz * sc + sc - 1 + (iz == D - 1 ? a : 0);
     {
-      const float gx = list_pos[3 * (int64_t)j], gy = list_pos[3 * (int64_t)j + 1], gz = list_pos[3 * (int64_t)j + 2];
+      const float gx = gcur.x, gy = gcur.y, gz = gcur.z;
       const int fx = (int)gx - bx, fy = (int)gy - by, fz = (int)gz - bz;
       if (R == 1) {
         // corners fx + {0, 1} per axis: offsets and x squares once per entry, then each corner's
