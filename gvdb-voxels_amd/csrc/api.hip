@@ -1348,6 +1348,7 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   A.svel = with_vel ? P<float4>(c->svel) : nullptr;
   A.nbm = P<int>(c->nbm);
   A.nbr = slot_map ? P<int>(c->nbr) : nullptr;
+  A.slot_pos = slot_map ? P<int4>(c->slot_pos) : nullptr;
   A.list_pos = nullptr;
   A.list_vel = nullptr;
   A.list_clr = nullptr;
@@ -1433,7 +1434,8 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
   const int br = cfg.res0 + 2 * c->apron;
   if (mode == 0 && !accum && cfg.res0 <= 8 && br * br * br <= kMaxPaddedVox) {
     density_brick(cfg, (int)c->used0, c->nsc, c->sclen, P<int>(c->map_leaf), P<int>(c->sc_off),
-                  P<float>(c->list_pos), P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]), c->apron,
+                  P<float>(c->list_pos), P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]),
+                  c->sc_map_current ? nullptr : P<int4>(c->slot_pos), c->apron,
                   c->apron > 0 ? P<int>(c->nbr) : nullptr, P<float>(c->chan[chan]), nullptr, c->stream);
     c->nbr_apron_ok[chan] = c->bound_ok[chan] = c->apron > 0;
     c->bound_val[chan] = 0.0f;
@@ -1543,8 +1545,8 @@ int gvdbx_gather_density_velocity(gvdbx_ctx* c, int sc, int64_t n, float r, cons
   const VelOut vo{P<int>(c->sc_cnt), P<unsigned long long>(c->vsum),
                   {P<float>(c->chan[cx]), P<float>(c->chan[cy]), P<float>(c->chan[cz])}};
   density_brick(cfg, (int)c->used0, c->nsc, c->sclen, P<int>(c->map_leaf), P<int>(c->sc_off), P<float>(c->list_pos),
-                P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]), c->apron, P<int>(c->nbr), P<float>(c->chan[chan]),
-                &vo, c->stream);
+                P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]), c->sc_map_current ? nullptr : P<int4>(c->slot_pos),
+                c->apron, P<int>(c->nbr), P<float>(c->chan[chan]), &vo, c->stream);
   for (int ch : {chan, cx, cy, cz}) {
     c->nbr_apron_ok[ch] = c->bound_ok[ch] = true;
     c->bound_val[ch] = 0.0f;

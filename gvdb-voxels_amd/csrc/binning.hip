@@ -210,10 +210,20 @@ __device__ __forceinline__ void add_vel_global(unsigned long long* vsum, int64_t
 // kernel of the same insert stored them); returns false for an unused map entry.
 __device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, BrickCtx& B, int D, int W,
                                             bool stored = false) {
-  const int leaf = A.map_leaf[m];
-  if (leaf < 0) return false;
-  const gvdbx_node* nd = A.T.nodes[0] + leaf;
-  const int gx = nd->pos[0], gy = nd->pos[1], gz = nd->pos[2];
+  int gx, gy, gz;
+  if (A.slot_pos) {  // slot map: brick m is slot m, which always holds a leaf
+    const int4 p = A.slot_pos[m];
+    gx = p.x;
+    gy = p.y;
+    gz = p.z;
+  } else {
+    const int leaf = A.map_leaf[m];
+    if (leaf < 0) return false;
+    const gvdbx_node* nd = A.T.nodes[0] + leaf;
+    gx = nd->pos[0];
+    gy = nd->pos[1];
+    gz = nd->pos[2];
+  }
   B.bx = gx & A.c.cmask;
   B.by = gy & A.c.cmask;
   B.bz = gz & A.c.cmask;
@@ -682,7 +692,8 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
                                                       const int* __restrict__ map_leaf, const int* __restrict__ off,
                                                       const float* __restrict__ list_pos,
                                                       const int* __restrict__ leaf_slot,
-                                                      const gvdbx_node* __restrict__ nodes0, int apron_rt,
+                                                      const gvdbx_node* __restrict__ nodes0,
+                                                      const int4* __restrict__ slot_pos, int apron_rt,
                                                       const int* __restrict__ nbr, float* __restrict__ chan,
                                                       VelOut vo) {
   // LDS sized for the compile-time geometry when there is one (more bricks fit on a CU)
@@ -704,16 +715,29 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
   const int D3 = D * D * D;
   const int64_t m = item_index();
   if (m >= nmap) return;
-  const int leaf = map_leaf[m];
-  if (leaf < 0) return;
-  const int slot = leaf_slot[leaf];
+  // brick corner and slot: with the slot map (slot_pos given) brick m is slot m, one load
+  int slot, gx0, gy0, gz0;
+  if (slot_pos) {
+    const int4 sp = slot_pos[m];
+    slot = (int)m;
+    gx0 = sp.x;
+    gy0 = sp.y;
+    gz0 = sp.z;
+  } else {
+    const int leaf = map_leaf[m];
+    if (leaf < 0) return;
+    slot = leaf_slot[leaf];
+    gx0 = nodes0[leaf].pos[0];
+    gy0 = nodes0[leaf].pos[1];
+    gz0 = nodes0[leaf].pos[2];
+  }
   const int nv = bw * bw * bw;
   const int64_t k0 = (int64_t)m * D3;
   for (int t = threadIdx.x; t < nv; t += blockDim.x) d2[t] = 0x7F800000u;
   for (int s = threadIdx.x; s <= D3; s += blockDim.x) soff[s] = (k0 + s < nsc) ? off[k0 + s] : (int)len;
   if (threadIdx.x < 27) nb[threadIdx.x] = (kWithApron && slot >= 0) ? nbr[(int64_t)slot * 27 + threadIdx.x] : -1;
   __syncthreads();
-  const int bx = nodes0[leaf].pos[0] & c.cmask, by = nodes0[leaf].pos[1] & c.cmask, bz = nodes0[leaf].pos[2] & c.cmask;
+  const int bx = gx0 & c.cmask, by = gy0 & c.cmask, bz = gz0 & c.cmask;
   if (kVel) {  // means of the window subcells (own D^3 and the one-subcell shell)
     const int W = D + 2, W3 = W * W * W;
     for (int w = threadIdx.x; w < W3; w += blockDim.x) {
@@ -836,15 +860,15 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
 }
 
 void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
-                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, const int* nbr,
-                   float* chan, const VelOut* vel, hipStream_t s) {
+                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, const int4* slot_pos,
+                   int apron, const int* nbr, float* chan, const VelOut* vel, hipStream_t s) {
   if (nmap <= 0) return;
   const int bs = brick_threads(len, nmap, 256);
   const bool fixed = c.res0 == 8 && c.D == 2 && apron == 1;
   const VelOut vo = vel ? *vel : VelOut{nullptr, nullptr, {nullptr, nullptr, nullptr}};
 #define GVDBX_DENSITY(R, A, D, AP, V)                                                                             \
   hipLaunchKernelGGL((k_density_brick<R, A, D, AP, V>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, \
-                     off, list_pos, leaf_slot, nodes0, apron, nbr, chan, vo)
+                     off, list_pos, leaf_slot, nodes0, slot_pos, apron, nbr, chan, vo)
   if (vel) {  // velocity means need the apron pass (a neighbour table)
     if (fixed) GVDBX_DENSITY(8, 1, 2, true, true);
     else GVDBX_DENSITY(0, 0, 0, true, true);

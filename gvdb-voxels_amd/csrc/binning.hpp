@@ -34,6 +34,7 @@ struct BinArgs {
   const float4* svel;   // sorted velocities, or null
   int* nbm;             // [nmap][27] neighbour subcell-map indices, written by the count kernel
   const int* nbr;       // the atlas neighbour table when the subcell map is the slot map, or null
+  const int4* slot_pos; // with nbr: the slots' leaf corners (w: leaf), so brick m's corner is one load
   float* list_pos;
   float* list_vel;
   uint32_t* list_clr;   // the listed points' colours, or null
@@ -62,9 +63,10 @@ struct VelOut {
   float* ch[3];
 };
 // vel: null, or the velocity means' inputs and channels (needs nbr: the apron is computed)
+// slot_pos: the slots' corners when the subcell map is the slot map (brick m = slot m), or null
 void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
-                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, int apron, const int* nbr,
-                   float* chan, const VelOut* vel, hipStream_t s);
+                   const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, const int4* slot_pos,
+                   int apron, const int* nbr, float* chan, const VelOut* vel, hipStream_t s);
 void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* vsum,
                     const int* leaf_slot, int apron, const int* nbr, float* cx, float* cy, float* cz, int accum,
                     hipStream_t s);
