@@ -63,13 +63,21 @@ def algorithmic_bytes(n, s_len, nsc, bricks, chans=4, vel=True):
             + 8 * chans * bricks * 488)
 
 
-def gather_bytes(s_len, nsc, bricks, vel=True):
+APRON_VOX = 10 ** 3 - 8 ** 3  # apron voxels of a padded 8^3 brick with apron 1
+
+
+def gather_bytes(s_len, nsc, bricks, vel=True, apron=True):
     """Algorithmic bytes of one gather launch: list positions read, per-subcell
     count/offset/corner/leaf read, interior voxels written; with the velocity means in the same
-    pass, per-subcell count and 64-bit sums read and 3 more channels' interior written."""
+    pass, per-subcell count and 64-bit sums read and 3 more channels' interior written; with the
+    apron computed in the same pass, its voxels written once per channel (the 4-byte write of
+    the apron term 8 C B 488 of the step's bytes, whose copy the pass makes unnecessary)."""
+    chans = 4 if vel else 1
     b = s_len * 12 + nsc * (4 + 4 + 12 + 4) + bricks * 512 * 4
     if vel:
         b += nsc * (4 + 24) + 3 * bricks * 512 * 4
+    if apron:
+        b += chans * bricks * APRON_VOX * 4
     return b
 
 
@@ -380,6 +388,7 @@ def main():
     bricks = v.getNumUsedNodes(0)
     step_bytes = algorithmic_bytes(n, s_len, nsc, bricks)
     kb = gather_bytes(s_len, nsc, bricks) if args.kernel == "gather" else None
+    kb_interior = gather_bytes(s_len, nsc, bricks, apron=False) if args.kernel == "gather" else None
     k_avg_ms = k_ms / max(k_launches, 1)
     achieved = kb / (k_avg_ms * 1e-3) / 1e9 if kb else None
 
@@ -407,7 +416,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": measured_traffic("k_density_brick") if args.kernel == "gather" else None,
                      "avg_launch_ms": round(k_avg_ms, 5), "launches": int(k_launches),
-                     "algorithmic_bytes_per_launch": int(kb) if kb else None},
+                     "algorithmic_bytes_per_launch": int(kb) if kb else None,
+                     "algorithmic_bytes_interiors_only": int(kb_interior) if kb_interior else None},
     }
     del dposs
     v.close()
