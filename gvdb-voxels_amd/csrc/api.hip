@@ -88,6 +88,13 @@ struct gvdbx_ctx {
   int64_t nsc = 0, sclen = 0;
   bool lists_have_vel = false;
   bool lists_have_clr = false;  // the insert stored the points' colours (AUX_SUBCELL_PNT_CLR)
+  // InsertPointsSubcell_FP16: list positions (and velocities) as ushort3 quantized to the
+  // rebuild's bounding box (and GetMinMaxVel's velocity box)
+  bool lists_fp16 = false;
+  Buf list_q16, list_vq16;
+  float q_min[3] = {0, 0, 0}, q_range[3] = {1, 1, 1};
+  bool vel_box = false;  // GetMinMaxVel ran
+  float vel_min[3] = {0, 0, 0}, vel_range[3] = {1, 1, 1};
   bool store_vel_lists = false;
   bool have_vsum = false;
   Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
@@ -845,6 +852,14 @@ int gather_checks(gvdbx_ctx* c, int chan) {
   return GVDBX_OK;
 }
 
+// gathers that read the list positions need the float lists of InsertPointsSubcell
+int float_lists(gvdbx_ctx* c) {
+  if (c->lists_fp16)
+    return fail(c, GVDBX_ERR_STATE, "the subcell lists hold FP16 positions (InsertPointsSubcell_FP16): "
+                                    "only GatherLevelSet_FP16 reads them");
+  return GVDBX_OK;
+}
+
 }  // namespace
 
 // ======================================================================= C ABI
@@ -905,7 +920,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
                  &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta, &c->box_region, &c->box_cnt,
                  &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->scatter_acc, &c->wvs,
-                 &c->list_clr, &c->csum, &c->cmean, &c->scatter_clr, &c->bmap,
+                 &c->list_clr, &c->csum, &c->cmean, &c->scatter_clr, &c->bmap, &c->list_q16, &c->list_vq16,
                  &c->cloud_id,
                  &c->cloud_off_dev};
   for (Buf* b : bufs)
@@ -1243,8 +1258,9 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   return launch_status(c, "UpdateAtlas");
 }
 
-int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t* len) {
-  CHK(check_points(c, n));  // InsertPointsSubcell (V.cpp:4971-5092)
+static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t* len, bool fp16) {
+  CHK(check_points(c, n));  // InsertPointsSubcell (V.cpp:4971-5092), _FP16 (V.cpp:5094-5189)
+  if (fp16 && c->batch) return fail(c, GVDBX_ERR_ARG, "InsertPointsSubcell_FP16 is not for batched clouds");
   if (c->batch) {
     if (r > c->batch_rmax) return fail(c, GVDBX_ERR_ARG, "radius %g exceeds the batch guard radius %g", (double)r,
                                        (double)c->batch_rmax);
@@ -1352,6 +1368,14 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   A.list_pos = nullptr;
   A.list_vel = nullptr;
   A.list_clr = nullptr;
+  A.list_q16 = nullptr;
+  A.list_vq16 = nullptr;
+  for (int a = 0; a < 3; ++a) {  // quantization boxes: the rebuild's bounding box, GetMinMaxVel's box
+    A.qmin[a] = c->posmin[a];
+    A.qrange[a] = c->posmax[a] - c->posmin[a];
+    A.vqmin[a] = c->vel_min[a];
+    A.vqrange[a] = c->vel_range[a];
+  }
   A.vsum = with_vel ? P<unsigned long long>(c->vsum) : nullptr;
   if (with_vel) CHK(ensure(c, c->wvs, (size_t)((int64_t)nmap * W * W * W * 3 + 1) * 8));
   A.wvs = with_vel ? P<unsigned long long>(c->wvs) : nullptr;
@@ -1371,6 +1395,16 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   c->nsc = nsc;
   c->lists_have_vel = with_vel && c->store_vel_lists;
   c->lists_have_clr = c->clr != nullptr;  // gvdbInsertSubcell stores colours when bound (P.cuh:268)
+  if (fp16 && c->lists_have_vel && !c->vel_box)
+    return fail(c, GVDBX_ERR_STATE, "GetMinMaxVel must run before an FP16 insert that stores velocities");
+  c->lists_fp16 = fp16;
+  for (int a = 0; a < 3; ++a) {
+    c->q_min[a] = A.qmin[a];
+    c->q_range[a] = A.qrange[a];
+  }
+  Buf& lpos = fp16 ? c->list_q16 : c->list_pos;
+  Buf& lvel = fp16 ? c->list_vq16 : c->list_vel;
+  const size_t ebytes = fp16 ? 6 : 12;  // per list entry: float3 or ushort3
   c->have_vsum = with_vel;
   // Lists sized for the largest possible length when that bound is modest (and below 2^31, so
   // the 32-bit offsets cannot wrap), so the insert is enqueued before the single length
@@ -1378,12 +1412,12 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   // bound of 2^32 on, the 32-bit scan total can wrap, so the length comes from a 64-bit sum.
   int64_t per_axis = (int64_t)((2.0f * r + 1.0f) / (float)sc) + 2;
   const int64_t bound = per_axis * per_axis * per_axis * n;
-  const size_t room = c->list_pos.cap > ((size_t)1 << 30) ? c->list_pos.cap : ((size_t)1 << 30);
-  const bool early = bound <= INT32_MAX && (size_t)(bound + 1) * 12 <= room;
+  const size_t room = lpos.cap > ((size_t)1 << 30) ? lpos.cap : ((size_t)1 << 30);
+  const bool early = bound <= INT32_MAX && (size_t)(bound + 1) * ebytes <= room;
   int64_t ln = -1;
   if (early) {
-    CHK(ensure(c, c->list_pos, (size_t)(bound + 1) * 12));
-    if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(bound + 1) * 12));
+    CHK(ensure(c, lpos, (size_t)(bound + 1) * ebytes));
+    if (c->lists_have_vel) CHK(ensure(c, lvel, (size_t)(bound + 1) * ebytes));
     if (c->lists_have_clr) CHK(ensure(c, c->list_clr, (size_t)(bound + 1) * 4));
   } else {
     const bool wide = bound >= ((int64_t)1 << 32);
@@ -1397,12 +1431,14 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
       ln = (uint32_t)c->h_meta[M_TOTAL];
     }
     if (ln > INT32_MAX || ln > bound) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);
-    CHK(ensure(c, c->list_pos, (size_t)(ln + 1) * 12));
-    if (c->lists_have_vel) CHK(ensure(c, c->list_vel, (size_t)(ln + 1) * 12));
+    CHK(ensure(c, lpos, (size_t)(ln + 1) * ebytes));
+    if (c->lists_have_vel) CHK(ensure(c, lvel, (size_t)(ln + 1) * ebytes));
     if (c->lists_have_clr) CHK(ensure(c, c->list_clr, (size_t)(ln + 1) * 4));
   }
-  A.list_pos = P<float>(c->list_pos);
-  A.list_vel = c->lists_have_vel ? P<float>(c->list_vel) : nullptr;
+  A.list_pos = fp16 ? nullptr : P<float>(c->list_pos);
+  A.list_vel = (c->lists_have_vel && !fp16) ? P<float>(c->list_vel) : nullptr;
+  A.list_q16 = fp16 ? P<uint16_t>(c->list_q16) : nullptr;
+  A.list_vq16 = (c->lists_have_vel && fp16) ? P<uint16_t>(c->list_vq16) : nullptr;
   A.list_clr = c->lists_have_clr ? P<uint32_t>(c->list_clr) : nullptr;
   if (early) CHK(read_meta_begin(c));  // the length is final after the scan
   {
@@ -1420,6 +1456,40 @@ int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const 
   return launch_status(c, "InsertPointsSubcell");
 }
 
+int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t* len) {
+  return insert_impl(c, sc, n, r, trans, len, false);
+}
+
+int gvdbx_insert_points_subcell_fp16(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t* len) {
+  return insert_impl(c, sc, n, r, trans, len, true);
+}
+
+// GetMinMaxVel (V.cpp:5470-5522): the velocity box of the first n bound points, kept for the
+// FP16 insert's velocity quantization (min, and range = max - min); out[6] = min xyz, max xyz.
+int gvdbx_get_min_max_vel(gvdbx_ctx* c, int64_t n, float out[6]) {
+  CHK(check_points(c, n));
+  if (!c->vel) return fail(c, GVDBX_ERR_STATE, "no velocities bound (gvdbx_set_points)");
+  CHK(ensure(c, c->bbox_part, (size_t)kBBoxParts * 6 * sizeof(float)));
+  PointsView pv = points_of(c, n, nullptr);
+  pv.pos = c->vel;
+  pv.pos_off = c->vel_off;
+  pv.pos_stride = c->vel_stride;
+  CHK(zero_meta(c));
+  bbox_two_pass(pv, P<float>(c->bbox_part), P<int>(c->meta), c->stream);
+  CHK(read_meta(c));
+  for (int a = 0; a < 3; ++a) {
+    const float mn = ordered_to_float(c->h_meta[M_BBOX + a]), mx = ordered_to_float(c->h_meta[M_BBOX + 3 + a]);
+    c->vel_min[a] = mn;
+    c->vel_range[a] = mx - mn;
+    if (out) {
+      out[a] = mn;
+      out[3 + a] = mx;
+    }
+  }
+  c->vel_box = true;
+  return GVDBX_OK;
+}
+
 int gvdbx_set_subcell_options(gvdbx_ctx* c, int store_velocity_lists) {
   if (!c) return GVDBX_ERR_ARG;
   c->store_vel_lists = store_velocity_lists != 0;
@@ -1428,6 +1498,7 @@ int gvdbx_set_subcell_options(gvdbx_ctx* c, int store_velocity_lists) {
 
 static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
   CHK(gather_checks(c, chan));
+  CHK(float_lists(c));
   if (c->nsc == 0) return GVDBX_OK;
   const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
   TimedRegion tr(c, "gather");
@@ -1495,6 +1566,27 @@ int gvdbx_gather_levelset(gvdbx_ctx* c, int sc, int64_t n, float r, const float 
   return gather_with_colour(c, sc, r, chan, chan_clr, accum, 1);
 }
 
+// GatherLevelSet_FP16 (V.cpp:5375-5401, kernel P.cuh:775-811): the level set from the FP16 lists,
+// min(3, |p - v| - r) per voxel with p decoded from the list's ushort3; no accumulation.  The
+// reference's colour part reads an uninitialised accumulator: not built (chan_clr must be < 0).
+int gvdbx_gather_levelset_fp16(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t len, int chan,
+                               int chan_clr) {
+  (void)n; (void)trans; (void)len;
+  if (!c) return GVDBX_ERR_ARG;
+  if (sc != c->sc) return fail(c, GVDBX_ERR_ARG, "subcell size %d differs from the insert (%d)", sc, c->sc);
+  CHK(gather_checks(c, chan));
+  if (!c->lists_fp16) return fail(c, GVDBX_ERR_STATE, "GatherLevelSet_FP16 needs InsertPointsSubcell_FP16 lists");
+  if (chan_clr >= 0) return fail(c, GVDBX_ERR_ARG, "the FP16 level set takes no colour channel");
+  if (c->nsc == 0) return GVDBX_OK;
+  const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
+  TimedRegion tr(c, "gather");
+  levelset_field_q16(cfg, c->nsc, P<int>(c->sc_pos), P<int>(c->sc_nid), P<int>(c->sc_cnt), P<int>(c->sc_off),
+                     P<uint16_t>(c->list_q16), c->q_min, c->q_range, P<int>(c->leaf_slot), c->apron,
+                     P<float>(c->chan[chan]), 3.0f, c->stream);
+  c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;
+  return launch_status(c, "GatherLevelSet_FP16");
+}
+
 int gvdbx_gather_velocity(gvdbx_ctx* c, int cx, int cy, int cz, int accum) {
   CHK(gather_checks(c, cx));
   CHK(chan_ok(c, cy));
@@ -1533,6 +1625,7 @@ int gvdbx_gather_density_velocity(gvdbx_ctx* c, int sc, int64_t n, float r, cons
   if (chan == cx || chan == cy || chan == cz || cx == cy || cx == cz || cy == cz)
     return fail(c, GVDBX_ERR_ARG, "density and velocity channels must be four different channels");
   if (!c->have_vsum) return fail(c, GVDBX_ERR_STATE, "no velocities were bound at InsertPointsSubcell");
+  CHK(float_lists(c));
   if (c->nsc == 0) return GVDBX_OK;
   const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
   const int br = cfg.res0 + 2 * c->apron;
@@ -1556,6 +1649,7 @@ int gvdbx_gather_density_velocity(gvdbx_ctx* c, int sc, int64_t n, float r, cons
 
 int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {
   CHK(gather_checks(c, chan));
+  CHK(float_lists(c));
   if (c->nsc == 0) return GVDBX_OK;
   const SubcellCfg cfg = subcell_cfg(c, c->sc, r);
   TimedRegion tr(c, "scatter");
@@ -1850,9 +1944,11 @@ int gvdbx_copy_subcell_array(gvdbx_ctx* c, int which, void* out, int64_t max_byt
     case 2: src = c->sc_off.p; bytes = c->nsc * 4; break;
     case 3: src = c->sc_pos.p; bytes = c->nsc * 12; break;
     case 4: src = c->sc_nid.p; bytes = c->nsc * 4; break;
-    case 5: src = c->list_pos.p; bytes = c->sclen * 12; break;
-    case 6: src = c->lists_have_vel ? c->list_vel.p : nullptr; bytes = c->lists_have_vel ? c->sclen * 12 : 0; break;
+    case 5: src = c->lists_fp16 ? nullptr : c->list_pos.p; bytes = c->sclen * 12; break;
+    case 6: src = (c->lists_have_vel && !c->lists_fp16) ? c->list_vel.p : nullptr; bytes = c->sclen * 12; break;
     case 7: src = c->lists_have_clr ? c->list_clr.p : nullptr; bytes = c->lists_have_clr ? c->sclen * 4 : 0; break;
+    case 8: src = c->lists_fp16 ? c->list_q16.p : nullptr; bytes = c->sclen * 6; break;
+    case 9: src = (c->lists_have_vel && c->lists_fp16) ? c->list_vq16.p : nullptr; bytes = c->sclen * 6; break;
     default: return fail(c, GVDBX_ERR_ARG, "unknown subcell array %d", which);
   }
   if (bytes > max_bytes) bytes = max_bytes;

@@ -377,6 +377,31 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_
   }
 }
 
+// One list entry: the position (float3, or ushort3 quantized for the FP16 insert), the velocity
+// when velocity lists are kept, the colour when colours are bound.
+__device__ __forceinline__ void put_entry(const BinArgs& A, int64_t k, const float q[3], float vx, float vy, float vz,
+                                          uint32_t clr) {
+  if (A.list_q16) {
+    A.list_q16[3 * k + 0] = quant16(q[0], A.qmin[0], A.qrange[0]);
+    A.list_q16[3 * k + 1] = quant16(q[1], A.qmin[1], A.qrange[1]);
+    A.list_q16[3 * k + 2] = quant16(q[2], A.qmin[2], A.qrange[2]);
+  } else {
+    A.list_pos[3 * k + 0] = q[0];
+    A.list_pos[3 * k + 1] = q[1];
+    A.list_pos[3 * k + 2] = q[2];
+  }
+  if (A.list_vq16) {
+    A.list_vq16[3 * k + 0] = quant16(vx, A.vqmin[0], A.vqrange[0]);
+    A.list_vq16[3 * k + 1] = quant16(vy, A.vqmin[1], A.vqrange[1]);
+    A.list_vq16[3 * k + 2] = quant16(vz, A.vqmin[2], A.vqrange[2]);
+  } else if (A.list_vel) {
+    A.list_vel[3 * k + 0] = vx;
+    A.list_vel[3 * k + 1] = vy;
+    A.list_vel[3 * k + 2] = vz;
+  }
+  if (A.list_clr) A.list_clr[k] = clr;
+}
+
 // kTwo (2r + 2 <= sc): a point's subcell box [trunc(q - r), trunc(q + 1 + r)] spans under one
 // subcell per axis, so it reaches at most 2 subcells per axis, all inside its brick's window; the
 // up to 8 entries then take their list slots with independent LDS atomics issued back to back.
@@ -404,7 +429,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
     const float4 p = A.spos[j];
     const float q[3] = {p.x, p.y, p.z};
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (A.list_vel) v = A.svel[j];
+    if (A.list_vel || A.list_vq16) v = A.svel[j];
     const uint32_t clr = A.list_clr ? load_u32(A.P.clr, __float_as_uint(p.w), A.P.clr_off, A.P.clr_stride) : 0u;
     int lo[3], hi[3];
     subcell_range(q, c.r, c.sc, lo, hi);
@@ -422,16 +447,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         if (!act[e]) continue;
-        const int64_t k = slot[e];
-        A.list_pos[3 * k + 0] = q[0];
-        A.list_pos[3 * k + 1] = q[1];
-        A.list_pos[3 * k + 2] = q[2];
-        if (A.list_vel) {
-          A.list_vel[3 * k + 0] = v.x;
-          A.list_vel[3 * k + 1] = v.y;
-          A.list_vel[3 * k + 2] = v.z;
-        }
-        if (A.list_clr) A.list_clr[k] = clr;
+        put_entry(A, slot[e], q, v.x, v.y, v.z, clr);
       }
       continue;
     }
@@ -449,15 +465,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
             if (g < 0) continue;
             k = (int64_t)A.off[g] + A.cnt_in[g] + atomicAdd(&A.ocur[g], 1);
           }
-          A.list_pos[3 * k + 0] = q[0];
-          A.list_pos[3 * k + 1] = q[1];
-          A.list_pos[3 * k + 2] = q[2];
-          if (A.list_vel) {
-            A.list_vel[3 * k + 0] = v.x;
-            A.list_vel[3 * k + 1] = v.y;
-            A.list_vel[3 * k + 2] = v.z;
-          }
-          if (A.list_clr) A.list_clr[k] = clr;
+          put_entry(A, k, q, v.x, v.y, v.z, clr);
         }
   }
 }
@@ -615,7 +623,7 @@ __global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __rest
     const int3 o = cloud_origin(A.P, i);
     float3 v = make_float3(0.f, 0.f, 0.f);
     long long fv[3] = {0, 0, 0};
-    if (A.P.vel && (vel || A.list_vel)) {
+    if (A.P.vel && (vel || A.list_vel || A.list_vq16)) {
       v = load_f3(A.P.vel, i, A.P.vel_off, A.P.vel_stride);
       fv[0] = vel_fixed(v.x);
       fv[1] = vel_fixed(v.y);
@@ -635,15 +643,7 @@ __global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __rest
             continue;
           }
           const int64_t k = (int64_t)A.off[g] + A.cnt_in[g] + atomicAdd(&A.ocur[g], 1);
-          A.list_pos[3 * k + 0] = q[0];
-          A.list_pos[3 * k + 1] = q[1];
-          A.list_pos[3 * k + 2] = q[2];
-          if (A.list_vel) {
-            A.list_vel[3 * k + 0] = v.x;
-            A.list_vel[3 * k + 1] = v.y;
-            A.list_vel[3 * k + 2] = v.z;
-          }
-          if (A.list_clr) A.list_clr[k] = clr;
+          put_entry(A, k, q, v.x, v.y, v.z, clr);
         }
   }
 }

@@ -38,6 +38,11 @@ struct BinArgs {
   float* list_pos;
   float* list_vel;
   uint32_t* list_clr;   // the listed points' colours, or null
+  // InsertPointsSubcell_FP16: positions (and velocities) stored as ushort3, quantized to the
+  // box [qmin, qmin + qrange] (list_pos / list_vel are null then)
+  uint16_t* list_q16;
+  uint16_t* list_vq16;
+  float qmin[3], qrange[3], vqmin[3], vqrange[3];
   unsigned long long* vsum;
   unsigned long long* wvs;  // [nmap][W^3][3]: the brick's window velocity sums (where its count > 0)
 };

@@ -127,6 +127,17 @@ __device__ __forceinline__ float3 load_f3(const char* base, int64_t i, int off, 
   return make_float3(p[0], p[1], p[2]);
 }
 
+// ushort((v - lo) / range * 65535) of gvdbInsertSubcell_fp16 (P.cuh:200-207), in IEEE single
+// operations in that order, saturated to [0, 65535] (NaN -> 0)
+__device__ __forceinline__ uint16_t quant16(float v, float lo, float range) {
+  const float f = __fmul_rn(__fdiv_rn(__fsub_rn(v, lo), range), 65535.0f);
+  return (uint16_t)(f >= 65535.0f ? 65535u : (f >= 0.0f ? (uint32_t)f : 0u));
+}
+// u / 65535 * range + lo of gvdbGatherLevelSet_fp16 (P.cuh:791-793), unfused
+__device__ __forceinline__ float dequant16(uint16_t u, float lo, float range) {
+  return __fadd_rn(__fmul_rn(__fdiv_rn((float)u, 65535.0f), range), lo);
+}
+
 __device__ __forceinline__ uint32_t load_u32(const char* base, int64_t i, int off, int stride) {
   return *(const uint32_t*)(base + i * (int64_t)stride + off);
 }

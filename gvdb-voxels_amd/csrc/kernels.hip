@@ -622,10 +622,14 @@ __device__ __forceinline__ int64_t atlas_voxel(int slot, int br, int lx, int ly,
   return (((int64_t)slot * br + lz) * br + ly) * br + lx;
 }
 
+// kQ16: the list holds ushort3 positions quantized to the box [qmin, qmin + qrange]
+// (InsertPointsSubcell_FP16), decoded as gvdbGatherLevelSet_fp16 does (P.cuh:791-793)
+template <bool kQ16>
 __global__ void __launch_bounds__(512) k_gather_field(SubcellCfg c, int64_t nsc, const int* __restrict__ sc_pos,
                                                       const int* __restrict__ sc_nid, const int* __restrict__ cnt,
                                                       const int* __restrict__ off, const float* __restrict__ list_pos,
-                                                      const int* __restrict__ leaf_slot, int apron,
+                                                      const uint16_t* __restrict__ list_q16, float3 qmin,
+                                                      float3 qrange, const int* __restrict__ leaf_slot, int apron,
                                                       float* __restrict__ chan, int accum, int mode, float init) {
   __shared__ float sx[512], sy[512], sz[512];
   const int64_t k = item_index();
@@ -640,9 +644,15 @@ __global__ void __launch_bounds__(512) k_gather_field(SubcellCfg c, int64_t nsc,
   for (int base = 0; base < n; base += nt) {
     const int j = base + t;
     if (j < n) {
-      sx[t] = list_pos[3 * (int64_t)(o + j) + 0];
-      sy[t] = list_pos[3 * (int64_t)(o + j) + 1];
-      sz[t] = list_pos[3 * (int64_t)(o + j) + 2];
+      if (kQ16) {
+        sx[t] = dequant16(list_q16[3 * (int64_t)(o + j) + 0], qmin.x, qrange.x);
+        sy[t] = dequant16(list_q16[3 * (int64_t)(o + j) + 1], qmin.y, qrange.y);
+        sz[t] = dequant16(list_q16[3 * (int64_t)(o + j) + 2], qmin.z, qrange.z);
+      } else {
+        sx[t] = list_pos[3 * (int64_t)(o + j) + 0];
+        sy[t] = list_pos[3 * (int64_t)(o + j) + 1];
+        sz[t] = list_pos[3 * (int64_t)(o + j) + 2];
+      }
     }
     __syncthreads();
     const int lim = min(nt, n - base);
@@ -672,8 +682,18 @@ void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_ni
                   const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode, float init,
                   hipStream_t s) {
   if (nsc <= 0) return;
-  hipLaunchKernelGGL(k_gather_field, item_grid(nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, nsc, sc_pos, sc_nid, cnt, off,
-                     list_pos, leaf_slot, apron, chan, accum, mode, init);
+  hipLaunchKernelGGL(k_gather_field<false>, item_grid(nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, nsc, sc_pos, sc_nid,
+                     cnt, off, list_pos, (const uint16_t*)nullptr, make_float3(0.f, 0.f, 0.f),
+                     make_float3(1.f, 1.f, 1.f), leaf_slot, apron, chan, accum, mode, init);
+}
+
+void levelset_field_q16(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
+                        const int* off, const uint16_t* list_q16, const float qmin[3], const float qrange[3],
+                        const int* leaf_slot, int apron, float* chan, float init, hipStream_t s) {
+  if (nsc <= 0) return;
+  hipLaunchKernelGGL(k_gather_field<true>, item_grid(nsc), dim3(c.sc * c.sc * c.sc), 0, s, c, nsc, sc_pos, sc_nid,
+                     cnt, off, (const float*)nullptr, list_q16, make_float3(qmin[0], qmin[1], qmin[2]),
+                     make_float3(qrange[0], qrange[1], qrange[2]), leaf_slot, apron, chan, 0, 1, init);
 }
 
 // Level set over 4^3 subcells (GatherLevelSet, P.cuh:736-773, and ScatterReduceLevelSet, which is

@@ -106,6 +106,11 @@ void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map
 void gather_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
                   const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, int mode, float init,
                   hipStream_t s);
+// GatherLevelSet_FP16 (P.cuh:775-811): the level set from ushort3 list positions quantized to
+// [qmin, qmin + qrange], starting from init, without accumulation
+void levelset_field_q16(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt,
+                        const int* off, const uint16_t* list_q16, const float qmin[3], const float qrange[3],
+                        const int* leaf_slot, int apron, float* chan, float init, hipStream_t s);
 // level set (GatherLevelSet and ScatterReduceLevelSet): segmented min over the sorted pairs
 void levelset_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_nid, const int* cnt, const int* off,
                     const float* list_pos, const int* leaf_slot, int apron, float* chan, int accum, float init,

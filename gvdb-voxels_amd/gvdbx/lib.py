@@ -50,6 +50,9 @@ SIGNATURES = {
     "gvdbx_insert_points_subcell": [_P, _I, _I64, _F, _PF, C.POINTER(_I64)],
     "gvdbx_gather_density": [_P, _I, _I64, _F, _PF, _I64, _I, _I, _I],
     "gvdbx_gather_levelset": [_P, _I, _I64, _F, _PF, _I64, _I, _I, _I],
+    "gvdbx_insert_points_subcell_fp16": [_P, _I, _I64, _F, _PF, C.POINTER(_I64)],
+    "gvdbx_gather_levelset_fp16": [_P, _I, _I64, _F, _PF, _I64, _I, _I],
+    "gvdbx_get_min_max_vel": [_P, _I64, _PF],
     "gvdbx_gather_velocity": [_P, _I, _I, _I, _I],
     "gvdbx_set_subcell_options": [_P, _I],
     "gvdbx_scatter_reduce_levelset": [_P, _F, _I, _F],

@@ -160,6 +160,24 @@ class VolumeGVDB:
         self._chk(self.L.gvdbx_gather_levelset(self.h, subcell, n, radius, fptr(trans), sc_len, chan, chan_clr,
                                                int(bool(accum))), "GatherLevelSet")
 
+    def InsertPointsSubcell_FP16(self, subcell, n, radius, trans=(0, 0, 0)):
+        """InsertPointsSubcell_FP16 (V.cpp:5094-5189): list positions as ushort3 over the rebuild's box."""
+        ln = C.c_int64(0)
+        self._chk(self.L.gvdbx_insert_points_subcell_fp16(self.h, subcell, n, radius, fptr(trans), C.byref(ln)),
+                  "InsertPointsSubcell_FP16")
+        return ln.value
+
+    def GatherLevelSet_FP16(self, subcell, n, radius, trans, sc_len, chan, chan_clr=-1):
+        self._chk(self.L.gvdbx_gather_levelset_fp16(self.h, subcell, n, radius, fptr(trans), sc_len, chan, chan_clr),
+                  "GatherLevelSet_FP16")
+
+    def GetMinMaxVel(self, n):
+        """GetMinMaxVel (V.cpp:5470-5522): (min xyz, max xyz) of the bound velocities."""
+        out = np.zeros(6, np.float32)
+        self._chk(self.L.gvdbx_get_min_max_vel(self.h, n, out.ctypes.data_as(C.POINTER(C.c_float))),
+                  "GetMinMaxVel")
+        return out[:3].copy(), out[3:].copy()
+
     def GatherVelocity(self, chan_x, chan_y, chan_z, accum=False):
         self._chk(self.L.gvdbx_gather_velocity(self.h, chan_x, chan_y, chan_z, int(bool(accum))), "GatherVelocity")
 
@@ -316,6 +334,7 @@ class VolumeGVDB:
             pos=grab(3, nsc * 3, np.int32, (nsc, 3)), nid=grab(4, nsc, np.int32),
             list_pos=grab(5, ln * 3, np.float32, (ln, 3)), list_vel=grab(6, ln * 3, np.float32, (ln, 3)),
             list_clr=grab(7, ln, np.uint32),
+            list_q16=grab(8, ln * 3, np.uint16, (ln, 3)), list_vq16=grab(9, ln * 3, np.uint16, (ln, 3)),
         )
 
     # ------------------------------------------------------------ timing

@@ -138,6 +138,22 @@ class VolumeGVDB {
                                 bAccum ? 1 : 0),
           "GatherLevelSet", true);
   }
+  // FP16 subcell lists and their level set (V.cpp:5094-5189, 5375-5401), GetMinMaxVel (V.cpp:5470-5522)
+  void InsertPointsSubcell_FP16(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength) {
+    const float t[3] = {trans.x, trans.y, trans.z};
+    int64_t len = 0;
+    check(gvdbx_insert_points_subcell_fp16(ctx_, subcell_size, num_pnts, radius, t, &len), "InsertPointsSubcell_FP16",
+          true);
+    pSCPntsLength = (int)len;
+  }
+  void GatherLevelSet_FP16(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength,
+                           int chanLevelSet, int chanClr) {
+    const float t[3] = {trans.x, trans.y, trans.z};
+    (void)chanClr;  // the reference's FP16 colour accumulator is uninitialised: no colour here
+    check(gvdbx_gather_levelset_fp16(ctx_, subcell_size, num_pnts, radius, t, pSCPntsLength, chanLevelSet, -1),
+          "GatherLevelSet_FP16", true);
+  }
+  void GetMinMaxVel(int num_pnts) { check(gvdbx_get_min_max_vel(ctx_, num_pnts, nullptr), "GetMinMaxVel", true); }
   // ScatterDensity (V.cpp:5625-5659) into the reference kernel's channels: density 0, colours
   // (when bound) 1 with avgColor, else 3.
   void ScatterDensity(int num_pnts, float radius, float amp, Vector3DF trans, bool expand = true,

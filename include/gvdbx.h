@@ -131,6 +131,22 @@ int gvdbx_gather_density(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, c
 /* colour: as gvdbx_gather_density (the same rule, P.cuh:752-770) */
 int gvdbx_gather_levelset(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, const float trans[3],
                           int64_t sc_pnts_len, int chan, int chan_clr, int accum);
+/* ---- FP16 subcell lists: InsertPointsSubcell_FP16 (V.cpp:5094-5189, kernel gvdbInsertSubcell_fp16
+ *      P.cuh:154-216) stores each listed position as ushort3, ushort((q - posmin) / posrange *
+ *      65535) over the last rebuild's bounding box (gvdbx_get_depth posmin / posmax; IEEE single
+ *      operations in that order, saturated to [0, 65535]), and, when velocity lists are kept
+ *      (gvdbx_set_subcell_options), each velocity the same way over GetMinMaxVel's box;
+ *      counts, offsets and colours as gvdbx_insert_points_subcell.  Only
+ *      gvdbx_gather_levelset_fp16 reads these lists (the float gathers refuse them);
+ *      it decodes p = u / 65535 * posrange + posmin (unfused) and writes min(3, |p - v| - r)
+ *      (P.cuh:775-811).  chan_clr must be < 0: the reference's FP16 colour accumulator is
+ *      uninitialised.  GetMinMaxVel (V.cpp:5470-5522): out = min xyz, max xyz of the first n
+ *      velocities. */
+int gvdbx_insert_points_subcell_fp16(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, const float trans[3],
+                                     int64_t* sc_pnts_len);
+int gvdbx_gather_levelset_fp16(gvdbx_ctx* ctx, int subcell, int64_t n, float radius, const float trans[3],
+                               int64_t sc_pnts_len, int chan, int chan_clr);
+int gvdbx_get_min_max_vel(gvdbx_ctx* ctx, int64_t n, float out[6]);
 /* Velocity channels: the reference colour-mean rule (P.cuh:717-731) applied to the binned
  * float3 velocities (stored by gvdbInsertSubcell, P.cuh:266-267, read by no reference gather).
  * Sums are exact 64-bit fixed point (2^-24 units), so the result is order independent. */
@@ -203,7 +219,7 @@ int gvdbx_copy_channel(gvdbx_ctx* ctx, int chan, float* host_out, int64_t max_fl
 int gvdbx_get_channel_ptr(gvdbx_ctx* ctx, int chan, void** dev_ptr, int64_t* floats);
 int gvdbx_get_subcell_info(const gvdbx_ctx* ctx, int64_t* num_subcells, int64_t* list_len, int* subcell);
 /* which: 0 map(int32 per leaf) 1 cnt 2 off 3 pos(int32x3) 4 nid 5 list pos(float x3) 6 list vel(float x3)
- *        7 list colour (uint32 = uchar4) */
+ *        7 list colour (uint32 = uchar4) 8 FP16 list pos (uint16 x3) 9 FP16 list vel (uint16 x3) */
 int gvdbx_copy_subcell_array(gvdbx_ctx* ctx, int which, void* host_out, int64_t max_bytes);
 
 /* ---- device memory helpers: AllocData/CommitData/RetrieveData (V.cpp:4822-4860, A.cpp:263-298) */

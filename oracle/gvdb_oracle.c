@@ -655,6 +655,59 @@ static int gather_common(orc_tree* t, float r, int chan, int accum, int mode) {
 }
 
 int orc_gather_density(orc_tree* t, float r, int chan, int accum) { return gather_common(t, r, chan, accum, 0); }
+
+/* FP16 subcell lists (gvdbInsertSubcell_fp16, P.cuh:200-207): ushort((v - lo) / range * 65535)
+ * in single precision, each operation rounded in that order, saturated to [0, 65535]. */
+static uint16_t quant16(float v, float lo, float range) {
+  volatile float d = v - lo;
+  volatile float q = d / range;
+  float f = q * 65535.0f;
+  if (!(f >= 0.0f)) return 0;
+  if (f >= 65535.0f) return 65535;
+  return (uint16_t)f;
+}
+
+/* The listed positions quantized to [qmin, qmin + qrange]: out[3 * len] */
+int orc_quantize_list_pos(const orc_tree* t, const float* qmin, const float* qrange, uint16_t* out) {
+  if (!t->sc_cnt) return ORC_ERR_STATE;
+  int64_t len = t->nsc > 0 ? (int64_t)t->sc_off[t->nsc - 1] + t->sc_cnt[t->nsc - 1] : 0;
+  for (int64_t j = 0; j < len; j++)
+    for (int a = 0; a < 3; a++) out[3 * j + a] = quant16(t->sc_list_pos[3 * j + a], qmin[a], qrange[a]);
+  return ORC_OK;
+}
+
+/* The listed points' velocities (vel: float3 rows of `st` floats) quantized the same way */
+int orc_quantize_list_vel(const orc_tree* t, const float* vel, int64_t st, const float* vmin, const float* vrange,
+                          uint16_t* out) {
+  if (!t->sc_cnt) return ORC_ERR_STATE;
+  int64_t len = t->nsc > 0 ? (int64_t)t->sc_off[t->nsc - 1] + t->sc_cnt[t->nsc - 1] : 0;
+  for (int64_t j = 0; j < len; j++)
+    for (int a = 0; a < 3; a++) out[3 * j + a] = quant16(vel[t->sc_list[j] * st + a], vmin[a], vrange[a]);
+  return ORC_OK;
+}
+
+/* gvdbGatherLevelSet_fp16 (P.cuh:775-811): each listed position decoded as
+ * u / 65535 * range + lo (single precision, unfused), then the level set min(3, |p - v| - r)
+ * without accumulation. q16: the quantized list (3 per entry). */
+int orc_gather_levelset_q16(orc_tree* t, float r, int chan, const uint16_t* q16, const float* qmin,
+                            const float* qrange) {
+  if (!t->chan[chan] || !t->sc_cnt) return ORC_ERR_STATE;
+  int64_t len = t->nsc > 0 ? (int64_t)t->sc_off[t->nsc - 1] + t->sc_cnt[t->nsc - 1] : 0;
+  float* dec = (float*)malloc((size_t)(len + 1) * 3 * sizeof(float));
+  if (!dec) return ORC_ERR_NOMEM;
+  for (int64_t j = 0; j < 3 * len; j++) {
+    int a = (int)(j % 3);
+    volatile float f = (float)q16[j] / 65535.0f;
+    volatile float g = f * qrange[a];
+    dec[j] = g + qmin[a];
+  }
+  float* keep = t->sc_list_pos;
+  t->sc_list_pos = dec;
+  int s = gather_common(t, r, chan, 0, 1);
+  t->sc_list_pos = keep;
+  free(dec);
+  return s;
+}
 int orc_gather_levelset(orc_tree* t, float r, int chan, int accum) { return gather_common(t, r, chan, accum, 1); }
 
 /* Velocity channels: the reference colour rule (P.cuh:717-731) restated for float3:

@@ -103,6 +103,12 @@ int orc_gather_velocity(orc_tree* t, const float* vel, int64_t stride, int chan_
 /* colour means of the gathers (P.cuh:713-731): packed uchar4 words stored as doubles */
 int orc_gather_colour(orc_tree* t, const uint32_t* clr, int64_t stride_words, int chan, int accum);
 int orc_update_apron(orc_tree* t, int chan, double boundval);                 /* O.cuh:72-126 */
+/* FP16 subcell lists (P.cuh:154-216) and their level set (P.cuh:775-811) */
+int orc_quantize_list_pos(const orc_tree* t, const float* qmin, const float* qrange, uint16_t* out);
+int orc_quantize_list_vel(const orc_tree* t, const float* vel, int64_t stride, const float* vmin, const float* vrange,
+                          uint16_t* out);
+int orc_gather_levelset_q16(orc_tree* t, float r, int chan, const uint16_t* q16, const float* qmin,
+                            const float* qrange);
 /* colour part of ScatterDensity (P.cuh:553-572, 610-641); clr: one uint32 word per point */
 int orc_scatter_colour(orc_tree* t, int chan, const float* pos, int64_t n, int64_t stride, const float* trans,
                        const uint32_t* clr, int avg);

@@ -70,6 +70,9 @@ def lib():
         L.orc_gather_levelset.argtypes = [P, f32, C.c_int, C.c_int]
         L.orc_gather_velocity.argtypes = [P, P, i64, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_gather_colour.argtypes = [P, P, i64, C.c_int, C.c_int]
+        L.orc_quantize_list_pos.argtypes = [P, P, P, P]
+        L.orc_quantize_list_vel.argtypes = [P, P, i64, P, P, P]
+        L.orc_gather_levelset_q16.argtypes = [P, f32, C.c_int, P, P, P]
         L.orc_scatter_colour.argtypes = [P, C.c_int, P, i64, i64, P, P, C.c_int]
         L.orc_update_apron.argtypes = [P, C.c_int, C.c_double]
         L.orc_scatter_reduce_levelset.argtypes = [P, f32, C.c_int, C.c_double]
@@ -264,6 +267,28 @@ class Tree:
         tr = np.asarray(trans, np.float32)
         _chk(lib().orc_scatter_colour(self.h, chan, _ptr(pos), len(pos), st * 4, _ptr(tr), _ptr(clr), int(avg)),
              "scatter_colour")
+
+    def quantize_list_pos(self, qmin, qrange):
+        """The listed positions as FP16-path ushort3 (len x 3) over [qmin, qmin + qrange]."""
+        ln = int(self.subcells()["cnt"].sum())
+        out = np.zeros((ln, 3), np.uint16)
+        lo, rg = np.asarray(qmin, np.float32), np.asarray(qrange, np.float32)
+        _chk(lib().orc_quantize_list_pos(self.h, _ptr(lo), _ptr(rg), _ptr(out)), "quantize_list_pos")
+        return out
+
+    def quantize_list_vel(self, vel, vmin, vrange):
+        ln = int(self.subcells()["cnt"].sum())
+        out = np.zeros((ln, 3), np.uint16)
+        vel, st = _pts(vel)
+        lo, rg = np.asarray(vmin, np.float32), np.asarray(vrange, np.float32)
+        _chk(lib().orc_quantize_list_vel(self.h, _ptr(vel), st, _ptr(lo), _ptr(rg), _ptr(out)), "quantize_list_vel")
+        return out
+
+    def gather_levelset_q16(self, radius, chan, q16, qmin, qrange):
+        q16 = np.ascontiguousarray(q16, np.uint16)
+        lo, rg = np.asarray(qmin, np.float32), np.asarray(qrange, np.float32)
+        _chk(lib().orc_gather_levelset_q16(self.h, radius, chan, _ptr(q16), _ptr(lo), _ptr(rg)),
+             "gather_levelset_q16")
 
     def update_apron(self, chan, boundval):
         _chk(lib().orc_update_apron(self.h, chan, boundval), "update_apron")

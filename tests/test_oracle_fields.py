@@ -393,3 +393,39 @@ def test_scatter_colour_hand_computed(oracle_mod):
         assert b[4 + a, 3 + a, 2 + a] == want0
         assert b[5 + a, 5 + a, 5 + a] == want1
         assert np.count_nonzero(b != 77.0) == 2
+
+
+def test_fp16_lists_hand_computed_and_bounded(oracle_mod):
+    """FP16 subcell lists (P.cuh:200-207): ushort((q - lo) / range * 65535) in single precision.
+    Over lo = (2, 2, 2), range = (100, 100, 100): 10.5 -> 8.5 / 100 * 65535 = 5570.47 -> 5570;
+    q = lo -> 0; q = lo + range -> 65535.  The FP16 level set (P.cuh:775-811) decodes
+    u / 65535 * range + lo, at most range / 65535 per axis from the listed position, so it stays
+    within sqrt(3) * 100 / 65535 (plus rounding) of the float level set on every voxel."""
+    p = np.array([[10.5, 2.0, 102.0], [40.25, 30.5, 60.75], [150.0, 150.0, 150.0]], np.float32)
+    t = oracle_mod.Tree((3, 3, 3, 3, 5), 16, 16, 1)
+    t.rebuild_topology(p, 1.0)
+    t.add_channel(0)
+    t.add_channel(1)
+    t.update_atlas()
+    t.insert_points_subcell(4, p, 1.0)
+    lo, rg = (2.0, 2.0, 2.0), (100.0, 100.0, 100.0)
+    q = t.quantize_list_pos(lo, rg)
+    s = t.subcells()
+    first = q[np.nonzero(s["list"] == 0)[0]]
+    assert len(first) > 0 and (first == np.array([5570, 0, 65535], np.uint16)).all()
+    # saturation: a listed coordinate beyond the box maps to 65535
+    assert (t.quantize_list_pos(lo, (50.0, 50.0, 50.0))[np.nonzero(s["list"] == 2)[0]] == 65535).all()
+    g = np.random.default_rng(4).uniform(20, 90, (3000, 3)).astype(np.float32)
+    t = oracle_mod.Tree((3, 3, 3, 3, 5), 16, 16, 1)
+    t.rebuild_topology(g, 1.0)
+    t.add_channel(0)
+    t.add_channel(1)
+    t.update_atlas()
+    t.insert_points_subcell(4, g, 1.0)
+    lo = np.float32([18.0, 18.0, 18.0])
+    rg = np.float32([80.0, 80.0, 80.0])
+    t.gather_levelset(1.0, 0)
+    t.gather_levelset_q16(1.0, 1, t.quantize_list_pos(lo, rg), lo, rg)
+    d = np.abs(t.channel(0) - t.channel(1))
+    assert d.max() <= np.sqrt(3.0) * 80.0 / 65535.0 * 1.01 + 1e-6
+    assert d.max() > 0  # the quantization shows
