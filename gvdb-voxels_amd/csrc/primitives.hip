@@ -238,15 +238,32 @@ __global__ void __launch_bounds__(256) k_radix_up(const K* __restrict__ keys, in
   counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Scatter of one tile: keys are ranked stably by digit into LDS first (local digit starts from
-// the scanned counts: a digit's tile count is the next offset minus this one, the offsets being
-// a digit-major exclusive scan), then written out in local order, so consecutive threads store
-// consecutive addresses within each digit's run instead of scattering single keys.
+// Per digit (one workgroup each): exclusive scan of its ntiles tile counts in place, and the
+// digit's total into totals[digit].
+__global__ void __launch_bounds__(256) k_digit_scan(int* __restrict__ counts, int ntiles, int* __restrict__ totals) {
+  int* row = counts + (int64_t)blockIdx.x * ntiles;
+  int carry = 0;
+  for (int base = 0; base < ntiles; base += 256) {
+    const int i = base + (int)threadIdx.x;
+    const int v = i < ntiles ? row[i] : 0;
+    int tot;
+    const int pre = block_exclusive_scan_256(v, &tot);
+    if (i < ntiles) row[i] = carry + pre;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+// Scatter of one tile: keys are ranked stably by digit into LDS first, then written out in
+// local order, so consecutive threads store consecutive addresses within each digit's run instead
+// of scattering single keys.  A digit's start in the output is the exclusive scan of the digit
+// totals (formed here, one value per thread) plus its start among the tiles (k_digit_scan); its
+// count in this tile is the next tile's start minus this one (the total for the last tile).
 template <typename K, bool kHasVal>
 __global__ void __launch_bounds__(256) k_radix_down(const K* __restrict__ kin, K* __restrict__ kout,
                                                    const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                    int64_t n, int shift, const int* __restrict__ offsets,
-                                                   int ntiles) {
+                                                   int ntiles, const int* __restrict__ totals) {
   __shared__ K sk[kRadixTile];
   __shared__ uint32_t sv[kHasVal ? kRadixTile : 1];
   __shared__ int base_of[256], lstart[256], run[256];
@@ -254,12 +271,13 @@ __global__ void __launch_bounds__(256) k_radix_down(const K* __restrict__ kin, K
   const int tid = threadIdx.x, wave = tid >> 6;
   const int64_t tile = blockIdx.x;
   const int64_t tbase = tile * kRadixTile;
-  const int mine = offsets[(int64_t)tid * ntiles + tile];
-  const int next = tile + 1 < ntiles ? offsets[(int64_t)tid * ntiles + tile + 1]
-                                     : (tid < 255 ? offsets[(int64_t)(tid + 1) * ntiles] : (int)n);
+  const int within = offsets[(int64_t)tid * ntiles + tile];
+  const int dtot = totals[tid];
+  const int next = tile + 1 < ntiles ? offsets[(int64_t)tid * ntiles + tile + 1] : dtot;
   int total;
-  const int ls = block_exclusive_scan_256(next - mine, &total);
-  base_of[tid] = mine;
+  const int dbase = block_exclusive_scan_256(dtot, &total);
+  const int ls = block_exclusive_scan_256(next - within, &total);
+  base_of[tid] = dbase + within;
   lstart[tid] = ls;
   run[tid] = 0;
   const uint64_t lt = lanemask_lt();
@@ -303,7 +321,8 @@ __global__ void __launch_bounds__(256) k_radix_down(const K* __restrict__ kin, K
   }
 }
 
-int64_t radix_count_ints(int64_t n) { return 256 * ((n + kRadixTile - 1) / kRadixTile); }
+// the digit-major tile counts, then the 256 digit totals
+int64_t radix_count_ints(int64_t n) { return 256 * ((n + kRadixTile - 1) / kRadixTile) + 256; }
 
 template <typename K>
 static int radix_sort_impl(K* keys, K* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int begin_bit,
@@ -316,15 +335,16 @@ static int radix_sort_impl(K* keys, K* keys_alt, uint32_t* vals, uint32_t* vals_
   K* kout = keys_alt;
   uint32_t* vin = vals;
   uint32_t* vout = vals_alt;
+  int* totals = counts + ncnt;
   for (int shift = begin_bit; shift < end_bit; shift += 8) {
     hipLaunchKernelGGL(k_radix_up<K>, dim3(ntiles), dim3(256), 0, s, kin, n, shift, counts, ntiles);
-    exclusive_scan(counts, counts, ncnt, tile_buf, nullptr, s);
+    hipLaunchKernelGGL(k_digit_scan, dim3(256), dim3(256), 0, s, counts, ntiles, totals);
     if (vals)
       hipLaunchKernelGGL((k_radix_down<K, true>), dim3(ntiles), dim3(256), 0, s, kin, kout, vin, vout, n, shift, counts,
-                         ntiles);
+                         ntiles, (const int*)totals);
     else
       hipLaunchKernelGGL((k_radix_down<K, false>), dim3(ntiles), dim3(256), 0, s, kin, kout, nullptr, nullptr, n, shift,
-                         counts, ntiles);
+                         counts, ntiles, (const int*)totals);
     K* t = kin; kin = kout; kout = t;
     uint32_t* tv = vin; vin = vout; vout = tv;
     parity ^= 1;
