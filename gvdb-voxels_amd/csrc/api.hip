@@ -1300,7 +1300,10 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   CHK(ensure(c, c->radix_counts, (size_t)ncnt * 4));
   int64_t big = nsc_cap > total ? nsc_cap : total;
   if (ncnt > big) big = ncnt;
-  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(big + 1) + 16) * 4));
+  // scan tiles of the map scan, and the 256-subcell block sums of the offsets scan
+  int64_t tiles = scan_tile_count(big + 1);
+  if ((nsc_cap + 255) / 256 > tiles) tiles = (nsc_cap + 255) / 256;
+  CHK(ensure(c, c->tile_buf, (size_t)(tiles + 16) * 4));
   DevTree T = tree_of(c);
   PointsView pv = points_of(c, n, trans);
 
@@ -1385,11 +1388,13 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     bin_count(A, nmap, c->stream);
     bin_points_count(A, tail, c->stream);
   }
-  cell_totals(A, nsc, P<int>(c->sc_cnt), c->stream);
+  // subcell offsets: block sums from the totals pass, their scan, the block scans in the bases pass
+  const int64_t nblk = (nsc + 255) / 256;
+  CHK(ensure(c, c->tile_buf, (size_t)(nblk + 16) * 4));
+  cell_totals(A, nsc, P<int>(c->sc_cnt), P<int>(c->tile_buf), c->stream);
   CHK(zero_meta(c));
-  exclusive_scan(P<int>(c->sc_cnt), P<int>(c->sc_off), nsc, P<int>(c->tile_buf), P<int>(c->meta) + M_TOTAL,
-                 c->stream);
-  cell_bases(A, nsc, c->stream);
+  scan_tile_sums(P<int>(c->tile_buf), nblk, P<int>(c->meta) + M_TOTAL, c->stream);
+  cell_bases(A, nsc, P<int>(c->sc_cnt), P<int>(c->tile_buf), P<int>(c->sc_off), c->stream);
   c->sc = sc;
   c->scdim = cfg.D;
   c->nsc = nsc;

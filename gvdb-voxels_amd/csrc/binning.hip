@@ -545,64 +545,86 @@ void gather_sorted(const PointsView& P, const uint32_t* svals, int64_t n, float4
 
 // Walks the bricks whose window reaches subcell g = n*D^3 + l: per axis the brick itself and,
 // for l on the first/last layer, the neighbour below/above (window cell l - delta*D).
+// The subcell offsets (exclusive scan of cnt) come from the two passes themselves: the totals
+// pass writes each 256-subcell block's sum to tile[block], one workgroup scans those, and the
+// bases pass adds the block's prefix to a scan of its own counts (cnt = the counts, out = off).
 template <bool kBases, int kD>
-__global__ void k_cell_walk(BinArgs A, int64_t nsc, int* __restrict__ cnt) {
+__global__ void __launch_bounds__(256) k_cell_walk(BinArgs A, int64_t nsc, int* __restrict__ cnt,
+                                                   int* __restrict__ tile, int* __restrict__ off_out) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nsc) return;
+  const bool live = g < nsc;
   const int D = kD ? kD : A.c.D, D3 = D * D * D, W = D + 2, W3 = W * W * W;
-  const int64_t n = g / D3;
-  const int l = (int)(g - n * D3);
-  const int lx = l % D, ly = (l / D) % D, lz = l / (D * D);
-  int p = 0;
+  // every lane reaches the one block scan of each pass (it holds barriers): the bases pass scans
+  // first, the totals pass after its walk, both outside any lane-dependent branch
   int base = 0;
   if (kBases) {
-    base = A.off[g];
-    A.ocur[g] = 0;  // the insert's one-by-one cursor starts at zero
+    int tot;
+    const int c0 = live ? cnt[g] : 0;
+    base = tile[blockIdx.x] + block_exclusive_scan_256(c0, &tot);
   }
-  const bool vel = !kBases && A.vsum != nullptr && A.P.vel != nullptr;
+  int p = 0;
   unsigned long long vs[3] = {0ull, 0ull, 0ull};
-  for (int dz = (lz == 0 ? -1 : 0); dz <= (lz == D - 1 ? 1 : 0); ++dz)
-    for (int dy = (ly == 0 ? -1 : 0); dy <= (ly == D - 1 ? 1 : 0); ++dy)
-      for (int dx = (lx == 0 ? -1 : 0); dx <= (lx == D - 1 ? 1 : 0); ++dx) {
-        const int b = A.nbm[n * 27 + (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-        if (b < 0) continue;
-        const int w = ((lz - dz * D + 1) * W + (ly - dy * D + 1)) * W + (lx - dx * D + 1);
-        int* cell = A.wgb + (int64_t)b * W3 + w;
-        const int c = *cell;
-        if (kBases) *cell = base + p;
-        p += c;
-        if (vel && c > 0) {
-          const unsigned long long* v = A.wvs + ((int64_t)b * W3 + w) * 3;
-          vs[0] += v[0];
-          vs[1] += v[1];
-          vs[2] += v[2];
-        }
-      }
-  if (!kBases) {
-    A.cnt_in[g] = p;
-    cnt[g] = p + A.ocnt[g];
-    if (vel && p > 0) {  // the pairs placed one by one added theirs with atomics before
-      A.vsum[3 * g + 0] += vs[0];
-      A.vsum[3 * g + 1] += vs[1];
-      A.vsum[3 * g + 2] += vs[2];
+  const bool vel = !kBases && A.vsum != nullptr && A.P.vel != nullptr;
+  if (live) {
+    if (kBases) {
+      off_out[g] = base;
+      A.ocur[g] = 0;  // the insert's one-by-one cursor starts at zero
     }
+    const int64_t n = g / D3;
+    const int l = (int)(g - n * D3);
+    const int lx = l % D, ly = (l / D) % D, lz = l / (D * D);
+    for (int dz = (lz == 0 ? -1 : 0); dz <= (lz == D - 1 ? 1 : 0); ++dz)
+      for (int dy = (ly == 0 ? -1 : 0); dy <= (ly == D - 1 ? 1 : 0); ++dy)
+        for (int dx = (lx == 0 ? -1 : 0); dx <= (lx == D - 1 ? 1 : 0); ++dx) {
+          const int b = A.nbm[n * 27 + (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+          if (b < 0) continue;
+          const int w = ((lz - dz * D + 1) * W + (ly - dy * D + 1)) * W + (lx - dx * D + 1);
+          int* cell = A.wgb + (int64_t)b * W3 + w;
+          const int c = *cell;
+          if (kBases) *cell = base + p;
+          p += c;
+          if (vel && c > 0) {
+            const unsigned long long* v = A.wvs + ((int64_t)b * W3 + w) * 3;
+            vs[0] += v[0];
+            vs[1] += v[1];
+            vs[2] += v[2];
+          }
+        }
+  }
+  if (!kBases) {
+    int total = 0;
+    if (live) {
+      A.cnt_in[g] = p;
+      total = p + A.ocnt[g];
+      cnt[g] = total;
+      if (vel && p > 0) {  // the pairs placed one by one added theirs with atomics before
+        A.vsum[3 * g + 0] += vs[0];
+        A.vsum[3 * g + 1] += vs[1];
+        A.vsum[3 * g + 2] += vs[2];
+      }
+    }
+    int tot;
+    (void)block_exclusive_scan_256(total, &tot);
+    if (threadIdx.x == 0) tile[blockIdx.x] = tot;
   }
 }
 
-void cell_totals(const BinArgs& A, int64_t nsc, int* cnt, hipStream_t s) {
+void cell_totals(const BinArgs& A, int64_t nsc, int* cnt, int* tile, hipStream_t s) {
   if (nsc <= 0) return;
   if (A.c.D == 2)
-    hipLaunchKernelGGL((k_cell_walk<false, 2>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt);
+    hipLaunchKernelGGL((k_cell_walk<false, 2>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt, tile,
+                       (int*)nullptr);
   else
-    hipLaunchKernelGGL((k_cell_walk<false, 0>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt);
+    hipLaunchKernelGGL((k_cell_walk<false, 0>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt, tile,
+                       (int*)nullptr);
 }
 
-void cell_bases(const BinArgs& A, int64_t nsc, hipStream_t s) {
+void cell_bases(const BinArgs& A, int64_t nsc, int* cnt, int* tile, int* off, hipStream_t s) {
   if (nsc <= 0) return;
   if (A.c.D == 2)
-    hipLaunchKernelGGL((k_cell_walk<true, 2>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, nullptr);
+    hipLaunchKernelGGL((k_cell_walk<true, 2>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt, tile, off);
   else
-    hipLaunchKernelGGL((k_cell_walk<true, 0>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, nullptr);
+    hipLaunchKernelGGL((k_cell_walk<true, 0>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt, tile, off);
 }
 
 // ------------------------------------------------------------------ per-point path

@@ -54,9 +54,11 @@ void bin_points_insert(const BinArgs& A, const int* begin, hipStream_t s);
 void sum_counts(const int* a, const int* b, int* out, int64_t n, hipStream_t s);
 void gather_sorted(const PointsView& P, const uint32_t* svals, int64_t n, float4* spos, float4* svel, hipStream_t s);
 // per subcell: cnt_in = sum of the brick-window counts of the (up to 8) bricks that reach it,
-// cnt = cnt_in + ocnt; then, after the scan, each window's slice base = off + prefix
-void cell_totals(const BinArgs& A, int64_t nsc, int* cnt, hipStream_t s);
-void cell_bases(const BinArgs& A, int64_t nsc, hipStream_t s);
+// cnt = cnt_in + ocnt, and each 256-subcell block's sum of cnt into tile[block]; then, once tile
+// holds the exclusive scan of those sums (scan_tile_sums), off = exclusive scan of cnt and each
+// window's slice base = off + prefix
+void cell_totals(const BinArgs& A, int64_t nsc, int* cnt, int* tile, hipStream_t s);
+void cell_bases(const BinArgs& A, int64_t nsc, int* cnt, int* tile, int* off, hipStream_t s);
 
 constexpr int kMaxWindowCells = 1000;  // (D + 2)^3 for D <= 8
 constexpr int kMaxPaddedVox = 1728;  // (8 + 2*2)^3: brick-window kernels handle res0 <= 8, apron <= 2

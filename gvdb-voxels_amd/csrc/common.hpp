@@ -110,6 +110,30 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Exclusive scan of a 256-thread block's values; returns the block total in *total.
+__device__ __forceinline__ int block_exclusive_scan_256(int v, int* total) {
+  __shared__ int ws[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[wave] = x;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    int s = ws[w];
+    pre += (w < wave) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + x - v;
+}
+
 // Monotonic float <-> int mapping for atomic min/max on floats.
 __device__ __forceinline__ int float_to_ordered(float f) {
   int i = __float_as_int(f);

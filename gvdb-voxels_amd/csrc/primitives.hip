@@ -56,29 +56,6 @@ void sum_wide(const int* in, int64_t n, unsigned long long* total, hipStream_t s
 
 // ------------------------------------------------------------------ scan
 
-// Exclusive scan of a 256-thread block's values; returns the block total in *total.
-__device__ __forceinline__ int block_exclusive_scan_256(int v, int* total) {
-  __shared__ int ws[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) ws[wave] = x;
-  __syncthreads();
-  int pre = 0, tot = 0;
-#pragma unroll
-  for (int w = 0; w < 4; ++w) {
-    int s = ws[w];
-    pre += (w < wave) ? s : 0;
-    tot += s;
-  }
-  __syncthreads();
-  *total = tot;
-  return pre + x - v;
-}
 
 // Tile = 4 rounds x 256 threads x int4 = 4096 ints.
 __global__ void __launch_bounds__(256) k_scan_reduce(const int* __restrict__ in, int64_t n, int* __restrict__ tile_sums) {
@@ -204,6 +181,11 @@ __global__ void __launch_bounds__(1024) k_scan_small(const int* __restrict__ in,
 }
 
 int64_t scan_tile_count(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+void scan_tile_sums(int* sums, int64_t ntiles, int* total, hipStream_t s) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, s, sums, ntiles, total);
+}
 
 void exclusive_scan(const int* in, int* out, int64_t n, int* tile_buf, int* total, hipStream_t s) {
   if (n <= 0) {

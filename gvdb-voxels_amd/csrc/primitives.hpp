@@ -15,6 +15,8 @@ void fill_f32(float* p, int64_t n, float v, hipStream_t s);
 // ints; *total (device) receives the sum when non-null.
 int64_t scan_tile_count(int64_t n);
 void exclusive_scan(const int* in, int* out, int64_t n, int* tile_buf, int* total, hipStream_t s);
+// Exclusive scan of ntiles block sums in place (one workgroup); their total to *total.
+void scan_tile_sums(int* sums, int64_t ntiles, int* total, hipStream_t s);
 
 // *total (device, zeroed by the caller) += sum of n non-negative int32 values, in 64 bits: the
 // scan's 32-bit total wraps once the sum passes 2^32.
