@@ -450,12 +450,16 @@ int alloc_unique(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, const K* sorted, bool
     CHK(reserve_level(c, l, levcnt[l]));
   }
   DevTree T = tree_of(c);
-  setup_nodes<K>(T, sorted, P<int>(c->flags), P<int>(c->rank), m, f, plan, c->stream);
+  // only new leaves (every inner node exists and is linked): the set-up launch links them too
+  bool leaves_only = levcnt[0] > 0 && c->cnt[d] > 0;
+  for (int l = 1; l < d; ++l) leaves_only = leaves_only && levcnt[l] == 0;
+  setup_nodes<K>(T, sorted, P<int>(c->flags), P<int>(c->rank), m, f, plan, c->stream, leaves_only);
   for (int l = 1; l < d; ++l)
     if (levcnt[l])
       fill_u64(P<uint64_t>(c->clist[l]) + c->cnt[l] * clist_per(c, l), levcnt[l] * clist_per(c, l), kUndef64,
                c->stream);
   for (int l = 0; l < kMaxLev; ++l) c->new_nodes[l] = l < d ? levcnt[l] : 0;
+  if (leaves_only) c->new_nodes[0] = 0;  // linked by the set-up launch
   for (int l = 0; l < d; ++l) c->cnt[l] += levcnt[l];
   c->used0 += levcnt[0];
   return GVDBX_OK;

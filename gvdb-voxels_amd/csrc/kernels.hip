@@ -371,9 +371,11 @@ __device__ __forceinline__ void write_node(gvdbx_node* nd, int lev, int x, int y
   w[3] = make_uint4((uint32_t)childlist, (uint32_t)(childlist >> 32), 0u, 0u);
 }
 
+// link_leaves: every new node is a leaf whose parent is linked already (no new inner node), so
+// each new leaf is also linked here (k_link_level's work for level 0, one launch fewer).
 template <typename K>
 __global__ void k_setup_nodes(DevTree T, const K* __restrict__ k, const int* __restrict__ flags,
-                              const int* __restrict__ rank, int64_t m, KeyFmt f, AllocPlan plan) {
+                              const int* __restrict__ rank, int64_t m, KeyFmt f, AllocPlan plan, int link_leaves) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m || !flags[i]) return;
   const K v = k[i];
@@ -383,14 +385,22 @@ __global__ void k_setup_nodes(DevTree T, const K* __restrict__ k, const int* __r
   const int sh = T.lgR[lev];
   const int64_t idx = plan.base[lev] + (rank[i] - plan.levstart[lev]);
   const uint64_t cl = lev > 0 ? (1ull | ((uint64_t)lev << 8) | ((uint64_t)idx << 16)) : kUndefL;
-  write_node(T.nodes[lev] + idx, lev, ((T.root[0] >> sh) + xr) << sh, ((T.root[1] >> sh) + yr) << sh,
-             ((T.root[2] >> sh) + zr) << sh, cl);
+  const int x = ((T.root[0] >> sh) + xr) << sh, y = ((T.root[1] >> sh) + yr) << sh, z = ((T.root[2] >> sh) + zr) << sh;
+  gvdbx_node* nd = T.nodes[lev] + idx;
+  write_node(nd, lev, x, y, z, cl);
+  if (link_leaves && lev == 0) {
+    const int64_t pi = descend(T, x, y, z, 1);
+    if (pi < 0) return;
+    nd->parent = elem0(1, (uint64_t)pi);
+    T.clist[1][((uint64_t)pi << (3 * T.ld[1])) + (uint64_t)child_bit(T, 1, x, y, z)] = elem0(0, (uint64_t)idx);
+  }
 }
 
 template <typename K>
 void setup_nodes(const DevTree& T, const K* k, const int* flags, const int* rank, int64_t m, KeyFmt f,
-                 const AllocPlan& plan, hipStream_t s) {
-  hipLaunchKernelGGL(k_setup_nodes<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, T, k, flags, rank, m, f, plan);
+                 const AllocPlan& plan, hipStream_t s, bool link_leaves) {
+  hipLaunchKernelGGL(k_setup_nodes<K>, dim3(blocks_for(m, 256)), dim3(256), 0, s, T, k, flags, rank, m, f, plan,
+                     link_leaves ? 1 : 0);
 }
 
 __global__ void k_setup_root(DevTree T) {
@@ -809,8 +819,8 @@ template void unique_flags<uint64_t>(const uint64_t*, int64_t, KeyFmt, int, int*
 template void level_hist<uint32_t>(const uint32_t*, const int*, int64_t, KeyFmt, int*, hipStream_t);
 template void level_hist<uint64_t>(const uint64_t*, const int*, int64_t, KeyFmt, int*, hipStream_t);
 template void setup_nodes<uint32_t>(const DevTree&, const uint32_t*, const int*, const int*, int64_t, KeyFmt,
-                                    const AllocPlan&, hipStream_t);
+                                    const AllocPlan&, hipStream_t, bool);
 template void setup_nodes<uint64_t>(const DevTree&, const uint64_t*, const int*, const int*, int64_t, KeyFmt,
-                                    const AllocPlan&, hipStream_t);
+                                    const AllocPlan&, hipStream_t, bool);
 
 }  // namespace gvdbx

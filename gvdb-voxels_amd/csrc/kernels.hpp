@@ -66,8 +66,9 @@ struct AllocPlan {
   int64_t levstart[kMaxLev];  // first unique rank of each level
 };
 template <typename K>
+// link_leaves: the new nodes are all leaves under linked parents; they are linked here as well
 void setup_nodes(const DevTree& T, const K* k, const int* flags, const int* rank, int64_t m, KeyFmt f,
-                 const AllocPlan& plan, hipStream_t s);
+                 const AllocPlan& plan, hipStream_t s, bool link_leaves = false);
 void setup_root(const DevTree& T, hipStream_t s);
 // gvdbLinkBricks over the level-lev nodes [first, first + count)
 void link_level(const DevTree& T, int lev, int64_t first, int64_t count, hipStream_t s);
