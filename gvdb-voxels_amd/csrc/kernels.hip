@@ -727,12 +727,26 @@ __global__ void __launch_bounds__(256) k_levelset_sc4(SubcellCfg c, int64_t nsc,
   const float* __restrict__ lp = list_pos + 3 * (int64_t)off[k];
   float m0 = INFINITY, m1 = INFINITY;
   int j = 0;
-  for (; j + 4 <= n; j += 4) {
-    const float* p = lp + 3 * j;
-    m0 = fminf(m0, dist2(p[0] - vx, p[1] - vy, p[2] - vz));
-    m1 = fminf(m1, dist2(p[3] - vx, p[4] - vy, p[5] - vz));
-    m0 = fminf(m0, dist2(p[6] - vx, p[7] - vy, p[8] - vz));
-    m1 = fminf(m1, dist2(p[9] - vx, p[10] - vy, p[11] - vz));
+  if (n >= 4) {
+    // four entries per step; the next step's entries are loaded before this step's distances,
+    // so the scalar load latency overlaps the arithmetic
+    float nx[12];
+#pragma unroll
+    for (int e = 0; e < 12; ++e) nx[e] = lp[e];
+    for (; j + 4 <= n; j += 4) {
+      float p[12];
+#pragma unroll
+      for (int e = 0; e < 12; ++e) p[e] = nx[e];
+      if (j + 8 <= n) {
+        const float* q = lp + 3 * (j + 4);
+#pragma unroll
+        for (int e = 0; e < 12; ++e) nx[e] = q[e];
+      }
+      m0 = fminf(m0, dist2(p[0] - vx, p[1] - vy, p[2] - vz));
+      m1 = fminf(m1, dist2(p[3] - vx, p[4] - vy, p[5] - vz));
+      m0 = fminf(m0, dist2(p[6] - vx, p[7] - vy, p[8] - vz));
+      m1 = fminf(m1, dist2(p[9] - vx, p[10] - vy, p[11] - vz));
+    }
   }
   for (; j < n; ++j) m0 = fminf(m0, dist2(lp[3 * j] - vx, lp[3 * j + 1] - vy, lp[3 * j + 2] - vz));
   const float md2 = fminf(m0, m1);
