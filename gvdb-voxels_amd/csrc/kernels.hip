@@ -727,26 +727,35 @@ __global__ void __launch_bounds__(256) k_levelset_sc4(SubcellCfg c, int64_t nsc,
   const float* __restrict__ lp = list_pos + 3 * (int64_t)off[k];
   float m0 = INFINITY, m1 = INFINITY;
   int j = 0;
-  if (n >= 4) {
-    // four entries per step; the next step's entries are loaded before this step's distances,
-    // so the scalar load latency overlaps the arithmetic
-    float nx[12];
+  constexpr int kStep = 8;  // entries per step (24 floats in scalar registers, twice over)
+  if (n >= kStep) {
+    // the next step's entries are loaded before this step's distances, so the scalar load
+    // latency overlaps the arithmetic
+    float nx[3 * kStep];
 #pragma unroll
-    for (int e = 0; e < 12; ++e) nx[e] = lp[e];
-    for (; j + 4 <= n; j += 4) {
-      float p[12];
+    for (int e = 0; e < 3 * kStep; ++e) nx[e] = lp[e];
+    for (; j + kStep <= n; j += kStep) {
+      float p[3 * kStep];
 #pragma unroll
-      for (int e = 0; e < 12; ++e) p[e] = nx[e];
-      if (j + 8 <= n) {
-        const float* q = lp + 3 * (j + 4);
+      for (int e = 0; e < 3 * kStep; ++e) p[e] = nx[e];
+      if (j + 2 * kStep <= n) {
+        const float* q = lp + 3 * (j + kStep);
 #pragma unroll
-        for (int e = 0; e < 12; ++e) nx[e] = q[e];
+        for (int e = 0; e < 3 * kStep; ++e) nx[e] = q[e];
       }
-      m0 = fminf(m0, dist2(p[0] - vx, p[1] - vy, p[2] - vz));
-      m1 = fminf(m1, dist2(p[3] - vx, p[4] - vy, p[5] - vz));
-      m0 = fminf(m0, dist2(p[6] - vx, p[7] - vy, p[8] - vz));
-      m1 = fminf(m1, dist2(p[9] - vx, p[10] - vy, p[11] - vz));
+#pragma unroll
+      for (int e = 0; e < kStep; e += 2) {
+        m0 = fminf(m0, dist2(p[3 * e] - vx, p[3 * e + 1] - vy, p[3 * e + 2] - vz));
+        m1 = fminf(m1, dist2(p[3 * e + 3] - vx, p[3 * e + 4] - vy, p[3 * e + 5] - vz));
+      }
     }
+  }
+  for (; j + 4 <= n; j += 4) {
+    const float* p = lp + 3 * j;
+    m0 = fminf(m0, dist2(p[0] - vx, p[1] - vy, p[2] - vz));
+    m1 = fminf(m1, dist2(p[3] - vx, p[4] - vy, p[5] - vz));
+    m0 = fminf(m0, dist2(p[6] - vx, p[7] - vy, p[8] - vz));
+    m1 = fminf(m1, dist2(p[9] - vx, p[10] - vy, p[11] - vz));
   }
   for (; j < n; ++j) m0 = fminf(m0, dist2(lp[3 * j] - vx, lp[3 * j + 1] - vy, lp[3 * j + 2] - vz));
   const float md2 = fminf(m0, m1);
