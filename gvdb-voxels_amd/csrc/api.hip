@@ -708,9 +708,6 @@ bool wide_keys(const gvdbx_ctx* c, int d) { return key_fmt(c, d).nbits > 32; }
 
 int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accumulate, int min_depth) {
   CHK(check_points(c, n));
-  c->topo_version++;
-  c->sc = 0;
-  c->last_r = r;
   // One read-back returns the bounding box and, when the tree is expected to stay at its
   // depth, the key count of the speculatively launched incremental box pass (it only writes
   // keys and markers, so it is simply dropped when a full rebuild follows).
@@ -746,6 +743,11 @@ int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accum
   if (depth == 0) return fail(c, GVDBX_ERR_DEPTH0, "Points extents smaller than 1 brick");
   if (depth < 0 || depth > 5 || depth >= c->levs)
     return fail(c, GVDBX_ERR_DEPTH_MAX, "Point extents exceed the configured levels (depth %d)", depth);
+  // a rejected call returns above with the tree, atlas and subcells untouched (the speculative
+  // pass writes scratch only); from here the topology may change
+  c->topo_version++;
+  c->sc = 0;
+  c->last_r = r;
   if (accumulate && !c->rebuild) {  // AccumulateTopology (V.cpp:1158-1178)
     return wide_keys(c, c->depth) ? incre_pass<uint64_t>(c, n, orig, r, c->depth, true, spec)
                                   : incre_pass<uint32_t>(c, n, orig, r, c->depth, true, spec);

@@ -131,6 +131,30 @@ def test_depth_errors_on_gpu():
     assert e.value.code in (1, 6)
 
 
+def test_rejected_rebuild_keeps_state(oracle_mod):
+    """A RebuildTopology that ends in a depth error (V.cpp:1187-1194 returns before touching the
+    tree) leaves the tree, atlas and subcell lists usable: a gather afterwards still matches."""
+    from gvdbx import GvdbxError
+    U.log("case rejected rebuild keeps state")
+    p = rand_points(4000, 31, 30, 110)
+    v, t = U.new_volume(), U.new_tree(oracle_mod)
+    dpos, _ = U.run_pipeline(v, t, p, 1.0, 4)
+    before = v.depth()[0], v.node_count(0), v.subcell_info()
+    small = v.to_device(np.full((16, 3), 3.5, np.float32))
+    v.SetPoints(small, n=16)
+    with pytest.raises(GvdbxError) as e:
+        v.RebuildTopology(16, 1.0)
+    assert e.value.code == 2
+    assert (v.depth()[0], v.node_count(0), v.subcell_info()) == before
+    v.SetPoints(dpos, n=len(p))
+    v.FillChannel(0, 0.0)
+    t.fill_channel(0, 0.0)
+    v.GatherDensity(4, len(p), 1.0, (0, 0, 0), before[2][1], 0)
+    t.gather_density(1.0, 0)
+    U.compare_channel(v, t, 0, interior_only=True)
+    v.close()
+
+
 @pytest.mark.parametrize("radius,sc", [(2.5, 2), (3.5, 4), (1.0, 1)])
 def test_window_overflow_paths(oracle_mod, radius, sc):
     """r + 1 > sc: pairs outside the per-brick LDS window take the global path."""
