@@ -427,7 +427,7 @@ int sort_unique_alloc(gvdbx_ctx* c, int64_t m, KeyFmt f, int d, int64_t count_le
     level_hist<K>(sorted, P<int>(c->flags), m, f, P<int>(c->meta) + M_LEV, c->stream);
   }
   if (count_leaves >= 0)
-    count_leaf_flags(tree_of(c), count_leaves, markers, nullptr, P<int>(c->meta) + M_USED, c->stream);
+    count_leaf_flags(tree_of(c), count_leaves, markers, P<int>(c->meta) + M_USED, c->stream);
   CHK(read_meta(c));
   return alloc_unique<K>(c, m, f, d, sorted, count_leaves >= 0);
 }
@@ -639,9 +639,10 @@ int incre_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool 
   if (n > 0) CHK(box_launch<K>(c, 0, g.chunk, tr, r, 1.0f, d, P<int>(c->markers), g));
   if (n > 0 && g.chunk >= n) {
     // One box launch covers the points: compact its keys, sort and unique them in one workgroup
-    // and apply the markers with the used count, all reading the key count on the device, so
-    // the read-back that returns it also returns the level and used counts.  A count of 0 or
-    // above the one-workgroup limit leaves this to the host path (markers applied only if > 0).
+    // and count the marked leaves, all reading the key count on the device, so the read-back
+    // that returns it also returns the level and used counts.  A count above the one-workgroup
+    // limit leaves this to the host path.  Nothing here writes a node: the call may still be
+    // rejected (depth), and the markers are applied by the delink launch once it goes ahead.
     CHK(ensure(c, c->keys, (size_t)kSmallSortKeys * sizeof(K)));
     CHK(ensure(c, c->keys_alt, (size_t)kSmallSortKeys * sizeof(K)));
     CHK(ensure(c, c->flags, (size_t)kSmallSortKeys * 4));
@@ -652,7 +653,7 @@ int incre_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool 
     sort_unique_small<K>(P<K>(c->keys), 0, key_fmt(c, d), d, P<K>(c->keys_alt), P<int>(c->flags), P<int>(c->rank),
                          P<int>(c->meta) + M_UNIQUE, P<int>(c->meta) + M_LEV, P<int>(c->meta) + M_USED, mdev,
                          c->stream);
-    count_leaf_flags(T, total0, P<int>(c->markers), mdev, P<int>(c->meta) + M_USED, c->stream);
+    count_leaf_flags(T, total0, P<int>(c->markers), P<int>(c->meta) + M_USED, c->stream);
     c->spec_sorted = true;
   }
   return GVDBX_OK;
@@ -677,7 +678,7 @@ int incre_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool ac
       CHK(alloc_unique<K>(c, m, key_fmt(c, d), d, P<K>(c->keys_alt), true));
       link_all(c, d);
       DevTree T = tree_of(c);
-      delink_leaves(T, c->cnt[0], c->stream);
+      delink_leaves(T, c->cnt[0], P<int>(c->markers), total0, c->stream);
       for (int l = 1; l < d; ++l) delink_level(T, l, c->cnt[l], c->stream);
       return GVDBX_OK;
     }
@@ -685,12 +686,13 @@ int incre_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool ac
   CHK(box_pass<K>(c, n, tr, r, 1.0f, d, P<int>(c->markers), &m, true));
   if (m == 0) return GVDBX_OK;  // no new node: topology unchanged (V.cpp:1066)
   // flags = markers, then the used count (V.cpp:1144-1147): the flagged leaves are the marked old
-  // ones (counted with the markers' application, read back with the level counts) plus the new
-  // ones (sort_unique_alloc adds those); delinking clears no flag
+  // ones (counted from the markers, read back with the level counts) plus the new ones
+  // (sort_unique_alloc adds those); the delink launch applies the markers, then unlinks the
+  // leaves left unmarked
   CHK(sort_unique_alloc<K>(c, m, key_fmt(c, d), d, total0, P<int>(c->markers)));
   link_all(c, d);
   DevTree T = tree_of(c);
-  delink_leaves(T, c->cnt[0], c->stream);
+  delink_leaves(T, c->cnt[0], P<int>(c->markers), total0, c->stream);
   for (int l = 1; l < d; ++l) delink_level(T, l, c->cnt[l], c->stream);
   return GVDBX_OK;
 }
@@ -1277,9 +1279,22 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   if (sc <= 0 || (sc & (sc - 1)) || res0 % sc || sc * sc * sc > 512)
     return fail(c, GVDBX_ERR_ARG, "subcell size %d must be a power of two dividing %d", sc, res0);
   if (!(r >= 0.0f) || r > 1.0e6f) return fail(c, GVDBX_ERR_ARG, "radius %g out of range", (double)r);
+  const bool with_vel = c->vel != nullptr;
+  const bool keep_vel_lists = with_vel && c->store_vel_lists;
+  if (fp16 && keep_vel_lists && !c->vel_box)
+    return fail(c, GVDBX_ERR_STATE, "GetMinMaxVel must run before an FP16 insert that stores velocities");
+  if (c->used0 > 0 && c->depth < 1) return fail(c, GVDBX_ERR_DEGENERATE, "no tree");
   if (len) *len = 0;
-  if (c->used0 == 0) return GVDBX_OK;
-  if (c->depth < 1) return fail(c, GVDBX_ERR_DEGENERATE, "no tree");
+  // Every check that needs no device result is above.  From here the subcell arrays change, so
+  // the gathers refuse (c->sc = 0) until this insert has completed.
+  c->sc = 0;
+  if (c->used0 == 0) {
+    c->sc = sc;
+    c->scdim = res_of(c, 0) / sc;
+    c->nsc = 0;
+    c->sclen = 0;
+    return GVDBX_OK;
+  }
   const SubcellCfg cfg = subcell_cfg(c, sc, r);
   const int64_t per = (int64_t)cfg.D * cfg.D * cfg.D;
   const int64_t total = c->cnt[0];
@@ -1295,7 +1310,6 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   CHK(ensure(c, c->sc_off, (size_t)(nsc_cap + 1) * 4));
   CHK(ensure(c, c->sc_pos, (size_t)(nsc_cap + 1) * 12));
   CHK(ensure(c, c->sc_nid, (size_t)(nsc_cap + 1) * 4));
-  const bool with_vel = c->vel != nullptr;
   if (with_vel) CHK(ensure(c, c->vsum, (size_t)(nsc_cap + 1) * 24));
   CHK(ensure(c, c->hkeys, (size_t)n * 4));
   CHK(ensure(c, c->hkeys_alt, (size_t)n * 4));
@@ -1401,22 +1415,10 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   CHK(zero_meta(c));
   scan_tile_sums(P<int>(c->tile_buf), nblk, P<int>(c->meta) + M_TOTAL, c->stream);
   cell_bases(A, nsc, P<int>(c->sc_cnt), P<int>(c->tile_buf), P<int>(c->sc_off), c->stream);
-  c->sc = sc;
-  c->scdim = cfg.D;
-  c->nsc = nsc;
-  c->lists_have_vel = with_vel && c->store_vel_lists;
-  c->lists_have_clr = c->clr != nullptr;  // gvdbInsertSubcell stores colours when bound (P.cuh:268)
-  if (fp16 && c->lists_have_vel && !c->vel_box)
-    return fail(c, GVDBX_ERR_STATE, "GetMinMaxVel must run before an FP16 insert that stores velocities");
-  c->lists_fp16 = fp16;
-  for (int a = 0; a < 3; ++a) {
-    c->q_min[a] = A.qmin[a];
-    c->q_range[a] = A.qrange[a];
-  }
+  const bool keep_clr = c->clr != nullptr;  // gvdbInsertSubcell stores colours when bound (P.cuh:268)
   Buf& lpos = fp16 ? c->list_q16 : c->list_pos;
   Buf& lvel = fp16 ? c->list_vq16 : c->list_vel;
   const size_t ebytes = fp16 ? 6 : 12;  // per list entry: float3 or ushort3
-  c->have_vsum = with_vel;
   // Lists sized for the largest possible length when that bound is modest (and below 2^31, so
   // the 32-bit offsets cannot wrap), so the insert is enqueued before the single length
   // read-back; otherwise the length is read and checked before anything is inserted.  From a
@@ -1428,8 +1430,8 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   int64_t ln = -1;
   if (early) {
     CHK(ensure(c, lpos, (size_t)(bound + 1) * ebytes));
-    if (c->lists_have_vel) CHK(ensure(c, lvel, (size_t)(bound + 1) * ebytes));
-    if (c->lists_have_clr) CHK(ensure(c, c->list_clr, (size_t)(bound + 1) * 4));
+    if (keep_vel_lists) CHK(ensure(c, lvel, (size_t)(bound + 1) * ebytes));
+    if (keep_clr) CHK(ensure(c, c->list_clr, (size_t)(bound + 1) * 4));
   } else {
     const bool wide = bound >= ((int64_t)1 << 32);
     if (wide) sum_wide(P<int>(c->sc_cnt), nsc, (unsigned long long*)(P<int>(c->meta) + M_WIDE), c->stream);
@@ -1443,14 +1445,14 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     }
     if (ln > INT32_MAX || ln > bound) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);
     CHK(ensure(c, lpos, (size_t)(ln + 1) * ebytes));
-    if (c->lists_have_vel) CHK(ensure(c, lvel, (size_t)(ln + 1) * ebytes));
-    if (c->lists_have_clr) CHK(ensure(c, c->list_clr, (size_t)(ln + 1) * 4));
+    if (keep_vel_lists) CHK(ensure(c, lvel, (size_t)(ln + 1) * ebytes));
+    if (keep_clr) CHK(ensure(c, c->list_clr, (size_t)(ln + 1) * 4));
   }
   A.list_pos = fp16 ? nullptr : P<float>(c->list_pos);
-  A.list_vel = (c->lists_have_vel && !fp16) ? P<float>(c->list_vel) : nullptr;
+  A.list_vel = (keep_vel_lists && !fp16) ? P<float>(c->list_vel) : nullptr;
   A.list_q16 = fp16 ? P<uint16_t>(c->list_q16) : nullptr;
-  A.list_vq16 = (c->lists_have_vel && fp16) ? P<uint16_t>(c->list_vq16) : nullptr;
-  A.list_clr = c->lists_have_clr ? P<uint32_t>(c->list_clr) : nullptr;
+  A.list_vq16 = (keep_vel_lists && fp16) ? P<uint16_t>(c->list_vq16) : nullptr;
+  A.list_clr = keep_clr ? P<uint32_t>(c->list_clr) : nullptr;
   if (early) CHK(read_meta_begin(c));  // the length is final after the scan
   {
     TimedRegion tr(c, "insert");
@@ -1462,9 +1464,21 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     ln = (uint32_t)c->h_meta[M_TOTAL];
   }
   if (ln > INT32_MAX || ln > bound) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);
+  CHK(launch_status(c, "InsertPointsSubcell"));
+  c->sc = sc;
+  c->scdim = cfg.D;
+  c->nsc = nsc;
   c->sclen = ln;
+  c->lists_have_vel = keep_vel_lists;
+  c->lists_have_clr = keep_clr;
+  c->lists_fp16 = fp16;
+  c->have_vsum = with_vel;
+  for (int a = 0; a < 3; ++a) {
+    c->q_min[a] = A.qmin[a];
+    c->q_range[a] = A.qrange[a];
+  }
   if (len) *len = ln;
-  return launch_status(c, "InsertPointsSubcell");
+  return GVDBX_OK;
 }
 
 int gvdbx_insert_points_subcell(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t* len) {

@@ -427,16 +427,6 @@ void link_level(const DevTree& T, int lev, int64_t first, int64_t count, hipStre
   hipLaunchKernelGGL(k_link_level, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, lev, first, count);
 }
 
-// Leaf flags <- markers (V.cpp:1075-1081).
-__global__ void k_apply_markers(DevTree T, const int* __restrict__ markers, int64_t count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) T.nodes[0][i].flags = markers[i] ? 1 : 0;
-}
-void apply_markers(const DevTree& T, const int* markers, int64_t count, hipStream_t s) {
-  if (count <= 0) return;
-  hipLaunchKernelGGL(k_apply_markers, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, markers, count);
-}
-
 __global__ void k_flags_to_markers(DevTree T, int64_t count, int* __restrict__ markers) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) markers[i] = T.nodes[0][i].flags;
@@ -446,11 +436,14 @@ void copy_flags_to_markers(const DevTree& T, int64_t count, int* markers, hipStr
   hipLaunchKernelGGL(k_flags_to_markers, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, count, markers);
 }
 
-// gvdbDelinkLeafBricks (N.cuh:401-429).
-__global__ void k_delink_leaves(DevTree T, int64_t count) {
+// gvdbDelinkLeafBricks (N.cuh:401-429).  markers (the first nmark leaves): each leaf's flag is
+// first set to its marker (the incremental pass's marker application, V.cpp:1075-1081), here
+// rather than before the new nodes are allocated, since nothing in between reads old leaves' flags.
+__global__ void k_delink_leaves(DevTree T, int64_t count, const int* __restrict__ markers, int64_t nmark) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   gvdbx_node* nd = T.nodes[0] + i;
+  if (markers && i < nmark) nd->flags = markers[i] ? 1 : 0;
   if (nd->flags) return;
   if (nd->parent == kUndef64) return;
   const int x = nd->pos[0], y = nd->pos[1], z = nd->pos[2];
@@ -459,9 +452,9 @@ __global__ void k_delink_leaves(DevTree T, int64_t count) {
   if (pi < 0) return;
   T.clist[1][((uint64_t)pi << (3 * T.ld[1])) + (uint64_t)child_bit(T, 1, x, y, z)] = kUndef64;
 }
-void delink_leaves(const DevTree& T, int64_t count, hipStream_t s) {
+void delink_leaves(const DevTree& T, int64_t count, const int* markers, int64_t nmark, hipStream_t s) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_delink_leaves, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, count);
+  hipLaunchKernelGGL(k_delink_leaves, dim3(blocks_for(count, 256)), dim3(256), 0, s, T, count, markers, nmark);
 }
 
 // gvdbDelinkBricks (N.cuh:431-467): one wave per inner node scans its child list.
@@ -523,30 +516,20 @@ void count_flags(const int* flags, int64_t n, int* out, hipStream_t s) {
   hipLaunchKernelGGL(k_count_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, flags, n, out);
 }
 
-// *out (zeroed by the caller) += flagged leaves among the first n; with markers, each leaf's
-// flag is first set to its marker (k_apply_markers folded in)
+// *out (zeroed by the caller) += flagged leaves among the first n; with markers, the leaves whose
+// marker is set (the flags they get once the markers are applied).  Writes no node: the count is
+// taken before the incremental pass is known to go ahead.
 __global__ void __launch_bounds__(256) k_count_leaf_flags(DevTree T, int64_t n, const int* __restrict__ markers,
-                                                         const int* __restrict__ mcount, int* out) {
-  // mcount: apply the markers only when the box pass found new keys (without any the step
-  // leaves the topology, flags included, as it was: V.cpp:1066)
-  const bool apply = markers != nullptr && (mcount == nullptr || *mcount > 0);
+                                                         int* out) {
   int c = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    int f;
-    if (apply) {
-      f = markers[i] ? 1 : 0;
-      T.nodes[0][i].flags = (uint8_t)f;
-    } else {
-      f = T.nodes[0][i].flags != 0;
-    }
-    c += f;
-  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    c += markers ? (markers[i] != 0) : (T.nodes[0][i].flags != 0);
   c = wave_sum(c);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
 }
-void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, const int* mcount, int* out, hipStream_t s) {
+void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, int* out, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_count_leaf_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, T, n, markers, mcount, out);
+  hipLaunchKernelGGL(k_count_leaf_flags, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, s, T, n, markers, out);
 }
 
 // ================================================================ atlas

@@ -41,10 +41,9 @@ template <typename K>
 // mdev: read m from device memory instead (meta_unique = -1 when it is 0 or above the limit)
 void sort_unique_small(const K* in, int64_t m, KeyFmt f, int depth, K* out, int* flags, int* rank, int* meta_unique,
                        int* meta_lev, int* meta_zero, const int* mdev, hipStream_t s);
-// *out (zeroed by the caller) += flagged leaves among the first n (markers: set each flag to
-// its marker first)
-// (mcount: apply the markers only when *mcount > 0)
-void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, const int* mcount, int* out, hipStream_t s);
+// *out (zeroed by the caller) += flagged leaves among the first n, or with markers the leaves
+// whose marker is set; writes no node
+void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, int* out, hipStream_t s);
 template <typename K>
 void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const int* rank, int64_t n, KeyFmt f, K* out,
                       hipStream_t s);
@@ -72,8 +71,9 @@ void setup_nodes(const DevTree& T, const K* k, const int* flags, const int* rank
 void setup_root(const DevTree& T, hipStream_t s);
 // gvdbLinkBricks over the level-lev nodes [first, first + count)
 void link_level(const DevTree& T, int lev, int64_t first, int64_t count, hipStream_t s);
-void apply_markers(const DevTree& T, const int* markers, int64_t count, hipStream_t s);
-void delink_leaves(const DevTree& T, int64_t count, hipStream_t s);
+// gvdbDelinkLeafBricks over the first count leaves; with markers, the flags of the first nmark
+// leaves are set to their markers first
+void delink_leaves(const DevTree& T, int64_t count, const int* markers, int64_t nmark, hipStream_t s);
 void delink_level(const DevTree& T, int lev, int64_t count, hipStream_t s);
 // Leaf lookup table being built (DevTree::bmap): cells [lo, lo + n) of 2^lg0-voxel bricks, or none
 struct BrickMapOut {

@@ -96,6 +96,8 @@ def test_fp16_lists_and_levelset(oracle_mod, sc, r, seed):
 
 
 def test_fp16_velocity_lists_need_min_max_vel(oracle_mod):
+    """The refusal comes before anything is launched: the lists of the insert before stay as they
+    were, so a gather after the refused call gives the same channel, bit for bit."""
     from gvdbx import GvdbxError
     U.log("case fp16 velocity box required")
     p, vel = cloud(5, 3000)
@@ -104,8 +106,16 @@ def test_fp16_velocity_lists_need_min_max_vel(oracle_mod):
     v.SetPoints(dp, dv, n=len(p))
     v.RebuildTopology(len(p), 1.0)
     v.UpdateAtlas()
+    ln = v.InsertPointsSubcell(4, len(p), 1.0)
+    v.GatherLevelSet(4, len(p), 1.0, (0, 0, 0), ln, 0)
+    first = v.get_channel(0).tobytes()
+    info = v.subcell_info()
     v.SetSubcellOptions(store_velocity_lists=True)
     with pytest.raises(GvdbxError) as e:
         v.InsertPointsSubcell_FP16(4, len(p), 1.0)
     assert e.value.code == 6
+    assert v.subcell_info() == info
+    v.FillChannel(0, 0.0)
+    v.GatherLevelSet(4, len(p), 1.0, (0, 0, 0), ln, 0)
+    assert v.get_channel(0).tobytes() == first
     v.close()

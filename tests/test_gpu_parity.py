@@ -133,23 +133,43 @@ def test_depth_errors_on_gpu():
 
 def test_rejected_rebuild_keeps_state(oracle_mod):
     """A RebuildTopology that ends in a depth error (V.cpp:1187-1194 returns before touching the
-    tree) leaves the tree, atlas and subcell lists usable: a gather afterwards still matches."""
+    tree) leaves the pools, child lists, atlas and subcell lists as they were: a gather afterwards
+    still matches, and the next (incremental) rebuild of the original points matches the
+    oracle's, which never saw the rejected call.  The rejected call comes after two full
+    rebuilds, so it runs the speculative incremental box pass, whose new keys (the 16 points
+    reach a missing brick) must not reach the node flags."""
     from gvdbx import GvdbxError
     U.log("case rejected rebuild keeps state")
     p = rand_points(4000, 31, 30, 110)
     v, t = U.new_volume(), U.new_tree(oracle_mod)
-    dpos, _ = U.run_pipeline(v, t, p, 1.0, 4)
+    dpos, _ = U.run_pipeline(v, t, p, 1.0, 4, steps=2)
     before = v.depth()[0], v.node_count(0), v.subcell_info()
+    pools = [v.get_nodes(l).tobytes() for l in range(len(v.log2dims))]
+    lists = [v.get_childlists(l).tobytes() for l in range(1, before[0] + 1)]
     small = v.to_device(np.full((16, 3), 3.5, np.float32))
     v.SetPoints(small, n=16)
     with pytest.raises(GvdbxError) as e:
         v.RebuildTopology(16, 1.0)
     assert e.value.code == 2
     assert (v.depth()[0], v.node_count(0), v.subcell_info()) == before
+    assert [v.get_nodes(l).tobytes() for l in range(len(v.log2dims))] == pools, "a rejected call changed a pool"
+    assert [v.get_childlists(l).tobytes() for l in range(1, before[0] + 1)] == lists
     v.SetPoints(dpos, n=len(p))
     v.FillChannel(0, 0.0)
     t.fill_channel(0, 0.0)
     v.GatherDensity(4, len(p), 1.0, (0, 0, 0), before[2][1], 0)
+    t.gather_density(1.0, 0)
+    U.compare_channel(v, t, 0, interior_only=True)
+    # the next call is the state machine's first incremental rebuild, on both sides
+    v.RebuildTopology(len(p), 1.0)
+    t.rebuild_topology(p, 1.0)
+    v.UpdateAtlas()
+    t.update_atlas()
+    U.compare_topology(v, t)
+    ln = v.InsertPointsSubcell(4, len(p), 1.0)
+    assert ln == t.insert_points_subcell(4, p, 1.0)
+    U.compare_subcells(v, t)
+    v.GatherDensity(4, len(p), 1.0, (0, 0, 0), ln, 0)
     t.gather_density(1.0, 0)
     U.compare_channel(v, t, 0, interior_only=True)
     v.close()
