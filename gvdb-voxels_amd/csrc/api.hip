@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -28,6 +29,53 @@ struct Buf {
 };
 
 enum MetaSlot { M_BBOX = 0, M_UNIQUE = 8, M_COUNT = 9, M_USED = 10, M_TOTAL = 11, M_LEV = 16, M_WIDE = 32, M_INTS = 64 };
+
+// Scratch of gvdbx_splat_trilinear_noalloc, for callers that cannot allocate device memory: the
+// splat's counting sort (key counts and starts, scan tiles, per-point key/rank, sorted payload
+// and permutation) sized for kNoAllocPoints points and kNoAllocKeys keys (27 per brick + 1).
+constexpr int64_t kNoAllocPoints = (int64_t)1 << 20;
+constexpr int64_t kNoAllocKeys = (int64_t)1 << 20;
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kScrCnt = align256((size_t)(kNoAllocKeys + 1) * 4);
+constexpr size_t kScrStart = align256((size_t)(kNoAllocKeys + 1) * 4);
+constexpr size_t kScrTiles = align256((size_t)((kNoAllocKeys + 1 + kScanTile - 1) / kScanTile + 16) * 4);
+constexpr size_t kScrKeyrank = align256((size_t)(kNoAllocPoints + 1) * 8);
+constexpr size_t kScrPayload = align256((size_t)(kNoAllocPoints + 1) * 16);
+constexpr size_t kScrPerm = align256((size_t)(kNoAllocPoints + 1) * 4);
+constexpr size_t kScratchBytes = kScrCnt + kScrStart + kScrTiles + kScrKeyrank + kScrPayload + kScrPerm;
+
+}  // namespace
+
+// One copy per GPU, held until the process ends.  Host code never uses this name as an address:
+// each device's copy is reached through the pointer hipGetSymbolAddress returns for it
+// (scratch_address), and every call on one GPU shares that GPU's copy.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, resolved once per device with that device
+// current (the caller has made it current).  Returns the resolution's status.
+hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) (void)hipGetLastError();
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    g_scratch_err[device] = e;
+    g_scratch_done[device] = true;
+  }
+  *out = g_scratch_ptr[device];
+  return g_scratch_err[device];
+}
 
 }  // namespace
 
@@ -124,6 +172,10 @@ struct gvdbx_ctx {
   bool bm_valid = false;
   int bm_lo[3] = {0, 0, 0}, bm_n[3] = {0, 0, 0};
   float last_r = 0.f;  // radius of the last rebuild
+
+  // this device's copy of `scratch` (gvdbx_splat_trilinear_noalloc), resolved at creation
+  void* scratch = nullptr;
+  hipError_t scratch_err = hipSuccess;
 
   // named-kernel timer: event pairs recorded on the stream, summed when read
   std::string timer_name;
@@ -300,7 +352,9 @@ hipEvent_t next_event(gvdbx_ctx* c) {
 struct TimedRegion {
   gvdbx_ctx* c;
   bool on;
-  TimedRegion(gvdbx_ctx* ctx, const char* name) : c(ctx), on(ctx->timer_on && ctx->timer_name == name) {
+  // allowed = false: never times (no event is created, for the entry point that allocates nothing)
+  TimedRegion(gvdbx_ctx* ctx, const char* name, bool allowed = true)
+      : c(ctx), on(allowed && ctx->timer_on && ctx->timer_name == name) {
     if (on) {
       hipEvent_t e = next_event(c);
       if (e) (void)hipEventRecord(e, c->stream);
@@ -909,6 +963,7 @@ int gvdbx_ctx_create(int device, void* hip_stream, gvdbx_ctx** out) {
     }
     c->own_stream = true;
   }
+  c->scratch_err = scratch_address(device, &c->scratch);
   if (ensure(c, c->meta, M_INTS * sizeof(int)) != GVDBX_OK ||
       hipHostMalloc((void**)&c->h_meta, M_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&c->meta_ev, hipEventDisableTiming) != hipSuccess) {
@@ -1769,39 +1824,63 @@ int field_checks(gvdbx_ctx* c, int chan, const void* q, int64_t n) {
   return GVDBX_OK;
 }
 
-// Trilinear splat of (q, mass) into the interior of chan (sorted pull, see splat.hip).
-int splat_impl(gvdbx_ctx* c, int chan, const float* q, const float* mass, int64_t n, int accum) {
+// Scratch of one splat: key counts and starts, scan tiles, per-point (key, rank), the sorted
+// (q, m) payload and its permutation.
+struct SplatWork {
+  uint32_t* cnt;
+  int* start;
+  int* tiles;
+  uint2* keyrank;
+  float4* payload;
+  uint32_t* perm;
+};
+
+int splat_limits(gvdbx_ctx* c) {
   if (!splat_brick_supported(c->ld[0]))
     return fail(c, GVDBX_ERR_CAPACITY, "splat supports leaf bricks of 2^1..2^4 voxels");
   if (c->nslots * 27 + 2 > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "too many bricks for the splat keys");
-  if (c->nslots == 0) return GVDBX_OK;
+  return GVDBX_OK;
+}
+
+// Trilinear splat of (q, mass) into the interior of chan (sorted pull, see splat.hip), with the
+// sort's scratch in w (sized for n points and nslots * 27 + 1 keys).
+int splat_run(gvdbx_ctx* c, int chan, const float* q, const float* mass, int64_t n, int accum, const SplatWork& w,
+              bool timed) {
   const uint32_t sentinel = (uint32_t)(c->nslots * 27);
   const int64_t nkeys = (int64_t)sentinel + 1;
+  {
+    TimedRegion tr(c, "splat_sort", timed);
+    fill_u32(w.cnt, nkeys, 0u, c->stream);  // a fill kernel: the scratch may hold a previous call's counts
+    splat_count(tree_of(c), q, n, P<int>(c->leaf_slot), c->ld[0], sentinel, w.cnt, w.keyrank, c->stream);
+    exclusive_scan((const int*)w.cnt, w.start, nkeys, w.tiles, nullptr, c->stream);
+    splat_place(q, mass, n, w.keyrank, w.start, w.payload, w.perm, c->stream);
+  }
+  {
+    TimedRegion tr(c, "splat_brick", timed);
+    splat_brick(c->nslots, w.start, w.payload, P<int>(c->nbr), P<int4>(c->slot_pos), c->ld[0], c->apron,
+                P<float>(c->chan[chan]), accum, c->stream);
+  }
+  c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;
+  return launch_status(c, "SplatTrilinear");
+}
+
+int splat_impl(gvdbx_ctx* c, int chan, const float* q, const float* mass, int64_t n, int accum) {
+  CHK(splat_limits(c));
+  if (c->nslots == 0) return GVDBX_OK;
+  const int64_t nkeys = c->nslots * 27 + 1;
   CHK(ensure(c, c->sp_cnt, (size_t)(nkeys + 1) * 4));
   CHK(ensure(c, c->sp_start, (size_t)(nkeys + 1) * 4));
   CHK(ensure(c, c->sp_keyrank, (size_t)(n + 1) * 8));
   CHK(ensure(c, c->sp_payload, (size_t)(n + 1) * 16));
   CHK(ensure(c, c->sp_perm, (size_t)(n + 1) * 4));
   CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(nkeys + 1) + 16) * 4));
-  {
-    TimedRegion tr(c, "splat_sort");
-    fill_u32(P<uint32_t>(c->sp_cnt), nkeys, 0u, c->stream);
-    splat_count(tree_of(c), q, n, P<int>(c->leaf_slot), c->ld[0], sentinel, P<uint32_t>(c->sp_cnt),
-                P<uint2>(c->sp_keyrank), c->stream);
-    exclusive_scan(P<int>(c->sp_cnt), P<int>(c->sp_start), nkeys, P<int>(c->tile_buf), nullptr, c->stream);
-    splat_place(q, mass, n, P<uint2>(c->sp_keyrank), P<int>(c->sp_start), P<float4>(c->sp_payload),
-                P<uint32_t>(c->sp_perm), c->stream);
-  }
-  {
-    TimedRegion tr(c, "splat_brick");
-    splat_brick(c->nslots, P<int>(c->sp_start), P<float4>(c->sp_payload), P<int>(c->nbr), P<int4>(c->slot_pos),
-                c->ld[0], c->apron, P<float>(c->chan[chan]), accum, c->stream);
-  }
+  const SplatWork w{P<uint32_t>(c->sp_cnt), P<int>(c->sp_start), P<int>(c->tile_buf),
+                    P<uint2>(c->sp_keyrank), P<float4>(c->sp_payload), P<uint32_t>(c->sp_perm)};
+  CHK(splat_run(c, chan, q, mass, n, accum, w, true));
   c->sp_order_q = n > 0 ? q : nullptr;
   c->sp_order_n = n;
   c->sp_order = n > 0 ? P<uint32_t>(c->sp_perm) : nullptr;
-  c->nbr_apron_ok[chan] = c->bound_ok[chan] = false;
-  return launch_status(c, "SplatTrilinear");
+  return GVDBX_OK;
 }
 
 }  // namespace
@@ -1810,6 +1889,34 @@ int gvdbx_splat_trilinear(gvdbx_ctx* c, int chan, const void* q, const void* mas
   CHK(field_checks(c, chan, q, n));
   TimedRegion tr(c, "splat");
   return splat_impl(c, chan, (const float*)q, (const float*)mass, n, accum);
+}
+
+// gvdbx_splat_trilinear with its scratch in this device's copy of `scratch`: no allocation,
+// sizes capped by the array (kNoAllocPoints points, kNoAllocKeys splat keys).  Calls that overlap
+// on one GPU (any context, any stream) race on that copy.
+int gvdbx_splat_trilinear_noalloc(gvdbx_ctx* c, int chan, const void* q, const void* mass, int64_t n, int accum) {
+  CHK(field_checks(c, chan, q, n));
+  CHK(splat_limits(c));
+  if (n > kNoAllocPoints)
+    return fail(c, GVDBX_ERR_CAPACITY, "SplatTrilinear without allocation takes at most %lld points per call (%lld given)",
+                (long long)kNoAllocPoints, (long long)n);
+  if (c->nslots * 27 + 1 > kNoAllocKeys)
+    return fail(c, GVDBX_ERR_CAPACITY, "SplatTrilinear without allocation takes at most %lld bricks (%lld active)",
+                (long long)((kNoAllocKeys - 1) / 27), (long long)c->nslots);
+  if (!c->scratch)
+    return fail(c, GVDBX_ERR_HIP, "no scratch array on device %d: %s", c->device, hipGetErrorString(c->scratch_err));
+  if (c->nslots == 0) return GVDBX_OK;
+  unsigned char* base = (unsigned char*)c->scratch;
+  const SplatWork w{(uint32_t*)base, (int*)(base + kScrCnt), (int*)(base + kScrCnt + kScrStart),
+                    (uint2*)(base + kScrCnt + kScrStart + kScrTiles),
+                    (float4*)(base + kScrCnt + kScrStart + kScrTiles + kScrKeyrank),
+                    (uint32_t*)(base + kScrCnt + kScrStart + kScrTiles + kScrKeyrank + kScrPayload)};
+  CHK(splat_run(c, chan, (const float*)q, (const float*)mass, n, accum, w, false));
+  // the permutation lives in the shared scratch: the backward does not reuse it
+  c->sp_order_q = nullptr;
+  c->sp_order_n = 0;
+  c->sp_order = nullptr;
+  return GVDBX_OK;
 }
 
 int gvdbx_splat_backward(gvdbx_ctx* c, int chan_grad, const void* q, const void* mass, int64_t n, void* dq,

@@ -60,6 +60,7 @@ SIGNATURES = {
     "gvdbx_update_apron_channels": [_P, _P, _P, _I],
     "gvdbx_sample_trilinear": [_P, _I, _P, _I64, _P],
     "gvdbx_splat_trilinear": [_P, _I, _P, _P, _I64, _I],
+    "gvdbx_splat_trilinear_noalloc": [_P, _I, _P, _P, _I64, _I],
     "gvdbx_scatter_density": [_P, _I64, _F, _F, _P, _I, _I, _I, _I],
     "gvdbx_gather_density_velocity": [_P, _I, _I64, _F, _P, _I64, _I, _I, _I, _I],
     "gvdbx_splat_backward": [_P, _I, _P, _P, _I64, _P, _P],

@@ -233,6 +233,12 @@ class VolumeGVDB:
         self._chk(self.L.gvdbx_splat_trilinear(self.h, chan, q_dev.ptr, self._ptr(mass_dev), n, int(accum)),
                   "SplatTrilinear")
 
+    def SplatTrilinearNoAlloc(self, chan, q_dev, n, mass_dev=None, accum=False):
+        """SplatTrilinear with its scratch in the library's fixed per-GPU array (no allocation;
+        at most 1,048,576 points per call)."""
+        self._chk(self.L.gvdbx_splat_trilinear_noalloc(self.h, chan, q_dev.ptr, self._ptr(mass_dev), n, int(accum)),
+                  "SplatTrilinearNoAlloc")
+
     def SplatBackward(self, chan_grad, q_dev, n, mass_dev=None, dq_dev=None, dmass_dev=None):
         """dL/dq and dL/dmass of the splat from dL/drho held in chan_grad."""
         self._chk(self.L.gvdbx_splat_backward(self.h, chan_grad, q_dev.ptr, self._ptr(mass_dev), n, self._ptr(dq_dev),

@@ -198,6 +198,15 @@ int gvdbx_scatter_density(gvdbx_ctx* ctx, int64_t n, float radius, float amp, co
  *      unit masses.  Deterministic: points sorted by (brick, region), exact 64-bit fixed-point
  *      sums per brick.  The apron is not written (UpdateApron afterwards). */
 int gvdbx_splat_trilinear(gvdbx_ctx* ctx, int chan, const void* q, const void* mass, int64_t n, int accum);
+/* gvdbx_splat_trilinear without any device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The splat's sort scratch lives in a fixed array of about 36 MiB per
+ * GPU, allocated when the library is loaded on that GPU and held until the process ends; so
+ * n <= 1,048,576 points per call and at most 38,836 active bricks (GVDBX_ERR_CAPACITY beyond;
+ * splat larger clouds in several calls with accum = 1).  Same result as gvdbx_splat_trilinear,
+ * bit for bit.  All calls on one GPU share the array: calls that overlap on one GPU (any
+ * context or stream) race. */
+int gvdbx_splat_trilinear_noalloc(gvdbx_ctx* ctx, int chan, const void* q, const void* mass, int64_t n, int accum);
 /* Backward of the splat given G = dL/drho in chan_grad's interior: dmass_i = sum_g w_g(q_i) G(g),
  * dq_i = m_i sum_g grad w_g(q_i) G(g).  dq: device float3[n] or NULL; dmass: float[n] or NULL. */
 int gvdbx_splat_backward(gvdbx_ctx* ctx, int chan_grad, const void* q, const void* mass, int64_t n, void* dq,
