@@ -16,6 +16,20 @@
  *   - positions outside the root are "missing" (N.cuh:275 checks '>' only and never
  *     checks negatives, so such lookups read outside the child list);
  *   - extra/incremental box nodes outside the root are not emitted.
+ *
+ * Pinning: every function below is checked against an independent restatement in tests/
+ * (set enumeration, closed forms, brute force, hand-computed words, finite differences):
+ *   orc_rebuild_topology (full, extra, incremental, state machine, depth errors) and
+ *     orc_update_atlas .............................. tests/test_oracle_topology.py
+ *   orc_accumulate_topology ............................ tests/test_oracle_pins.py
+ *   orc_insert_subcell, orc_gather_density / _levelset (plain), orc_gather_velocity (plain),
+ *     orc_update_apron, orc_sample_trilinear, orc_splat_trilinear, orc_splat_backward,
+ *     orc_sample_backward, orc_scatter_density, orc_gather_colour, orc_scatter_colour,
+ *     orc_quantize_list_pos, orc_gather_levelset_q16 ... tests/test_oracle_fields.py
+ *   the accumulating forms of orc_gather_density / _levelset / _velocity,
+ *     orc_quantize_list_vel, orc_scatter_reduce_levelset .. tests/test_oracle_pins.py
+ *   orc_cpu_activate_space ............................. tests/test_oracle_topology.py
+ * No function is left unpinned.
  */
 #include "gvdb_oracle.h"
 
