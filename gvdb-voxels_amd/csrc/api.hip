@@ -99,6 +99,9 @@ struct gvdbx_ctx {
   float posmin[3] = {0, 0, 0}, posmax[3] = {0, 0, 0};
 
   int axis_cx = 16, axis_cy = 16, axis_cz = 1;
+  // atlas bricks along z of the reference's 3D atlas (its AddChannel count, grown by UpdateAtlas
+  // to hold the used leaves, A.cpp:622-650); only the VBX layout needs it
+  int atlas_cz = 0;
   int apron = -1;
   bool chan_on[32] = {};
   int chan_type[32] = {};  // GVDBX_T_FLOAT or GVDBX_T_UCHAR4 (32-bit words either way)
@@ -1062,6 +1065,7 @@ int gvdbx_add_channel(gvdbx_ctx* c, int chan, int dtype, int apron) {  // AddCha
     return fail(c, GVDBX_ERR_ARG, "channel %d exists with another type", chan);
   if (c->apron >= 0 && apron != c->apron)
     return fail(c, GVDBX_ERR_ARG, "all channels share one apron width (%d)", c->apron);
+  if (c->apron < 0) c->atlas_cz = c->axis_cz;
   c->apron = apron;
   if (!c->chan_on[chan]) {
     c->chan_on[chan] = true;
@@ -1305,6 +1309,11 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
                P<int>(c->leaf_slot), P<int>(c->slot_leaf), P<int4>(c->slot_pos), c->stream,
                use_tab ? &tab : nullptr);
   c->nslots = c->used0;
+  {
+    const int64_t layer = (int64_t)c->axis_cx * c->axis_cy;
+    const int64_t need = (c->used0 + layer - 1) / layer;
+    if (need > c->atlas_cz) c->atlas_cz = (int)need;
+  }
   for (int ch = 0; ch < 32; ++ch) c->nbr_apron_ok[ch] = c->bound_ok[ch] = false;
   if (use_tab) {
     for (int a = 0; a < 3; ++a) {
@@ -2086,6 +2095,137 @@ int gvdbx_copy_subcell_array(gvdbx_ctx* c, int which, void* out, int64_t max_byt
   if (bytes > max_bytes) bytes = max_bytes;
   if (!src || bytes <= 0) return GVDBX_OK;
   return copy_out(c, out, src, (size_t)bytes);
+}
+
+// SaveVBX (V.cpp:1626-1768): pools and channels copied to the host and written by the codec
+// (vbx.cpp).  The channels must be laid out for the current topology.
+int gvdbx_save_vbx(gvdbx_ctx* c, const char* path) {
+  if (!c || !c->configured || !path) return fail(c, GVDBX_ERR_ARG, "context not configured or no path");
+  int nchan = 0;
+  while (nchan < 32 && c->chan_on[nchan]) ++nchan;
+  for (int k = nchan; k < 32; ++k)
+    if (c->chan_on[k]) return fail(c, GVDBX_ERR_ARG, "VBX files hold channels 0 .. n-1; channel %d follows a gap", k);
+  if (nchan > 0) CHK(atlas_current(c));
+  gvdbx_vbx_desc d;
+  memset(&d, 0, sizeof(d));
+  d.levels = c->levs;
+  for (int l = 0; l < c->levs; ++l) d.log2dim[l] = c->ld[l];
+  d.apron = c->apron < 0 ? 0 : c->apron;
+  d.axis_count[0] = c->axis_cx;
+  d.axis_count[1] = c->axis_cy;
+  d.axis_count[2] = c->atlas_cz > 0 ? c->atlas_cz : c->axis_cz;
+  d.nchan = nchan;
+  for (int k = 0; k < nchan; ++k) d.chan_type[k] = c->chan_type[k];
+  d.transform[6] = d.transform[7] = d.transform[8] = 1.0f;  // identity grid transform (V.cpp:77)
+  d.root = c->depth >= 0 && c->cnt[c->depth] > 0 ? (uint64_t)c->depth << 8 : kUndefL;  // Elem(0, depth, 0)
+  d.slots = nchan > 0 ? c->nslots : 0;
+  std::vector<std::vector<gvdbx_node>> nodes(c->levs);
+  std::vector<std::vector<uint64_t>> lists(c->levs);
+  std::vector<std::vector<uint32_t>> chans(nchan);
+  for (int l = 0; l < c->levs; ++l) {
+    d.node_count[l] = c->cnt[l];
+    d.clist_count[l] = l > 0 ? c->cnt[l] : 0;
+    nodes[l].resize((size_t)c->cnt[l]);
+    d.nodes[l] = nodes[l].data();
+    CHK(copy_out(c, nodes[l].data(), c->nodes[l].p, nodes[l].size() * sizeof(gvdbx_node)));
+    if (l > 0) {
+      lists[l].resize((size_t)(c->cnt[l] * clist_per(c, l)));
+      d.clists[l] = lists[l].data();
+      CHK(copy_out(c, lists[l].data(), c->clist[l].p, lists[l].size() * 8));
+    }
+  }
+  for (int k = 0; k < nchan; ++k) {
+    chans[k].resize((size_t)(d.slots * brick_floats(c)));
+    d.chan_data[k] = chans[k].data();
+    CHK(copy_out(c, chans[k].data(), c->chan[k].p, chans[k].size() * 4));
+  }
+  const int st = gvdbx_vbx_write(path, &d);
+  if (st != GVDBX_OK) return fail(c, st, "SaveVBX: cannot write %s", path);
+  return GVDBX_OK;
+}
+
+// LoadVBX (V.cpp:507-685): the file's first grid replaces the tree and the channels.
+int gvdbx_load_vbx(gvdbx_ctx* c, const char* path) {
+  if (!c || !path) return fail(c, GVDBX_ERR_ARG, "no context or path");
+  gvdbx_vbx_desc d;
+  int st = gvdbx_vbx_probe(path, &d);
+  if (st != GVDBX_OK) return fail(c, st, "LoadVBX: %s is not a VBX file this library reads", path);
+  if (d.levels < 2) return fail(c, GVDBX_ERR_ARG, "LoadVBX: %d levels (at least 2 needed)", d.levels);
+  const int depth = d.root == kUndefL ? -1 : (int)((d.root >> 8) & 0xFF);
+  if (depth >= d.levels || (depth >= 0 && d.node_count[depth] < 1))
+    return fail(c, GVDBX_ERR_ARG, "LoadVBX: root id %llx does not name a stored node", (unsigned long long)d.root);
+  for (int l = 1; l < d.levels; ++l)
+    if (d.clist_count[l] != d.node_count[l])
+      return fail(c, GVDBX_ERR_ARG, "LoadVBX: level %d holds %lld child lists for %lld nodes", l,
+                  (long long)d.clist_count[l], (long long)d.node_count[l]);
+  const int64_t br = ((int64_t)1 << d.log2dim[0]) + 2 * d.apron;
+  std::vector<std::vector<gvdbx_node>> nodes(d.levels);
+  std::vector<std::vector<uint64_t>> lists(d.levels);
+  std::vector<std::vector<uint32_t>> chans(d.nchan);
+  for (int l = 0; l < d.levels; ++l) {
+    nodes[l].resize((size_t)d.node_count[l]);
+    d.nodes[l] = nodes[l].data();
+    if (l > 0) {
+      lists[l].resize((size_t)(d.clist_count[l] << (3 * d.log2dim[l])));
+      d.clists[l] = lists[l].data();
+    }
+  }
+  for (int k = 0; k < d.nchan; ++k) {
+    chans[k].resize((size_t)(d.slots * br * br * br));
+    d.chan_data[k] = chans[k].data();
+  }
+  st = gvdbx_vbx_read(path, &d);
+  if (st != GVDBX_OK) return fail(c, st, "LoadVBX: cannot read %s", path);
+  // child list i must belong to node i (the layout every descent here assumes)
+  for (int l = 1; l < d.levels; ++l)
+    for (int64_t i = 0; i < d.node_count[l]; ++i)
+      if ((nodes[l][i].childlist >> 16) != (uint64_t)i || ((nodes[l][i].childlist >> 8) & 0xFF) != (uint64_t)l)
+        return fail(c, GVDBX_ERR_ARG, "LoadVBX: node %lld of level %d does not own child list %lld", (long long)i, l,
+                    (long long)i);
+  // DestroyChannels + Configure (Clear): the context starts over with the file's geometry
+  for (int k = 0; k < 32; ++k) c->chan_on[k] = false;
+  c->apron = -1;
+  c->chan_slot_cap = 0;
+  CHK(gvdbx_configure(c, d.levels, d.log2dim));
+  c->axis_cx = d.axis_count[0];
+  c->axis_cy = d.axis_count[1];
+  c->axis_cz = 1;
+  for (int k = 0; k < d.nchan; ++k) CHK(gvdbx_add_channel(c, k, d.chan_type[k], d.apron));
+  c->atlas_cz = d.axis_count[2];
+  for (int l = 0; l < d.levels; ++l) {
+    CHK(reserve_level(c, l, d.node_count[l]));
+    HIPCHK(c, hipMemcpyAsync(c->nodes[l].p, nodes[l].data(), nodes[l].size() * sizeof(gvdbx_node),
+                             hipMemcpyHostToDevice, c->stream));
+    if (l > 0)
+      HIPCHK(c, hipMemcpyAsync(c->clist[l].p, lists[l].data(), lists[l].size() * 8, hipMemcpyHostToDevice, c->stream));
+    c->cnt[l] = d.node_count[l];
+  }
+  c->depth = depth;
+  c->used0 = d.slots;
+  c->topo_version++;
+  const int res0 = 1 << d.log2dim[0];
+  bool any = false;
+  for (int a = 0; a < 3; ++a) c->posmin[a] = c->posmax[a] = 0.f;
+  if (depth >= 0)
+    for (int a = 0; a < 3; ++a) c->root[a] = nodes[depth][0].pos[a];
+  for (const gvdbx_node& n : nodes[0]) {  // FinishTopology's bounds of the flagged leaves
+    if (!n.flags) continue;
+    for (int a = 0; a < 3; ++a) {
+      const float lo = (float)n.pos[a], hi = (float)(n.pos[a] + res0);
+      c->posmin[a] = any ? std::fmin(c->posmin[a], lo) : lo;
+      c->posmax[a] = any ? std::fmax(c->posmax[a], hi) : hi;
+    }
+    any = true;
+  }
+  if (d.nchan > 0) {
+    CHK(gvdbx_update_atlas(c));
+    if (c->nslots != d.slots) return fail(c, GVDBX_ERR_STATE, "LoadVBX: %lld slots for %lld stored bricks",
+                                          (long long)c->nslots, (long long)d.slots);
+    for (int k = 0; k < d.nchan; ++k)
+      HIPCHK(c, hipMemcpyAsync(c->chan[k].p, chans[k].data(), chans[k].size() * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // the host copies above are released on return
+  return launch_status(c, "LoadVBX");
 }
 
 int gvdbx_dev_alloc(gvdbx_ctx* c, int64_t bytes, void** p) {

@@ -20,6 +20,21 @@ STATUS = {
     8: "GVDBX_ERR_DEGENERATE",
 }
 
+VBX_MAXLEV = 10
+VBX_MAXCHAN = 32
+
+
+class VbxDesc(C.Structure):
+    """gvdbx_vbx_desc (include/gvdbx.h): one grid of a VBX file on host memory."""
+    _fields_ = [
+        ("levels", C.c_int32), ("log2dim", C.c_int32 * VBX_MAXLEV), ("apron", C.c_int32),
+        ("axis_count", C.c_int32 * 3), ("nchan", C.c_int32), ("chan_type", C.c_int32 * VBX_MAXCHAN),
+        ("transform", C.c_float * 12), ("root", C.c_uint64), ("node_count", C.c_int64 * VBX_MAXLEV),
+        ("clist_count", C.c_int64 * VBX_MAXLEV), ("slots", C.c_int64), ("nodes", C.c_void_p * VBX_MAXLEV),
+        ("clists", C.c_void_p * VBX_MAXLEV), ("chan_data", C.c_void_p * VBX_MAXCHAN),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _I64 = C.c_int64
@@ -74,6 +89,11 @@ SIGNATURES = {
     "gvdbx_get_channel_ptr": [_P, _I, C.POINTER(_P), C.POINTER(_I64)],
     "gvdbx_get_subcell_info": [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I)],
     "gvdbx_copy_subcell_array": [_P, _I, _P, _I64],
+    "gvdbx_save_vbx": [_P, C.c_char_p],
+    "gvdbx_load_vbx": [_P, C.c_char_p],
+    "gvdbx_vbx_write": [C.c_char_p, C.POINTER(VbxDesc)],
+    "gvdbx_vbx_probe": [C.c_char_p, C.POINTER(VbxDesc)],
+    "gvdbx_vbx_read": [C.c_char_p, C.POINTER(VbxDesc)],
     "gvdbx_dev_alloc": [_P, _I64, C.POINTER(_P)],
     "gvdbx_dev_free": [_P, _P],
     "gvdbx_commit": [_P, _P, _P, _I64],

@@ -249,6 +249,21 @@ class VolumeGVDB:
         self._chk(self.L.gvdbx_sample_trilinear_backward(self.h, chan, q_dev.ptr, dout_dev.ptr, n, self._ptr(dq_dev),
                                                          chan_grad, int(accum_grad)), "SampleTrilinearBackward")
 
+    def SaveVBX(self, path):
+        """Write the pools and channels as a VBX file (SaveVBX, V.cpp:1626-1768)."""
+        self._chk(self.L.gvdbx_save_vbx(self.h, str(path).encode()), "SaveVBX")
+
+    def LoadVBX(self, path):
+        """Replace the tree and channels by a VBX file's first grid (LoadVBX, V.cpp:507-685)."""
+        from .lib import VbxDesc
+        d = VbxDesc()
+        self._chk(self.L.gvdbx_vbx_probe(str(path).encode(), C.byref(d)), "LoadVBX")
+        self._chk(self.L.gvdbx_load_vbx(self.h, str(path).encode()), "LoadVBX")
+        self.log2dims = tuple(int(d.log2dim[l]) for l in range(d.levels))
+        self.apron = int(d.apron)
+        self._channels = list(range(d.nchan))
+        self._types = {c: int(d.chan_type[c]) for c in range(d.nchan)}
+
     def set_channel(self, chan, data):
         """Overwrite a channel (float[slots][br][br][br]) from host memory."""
         ptr, floats = C.c_void_p(), C.c_int64(0)

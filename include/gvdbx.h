@@ -231,6 +231,46 @@ int gvdbx_get_subcell_info(const gvdbx_ctx* ctx, int64_t* num_subcells, int64_t*
  *        7 list colour (uint32 = uchar4) 8 FP16 list pos (uint16 x3) 9 FP16 list vel (uint16 x3) */
 int gvdbx_copy_subcell_array(gvdbx_ctx* ctx, int which, void* host_out, int64_t max_bytes);
 
+/* ---- VBX files (GVDB_FILESPEC.txt): SaveVBX (V.cpp:1626-1768), LoadVBX (V.cpp:507-685).
+ *      gvdbx_save_vbx writes the context's pools and channels as the single-grid file SaveVBX
+ *      writes (version 1.11, GVDB topology, atlas layout: channel c's bricks embedded in one
+ *      (ax br) x (ay br) x (az br) volume, brick s at (s % ax, s / ax % ay, s / (ax ay)), the
+ *      atlas depth az = the bricks per ax * ay layer that UpdateAtlas has needed so far).
+ *      Channels 0 .. n-1 must all exist.  gvdbx_load_vbx replaces the context's tree and
+ *      channels by a file's first grid as LoadVBX does (configure from the file's levels, pools
+ *      as stored, one channel per stored channel with the file's apron and atlas layout,
+ *      UpdateAtlas); the next RebuildTopology is a full one. */
+int gvdbx_save_vbx(gvdbx_ctx* ctx, const char* path);
+int gvdbx_load_vbx(gvdbx_ctx* ctx, const char* path);
+
+/* The VBX codec on host memory (no GPU needed): the pools and the brick-linear channels of one
+ * grid.  slots = bricks carrying data (the flagged leaves; atlas bricks 0 .. slots-1), each
+ * channel slots * br^3 32-bit words (br = 2^log2dim[0] + 2 apron).  gvdbx_vbx_probe fills the
+ * counts, types and layout of a file (pointers left NULL); the caller allocates the arrays and
+ * gvdbx_vbx_read fills them.  Files with bitmask nodes, compression, brick layout or topology
+ * reuse are refused (GVDBX_ERR_ARG), as LoadVBX cannot read them either. */
+#define GVDBX_VBX_MAXLEV 10
+#define GVDBX_VBX_MAXCHAN 32
+typedef struct gvdbx_vbx_desc {
+  int32_t levels;
+  int32_t log2dim[GVDBX_VBX_MAXLEV];
+  int32_t apron;
+  int32_t axis_count[3];                 /* atlas bricks per axis */
+  int32_t nchan;
+  int32_t chan_type[GVDBX_VBX_MAXCHAN];  /* GVDBX_T_FLOAT or GVDBX_T_UCHAR4 */
+  float transform[12];                   /* pretranslation, XYZ angles, scale, translation */
+  uint64_t root;                         /* pool id of the root: Elem(0, depth, 0) */
+  int64_t node_count[GVDBX_VBX_MAXLEV];  /* pool 0 rows */
+  int64_t clist_count[GVDBX_VBX_MAXLEV]; /* pool 1 rows (0 at level 0) */
+  int64_t slots;
+  gvdbx_node* nodes[GVDBX_VBX_MAXLEV];
+  uint64_t* clists[GVDBX_VBX_MAXLEV];    /* clist_count[l] * 8^log2dim[l] entries */
+  uint32_t* chan_data[GVDBX_VBX_MAXCHAN];
+} gvdbx_vbx_desc;
+int gvdbx_vbx_write(const char* path, const gvdbx_vbx_desc* desc);
+int gvdbx_vbx_probe(const char* path, gvdbx_vbx_desc* desc);
+int gvdbx_vbx_read(const char* path, gvdbx_vbx_desc* desc);
+
 /* ---- device memory helpers: AllocData/CommitData/RetrieveData (V.cpp:4822-4860, A.cpp:263-298) */
 int gvdbx_dev_alloc(gvdbx_ctx* ctx, int64_t bytes, void** dev_ptr);
 int gvdbx_dev_free(gvdbx_ctx* ctx, void* dev_ptr);

@@ -140,3 +140,77 @@ def test_batch_rejects_cloud_beyond_its_cell():
     assert np.array_equal(pos[c0] + np.array([1 << CELL, 0, 0]), pos[c1])
     assert ch[c0].tobytes() == ch[c1].tobytes()
     v.close()
+
+
+def channel_rows(v, chan, slots, chunk=65536):
+    """Padded bricks of chan at the given slots (sorted), read in chunks of slots: the batch's
+    whole channel is too large to copy at once."""
+    ptr, _ = v.channel_ptr(chan)
+    s, br = v.num_slots()
+    bb = br ** 3 * 4
+    out = np.zeros((len(slots), br, br, br), np.float32)
+    buf = np.zeros(chunk * br ** 3, np.float32)
+    for start in range(0, s, chunk):
+        k = min(chunk, s - start)
+        sel = np.nonzero((slots >= start) & (slots < start + k))[0]
+        if not len(sel):
+            continue
+        v._chk(v.L.gvdbx_retrieve(v.h, buf.ctypes.data, ptr + start * bb, k * bb), "retrieve")
+        out[sel] = buf.reshape(chunk, br, br, br)[slots[sel] - start]
+    return out
+
+
+@pytest.mark.timeout(900)
+def test_batch_256_clouds_against_standalone_runs(oracle_mod):
+    """The batched workload at its size: the 256 blob clouds (log-uniform sizes in [4,096,
+    1,048,576], 8 Gaussian blobs each) in one batch with the autotuner's configuration (8^3
+    bricks, subcell 4), one step RebuildTopology + UpdateAtlas + InsertPointsSubcell +
+    GatherDensity + UpdateApron.  The smallest, the largest and four random clouds are run alone
+    and must match the batch bit for bit (bricks, subcell counts, padded channel bricks); the
+    largest standalone run is checked against the oracle."""
+    from gvdbx import autotune, synth
+    U.log("case batch of 256 clouds: generate")
+    cs = synth.blob_clouds()
+    counts = [len(c) for c in cs]
+    allp = np.concatenate(cs)
+    mc = autotune.max_valid_coord(cs)
+    vb = autotune.new_batch_volume(3)
+    d = vb.to_device(allp)
+    cell = autotune.bind(vb, 3, d, counts, 1.0, mc)
+    ln = autotune.run_batch(vb, len(allp), 1.0, 4)
+    bidx, bpos = live_leaves(vb)
+    bmap, bcnt = per_brick_counts(vb)
+    U.log(f"batch: {len(allp)} points, {len(bidx)} bricks, {ln} entries, cell 2^{cell}")
+    order = np.argsort(counts)
+    pick = [int(order[0]), int(order[-1])] + [int(k) for k in np.random.default_rng(256).choice(order[1:-1], 4,
+                                                                                             replace=False)]
+    for k in pick:
+        c = cs[k]
+        w = autotune.new_batch_volume(3)
+        dc = w.to_device(c)
+        w.SetPoints(dc, n=len(c))
+        autotune.run_batch(w, len(c), 1.0, 4)
+        sidx, spos = live_leaves(w)
+        org = vb.batch_origin(k)
+        inside = np.all((bpos >= org) & (bpos < org + (1 << cell)), axis=1)
+        assert np.array_equal(bpos[inside] - org, spos), f"cloud {k}: bricks differ"
+        bslots = np.nonzero(inside)[0]
+        assert channel_rows(vb, 0, bslots).tobytes() == w.get_channel(0).tobytes(), f"cloud {k}: channel differs"
+        smap, scnt = per_brick_counts(w)
+        a = bcnt[(bmap[bidx[bslots]][:, None] * 8 + np.arange(8)).ravel()]
+        b = scnt[(smap[sidx][:, None] * 8 + np.arange(8)).ravel()]
+        assert np.array_equal(a, b), f"cloud {k}: subcell counts differ"
+        U.log(f"  cloud {k}: {len(c)} points, {len(sidx)} bricks equal")
+        if k == pick[1]:  # the largest cloud against the oracle
+            t = U.new_tree(oracle_mod, LD)
+            t.rebuild_topology(c, 1.0)
+            t.update_atlas()
+            U.compare_topology(w, t)
+            assert t.insert_points_subcell(4, c, 1.0) == w.subcell_info()[1]
+            t.gather_density(1.0, 0)
+            t.update_apron(0, 0.0)
+            U.compare_channel(w, t, 0)
+            U.log(f"  cloud {k}: equals the oracle")
+            del t
+        w.close()
+    vb.close()
