@@ -23,15 +23,18 @@ static inline int nblocks(int64_t n, int bs) {
 constexpr float kVelScale = 16777216.0f;  // 2^24 fixed-point units
 __device__ __forceinline__ long long vel_fixed(float v) { return __float2ll_rn(v * kVelScale); }
 
-// Subcell corner range of a point (the exact FP32 operation order of P.cuh:238-243).
-__device__ __forceinline__ void subcell_range(const float q[3], float r, int sc, int lo[3], int hi[3]) {
+// C division of v by 2^lg (truncation toward zero), times 2^lg
+__device__ __forceinline__ int trunc_mult(int v, int lg) { return (v >= 0 ? v >> lg : -((-v) >> lg)) * (1 << lg); }
+
+// Subcell corner range of a point (the exact FP32 operation order of P.cuh:238-243); sc = 2^lgsc.
+__device__ __forceinline__ void subcell_range(const float q[3], float r, int lgsc, int lo[3], int hi[3]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float lq = q[a] - r;
     float hq = q[a] + 1.0f;
     hq = hq + r;
-    lo[a] = ((int)lq / sc) * sc;
-    hi[a] = ((int)hq / sc) * sc;
+    lo[a] = trunc_mult((int)lq, lgsc);
+    hi[a] = trunc_mult((int)hq, lgsc);
   }
 }
 
@@ -257,6 +260,7 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_
   const int64_t mi = item_index();
   if (mi >= nmap) return;
   const int m = (int)mi;
+  const int beg = A.start[m], end = A.start[m + 1];  // in flight with the brick's set-up loads
   const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W, D3 = D * D * D;
   if (!brick_setup(A, m, nb, B, D, W)) {  // no leaf: nothing reaches the cell walk through it
     if (threadIdx.x < 27) A.nbm[(int64_t)m * 27 + threadIdx.x] = -1;
@@ -280,7 +284,6 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_
   }
   __syncthreads();
   const SubcellCfg c = A.c;
-  const int beg = A.start[m], end = A.start[m + 1];
   for (int j = beg + threadIdx.x; j < end; j += blockDim.x) {
     const float4 p = A.spos[j];
     const float q[3] = {p.x, p.y, p.z};
@@ -292,7 +295,7 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_
       fv[2] = vel_fixed(v.z);
     }
     int lo[3], hi[3];
-    subcell_range(q, c.r, c.sc, lo, hi);
+    subcell_range(q, c.r, c.lgsc, lo, hi);
     if (use_bins) {
       const int hx = ((int)q[0] - B.bx) >> c.lgsc, hy = ((int)q[1] - B.by) >> c.lgsc, hz = ((int)q[2] - B.bz) >> c.lgsc;
       const int dlx = ((lo[0] - B.bx) >> c.lgsc) - hx, dhx = ((hi[0] - B.bx) >> c.lgsc) - hx;
@@ -413,6 +416,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
   const int64_t mi = item_index();
   if (mi >= nmap) return;
   const int m = (int)mi;
+  const int beg = A.start[m], end = A.start[m + 1];  // in flight with the brick's set-up loads
   const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W;
   if (!brick_setup(A, m, nb, B, D, W, true)) return;
   int* lbase = (int*)smem64;
@@ -424,7 +428,6 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
   }
   __syncthreads();
   const SubcellCfg c = A.c;
-  const int beg = A.start[m], end = A.start[m + 1];
   for (int j = beg + threadIdx.x; j < end; j += blockDim.x) {
     const float4 p = A.spos[j];
     const float q[3] = {p.x, p.y, p.z};
@@ -432,7 +435,7 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
     if (A.list_vel || A.list_vq16) v = A.svel[j];
     const uint32_t clr = A.list_clr ? load_u32(A.P.clr, __float_as_uint(p.w), A.P.clr_off, A.P.clr_stride) : 0u;
     int lo[3], hi[3];
-    subcell_range(q, c.r, c.sc, lo, hi);
+    subcell_range(q, c.r, c.lgsc, lo, hi);
     if (kTwo) {
       int slot[8];
       bool act[8];
@@ -653,7 +656,7 @@ __global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __rest
     }
     const uint32_t clr = (kInsert && A.list_clr) ? load_u32(A.P.clr, i, A.P.clr_off, A.P.clr_stride) : 0u;
     int lo[3], hi[3];
-    subcell_range(q, c.r, c.sc, lo, hi);
+    subcell_range(q, c.r, c.lgsc, lo, hi);
     for (int x = lo[0]; x <= hi[0]; x += c.sc)
       for (int y = lo[1]; y <= hi[1]; y += c.sc)
         for (int z = lo[2]; z <= hi[2]; z += c.sc) {

@@ -49,6 +49,18 @@ inline dim3 item_grid(int64_t items) {
 }
 __device__ __forceinline__ int64_t item_index() { return (int64_t)blockIdx.y * gridDim.x + blockIdx.x; }
 
+// XCD-aware items: workgroups are dispatched to the 8 XCDs round-robin, so workgroup w runs on
+// XCD w % 8.  Giving XCD k the contiguous items [k * chunk, (k + 1) * chunk) in order keeps items
+// that are neighbours in the item order (bricks in slot order) on one XCD, where they share its
+// L2.  The grid holds xcd_items(n) items; the caller skips those at or past n.
+constexpr int kXcds = 8;
+inline int64_t xcd_items(int64_t n) { return ((n + kXcds - 1) / kXcds) * kXcds; }
+__device__ __forceinline__ int64_t xcd_item(int64_t n) {
+  const int64_t w = item_index();
+  const int64_t chunk = (n + kXcds - 1) / kXcds;
+  return (w % kXcds) * chunk + w / kXcds;
+}
+
 __device__ __forceinline__ uint64_t elem0(int lev, uint64_t ndx) { return ((uint64_t)lev << 8) | (ndx << 16); }
 
 __device__ __forceinline__ bool in_root(const DevTree& T, int x, int y, int z) {

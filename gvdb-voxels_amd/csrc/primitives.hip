@@ -204,17 +204,30 @@ void exclusive_scan(const int* in, int* out, int64_t n, int* tile_buf, int* tota
 
 // ------------------------------------------------------------------ radix sort
 
+// Lanes of the wave whose digit (nbits <= 8 bits) equals this lane's (valid lanes only), from
+// one ballot per bit.
+__device__ __forceinline__ uint64_t match_digit(int d, bool valid, int nbits) {
+  uint64_t peers = __ballot(valid);
+  for (int b = 0; b < nbits; ++b) {
+    const bool bit = (d >> b) & 1;
+    const uint64_t bb = __ballot(bit);
+    peers &= bit ? bb : ~bb;
+  }
+  return peers;
+}
+
 template <typename K>
-__global__ void __launch_bounds__(256) k_radix_up(const K* __restrict__ keys, int64_t n, int shift,
+__global__ void __launch_bounds__(256) k_radix_up(const K* __restrict__ keys, int64_t n, int shift, int nbits,
                                                  int* __restrict__ counts, int ntiles) {
   __shared__ int hist[256];
   hist[threadIdx.x] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+  const K mask = ((K)1 << nbits) - 1;
 #pragma unroll 4
   for (int r = 0; r < kRadixTile / 256; ++r) {
     const int64_t i = base + r * 256 + threadIdx.x;
-    if (i < n) atomicAdd(&hist[(int)((keys[i] >> shift) & 255)], 1);
+    if (i < n) atomicAdd(&hist[(int)((keys[i] >> shift) & mask)], 1);
   }
   __syncthreads();
   counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
@@ -236,69 +249,86 @@ __global__ void __launch_bounds__(256) k_digit_scan(int* __restrict__ counts, in
   if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-// Scatter of one tile: keys are ranked stably by digit into LDS first, then written out in
-// local order, so consecutive threads store consecutive addresses within each digit's run instead
-// of scattering single keys.  A digit's start in the output is the exclusive scan of the digit
-// totals (formed here, one value per thread) plus its start among the tiles (k_digit_scan); its
-// count in this tile is the next tile's start minus this one (the total for the last tile).
+// Scatter of one tile.  Wave w ranks keys [w * 1024, (w + 1) * 1024) of the tile, 64 per round,
+// against its own digit counters in LDS (the lanes of a round that share a digit take
+// consecutive places, found with ballots), so the ranking needs no barrier; a per-digit scan over
+// the four waves then gives every key its stable place in the tile, the keys are placed in LDS
+// in digit order and written out in digit runs (consecutive threads store consecutive addresses).
+// A digit's start in the output is the exclusive scan of the digit totals (formed here, one
+// value per thread) plus its start among the tiles (k_digit_scan); its count in this tile is the
+// next tile's start minus this one (the total for the last tile).
 template <typename K, bool kHasVal>
 __global__ void __launch_bounds__(256) k_radix_down(const K* __restrict__ kin, K* __restrict__ kout,
                                                    const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
-                                                   int64_t n, int shift, const int* __restrict__ offsets,
+                                                   int64_t n, int shift, int nbits, const int* __restrict__ offsets,
                                                    int ntiles, const int* __restrict__ totals) {
+  constexpr int kPer = kRadixTile / 256;  // keys per lane
+  const K mask = ((K)1 << nbits) - 1;
   __shared__ K sk[kRadixTile];
   __shared__ uint32_t sv[kHasVal ? kRadixTile : 1];
-  __shared__ int base_of[256], lstart[256], run[256];
-  __shared__ int whist[4][256];
-  const int tid = threadIdx.x, wave = tid >> 6;
+  __shared__ int base_of[256], lstart[256];
+  __shared__ int wcnt[4][256];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int64_t tile = blockIdx.x;
   const int64_t tbase = tile * kRadixTile;
+  const int64_t wbase = tbase + (int64_t)wave * (kRadixTile / 4);
+  K k[kPer];
+  uint32_t v[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {  // all loads in flight before the first use
+    const int64_t i = wbase + r * 64 + lane;
+    k[r] = i < n ? kin[i] : (K)0;
+    if (kHasVal) v[r] = i < n ? vin[i] : 0u;
+  }
   const int within = offsets[(int64_t)tid * ntiles + tile];
   const int dtot = totals[tid];
   const int next = tile + 1 < ntiles ? offsets[(int64_t)tid * ntiles + tile + 1] : dtot;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) wcnt[w][tid] = 0;
   int total;
   const int dbase = block_exclusive_scan_256(dtot, &total);
   const int ls = block_exclusive_scan_256(next - within, &total);
   base_of[tid] = dbase + within;
   lstart[tid] = ls;
-  run[tid] = 0;
   const uint64_t lt = lanemask_lt();
-  for (int r = 0; r < kRadixTile / 256; ++r) {
-    const int64_t i = tbase + r * 256 + tid;
-    const bool valid = i < n;
-    const K k = valid ? kin[i] : (K)0;
-    uint32_t v = 0;
-    if (kHasVal && valid) v = vin[i];
-    const int d = (int)((k >> shift) & 255);
+  int place[kPer];
 #pragma unroll
-    for (int w = 0; w < 4; ++w) whist[w][tid] = 0;
-    __syncthreads();
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1;
-      const uint64_t bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
-    }
+  for (int r = 0; r < kPer; ++r) {
+    const bool valid = wbase + r * 64 + lane < n;
+    const int d = (int)((k[r] >> shift) & mask);
+    const uint64_t peers = match_digit(d, valid, nbits);
     const int rank = __popcll(peers & lt);
-    if (valid && rank == 0) whist[wave][d] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      int pos = lstart[d] + run[d] + rank;
-      for (int w = 0; w < wave; ++w) pos += whist[w][d];
-      sk[pos] = k;
-      if (kHasVal) sv[pos] = v;
+    const int c = valid ? wcnt[wave][d] : 0;
+    place[r] = c + rank;
+    // this wave's counters only: LDS accesses of one wave complete in program order
+    if (valid && rank == 0) wcnt[wave][d] = c + __popcll(peers);
+  }
+  __syncthreads();
+  {  // per digit: the waves' counts -> exclusive prefix over the waves, in place
+    int acc = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int c = wcnt[w][tid];
+      wcnt[w][tid] = acc;
+      acc += c;
     }
-    __syncthreads();
-    run[tid] += whist[0][tid] + whist[1][tid] + whist[2][tid] + whist[3][tid];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    if (wbase + r * 64 + lane >= n) continue;
+    const int d = (int)((k[r] >> shift) & mask);
+    const int pos = lstart[d] + wcnt[wave][d] + place[r];
+    sk[pos] = k[r];
+    if (kHasVal) sv[pos] = v[r];
   }
   __syncthreads();
   const int tn = (int)(n - tbase < kRadixTile ? n - tbase : kRadixTile);
   for (int j = tid; j < tn; j += 256) {
-    const K k = sk[j];
-    const int d = (int)((k >> shift) & 255);
+    const K key = sk[j];
+    const int d = (int)((key >> shift) & mask);
     const int64_t o = (int64_t)base_of[d] + (j - lstart[d]);
-    kout[o] = k;
+    kout[o] = key;
     if (kHasVal) vout[o] = sv[j];
   }
 }
@@ -318,15 +348,21 @@ static int radix_sort_impl(K* keys, K* keys_alt, uint32_t* vals, uint32_t* vals_
   uint32_t* vin = vals;
   uint32_t* vout = vals_alt;
   int* totals = counts + ncnt;
-  for (int shift = begin_bit; shift < end_bit; shift += 8) {
-    hipLaunchKernelGGL(k_radix_up<K>, dim3(ntiles), dim3(256), 0, s, kin, n, shift, counts, ntiles);
+  // digits of at most 8 bits; the first pass takes the remainder, so that 15 bits sort in 2
+  // passes (7 + 8) and 18 bits in 3 (2 + 8 + 8)
+  const int bits = end_bit - begin_bit;
+  const int passes = (bits + 7) / 8;
+  for (int pass = 0, shift = begin_bit; pass < passes; ++pass) {
+    const int nbits = pass == 0 ? bits - 8 * (passes - 1) : 8;
+    hipLaunchKernelGGL(k_radix_up<K>, dim3(ntiles), dim3(256), 0, s, kin, n, shift, nbits, counts, ntiles);
     hipLaunchKernelGGL(k_digit_scan, dim3(256), dim3(256), 0, s, counts, ntiles, totals);
     if (vals)
-      hipLaunchKernelGGL((k_radix_down<K, true>), dim3(ntiles), dim3(256), 0, s, kin, kout, vin, vout, n, shift, counts,
-                         ntiles, (const int*)totals);
+      hipLaunchKernelGGL((k_radix_down<K, true>), dim3(ntiles), dim3(256), 0, s, kin, kout, vin, vout, n, shift, nbits,
+                         counts, ntiles, (const int*)totals);
     else
       hipLaunchKernelGGL((k_radix_down<K, false>), dim3(ntiles), dim3(256), 0, s, kin, kout, nullptr, nullptr, n, shift,
-                         counts, ntiles, (const int*)totals);
+                         nbits, counts, ntiles, (const int*)totals);
+    shift += nbits;
     K* t = kin; kin = kout; kout = t;
     uint32_t* tv = vin; vin = vout; vout = tv;
     parity ^= 1;
