@@ -2,15 +2,18 @@
 
 Workload (the SPH-style step): 4,194,304 particles on a jittered lattice filling the
 lower 40% of a 256^3 box, velocities ~N(0,1), dt = 0.5, seed 1.  One step is
-  RebuildTopology(r=1) + UpdateAtlas + InsertPointsSubcell(sc=4, r=1)
+  q += v*dt (on the device, gvdbx_advance_points; not on step 0)
+  + RebuildTopology(r=1) + UpdateAtlas + InsertPointsSubcell(sc=4, r=1)
   + GatherDensity(ch0) + velocity means (ch1-3) + UpdateApron(ch0..3)
-on the positions of that step (p_k = p_{k-1} + v*dt, precomputed and resident in HBM
-before timing).  The rebuild state machine runs as in the reference: steps 0 and 1 are
-full rebuilds (inside the warmup), later steps are incremental.
+with the points and velocities resident in HBM.  The rebuild state machine runs as in the
+reference: steps 0 and 1 are full rebuilds (inside the warmup), later steps are incremental.
+The headline times the steps after the warmup; `late_steps` times steps 40-59 of the same
+60-step trajectory (the steps in between run untimed).
 
 Prints one JSON line with the throughput in Mpoints/s, the HBM roofline of the dominant
 kernel (HIP events on the library stream around that kernel) and a CPU baseline (the
-oracle restatement on the host, one thread).
+oracle restatement on the host: topology on one thread, binning / gathers / apron on one
+thread and on all threads).
 """
 import argparse
 import json
@@ -35,17 +38,12 @@ RADIUS = 1.0
 SUBCELL = 4
 
 
-def step_positions(p0, vel, count):
-    out = [p0]
-    cur = p0
-    for _ in range(count - 1):
-        cur = (cur + vel * DT).astype(np.float32)
-        out.append(cur)
-    return out
+LATE_FIRST, LATE_LAST = 40, 59  # the late window of the 60-step trajectory
 
 
-def run_step(v, dpos, dvel, n):
-    v.SetPoints(dpos, dvel, n=n)
+def run_step(v, n, advance):
+    if advance:
+        v.AdvancePoints(n, DT)  # q += v dt on the device
     v.RebuildTopology(n, RADIUS)
     v.UpdateAtlas()
     ln = v.InsertPointsSubcell(SUBCELL, n, RADIUS)
@@ -81,30 +79,105 @@ def gather_bytes(s_len, nsc, bricks, vel=True, apron=True):
     return b
 
 
-def cpu_baseline(p, vel, sample):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def omp_threads():
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def median_s(fn, reps=5, warm=1):
+    """Median wall time of `reps` calls after `warm` untimed ones."""
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline(p0, vel, sample):
+    """The oracle's C restatement on the host cores, on the first `sample` points of steps 0-2
+    of the trajectory: RebuildTopology + UpdateAtlas on one thread (the reference's CPU topology
+    is serial), then binning + density + 3 velocity means + 4 aprons on one thread and on all
+    threads; medians of 5 after 1 warm-up.  `value` = the incremental step 2 with the binning,
+    gathers and apron on all threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    ps = p[:sample]
-    vs = vel[:sample]
-    t0 = time.perf_counter()
-    t = O.Tree((3, 3, 3, 3, 5))
-    for c in range(4):
-        t.add_channel(c)
-    t.rebuild_topology(ps, RADIUS)
-    t.update_atlas()
-    t.insert_points_subcell(SUBCELL, ps, RADIUS)
-    t.gather_density(RADIUS, 0)
-    t.gather_velocity(vs, (1, 2, 3))
-    for c in range(4):
-        t.update_apron(c, 0.0)
-    dt = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    O.cpu_activate_space(ps)
-    dt_act = time.perf_counter() - t1
-    return {"value": round(sample / dt / 1e6, 4), "unit": "Mpoints/s", "cores": 1, "kind": "port",
-            "sample": f"first {sample} points of step 0 (full rebuild + atlas + subcell insert + density + "
-                      f"3 velocity means + 4 aprons), oracle C restatement, 1 thread, {dt:.2f} s",
-            "reference_cpu_activate_space_Mpoints_s": round(sample / dt_act / 1e6, 3)}
+    ps0, vs = p0[:sample], vel[:sample]
+    ps1 = (ps0 + vs * DT).astype(np.float32)
+    ps2 = (ps1 + vs * DT).astype(np.float32)
+    threads = omp_threads()
+
+    def tree():
+        t = O.Tree((3, 3, 3, 3, 5))
+        for c in range(4):
+            t.add_channel(c)
+        return t
+
+    def full():
+        t = tree()
+        t.rebuild_topology(ps0, RADIUS)
+        t.update_atlas()
+    t_full = median_s(full)
+
+    inc = []
+    for k in range(6):  # steps 0 and 1 (full) untimed, step 2 (incremental) timed; the first is warm-up
+        t = tree()
+        t.rebuild_topology(ps0, RADIUS)
+        t.rebuild_topology(ps1, RADIUS)
+        t0 = time.perf_counter()
+        t.rebuild_topology(ps2, RADIUS)
+        t.update_atlas()
+        if k:
+            inc.append(time.perf_counter() - t0)
+    t_inc = float(np.median(inc))
+
+    def bga():
+        t.insert_points_subcell(SUBCELL, ps2, RADIUS)
+        t.gather_density(RADIUS, 0)
+        t.gather_velocity(vs, (1, 2, 3))
+        for c in range(4):
+            t.update_apron(c, 0.0)
+    O.set_threads(1)
+    t_bga1 = median_s(bga)
+    O.set_threads(threads)
+    try:
+        t_bgan = median_s(bga)
+    finally:
+        O.set_threads(1)
+    del t
+    t_act = median_s(lambda: O.cpu_activate_space(ps0))
+    mps = lambda sec: round(sample / sec / 1e6, 3)
+    return {"value": mps(t_inc + t_bgan), "unit": "Mpoints/s", "cores": threads, "kind": "port",
+            "sample": f"first {sample} points of steps 0-2 of the trajectory; value = incremental step 2: "
+                      f"RebuildTopology + UpdateAtlas on 1 thread, InsertPointsSubcell + density + 3 velocity "
+                      f"means + 4 aprons on {threads} threads (OpenMP); oracle C restatement; medians of 5 after "
+                      f"1 warm-up",
+            "cpu_model": cpu_model(), "hardware_concurrency": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)), "omp_threads": threads,
+            "ms": {"full_topology_step0_1_thread": round(t_full * 1e3, 2),
+                   "incremental_topology_step2_1_thread": round(t_inc * 1e3, 2),
+                   "bin_gather_apron_1_thread": round(t_bga1 * 1e3, 2),
+                   f"bin_gather_apron_{threads}_threads": round(t_bgan * 1e3, 2)},
+            "Mpoints_s": {"incremental_step_1_thread": mps(t_inc + t_bga1),
+                          f"incremental_step_{threads}_threads": mps(t_inc + t_bgan),
+                          "full_step0_1_thread": mps(t_full + t_bga1),
+                          f"full_step0_{threads}_threads": mps(t_full + t_bgan)},
+            "reference_cpu_activate_space_Mpoints_s": mps(t_act)}
 
 
 def adjoint_bench(reps):
@@ -328,7 +401,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--points", type=int, default=N_POINTS)
-    ap.add_argument("--cpu-sample", type=int, default=N_POINTS)
+    ap.add_argument("--cpu-sample", type=int, default=1_048_576)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-levelset", action="store_true")
@@ -357,18 +430,18 @@ def main():
     p0, vel = synth.sph_block(n=args.points, seed=1)
     n = len(p0)
     total = args.warmup + args.steps
-    poss = step_positions(p0, vel, total)
 
     v = VolumeGVDB(0)
     v.Configure(5, 3, 3, 3, 3)
     v.SetChannelDefault(16, 16, 1)
     for c in range(4):
         v.AddChannel(c, apron=1)
+    dpos = v.to_device(p0)
     dvel = v.to_device(vel)
-    dposs = [v.to_device(p) for p in poss]
+    v.SetPoints(dpos, dvel, n=n)
 
     for k in range(args.warmup):
-        run_step(v, dposs[k], dvel, n)
+        run_step(v, n, k > 0)
     v.synchronize()
 
     # timed region: K steps, kernel-family events recorded on the library stream
@@ -376,16 +449,33 @@ def main():
     t0 = time.perf_counter()
     lens = []
     for k in range(args.warmup, total):
-        lens.append(run_step(v, dposs[k], dvel, n))
+        lens.append(run_step(v, n, True))
     v.synchronize()
     t1 = time.perf_counter()
     k_ms, k_launches = v.kernel_timer_read()
     v.kernel_timer(None, False)
+    nsc, s_len, _ = v.subcell_info()
+    bricks = v.getNumUsedNodes(0)
+
+    late = None
+    if total <= LATE_FIRST:
+        for k in range(total, LATE_FIRST):
+            run_step(v, n, True)
+        v.synchronize()
+        t2 = time.perf_counter()
+        for k in range(LATE_FIRST, LATE_LAST + 1):
+            run_step(v, n, True)
+        v.synchronize()
+        t3 = time.perf_counter()
+        late_ms = (t3 - t2) * 1e3 / (LATE_LAST - LATE_FIRST + 1)
+        late = {"steps": [LATE_FIRST, LATE_LAST], "ms_per_step": round(late_ms, 4),
+                "Mpoints_s": round(n / (late_ms * 1e-3) / 1e6, 2),
+                "bricks_last_step": int(v.getNumUsedNodes(0)),
+                "subcell_entries_last_step": int(v.subcell_info()[1])}
+        progress(f"late steps {LATE_FIRST}-{LATE_LAST}: {late_ms:.4f} ms")
 
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3) / 1e6
-    nsc, s_len, _ = v.subcell_info()
-    bricks = v.getNumUsedNodes(0)
     step_bytes = algorithmic_bytes(n, s_len, nsc, bricks)
     kb = gather_bytes(s_len, nsc, bricks) if args.kernel == "gather" else None
     kb_interior = gather_bytes(s_len, nsc, bricks, apron=False) if args.kernel == "gather" else None
@@ -404,7 +494,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded jittered lattice, N(0,1) velocities)",
+        "data": "synthetic (seeded jittered lattice, N(0,1) velocities; displaced on the device every step)",
         "config": {"workload": "sph_step_4M", "points": n, "box": 256, "fill": 0.4, "dt": 0.5,
                    "radius": RADIUS, "subcell": SUBCELL, "log2dims": [3, 3, 3, 3, 5], "channels": 4,
                    "apron": 1, "rebuild": "reference state machine (incremental after 2 full)",
@@ -419,7 +509,8 @@ def main():
                      "algorithmic_bytes_per_launch": int(kb) if kb else None,
                      "algorithmic_bytes_interiors_only": int(kb_interior) if kb_interior else None},
     }
-    del dposs
+    out["late_steps"] = late
+    del dpos, dvel
     v.close()
     progress(f"sph step done: {ms_per_step:.4f} ms")
     if not args.no_adjoint:
@@ -432,7 +523,7 @@ def main():
         out["batched"] = batched_bench(2)
         progress("batched done")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(poss[0], vel, min(args.cpu_sample, n))
+        out["cpu_baseline"] = cpu_baseline(p0, vel, min(args.cpu_sample, n))
         progress("cpu baseline done")
     print(json.dumps(out))
 

@@ -1131,6 +1131,16 @@ int gvdbx_set_points(gvdbx_ctx* c, int64_t n, const void* pos, int pos_off, int 
   return GVDBX_OK;
 }
 
+int gvdbx_advance_points(gvdbx_ctx* c, int64_t n, float dt) {
+  CHK(check_points(c, n));
+  if (c->batch) return fail(c, GVDBX_ERR_STATE, "batched clouds are moved by the caller and bound again");
+  if (!c->vel) return fail(c, GVDBX_ERR_STATE, "no velocities bound (gvdbx_set_points vel)");
+  if (!std::isfinite(dt)) return fail(c, GVDBX_ERR_ARG, "time step is not finite");
+  advance_points((char*)c->pos, c->pos_off, c->pos_stride, c->vel, c->vel_off, c->vel_stride, n, dt, c->stream);
+  HIPCHK(c, hipGetLastError());
+  return GVDBX_OK;
+}
+
 int gvdbx_batch_set_points(gvdbx_ctx* c, int ncloud, const int64_t* counts, const void* pos, int pos_off,
                            int pos_stride, const void* vel, int vel_off, int vel_stride, int cell_log2,
                            float max_radius) {

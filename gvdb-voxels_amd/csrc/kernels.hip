@@ -829,4 +829,28 @@ template void setup_nodes<uint32_t>(const DevTree&, const uint32_t*, const int*,
 template void setup_nodes<uint64_t>(const DevTree&, const uint64_t*, const int*, const int*, int64_t, KeyFmt,
                                     const AllocPlan&, hipStream_t, bool);
 
+// ------------------------------------------------------------------ displacement
+
+// q += v * dt in place: an FP32 product, then an FP32 sum (two roundings, no fused multiply-add),
+// so the positions equal those of the same arithmetic in FP32 on the host.
+__global__ void __launch_bounds__(256) k_advance(char* __restrict__ pos, int pos_off, int pos_stride,
+                                                const char* __restrict__ vel, int vel_off, int vel_stride,
+                                                int64_t n, float dt) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float* q = (float*)(pos + i * pos_stride + pos_off);
+    const float* v = (const float*)(vel + i * vel_stride + vel_off);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) q[a] = __fadd_rn(q[a], __fmul_rn(v[a], dt));
+  }
+}
+
+void advance_points(char* pos, int pos_off, int pos_stride, const char* vel, int vel_off, int vel_stride, int64_t n,
+                    float dt, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_advance, dim3((unsigned)g), dim3(256), 0, s, pos, pos_off, pos_stride, vel, vel_off,
+                     vel_stride, n, dt);
+}
+
 }  // namespace gvdbx

@@ -121,4 +121,8 @@ void levelset_field(SubcellCfg c, int64_t nsc, const int* sc_pos, const int* sc_
 void sample_trilinear(const DevTree& T, const int* leaf_slot, int res0, int apron, const float* chan,
                       const float* q, int64_t n, float* out, hipStream_t s);
 
+// q += v * dt in place over n points of the given byte views (FP32 product, then FP32 sum)
+void advance_points(char* pos, int pos_off, int pos_stride, const char* vel, int vel_off, int vel_stride, int64_t n,
+                    float dt, hipStream_t s);
+
 }  // namespace gvdbx

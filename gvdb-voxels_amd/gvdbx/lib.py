@@ -55,6 +55,7 @@ SIGNATURES = {
     "gvdbx_fill_channel": [_P, _I, _F],
     "gvdbx_fill_channel4": [_P, _I, _P],
     "gvdbx_set_points": [_P, _I64, _P, _I, _I, _P, _I, _I, _P, _I, _I],
+    "gvdbx_advance_points": [_P, _I64, _F],
     "gvdbx_batch_set_points": [_P, _I, _P, _P, _I, _I, _P, _I, _I, _I, _F],
     "gvdbx_batch_get_layout": [_P, C.POINTER(_I), C.POINTER(_I), _P],
     "gvdbx_rebuild_topology": [_P, _I64, _F, _PF],

@@ -99,6 +99,11 @@ class VolumeGVDB:
         self._chk(self.L.gvdbx_set_points(self.h, n, ptr(pos), pos_off, pos_stride, ptr(vel), vel_off, vel_stride,
                                           ptr(clr), clr_off, clr_stride), "set_points")
 
+    def AdvancePoints(self, n, dt):
+        """q += v * dt over the first n bound points, in place on the device (FP32 product, then
+        FP32 sum, as numpy's float32 arithmetic)."""
+        self._chk(self.L.gvdbx_advance_points(self.h, n, float(dt)), "advance_points")
+
     # ------------------------------------------------------------ batched clouds
     def BatchSetPoints(self, pos, counts, vel=None, cell_log2=12, max_radius=1.0, pos_off=0, pos_stride=12,
                        vel_off=0, vel_stride=12):

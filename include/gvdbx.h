@@ -89,6 +89,13 @@ int gvdbx_set_points(gvdbx_ctx* ctx, int64_t n,
                      const void* vel, int vel_off, int vel_stride,
                      const void* clr, int clr_off, int clr_stride);
 
+/* q += v * dt for the first n bound points, in place in the bound position buffer, on the
+ * context's stream: an FP32 product, then an FP32 sum (no fused multiply-add), so the result
+ * equals the same FP32 arithmetic on the host.  No reference counterpart: it keeps the SPH
+ * step's displacement (the gFluidSurface frame moves its particles before RebuildTopology,
+ * main_fluid_surface.cpp:320-356) on the device.  Needs bound velocities; not for batched clouds. */
+int gvdbx_advance_points(gvdbx_ctx* ctx, int64_t n, float dt);
+
 /* ---- batched clouds (no reference counterpart): ncloud independent clouds, each in its own
  *      frame, run through one RebuildTopology / UpdateAtlas / InsertPointsSubcell / Gather /
  *      UpdateApron sequence.  pos holds the clouds back to back (counts[k] points each, host

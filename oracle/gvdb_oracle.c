@@ -29,6 +29,8 @@
  *   the accumulating forms of orc_gather_density / _levelset / _velocity,
  *     orc_quantize_list_vel, orc_scatter_reduce_levelset .. tests/test_oracle_pins.py
  *   orc_cpu_activate_space ............................. tests/test_oracle_topology.py
+ *   the threaded forms (orc_set_threads > 1) against the serial ones, list for list and
+ *     voxel for voxel ................................... tests/test_oracle_threads.py
  * No function is left unpinned.
  */
 #include "gvdb_oracle.h"
@@ -36,6 +38,23 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the binning, gather and apron loops (orc_set_threads; 1 = the serial loops).  The
+ * threaded forms give the same results as the serial ones: the insert places each thread's
+ * contiguous chunk of points after the chunks before it, so every list keeps point order, and
+ * every other threaded loop writes disjoint voxels. */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 struct orc_tree {
   int levs;
@@ -498,10 +517,10 @@ const int64_t* orc_leaf_slot(const orc_tree* t) { return t->leaf_slot; }
  * corners from trunc(q-r)/sc*sc to trunc(q+1+r)/sc*sc.  Calls fn for each hit. */
 typedef void (*sc_fn)(orc_tree* t, int64_t idx, int64_t pt, const float* q, void* u);
 
-static void enum_subcells(orc_tree* t, int sc, const float* pos, int64_t n, int64_t st, float r, const float* tr,
-                          sc_fn fn, void* u) {
+static void enum_subcells(orc_tree* t, int sc, const float* pos, int64_t i0, int64_t i1, int64_t st, float r,
+                          const float* tr, sc_fn fn, void* u) {
   int D = t->scdim;
-  for (int64_t i = 0; i < n; i++) {
+  for (int64_t i = i0; i < i1; i++) {
     const float* p = pos + i * st;
     if (p[2] == ORC_NOHIT) continue;
     float q[3];
@@ -531,8 +550,8 @@ static void enum_subcells(orc_tree* t, int sc, const float* pos, int64_t n, int6
 }
 
 static void count_fn(orc_tree* t, int64_t idx, int64_t pt, const float* q, void* u) {
-  (void)pt; (void)q; (void)u;
-  t->sc_cnt[idx]++;
+  (void)pt; (void)q;
+  ((int32_t*)u)[idx]++;
 }
 
 typedef struct { int32_t* cursor; } ins_ctx;
@@ -570,20 +589,65 @@ int orc_insert_subcell(orc_tree* t, int sc, const float* pos, int64_t n, int64_t
   t->sc_off = (int32_t*)calloc((size_t)nsc + 1, sizeof(int32_t));
   t->sc_pos = (int32_t*)calloc((size_t)nsc * 3 + 3, sizeof(int32_t));
   t->sc_nid = (int32_t*)calloc((size_t)nsc + 1, sizeof(int32_t));
-  enum_subcells(t, sc, pos, n, st, r, tr, count_fn, NULL);
-  int64_t o = 0;
-  for (int64_t k = 0; k < nsc; k++) { t->sc_off[k] = (int32_t)o; o += t->sc_cnt[k]; }
-  t->sclen = o;
-  *len_out = o;
-  free(t->sc_list); free(t->sc_list_pos);
-  t->sc_list = (int64_t*)malloc((size_t)(o + 1) * sizeof(int64_t));
-  t->sc_list_pos = (float*)malloc((size_t)(o + 1) * 3 * sizeof(float));
-  ins_ctx c;
-  c.cursor = (int32_t*)calloc((size_t)nsc + 1, sizeof(int32_t));
-  enum_subcells(t, sc, pos, n, st, r, tr, insert_fn, &c);
-  free(c.cursor);
+  if (g_threads <= 1) {
+    enum_subcells(t, sc, pos, 0, n, st, r, tr, count_fn, t->sc_cnt);
+    int64_t o = 0;
+    for (int64_t k = 0; k < nsc; k++) { t->sc_off[k] = (int32_t)o; o += t->sc_cnt[k]; }
+    t->sclen = o;
+    *len_out = o;
+    free(t->sc_list); free(t->sc_list_pos);
+    t->sc_list = (int64_t*)malloc((size_t)(o + 1) * sizeof(int64_t));
+    t->sc_list_pos = (float*)malloc((size_t)(o + 1) * 3 * sizeof(float));
+    ins_ctx c;
+    c.cursor = (int32_t*)calloc((size_t)nsc + 1, sizeof(int32_t));
+    enum_subcells(t, sc, pos, 0, n, st, r, tr, insert_fn, &c);
+    free(c.cursor);
+  } else {
+    /* thread k counts, then inserts, points [n k / T, n (k + 1) / T); its entries of subcell s
+     * start after those of threads 0..k-1, so the lists equal the serial ones */
+    const int T = g_threads;
+    const int64_t row = nsc + 1;
+    int32_t* pc = (int32_t*)calloc((size_t)T * (size_t)row, sizeof(int32_t));
+#pragma omp parallel num_threads(T)
+    {
+#ifdef _OPENMP
+      const int id = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+      const int id = 0, nt = 1;
+#endif
+      int32_t* mine = pc + (int64_t)id * row;
+      const int64_t i0 = n * id / nt, i1 = n * (id + 1) / nt;
+      enum_subcells(t, sc, pos, i0, i1, st, r, tr, count_fn, mine);
+#pragma omp barrier
+#pragma omp for schedule(static)
+      for (int64_t k = 0; k < nsc; k++) {
+        int32_t sum = 0;
+        for (int th = 0; th < nt; th++) {
+          const int32_t v = pc[(int64_t)th * row + k];
+          pc[(int64_t)th * row + k] = sum;
+          sum += v;
+        }
+        t->sc_cnt[k] = sum;
+      }
+#pragma omp single
+      {
+        int64_t o = 0;
+        for (int64_t k = 0; k < nsc; k++) { t->sc_off[k] = (int32_t)o; o += t->sc_cnt[k]; }
+        t->sclen = o;
+        *len_out = o;
+        free(t->sc_list); free(t->sc_list_pos);
+        t->sc_list = (int64_t*)malloc((size_t)(o + 1) * sizeof(int64_t));
+        t->sc_list_pos = (float*)malloc((size_t)(o + 1) * 3 * sizeof(float));
+      }
+      ins_ctx c;
+      c.cursor = mine;
+      enum_subcells(t, sc, pos, i0, i1, st, r, tr, insert_fn, &c);
+    }
+    free(pc);
+  }
   /* gvdbCalcSubcellPos (P.cuh:677-694) */
   int64_t per = (int64_t)D * D * D;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
   for (int64_t i = 0; i < total; i++) {
     const orc_node* nd = &t->nodes[0][i];
     if (nd->parent == ORC_UNDEF64) continue;
@@ -643,6 +707,7 @@ static int gather_common(orc_tree* t, float r, int chan, int accum, int mode) {
   if (!t->chan[chan] || !t->sc_cnt) return ORC_ERR_STATE;
   int sc = t->sc, a = t->apron;
   double* ch = t->chan[chan];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads) if (g_threads > 1)
   for (int64_t k = 0; k < t->nsc; k++) {
     int64_t leaf = t->sc_nid[k];
     const orc_node* nd = &t->nodes[0][leaf];
@@ -732,6 +797,7 @@ int orc_gather_velocity(orc_tree* t, const float* vel, int64_t st, int cx, int c
   for (int c = 0; c < 3; c++) if (!t->chan[chans[c]]) return ORC_ERR_STATE;
   if (!t->sc_cnt) return ORC_ERR_STATE;
   int sc = t->sc, a = t->apron;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads) if (g_threads > 1)
   for (int64_t k = 0; k < t->nsc; k++) {
     int64_t leaf = t->sc_nid[k];
     const orc_node* nd = &t->nodes[0][leaf];
@@ -812,6 +878,7 @@ int orc_update_apron(orc_tree* t, int chan, double boundval) {
   if (a == 0) return ORC_OK;
   int res0 = res_of(t, 0), br = res0 + 2 * a;
   double* ch = t->chan[chan];
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads) if (g_threads > 1)
   for (int64_t i = 0; i < t->cnt[0]; i++) {
     const orc_node* nd = &t->nodes[0][i];
     int64_t slot = t->leaf_slot[i];

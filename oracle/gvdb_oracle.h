@@ -79,6 +79,10 @@ const int64_t* orc_leaf_slot(const orc_tree* t);   /* per leaf: atlas slot or -1
 
 /* InsertPointsSubcell (V.cpp:4971-5092, P.cuh:109-118, 218-315, 677-694).
  * Lists are in ascending point-index order (the build's deterministic order). */
+/* Threads of the subcell binning, gathers and apron (1, the default, runs the serial loops); the
+ * threaded forms return the same lists and fields.  orc_max_threads: OpenMP's thread limit. */
+void orc_set_threads(int n);
+int orc_max_threads(void);
 int orc_insert_subcell(orc_tree* t, int sc, const float* pos, int64_t n, int64_t stride, float radius,
                        const float* trans, int64_t* len_out);
 int64_t orc_sc_num(const orc_tree* t);

@@ -85,8 +85,19 @@ def lib():
         L.orc_find_leaf.argtypes = [P, P]
         L.orc_cpu_activate_space.restype = i64
         L.orc_cpu_activate_space.argtypes = [C.c_int, P, P, i64, i64, P, P]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_max_threads.restype = C.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Threads of the binning, gather and apron loops (1 = the serial loops; same results)."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
 
 
 def _ptr(a):

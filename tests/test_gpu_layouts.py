@@ -72,6 +72,13 @@ def test_points_inside_64_byte_records(oracle_mod):
         assert np.array_equal(got, want), "colour words differ"
         U.log(f"  frame {frame}: {v.node_count(0)[1]} bricks, {ln} entries")
     assert np.array_equal(pos[:5], rec["hit"][:5])
+    # the displacement writes the positions inside the records and nothing else
+    v.AdvancePoints(n, 0.25)
+    back = d.download(np.uint8).view(RAY)
+    want = (rec["hit"] + rec["dir"] * np.float32(0.25)).astype(np.float32)
+    assert back["hit"].tobytes() == want.tobytes()
+    for f in ("normal", "orig", "dir", "clr", "pnode", "pndx", "pad"):
+        assert back[f].tobytes() == rec[f].tobytes(), f
     v.close()
 
 
@@ -109,4 +116,27 @@ def test_brick32_fluid_surface_frames(oracle_mod):
         t.update_apron(1, 0.0)
         U.compare_channel(v, t, 1)
         U.log(f"  frame {frame}: {v.node_count(0)[1]} bricks of 32^3, {ln} entries")
+    v.close()
+
+
+def test_advance_points_range_and_refusals():
+    from gvdbx.lib import GvdbxError
+    U.log("case AdvancePoints refusals and partial range")
+    p = np.random.default_rng(5).uniform(10.0, 50.0, (1000, 3)).astype(np.float32)
+    v = U.new_volume()
+    d = v.to_device(p)
+    v.SetPoints(d, n=1000)
+    with pytest.raises(GvdbxError):
+        v.AdvancePoints(1000, 0.5)  # no velocities bound
+    dv = v.to_device(p)
+    v.SetPoints(d, dv, n=1000)
+    with pytest.raises(GvdbxError):
+        v.AdvancePoints(1001, 0.5)  # beyond the bound buffer
+    with pytest.raises(GvdbxError):
+        v.AdvancePoints(1000, float("nan"))
+    assert d.download(np.float32).tobytes() == p.tobytes(), "a refused call moved points"
+    v.AdvancePoints(500, 1.0)
+    got = d.download(np.float32).reshape(-1, 3)
+    assert got[:500].tobytes() == (p[:500] + p[:500]).tobytes()
+    assert got[500:].tobytes() == p[500:].tobytes()
     v.close()

@@ -1,6 +1,7 @@
 """The SPH workload at full size on the GPU: 4,194,304 particles on the jittered lattice of the
-lower 40 % of a 256^3 box, velocities N(0, 1), 60 steps of q += v dt (dt = 0.5), each step
-RebuildTopology(r = 1) (the reference state machine: full, full, then incremental) + UpdateAtlas +
+lower 40 % of a 256^3 box, velocities N(0, 1), 60 steps of q += v dt (dt = 0.5, on the device
+through gvdbx_advance_points; its positions must equal numpy's float32 q + v * dt bit for bit
+every step), each step RebuildTopology(r = 1) (the reference state machine: full, full, then incremental) + UpdateAtlas +
 InsertPointsSubcell(4) + GatherDensity with the three velocity means + UpdateApron of the four
 channels -- the call sequence of gFluidSurface's frame (main_fluid_surface.cpp:320-356).
 
@@ -116,7 +117,9 @@ def test_sph_4m_60_steps(oracle_mod):
     for step in range(STEPS):
         if step:
             cur = (cur + vel * DT).astype(np.float32)
-            dp.upload(cur)
+            v.AdvancePoints(n, DT)
+            got = dp.download(np.float32, 3 * n)
+            assert got.tobytes() == cur.tobytes(), f"step {step}: device displacement differs"
         v.RebuildTopology(n, R)
         v.UpdateAtlas()
         ln = v.InsertPointsSubcell(4, n, R)
