@@ -307,10 +307,15 @@ def levelset_bench(reps):
 def batched_bench(reps, tune=True, per_cloud=True):
     """The batched workload: 256 Gaussian-blob clouds with log-uniform sizes in
     [4,096, 1,048,576] (8 blobs each, sigma U(4, 32), centres U(96, 416)^3, seed 5), each with
-    its own grid.  Brick and subcell size come from the autotuner; then one step (full
-    RebuildTopology(r=1) + UpdateAtlas + InsertPointsSubcell + GatherDensity + UpdateApron) for
-    all clouds as one batch against the same step run cloud by cloud (one context per cloud,
-    created and warmed up beforehand).  Best of `reps` synchronised passes each."""
+    its own grid, r = 1.  The autotuner picks brick size, subcell size and launch shape (threads
+    per brick workgroup, grid or persistent form) on the graph-launched insert + gather + apron.
+    Timed, best of `reps` synchronised passes each:
+      - graph: InsertPointsSubcell + GatherDensity + UpdateApron for all clouds, recorded once on
+        the batch's topology and run as one graph launch (the configured form);
+      - eager: the same calls issued one by one on the batch;
+      - per cloud: the same calls on each cloud's own context (256 contexts, default launch
+        shape, topology built and every call run once beforehand), one after the other;
+      - full steps: RebuildTopology (full) + UpdateAtlas + the three calls, batched and per cloud."""
     from gvdbx import autotune, synth
     cs = synth.blob_clouds()
     counts = [len(c) for c in cs]
@@ -318,37 +323,66 @@ def batched_bench(reps, tune=True, per_cloud=True):
     n = len(allp)
     mc = autotune.max_valid_coord(cs)
     if tune:
-        tuned = autotune.tune(lambda v: v.to_device(allp), counts, mc, radius=RADIUS, reps=2, log=progress)
-    else:  # the configuration the autotuner picks for this batch
-        tuned = {"leaf_log2": 3, "subcell": 4, "cell_log2": autotune.cell_for(3, mc, RADIUS), "fixed": True}
+        tuned = autotune.tune(lambda v: v.to_device(allp), counts, mc, radius=RADIUS, reps=3, log=progress)
+    else:  # a configuration the autotuner picks for this batch
+        tuned = {"leaf_log2": 3, "subcell": 4, "block": 0, "persistent": False,
+                 "cell_log2": autotune.cell_for(3, mc, RADIUS), "fixed": True}
     leaf, sc = tuned["leaf_log2"], tuned["subcell"]
 
     v = autotune.new_batch_volume(leaf)
     d = v.to_device(allp)
     autotune.bind(v, leaf, d, counts, RADIUS, mc)
-    best_b = float("inf")
+    v.SetLaunch(tuned["block"], tuned["persistent"])
+    best_full = float("inf")
     for _ in range(reps + 1):
         v.synchronize()
         t0 = time.perf_counter()
         v.RequestFullRebuild(True)
         autotune.run_batch(v, n, RADIUS, sc)
         v.synchronize()
-        best_b = min(best_b, (time.perf_counter() - t0) * 1e3)
+        best_full = min(best_full, (time.perf_counter() - t0) * 1e3)
+    best_eager = float("inf")
+    for _ in range(reps + 1):
+        v.synchronize()
+        t0 = time.perf_counter()
+        autotune.insert_gather_apron(v, n, RADIUS, sc)
+        v.synchronize()
+        best_eager = min(best_eager, (time.perf_counter() - t0) * 1e3)
+    autotune.record_step(v, n, RADIUS, sc)
+    v.GraphLaunch()
+    best_graph = autotune.time_launches(v, reps + 1)
+    _, ln, _ = v.subcell_info()
     bricks = v.getNumUsedNodes(0)
+    v.GraphRelease()
     v.close()
     del d
+    out = {"workload": "batched_256 (256 blob clouds, sizes log-uniform in [4096, 1048576], own grid each, r=1)",
+           "points": n, "clouds": len(cs), "bricks": int(bricks), "subcell_entries": int(ln), "autotune": tuned,
+           "graph_ms": round(best_graph, 3), "graph_Mpoints_s": round(n / (best_graph * 1e-3) / 1e6, 1),
+           "eager_ms": round(best_eager, 3), "full_step_ms": round(best_full, 3),
+           "full_step_Mpoints_s": round(n / (best_full * 1e-3) / 1e6, 1)}
     if not per_cloud:
-        return {"workload": "batched_256", "points": n, "bricks": int(bricks), "autotune": tuned,
-                "batched_ms": round(best_b, 3), "batched_Mpoints_s": round(n / (best_b * 1e-3) / 1e6, 1)}
+        return out
 
     vols, devs = [], []
     for c in cs:
         w = autotune.new_batch_volume(leaf)
         dc = w.to_device(c)
         w.SetPoints(dc, n=len(c))
+        autotune.run_batch(w, len(c), RADIUS, sc)
         vols.append(w)
         devs.append(dc)
     best_p = float("inf")
+    for _ in range(reps + 1):
+        for w in vols:
+            w.synchronize()
+        t0 = time.perf_counter()
+        for w, c in zip(vols, cs):
+            autotune.insert_gather_apron(w, len(c), RADIUS, sc)
+        for w in vols:
+            w.synchronize()
+        best_p = min(best_p, (time.perf_counter() - t0) * 1e3)
+    best_pf = float("inf")
     for _ in range(reps + 1):
         for w in vols:
             w.synchronize()
@@ -358,16 +392,14 @@ def batched_bench(reps, tune=True, per_cloud=True):
             autotune.run_batch(w, len(c), RADIUS, sc)
         for w in vols:
             w.synchronize()
-        best_p = min(best_p, (time.perf_counter() - t0) * 1e3)
+        best_pf = min(best_pf, (time.perf_counter() - t0) * 1e3)
     del devs
     for w in vols:
         w.close()
-    return {"workload": "batched_256 (256 blob clouds, sizes log-uniform in [4096, 1048576], own grid each, r=1)",
-            "points": n, "clouds": len(cs), "bricks": int(bricks), "autotune": tuned,
-            "batched_ms": round(best_b, 3), "per_cloud_ms": round(best_p, 3),
-            "batched_Mpoints_s": round(n / (best_b * 1e-3) / 1e6, 1),
-            "per_cloud_Mpoints_s": round(n / (best_p * 1e-3) / 1e6, 1),
-            "speedup": round(best_p / best_b, 3)}
+    out.update({"per_cloud_ms": round(best_p, 3), "per_cloud_Mpoints_s": round(n / (best_p * 1e-3) / 1e6, 1),
+                "speedup": round(best_p / best_graph, 3),
+                "per_cloud_full_step_ms": round(best_pf, 3), "full_step_speedup": round(best_pf / best_full, 3)})
+    return out
 
 
 def calibrate_copy():

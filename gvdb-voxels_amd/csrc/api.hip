@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -180,6 +181,29 @@ struct gvdbx_ctx {
   void* scratch = nullptr;
   hipError_t scratch_err = hipSuccess;
 
+  // launch shape of the per-brick kernels (gvdbx_set_launch)
+  int launch_block = 0, launch_persist = 0, cus = 0;
+  Buf queue;  // the persistent kernels' brick queue counter
+
+  // list buffers are sized for the largest possible length (no read-back before the insert)
+  // while that takes at most this many bytes per list
+  size_t list_budget = (size_t)1 << 30;
+
+  // graph recording (gvdbx_graph_begin / _end / _launch)
+  bool recording = false;
+  int record_err = GVDBX_OK;     // first failure of a call while recording
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t graph_topo = 0, graph_atlas = 0;
+  const char* graph_pos = nullptr;
+  int64_t graph_npts = 0;
+  Buf graph_len;                 // the list length a recorded insert leaves on the device
+  bool sclen_dev = false;        // sclen is on the device (graph_len) until read back
+  int64_t sclen_hint = 0;        // the last list length known on the host
+  bool graph_insert = false;     // the recording holds an insert (its length is graph_len)
+  uint64_t buf_gen = 0;          // counts reallocations: a graph refers to the buffers of its recording
+  uint64_t graph_buf_gen = 0;
+
   // named-kernel timer: event pairs recorded on the stream, summed when read
   std::string timer_name;
   bool timer_on = false;
@@ -195,9 +219,19 @@ int fail(gvdbx_ctx* c, int code, const char* fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
-  if (c) c->err = buf;
+  if (c) {
+    c->err = buf;
+    if (c->recording && c->record_err == GVDBX_OK) c->record_err = code;  // the recording is void
+  }
   return code;
 }
+
+// Calls that need the host to wait for the device, or change the topology or bindings, cannot
+// be recorded into a graph.
+#define NOT_RECORDING(c)                                                                            \
+  do {                                                                                              \
+    if ((c) && (c)->recording) return fail((c), GVDBX_ERR_STATE, "%s cannot be recorded into a graph", __func__); \
+  } while (0)
 
 #define HIPCHK(c, expr)                                                                  \
   do {                                                                                   \
@@ -224,6 +258,8 @@ int launch_status(gvdbx_ctx* c, const char* what) {
 // buffer, so the stream is drained before it is freed (growth is rare in steady state).
 int ensure(gvdbx_ctx* c, Buf& b, size_t bytes) {
   if (bytes <= b.cap) return GVDBX_OK;
+  if (c->recording)
+    return fail(c, GVDBX_ERR_STATE, "a buffer would grow while recording: run the same calls once before recording");
   size_t nb = b.cap ? b.cap : 256;
   while (nb < bytes) nb += nb / 2 + 256;
   if (b.p) {
@@ -232,6 +268,7 @@ int ensure(gvdbx_ctx* c, Buf& b, size_t bytes) {
   }
   b.p = nullptr;
   b.cap = 0;
+  c->buf_gen++;
   hipError_t e = hipMalloc(&b.p, nb);
   if (e != hipSuccess) {
     b.p = nullptr;
@@ -245,6 +282,7 @@ int ensure(gvdbx_ctx* c, Buf& b, size_t bytes) {
 // Grow-only buffer keeping the first `keep` bytes (pool growth, A.cpp:168-187).
 int ensure_keep(gvdbx_ctx* c, Buf& b, size_t bytes, size_t keep) {
   if (bytes <= b.cap) return GVDBX_OK;
+  if (c->recording) return fail(c, GVDBX_ERR_STATE, "a pool would grow while recording");
   size_t nb = b.cap ? b.cap : 256;
   while (nb < bytes) nb *= 2;
   void* p = nullptr;
@@ -253,6 +291,7 @@ int ensure_keep(gvdbx_ctx* c, Buf& b, size_t bytes, size_t keep) {
     (void)hipGetLastError();
     return fail(c, GVDBX_ERR_NOMEM, "hipMalloc(%zu) failed: %s", nb, hipGetErrorString(e));
   }
+  c->buf_gen++;
   if (keep && b.p) HIPCHK(c, hipMemcpyAsync(p, b.p, keep, hipMemcpyDeviceToDevice, c->stream));
   if (b.p) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -266,6 +305,10 @@ int ensure_keep(gvdbx_ctx* c, Buf& b, size_t bytes, size_t keep) {
 template <typename T>
 T* P(Buf& b) {
   return (T*)b.p;
+}
+
+LaunchCfg launch_of(gvdbx_ctx* c) {
+  return LaunchCfg{c->launch_block, c->launch_persist, (int*)c->queue.p, c->cus};
 }
 
 int res_of(const gvdbx_ctx* c, int l) { return 1 << c->ld[l]; }
@@ -316,6 +359,7 @@ PointsView points_of(gvdbx_ctx* c, int64_t n, const float* tr) {
 }
 
 int read_meta(gvdbx_ctx* c) {
+  NOT_RECORDING(c);
   CHK(launch_status(c, "read-back"));
   HIPCHK(c, hipMemcpyAsync(c->h_meta, c->meta.p, M_INTS * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -325,6 +369,7 @@ int read_meta(gvdbx_ctx* c) {
 // Read-back split in two: the copy is enqueued where the values are final, more work can be
 // enqueued behind it, and the host waits only for the copy.
 int read_meta_begin(gvdbx_ctx* c) {
+  NOT_RECORDING(c);
   CHK(launch_status(c, "read-back"));
   HIPCHK(c, hipMemcpyAsync(c->h_meta, c->meta.p, M_INTS * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(c->meta_ev, c->stream));
@@ -357,7 +402,7 @@ struct TimedRegion {
   bool on;
   // allowed = false: never times (no event is created, for the entry point that allocates nothing)
   TimedRegion(gvdbx_ctx* ctx, const char* name, bool allowed = true)
-      : c(ctx), on(allowed && ctx->timer_on && ctx->timer_name == name) {
+      : c(ctx), on(allowed && !ctx->recording && ctx->timer_on && ctx->timer_name == name) {
     if (on) {
       hipEvent_t e = next_event(c);
       if (e) (void)hipEventRecord(e, c->stream);
@@ -766,6 +811,7 @@ int check_points(gvdbx_ctx* c, int64_t n) {
 bool wide_keys(const gvdbx_ctx* c, int d) { return key_fmt(c, d).nbits > 32; }
 
 int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accumulate, int min_depth) {
+  NOT_RECORDING(c);
   CHK(check_points(c, n));
   // One read-back returns the bounding box and, when the tree is expected to stay at its
   // depth, the key count of the speculatively launched incremental box pass (it only writes
@@ -967,7 +1013,13 @@ int gvdbx_ctx_create(int device, void* hip_stream, gvdbx_ctx** out) {
     c->own_stream = true;
   }
   c->scratch_err = scratch_address(device, &c->scratch);
-  if (ensure(c, c->meta, M_INTS * sizeof(int)) != GVDBX_OK ||
+  if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->cus = 0;
+  size_t mem_free = 0, mem_total = 0;
+  if (hipMemGetInfo(&mem_free, &mem_total) == hipSuccess && mem_total / 8 > c->list_budget)
+    c->list_budget = mem_total / 8;
+  (void)hipGetLastError();
+  if (ensure(c, c->meta, M_INTS * sizeof(int)) != GVDBX_OK || ensure(c, c->queue, 256) != GVDBX_OK ||
+      ensure(c, c->graph_len, 256) != GVDBX_OK ||
       hipHostMalloc((void**)&c->h_meta, M_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&c->meta_ev, hipEventDisableTiming) != hipSuccess) {
     gvdbx_ctx_destroy(c);
@@ -977,10 +1029,26 @@ int gvdbx_ctx_create(int device, void* hip_stream, gvdbx_ctx** out) {
   return GVDBX_OK;
 }
 
+static void release_graph(gvdbx_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_insert = false;
+}
+
 int gvdbx_ctx_destroy(gvdbx_ctx* c) {
   if (!c) return GVDBX_ERR_ARG;
   (void)hipSetDevice(c->device);
+  if (c->recording) {  // an unfinished recording is dropped
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    c->recording = false;
+    (void)hipGetLastError();
+  }
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  release_graph(c);
   Buf* bufs[] = {&c->leaf_slot, &c->slot_leaf, &c->slot_pos, &c->nbr, &c->apron_tab, &c->sc_map, &c->sc_cnt, &c->sc_off, &c->sc_pos,
                  &c->sc_nid, &c->list_pos, &c->list_vel, &c->map_leaf, &c->vsum, &c->hkeys, &c->hkeys_alt,
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
@@ -988,7 +1056,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
                  &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->scatter_acc, &c->wvs,
                  &c->list_clr, &c->csum, &c->cmean, &c->scatter_clr, &c->bmap, &c->list_q16, &c->list_vq16,
                  &c->cloud_id,
-                 &c->cloud_off_dev};
+                 &c->cloud_off_dev, &c->queue, &c->graph_len};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (int l = 0; l < kMaxLev; ++l) {
@@ -1012,6 +1080,7 @@ int gvdbx_last_error(const gvdbx_ctx* c, char* buf, int buflen) {
 }
 
 int gvdbx_synchronize(gvdbx_ctx* c) {
+  NOT_RECORDING(c);
   if (!c) return GVDBX_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return GVDBX_OK;
@@ -1023,7 +1092,8 @@ int gvdbx_get_stream(const gvdbx_ctx* c, void** s) {
   return GVDBX_OK;
 }
 
-int gvdbx_configure(gvdbx_ctx* c, int levs, const int* log2dims) {  // Configure (V.cpp:2364-2434)
+int gvdbx_configure(gvdbx_ctx* c, int levs, const int* log2dims) {
+  NOT_RECORDING(c);  // Configure (V.cpp:2364-2434)
   if (!c || !log2dims || levs < 2 || levs > kMaxLev) return fail(c, GVDBX_ERR_ARG, "bad level configuration");
   int sum = 0;
   for (int l = 0; l < levs; ++l) {
@@ -1048,7 +1118,8 @@ int gvdbx_configure(gvdbx_ctx* c, int levs, const int* log2dims) {  // Configure
   return GVDBX_OK;
 }
 
-int gvdbx_set_channel_default(gvdbx_ctx* c, int cx, int cy, int cz) {  // SetChannelDefault (V.h:390)
+int gvdbx_set_channel_default(gvdbx_ctx* c, int cx, int cy, int cz) {
+  NOT_RECORDING(c);  // SetChannelDefault (V.h:390)
   if (!c || cx <= 0 || cy <= 0 || cz <= 0) return fail(c, GVDBX_ERR_ARG, "bad channel default");
   c->axis_cx = cx;
   c->axis_cy = cy;
@@ -1056,7 +1127,8 @@ int gvdbx_set_channel_default(gvdbx_ctx* c, int cx, int cy, int cz) {  // SetCha
   return GVDBX_OK;
 }
 
-int gvdbx_add_channel(gvdbx_ctx* c, int chan, int dtype, int apron) {  // AddChannel (V.cpp:2437-2453)
+int gvdbx_add_channel(gvdbx_ctx* c, int chan, int dtype, int apron) {
+  NOT_RECORDING(c);  // AddChannel (V.cpp:2437-2453)
   if (!c || !c->configured) return fail(c, GVDBX_ERR_ARG, "context not configured");
   if (chan < 0 || chan >= 32 || (dtype != GVDBX_T_FLOAT && dtype != GVDBX_T_UCHAR4) || apron < 0 ||
       apron > res_of(c, 0))
@@ -1106,6 +1178,7 @@ int gvdbx_fill_channel(gvdbx_ctx* c, int chan, float v) {  // FillChannel / Clea
 
 int gvdbx_set_points(gvdbx_ctx* c, int64_t n, const void* pos, int pos_off, int pos_stride, const void* vel,
                      int vel_off, int vel_stride, const void* clr, int clr_off, int clr_stride) {
+  NOT_RECORDING(c);
   if (!c || !pos || n <= 0 || pos_stride < 12 || pos_off < 0 || (pos_off & 3) || (pos_stride & 3))
     return fail(c, GVDBX_ERR_ARG, "bad position buffer");
   if (vel && (vel_stride < 12 || vel_off < 0 || (vel_off & 3) || (vel_stride & 3)))
@@ -1144,6 +1217,7 @@ int gvdbx_advance_points(gvdbx_ctx* c, int64_t n, float dt) {
 int gvdbx_batch_set_points(gvdbx_ctx* c, int ncloud, const int64_t* counts, const void* pos, int pos_off,
                            int pos_stride, const void* vel, int vel_off, int vel_stride, int cell_log2,
                            float max_radius) {
+  NOT_RECORDING(c);
   if (!c || !c->configured) return fail(c, GVDBX_ERR_ARG, "context not configured");
   if (ncloud < 1 || ncloud > 65536 || !counts) return fail(c, GVDBX_ERR_ARG, "bad cloud count %d", ncloud);
   // radii up to 1 keep every extra-pass box inside the cloud's own root (its box
@@ -1297,7 +1371,8 @@ static int plan_brick_map(gvdbx_ctx* c, BrickMapOut* out, bool* use) {
   return GVDBX_OK;
 }
 
-int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
+int gvdbx_update_atlas(gvdbx_ctx* c) {
+  NOT_RECORDING(c);  // UpdateAtlas (V.cpp:2630-2704)
   if (!c || !c->configured) return GVDBX_ERR_ARG;
   if (c->apron < 0) return fail(c, GVDBX_ERR_STATE, "UpdateAtlas needs channel 0 (AddChannel)");
   const int64_t total = c->cnt[0];
@@ -1340,9 +1415,28 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {  // UpdateAtlas (V.cpp:2630-2704)
   return launch_status(c, "UpdateAtlas");
 }
 
+// Largest number of subcell corners one point reaches per axis.  The corners are the multiples of
+// sc in [trunc(q - r) / sc * sc, trunc(q + 1 + r) / sc * sc] (C division), a span below
+// 2r + 1 + sc + e voxels, e = the FP32 rounding of q - r and q + 1 + r (at most 1.5 units in the
+// last place of the largest coordinate that can reach a leaf), so there are at most
+// ceil((2r + 1 + e) / sc) + 1 of them.
+static int64_t subcells_per_axis(const gvdbx_ctx* c, float r, int sc) {
+  double xmax = 1.0;
+  if (c->batch) {
+    xmax = std::ldexp(1.0, c->cell_lg);  // positions stay in their cloud's frame
+  } else if (c->depth >= 0) {
+    for (int a = 0; a < 3; ++a)
+      xmax = std::max(xmax, std::fabs((double)c->root[a]) + std::ldexp(1.0, c->lgR[c->depth]));
+  }
+  xmax += (double)r + sc + 1.0;
+  const double e = 2.0 * std::ldexp(1.0, (int)std::ceil(std::log2(2.0 * xmax)) - 23);
+  return (int64_t)std::ceil((2.0 * (double)r + 1.0 + e) / (double)sc) + 1;
+}
+
 static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float trans[3], int64_t* len, bool fp16) {
   CHK(check_points(c, n));  // InsertPointsSubcell (V.cpp:4971-5092), _FP16 (V.cpp:5094-5189)
   if (fp16 && c->batch) return fail(c, GVDBX_ERR_ARG, "InsertPointsSubcell_FP16 is not for batched clouds");
+  if (fp16 && c->recording) return fail(c, GVDBX_ERR_STATE, "InsertPointsSubcell_FP16 cannot be recorded");
   if (c->batch) {
     if (r > c->batch_rmax) return fail(c, GVDBX_ERR_ARG, "radius %g exceeds the batch guard radius %g", (double)r,
                                        (double)c->batch_rmax);
@@ -1367,6 +1461,7 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     c->scdim = res_of(c, 0) / sc;
     c->nsc = 0;
     c->sclen = 0;
+    c->sclen_dev = false;
     return GVDBX_OK;
   }
   const SubcellCfg cfg = subcell_cfg(c, sc, r);
@@ -1444,6 +1539,7 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   const int* tail = P<int>(c->ranges) + nmap;  // points whose own leaf is missing, and invalid ones
   gather_sorted(pv, svals, n, P<float4>(c->spos), with_vel ? P<float4>(c->svel) : nullptr, c->stream);
   BinArgs A;
+  A.L = launch_of(c);
   A.T = T;
   A.P = pv;
   A.c = cfg;
@@ -1493,14 +1589,18 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   Buf& lpos = fp16 ? c->list_q16 : c->list_pos;
   Buf& lvel = fp16 ? c->list_vq16 : c->list_vel;
   const size_t ebytes = fp16 ? 6 : 12;  // per list entry: float3 or ushort3
-  // Lists sized for the largest possible length when that bound is modest (and below 2^31, so
-  // the 32-bit offsets cannot wrap), so the insert is enqueued before the single length
-  // read-back; otherwise the length is read and checked before anything is inserted.  From a
-  // bound of 2^32 on, the 32-bit scan total can wrap, so the length comes from a 64-bit sum.
-  int64_t per_axis = (int64_t)((2.0f * r + 1.0f) / (float)sc) + 2;
+  // Lists sized for the largest possible length when that bound fits the list budget (and lies
+  // below 2^31, so the 32-bit offsets cannot wrap), so the insert is enqueued before the single
+  // length read-back (and a recorded insert needs none); otherwise the length is read and checked
+  // before anything is inserted.  From a bound of 2^32 on, the 32-bit scan total can wrap, so the
+  // length comes from a 64-bit sum.
+  const int64_t per_axis = subcells_per_axis(c, r, sc);
   const int64_t bound = per_axis * per_axis * per_axis * n;
-  const size_t room = lpos.cap > ((size_t)1 << 30) ? lpos.cap : ((size_t)1 << 30);
+  const size_t room = lpos.cap > c->list_budget ? lpos.cap : c->list_budget;
   const bool early = bound <= INT32_MAX && (size_t)(bound + 1) * ebytes <= room;
+  if (c->recording && !early)
+    return fail(c, GVDBX_ERR_CAPACITY, "the list bound (%lld entries) does not fit a recorded insert",
+                (long long)bound);
   int64_t ln = -1;
   if (early) {
     CHK(ensure(c, lpos, (size_t)(bound + 1) * ebytes));
@@ -1527,22 +1627,30 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   A.list_q16 = fp16 ? P<uint16_t>(c->list_q16) : nullptr;
   A.list_vq16 = (keep_vel_lists && fp16) ? P<uint16_t>(c->list_vq16) : nullptr;
   A.list_clr = keep_clr ? P<uint32_t>(c->list_clr) : nullptr;
-  if (early) CHK(read_meta_begin(c));  // the length is final after the scan
+  if (c->recording) {  // the length stays on the device, where later recorded calls leave it alone
+    copy_u32((const uint32_t*)(P<int>(c->meta) + M_TOTAL), P<uint32_t>(c->graph_len), 1, c->stream);
+  } else if (early) {
+    CHK(read_meta_begin(c));  // the length is final after the scan
+  }
   {
     TimedRegion tr(c, "insert");
     bin_insert(A, nmap, c->stream);
     bin_points_insert(A, tail, c->stream);
   }
-  if (early) {
+  if (early && !c->recording) {
     CHK(read_meta_end(c));  // returns while the insert kernels still run
     ln = (uint32_t)c->h_meta[M_TOTAL];
   }
-  if (ln > INT32_MAX || ln > bound) return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);
+  if (!c->recording && (ln > INT32_MAX || ln > bound))
+    return fail(c, GVDBX_ERR_CAPACITY, "subcell list length %lld", (long long)ln);
   CHK(launch_status(c, "InsertPointsSubcell"));
   c->sc = sc;
   c->scdim = cfg.D;
   c->nsc = nsc;
-  c->sclen = ln;
+  c->sclen = c->recording ? -1 : ln;
+  c->sclen_dev = c->recording;
+  if (c->recording) c->graph_insert = true;
+  else c->sclen_hint = ln;
   c->lists_have_vel = keep_vel_lists;
   c->lists_have_clr = keep_clr;
   c->lists_fp16 = fp16;
@@ -1603,10 +1711,11 @@ static int gather_impl(gvdbx_ctx* c, float r, int chan, int accum, int mode) {
   TimedRegion tr(c, "gather");
   const int br = cfg.res0 + 2 * c->apron;
   if (mode == 0 && !accum && cfg.res0 <= 8 && br * br * br <= kMaxPaddedVox) {
-    density_brick(cfg, (int)c->used0, c->nsc, c->sclen, P<int>(c->map_leaf), P<int>(c->sc_off),
-                  P<float>(c->list_pos), P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]),
+    density_brick(cfg, (int)c->used0, c->nsc, c->sclen >= 0 ? c->sclen : c->sclen_hint, P<int>(c->map_leaf),
+                  P<int>(c->sc_off), P<float>(c->list_pos), P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]),
                   c->sc_map_current ? nullptr : P<int4>(c->slot_pos), c->apron,
-                  c->apron > 0 ? P<int>(c->nbr) : nullptr, P<float>(c->chan[chan]), nullptr, c->stream);
+                  c->apron > 0 ? P<int>(c->nbr) : nullptr, P<float>(c->chan[chan]), nullptr, launch_of(c),
+                  c->stream);
     c->nbr_apron_ok[chan] = c->bound_ok[chan] = c->apron > 0;
     c->bound_val[chan] = 0.0f;
   } else {
@@ -1736,9 +1845,10 @@ int gvdbx_gather_density_velocity(gvdbx_ctx* c, int sc, int64_t n, float r, cons
   TimedRegion tr(c, "gather");
   const VelOut vo{P<int>(c->sc_cnt), P<unsigned long long>(c->vsum),
                   {P<float>(c->chan[cx]), P<float>(c->chan[cy]), P<float>(c->chan[cz])}};
-  density_brick(cfg, (int)c->used0, c->nsc, c->sclen, P<int>(c->map_leaf), P<int>(c->sc_off), P<float>(c->list_pos),
-                P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]), c->sc_map_current ? nullptr : P<int4>(c->slot_pos),
-                c->apron, P<int>(c->nbr), P<float>(c->chan[chan]), &vo, c->stream);
+  density_brick(cfg, (int)c->used0, c->nsc, c->sclen >= 0 ? c->sclen : c->sclen_hint, P<int>(c->map_leaf),
+                P<int>(c->sc_off), P<float>(c->list_pos), P<int>(c->leaf_slot), P<gvdbx_node>(c->nodes[0]),
+                c->sc_map_current ? nullptr : P<int4>(c->slot_pos), c->apron, P<int>(c->nbr),
+                P<float>(c->chan[chan]), &vo, launch_of(c), c->stream);
   for (int ch : {chan, cx, cy, cz}) {
     c->nbr_apron_ok[ch] = c->bound_ok[ch] = true;
     c->bound_val[ch] = 0.0f;
@@ -1764,6 +1874,7 @@ int gvdbx_scatter_reduce_levelset(gvdbx_ctx* c, float r, int chan, float init) {
 static int apron_table_current(gvdbx_ctx* c) {
   const int res0 = res_of(c, 0);
   if (c->apron_tab_res == res0 && c->apron_tab_apron == c->apron) return GVDBX_OK;
+  NOT_RECORDING(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));  // an earlier upload may still read the host table
   apron_table(res0, c->apron, c->h_apron_tab);
   CHK(ensure(c, c->apron_tab, c->h_apron_tab.size() * 4 + 4));
@@ -2024,6 +2135,7 @@ int gvdbx_get_node_count(const gvdbx_ctx* c, int lev, int64_t* total, int64_t* u
 }
 
 static int copy_out(gvdbx_ctx* c, void* dst, const void* src, size_t bytes) {
+  NOT_RECORDING(c);
   if (bytes == 0) return GVDBX_OK;
   HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2066,8 +2178,22 @@ int gvdbx_get_channel_ptr(gvdbx_ctx* c, int chan, void** p, int64_t* floats) {
   return GVDBX_OK;
 }
 
-int gvdbx_get_subcell_info(const gvdbx_ctx* c, int64_t* nsc, int64_t* len, int* sc) {
+// The list length of a launched graph's insert, read back on request.
+static int resolve_sclen(gvdbx_ctx* c) {
+  if (!c->sclen_dev) return GVDBX_OK;
+  NOT_RECORDING(c);
+  uint32_t v = 0;
+  HIPCHK(c, hipMemcpyAsync(&v, c->graph_len.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->sclen = (int64_t)v;
+  c->sclen_hint = c->sclen;
+  c->sclen_dev = false;
+  return GVDBX_OK;
+}
+
+int gvdbx_get_subcell_info(gvdbx_ctx* c, int64_t* nsc, int64_t* len, int* sc) {
   if (!c) return GVDBX_ERR_ARG;
+  if (len) CHK(resolve_sclen(c));
   if (nsc) *nsc = c->nsc;
   if (len) *len = c->sclen;
   if (sc) *sc = c->sc;
@@ -2076,6 +2202,7 @@ int gvdbx_get_subcell_info(const gvdbx_ctx* c, int64_t* nsc, int64_t* len, int* 
 
 int gvdbx_copy_subcell_array(gvdbx_ctx* c, int which, void* out, int64_t max_bytes) {
   if (!c || !out) return GVDBX_ERR_ARG;
+  CHK(resolve_sclen(c));
   const void* src = nullptr;
   int64_t bytes = 0;
   switch (which) {
@@ -2239,6 +2366,7 @@ int gvdbx_load_vbx(gvdbx_ctx* c, const char* path) {
 }
 
 int gvdbx_dev_alloc(gvdbx_ctx* c, int64_t bytes, void** p) {
+  NOT_RECORDING(c);
   if (!c || !p || bytes < 0) return GVDBX_ERR_ARG;
   hipError_t e = hipMalloc(p, (size_t)(bytes > 0 ? bytes : 1));
   if (e != hipSuccess) {
@@ -2249,6 +2377,7 @@ int gvdbx_dev_alloc(gvdbx_ctx* c, int64_t bytes, void** p) {
 }
 
 int gvdbx_dev_free(gvdbx_ctx* c, void* p) {
+  NOT_RECORDING(c);
   if (!c) return GVDBX_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (p) HIPCHK(c, hipFree(p));
@@ -2256,6 +2385,7 @@ int gvdbx_dev_free(gvdbx_ctx* c, void* p) {
 }
 
 int gvdbx_commit(gvdbx_ctx* c, void* dst, const void* src, int64_t bytes) {
+  NOT_RECORDING(c);
   if (!c || bytes < 0) return GVDBX_ERR_ARG;
   if (bytes == 0) return GVDBX_OK;
   HIPCHK(c, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
@@ -2269,6 +2399,7 @@ int gvdbx_retrieve(gvdbx_ctx* c, void* dst, const void* src, int64_t bytes) {
 }
 
 int gvdbx_kernel_timer(gvdbx_ctx* c, const char* kernel, int enable) {
+  NOT_RECORDING(c);
   if (!c) return GVDBX_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->timer_on = enable && kernel;
@@ -2278,6 +2409,7 @@ int gvdbx_kernel_timer(gvdbx_ctx* c, const char* kernel, int enable) {
 }
 
 int gvdbx_kernel_timer_read(gvdbx_ctx* c, double* ms, int64_t* n) {
+  NOT_RECORDING(c);
   if (!c) return GVDBX_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double tot = 0;
@@ -2289,6 +2421,88 @@ int gvdbx_kernel_timer_read(gvdbx_ctx* c, double* ms, int64_t* n) {
   if (ms) *ms = tot;
   if (n) *n = (int64_t)(c->ev_used / 2);
   c->ev_used = 0;
+  return GVDBX_OK;
+}
+
+// ---------------------------------------------------------------- launch shape and graphs
+
+int gvdbx_set_launch(gvdbx_ctx* c, int block_threads, int persistent) {
+  if (!c) return GVDBX_ERR_ARG;
+  NOT_RECORDING(c);
+  if (block_threads != 0 && block_threads != 64 && block_threads != 128 && block_threads != 256)
+    return fail(c, GVDBX_ERR_ARG, "block size %d: 0 (automatic), 64, 128 or 256", block_threads);
+  if (persistent != 0 && persistent != 1) return fail(c, GVDBX_ERR_ARG, "persistent must be 0 or 1");
+  c->launch_block = block_threads;
+  c->launch_persist = persistent;
+  return GVDBX_OK;
+}
+
+int gvdbx_graph_begin(gvdbx_ctx* c) {
+  if (!c) return GVDBX_ERR_ARG;
+  NOT_RECORDING(c);
+  if (!c->configured) return fail(c, GVDBX_ERR_STATE, "context not configured");
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->recording = true;
+  c->record_err = GVDBX_OK;
+  c->graph_insert = false;
+  return GVDBX_OK;
+}
+
+int gvdbx_graph_end(gvdbx_ctx* c) {
+  if (!c) return GVDBX_ERR_ARG;
+  if (!c->recording) return fail(c, GVDBX_ERR_STATE, "no recording in progress (gvdbx_graph_begin)");
+  c->recording = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  const int rec = c->record_err;
+  const bool had_insert = c->graph_insert;
+  c->record_err = GVDBX_OK;
+  if (e != hipSuccess || rec != GVDBX_OK || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    c->graph_insert = false;
+    if (rec != GVDBX_OK) return rec;  // the failing call's message stays in place
+    return fail(c, GVDBX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  release_graph(c);
+  hipGraphExec_t x = nullptr;
+  const hipError_t e2 = hipGraphInstantiateWithFlags(&x, g, 0);
+  if (e2 != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return fail(c, GVDBX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e2));
+  }
+  c->graph = g;
+  c->graph_exec = x;
+  c->graph_insert = had_insert;
+  c->graph_topo = c->topo_version;
+  c->graph_atlas = c->atlas_version;
+  c->graph_pos = c->pos;
+  c->graph_npts = c->npts;
+  c->graph_buf_gen = c->buf_gen;
+  return GVDBX_OK;
+}
+
+int gvdbx_graph_launch(gvdbx_ctx* c) {
+  if (!c) return GVDBX_ERR_ARG;
+  NOT_RECORDING(c);
+  if (!c->graph_exec) return fail(c, GVDBX_ERR_STATE, "no graph recorded (gvdbx_graph_begin / _end)");
+  if (c->topo_version != c->graph_topo || c->atlas_version != c->graph_atlas)
+    return fail(c, GVDBX_ERR_STATE, "the topology or atlas changed since the graph was recorded");
+  if (c->pos != c->graph_pos || c->npts != c->graph_npts)
+    return fail(c, GVDBX_ERR_STATE, "the bound points changed since the graph was recorded");
+  if (c->buf_gen != c->graph_buf_gen)
+    return fail(c, GVDBX_ERR_STATE, "buffers the graph uses were reallocated since it was recorded");
+  HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+  if (c->graph_insert) c->sclen_dev = true;
+  return GVDBX_OK;
+}
+
+int gvdbx_graph_release(gvdbx_ctx* c) {
+  if (!c) return GVDBX_ERR_ARG;
+  NOT_RECORDING(c);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  release_graph(c);
   return GVDBX_OK;
 }
 

@@ -253,13 +253,9 @@ __device__ __forceinline__ bool brick_setup(const BinArgs& A, int m, int* nb, Br
 }
 
 template <int kD>
-__global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_bins) {
-  extern __shared__ unsigned long long smem64[];
-  __shared__ int nb[27];
+__device__ __forceinline__ void bin_count_brick(const BinArgs& A, int m, int use_bins, unsigned long long* smem64,
+                                                int* nb) {
   BrickCtx B;
-  const int64_t mi = item_index();
-  if (mi >= nmap) return;
-  const int m = (int)mi;
   const int beg = A.start[m], end = A.start[m + 1];  // in flight with the brick's set-up loads
   const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W, D3 = D * D * D;
   if (!brick_setup(A, m, nb, B, D, W)) {  // no leaf: nothing reaches the cell walk through it
@@ -380,6 +376,13 @@ __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_
   }
 }
 
+template <int kD, bool kPersist>
+__global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_bins) {
+  extern __shared__ unsigned long long smem64[];
+  __shared__ int nb[27];
+  for_each_brick<kPersist>(nmap, A.L.queue, [&](int m) { bin_count_brick<kD>(A, m, use_bins, smem64, nb); });
+}
+
 // One list entry: the position (float3, or ushort3 quantized for the FP16 insert), the velocity
 // when velocity lists are kept, the colour when colours are bound.
 __device__ __forceinline__ void put_entry(const BinArgs& A, int64_t k, const float q[3], float vx, float vy, float vz,
@@ -409,13 +412,8 @@ __device__ __forceinline__ void put_entry(const BinArgs& A, int64_t k, const flo
 // subcell per axis, so it reaches at most 2 subcells per axis, all inside its brick's window; the
 // up to 8 entries then take their list slots with independent LDS atomics issued back to back.
 template <int kD, bool kTwo>
-__global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
-  extern __shared__ unsigned long long smem64[];
-  __shared__ int nb[27];
+__device__ __forceinline__ void bin_insert_brick(const BinArgs& A, int m, unsigned long long* smem64, int* nb) {
   BrickCtx B;
-  const int64_t mi = item_index();
-  if (mi >= nmap) return;
-  const int m = (int)mi;
   const int beg = A.start[m], end = A.start[m + 1];  // in flight with the brick's set-up loads
   const int D = kD ? kD : A.c.D, W = D + 2, W3 = W * W * W;
   if (!brick_setup(A, m, nb, B, D, W, true)) return;
@@ -473,6 +471,13 @@ __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
   }
 }
 
+template <int kD, bool kTwo, bool kPersist>
+__global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
+  extern __shared__ unsigned long long smem64[];
+  __shared__ int nb[27];
+  for_each_brick<kPersist>(nmap, A.L.queue, [&](int m) { bin_insert_brick<kD, kTwo>(A, m, smem64, nb); });
+}
+
 int bin_uses_class_bins(int D) { return D <= 2 ? 1 : 0; }
 
 // D = 0: no class bins
@@ -485,36 +490,57 @@ size_t bin_count_smem(int W, int D, bool vel) {
 // Threads per brick workgroup.  The per-brick kernels stride every loop by blockDim, so any
 // multiple of 64 is valid: one wave when bricks hold few items (the time then goes to each
 // brick's chain of dependent loads, and one-wave workgroups keep four times as many bricks in
-// flight), four waves otherwise.
-static inline int brick_threads(int64_t items, int64_t bricks, int64_t per_brick) {
+// flight), four waves otherwise.  A launch shape that names the block size overrides this.
+static inline int brick_threads(const LaunchCfg& L, int64_t items, int64_t bricks, int64_t per_brick) {
+  if (L.block) return L.block;
   return items < per_brick * bricks ? 64 : 256;
+}
+
+// Grid form: one workgroup per brick.  Persistent form: as many workgroups as fit on the device
+// at once (at most one per kBrickChunk bricks), each taking bricks from the queue, which is
+// cleared first by a fill kernel (so the pair also records into a graph).
+template <typename... KArgs, typename... Args>
+static void launch_bricks(void (*grid_k)(KArgs...), void (*pers_k)(KArgs...), const LaunchCfg& L, int bs, size_t sm,
+                          int64_t items, hipStream_t s, Args... args) {
+  if (L.persist && L.queue) {
+    fill_u32((uint32_t*)L.queue, 1, 0u, s);
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pers_k, bs, sm) != hipSuccess || per < 1) per = 1;
+    int64_t g = (int64_t)per * (L.cus > 0 ? L.cus : 1);
+    const int64_t need = (items + kBrickChunk - 1) / kBrickChunk;
+    if (g > need) g = need;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(pers_k, dim3((unsigned)g), dim3(bs), sm, s, args...);
+  } else {
+    hipLaunchKernelGGL(grid_k, item_grid(items), dim3(bs), sm, s, args...);
+  }
 }
 
 void bin_count(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
-  const int bs = brick_threads(A.P.n, nmap, 64);
+  const int bs = brick_threads(A.L, A.P.n, nmap, 64);
   // class bins pay off when many points share a brick; sparse bricks count pairs directly
   const int bins = bs == 256 ? bin_uses_class_bins(A.c.D) : 0;
   const size_t sm = bin_count_smem(A.W, bins ? A.c.D : 0, A.vsum != nullptr && A.P.vel != nullptr);
   if (A.c.D == 2)
-    hipLaunchKernelGGL(k_bin_count<2>, item_grid(nmap), dim3(bs), sm, s, A, nmap, bins);
+    launch_bricks(k_bin_count<2, false>, k_bin_count<2, true>, A.L, bs, sm, nmap, s, A, nmap, bins);
   else
-    hipLaunchKernelGGL(k_bin_count<0>, item_grid(nmap), dim3(bs), sm, s, A, nmap, bins);
+    launch_bricks(k_bin_count<0, false>, k_bin_count<0, true>, A.L, bs, sm, nmap, s, A, nmap, bins);
 }
 
 void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
   if (nmap <= 0) return;
   const size_t sm = (size_t)2 * A.W * A.W * A.W * 4 + 8;
-  const int bs = brick_threads(A.P.n, nmap, 64);
+  const int bs = brick_threads(A.L, A.P.n, nmap, 64);
   const bool two = 2.0 * (double)A.c.r + 2.0 <= (double)A.c.sc;
   if (A.c.D == 2 && two)
-    hipLaunchKernelGGL((k_bin_insert<2, true>), item_grid(nmap), dim3(bs), sm, s, A, nmap);
+    launch_bricks(k_bin_insert<2, true, false>, k_bin_insert<2, true, true>, A.L, bs, sm, nmap, s, A, nmap);
   else if (A.c.D == 2)
-    hipLaunchKernelGGL((k_bin_insert<2, false>), item_grid(nmap), dim3(bs), sm, s, A, nmap);
+    launch_bricks(k_bin_insert<2, false, false>, k_bin_insert<2, false, true>, A.L, bs, sm, nmap, s, A, nmap);
   else if (two)
-    hipLaunchKernelGGL((k_bin_insert<0, true>), item_grid(nmap), dim3(bs), sm, s, A, nmap);
+    launch_bricks(k_bin_insert<0, true, false>, k_bin_insert<0, true, true>, A.L, bs, sm, nmap, s, A, nmap);
   else
-    hipLaunchKernelGGL((k_bin_insert<0, false>), item_grid(nmap), dim3(bs), sm, s, A, nmap);
+    launch_bricks(k_bin_insert<0, false, false>, k_bin_insert<0, false, true>, A.L, bs, sm, nmap, s, A, nmap);
 }
 
 __global__ void k_sum_counts(const int* __restrict__ a, const int* __restrict__ b, int* __restrict__ out, int64_t n) {
@@ -564,6 +590,7 @@ __global__ void __launch_bounds__(256) k_cell_walk(BinArgs A, int64_t nsc, int* 
     int tot;
     const int c0 = live ? cnt[g] : 0;
     base = tile[blockIdx.x] + block_exclusive_scan_256(c0, &tot);
+    if (g == nsc - 1) off_out[nsc] = base + c0;  // the list length closes the offsets
   }
   int p = 0;
   unsigned long long vs[3] = {0ull, 0ull, 0ull};
@@ -713,14 +740,14 @@ __device__ __forceinline__ float mean_of(const int* cnt, const unsigned long lon
 // written in the same pass, as the reference computes its colour channel inside the density
 // gather (P.cuh:717-731): window-cell means in LDS, three more stores per voxel.
 template <int kRes, int kApron, int kD, bool kWithApron, bool kVel>
-__global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len,
-                                                      const int* __restrict__ map_leaf, const int* __restrict__ off,
-                                                      const float* __restrict__ list_pos,
-                                                      const int* __restrict__ leaf_slot,
-                                                      const gvdbx_node* __restrict__ nodes0,
-                                                      const int4* __restrict__ slot_pos, int apron_rt,
-                                                      const int* __restrict__ nbr, float* __restrict__ chan,
-                                                      VelOut vo) {
+__device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t m, int64_t nsc, int64_t len,
+                                                   const int* __restrict__ map_leaf, const int* __restrict__ off,
+                                                   const float* __restrict__ list_pos,
+                                                   const int* __restrict__ leaf_slot,
+                                                   const gvdbx_node* __restrict__ nodes0,
+                                                   const int4* __restrict__ slot_pos, int apron_rt,
+                                                   const int* __restrict__ nbr, float* __restrict__ chan,
+                                                   const VelOut& vo) {
   // LDS sized for the compile-time geometry when there is one (more bricks fit on a CU)
   constexpr int kVox = kRes ? (kRes + 2 * kApron) * (kRes + 2 * kApron) * (kRes + 2 * kApron) : kMaxPaddedVox;
   constexpr int kOff = kRes ? kD * kD * kD + 1 : 513;
@@ -738,8 +765,6 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
   const int br = res0 + 2 * apron;        // channel brick layout
   const int bw = res0 + 2 * a;            // computed region
   const int D3 = D * D * D;
-  const int64_t m = item_index();
-  if (m >= nmap) return;
   // brick corner and slot: with the slot map (slot_pos given) brick m is slot m, one load
   int slot, gx0, gy0, gz0;
   if (slot_pos) {
@@ -759,7 +784,7 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
   const int nv = bw * bw * bw;
   const int64_t k0 = (int64_t)m * D3;
   for (int t = threadIdx.x; t < nv; t += blockDim.x) d2[t] = 0x7F800000u;
-  for (int s = threadIdx.x; s <= D3; s += blockDim.x) soff[s] = (k0 + s < nsc) ? off[k0 + s] : (int)len;
+  for (int s = threadIdx.x; s <= D3; s += blockDim.x) soff[s] = off[k0 + s];  // off[nsc] = the list length
   if (threadIdx.x < 27) nb[threadIdx.x] = (kWithApron && slot >= 0) ? nbr[(int64_t)slot * 27 + threadIdx.x] : -1;
   __syncthreads();
   const int bx = gx0 & c.cmask, by = gy0 & c.cmask, bz = gz0 & c.cmask;
@@ -884,16 +909,31 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
   }
 }
 
+template <int kRes, int kApron, int kD, bool kWithApron, bool kVel, bool kPersist>
+__global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len,
+                                                      const int* __restrict__ map_leaf, const int* __restrict__ off,
+                                                      const float* __restrict__ list_pos,
+                                                      const int* __restrict__ leaf_slot,
+                                                      const gvdbx_node* __restrict__ nodes0,
+                                                      const int4* __restrict__ slot_pos, int apron_rt,
+                                                      const int* __restrict__ nbr, float* __restrict__ chan,
+                                                      VelOut vo, int* queue) {
+  for_each_brick<kPersist>(nmap, queue, [&](int m) {
+    density_brick_body<kRes, kApron, kD, kWithApron, kVel>(c, m, nsc, len, map_leaf, off, list_pos, leaf_slot,
+                                                           nodes0, slot_pos, apron_rt, nbr, chan, vo);
+  });
+}
+
 void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
                    const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, const int4* slot_pos,
-                   int apron, const int* nbr, float* chan, const VelOut* vel, hipStream_t s) {
+                   int apron, const int* nbr, float* chan, const VelOut* vel, const LaunchCfg& L, hipStream_t s) {
   if (nmap <= 0) return;
-  const int bs = brick_threads(len, nmap, 256);
+  const int bs = brick_threads(L, len, nmap, 256);
   const bool fixed = c.res0 == 8 && c.D == 2 && apron == 1;
   const VelOut vo = vel ? *vel : VelOut{nullptr, nullptr, {nullptr, nullptr, nullptr}};
 #define GVDBX_DENSITY(R, A, D, AP, V)                                                                             \
-  hipLaunchKernelGGL((k_density_brick<R, A, D, AP, V>), item_grid(nmap), dim3(bs), 0, s, c, nmap, nsc, len, map_leaf, \
-                     off, list_pos, leaf_slot, nodes0, slot_pos, apron, nbr, chan, vo)
+  launch_bricks(k_density_brick<R, A, D, AP, V, false>, k_density_brick<R, A, D, AP, V, true>, L, bs, 0, nmap, s, c, \
+                nmap, nsc, len, map_leaf, off, list_pos, leaf_slot, nodes0, slot_pos, apron, nbr, chan, vo, L.queue)
   if (vel) {  // velocity means need the apron pass (a neighbour table)
     if (fixed) GVDBX_DENSITY(8, 1, 2, true, true);
     else GVDBX_DENSITY(0, 0, 0, true, true);

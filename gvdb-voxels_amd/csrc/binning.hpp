@@ -17,6 +17,7 @@ void home_keys(const DevTree& T, const PointsView& P, const int* map, int nmap, 
 void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStream_t s);
 
 struct BinArgs {
+  LaunchCfg L;   // launch shape of the count and insert kernels
   DevTree T;
   PointsView P;
   SubcellCfg c;
@@ -73,7 +74,7 @@ struct VelOut {
 // slot_pos: the slots' corners when the subcell map is the slot map (brick m = slot m), or null
 void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* map_leaf, const int* off,
                    const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, const int4* slot_pos,
-                   int apron, const int* nbr, float* chan, const VelOut* vel, hipStream_t s);
+                   int apron, const int* nbr, float* chan, const VelOut* vel, const LaunchCfg& L, hipStream_t s);
 void velocity_brick(SubcellCfg c, int nmap, const int* map_leaf, const int* cnt, const unsigned long long* vsum,
                     const int* leaf_slot, int apron, const int* nbr, float* cx, float* cy, float* cz, int accum,
                     hipStream_t s);

@@ -61,6 +61,41 @@ __device__ __forceinline__ int64_t xcd_item(int64_t n) {
   return (w % kXcds) * chunk + w / kXcds;
 }
 
+// Launch shape of the per-brick kernels (count, insert, density gather).
+struct LaunchCfg {
+  int block;    // threads per brick workgroup (64, 128 or 256); 0 = chosen from the items per brick
+  int persist;  // 1: a grid of resident workgroups takes bricks from a queue
+  int* queue;   // the queue counter (one device int), cleared before every persistent launch
+  int cus;      // compute units of the device (the persistent grid is a multiple of it)
+};
+
+// The bricks of a per-brick kernel: one per workgroup (grid form, m = the item index), or,
+// persistent, taken from the queue kBrickChunk at a time until it passes n.  The body may return
+// early only uniformly over the workgroup; a barrier follows every brick, so the body may reuse
+// its LDS from one brick to the next.  Every workgroup leaves once the queue passes n.
+constexpr int kBrickChunk = 4;
+template <bool kPersist, typename F>
+__device__ __forceinline__ void for_each_brick(int64_t n, int* queue, F&& body) {
+  if (!kPersist) {
+    const int64_t m = item_index();
+    if (m < n) body((int)m);
+    return;
+  }
+  __shared__ int first;
+  for (;;) {
+    if (threadIdx.x == 0) first = atomicAdd(queue, kBrickChunk);
+    __syncthreads();
+    const int f = first;
+    __syncthreads();
+    if ((int64_t)f >= n) return;
+    const int last = (int64_t)f + kBrickChunk < n ? f + kBrickChunk : (int)n;
+    for (int m = f; m < last; ++m) {
+      body(m);
+      __syncthreads();
+    }
+  }
+}
+
 __device__ __forceinline__ uint64_t elem0(int lev, uint64_t ndx) { return ((uint64_t)lev << 8) | (ndx << 16); }
 
 __device__ __forceinline__ bool in_root(const DevTree& T, int x, int y, int z) {

@@ -30,6 +30,16 @@ void fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill<uint64_t>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
 }
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void copy_u32(const uint32_t* src, uint32_t* dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, src, dst, n);
+}
+
 void fill_f32(float* p, int64_t n, float v, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);

@@ -10,6 +10,8 @@ constexpr int kRadixTile = 4096;
 void fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 void fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
 void fill_f32(float* p, int64_t n, float v, hipStream_t s);
+// device-to-device copy by a kernel (records into a graph as a kernel node)
+void copy_u32(const uint32_t* src, uint32_t* dst, int64_t n, hipStream_t s);
 
 // Exclusive scan of n int32 values (in may equal out).  tile_buf holds scan_tile_count(n)
 // ints; *total (device) receives the sum when non-null.

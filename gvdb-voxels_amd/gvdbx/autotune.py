@@ -1,18 +1,24 @@
-"""Autotuner of the batched-cloud path: picks the leaf brick size and the subcell size.
+"""Autotuner of the batched-cloud path: picks the leaf brick size, the subcell size and the launch
+shape of the per-brick kernels (threads per workgroup, grid or persistent form).
 
 Every candidate gives the exact field of its own topology (the gathers are exact for any
-brick / subcell size), so the choice is purely one of speed: each candidate runs the batched
-insert + gather + apron sequence a few times on the actual batch and the fastest wins.  The
-cell of each cloud is the smallest inner-level range that holds every cloud for that brick
-size (gvdbx_batch_set_points rejects a cell that is too small).
+brick / subcell size, and every launch shape gives the same bits), so the choice is purely one
+of speed.  For each (brick, subcell) pair the batch's topology is built once; then, per launch
+shape, InsertPointsSubcell + GatherDensity + UpdateApron is recorded as one graph and launched
+a few times, and the fastest launch wins.  The cell of each cloud is the smallest inner-level
+range that holds every cloud for that brick size (gvdbx_batch_set_points rejects a cell that is
+too small).
 """
 import time
 
 from .lib import GvdbxError
 from .volume import VolumeGVDB
 
-# (leaf log2, subcell) candidates; inner levels stay at 3 (8^3 children) and the root at 5
-CANDIDATES = ((3, 4), (3, 2), (2, 2), (2, 4))
+# (leaf log2, subcell) candidates: 8^3, 16^3 and 4^3 bricks; inner levels stay at 3 (8^3
+# children) and the root at 5
+CANDIDATES = ((3, 4), (3, 2), (4, 4), (4, 8), (2, 2), (2, 4))
+# (threads per brick workgroup, persistent); 0 = chosen from the items per brick
+SHAPES = ((0, False), (64, False), (128, False), (256, False), (64, True), (128, True), (256, True))
 
 
 def levels_for(leaf_log2):
@@ -35,10 +41,38 @@ def run_batch(v, n, radius, subcell, chan=0):
     UpdateApron; returns the subcell-list length."""
     v.RebuildTopology(n, radius)
     v.UpdateAtlas()
+    return insert_gather_apron(v, n, radius, subcell, chan)
+
+
+def insert_gather_apron(v, n, radius, subcell, chan=0):
+    """InsertPointsSubcell + GatherDensity + UpdateApron on the current topology."""
     ln = v.InsertPointsSubcell(subcell, n, radius)
     v.GatherDensity(subcell, n, radius, (0, 0, 0), ln, chan)
     v.UpdateApron(chan, 0.0)
     return ln
+
+
+def record_step(v, n, radius, subcell, chan=0):
+    """Run insert + gather + apron once (sizing every buffer), then record it as the volume's
+    graph (VolumeGVDB.GraphLaunch runs it)."""
+    insert_gather_apron(v, n, radius, subcell, chan)
+    v.GraphBegin()
+    try:
+        insert_gather_apron(v, n, radius, subcell, chan)
+    finally:
+        v.GraphEnd()
+
+
+def time_launches(v, reps):
+    """Best wall time (ms) of `reps` synchronised graph launches."""
+    best = float("inf")
+    for _ in range(reps):
+        v.synchronize()
+        t0 = time.perf_counter()
+        v.GraphLaunch()
+        v.synchronize()
+        best = min(best, (time.perf_counter() - t0) * 1e3)
+    return best
 
 
 def new_batch_volume(leaf_log2, device=0):
@@ -75,10 +109,11 @@ def bind(v, leaf_log2, dpos, counts, radius, max_coord):
     return cell
 
 
-def tune(upload, counts, max_coord, radius=1.0, candidates=CANDIDATES, reps=3, log=None):
+def tune(upload, counts, max_coord, radius=1.0, candidates=CANDIDATES, shapes=SHAPES, reps=3, log=None):
     """upload(v) -> device buffer of the concatenated clouds on volume v; max_coord = the largest
-    valid coordinate (max_valid_coord).  Returns {"leaf_log2", "subcell", "cell_log2", "ms"} of
-    the fastest candidate and every timing."""
+    valid coordinate (max_valid_coord).  Returns {"leaf_log2", "subcell", "block", "persistent",
+    "cell_log2", "ms"} of the fastest candidate (ms = one graph launch of insert + gather +
+    apron) and every timing."""
     n = int(sum(int(c) for c in counts))
     timings = []
     for leaf, sc in candidates:
@@ -88,16 +123,18 @@ def tune(upload, counts, max_coord, radius=1.0, candidates=CANDIDATES, reps=3, l
         try:
             d = upload(v)
             cell = bind(v, leaf, d, counts, radius, max_coord)
-            best = float("inf")
-            for _ in range(reps):
-                v.synchronize()
-                t0 = time.perf_counter()
-                run_batch(v, n, radius, sc)
-                v.synchronize()
-                best = min(best, (time.perf_counter() - t0) * 1e3)
-            timings.append({"leaf_log2": leaf, "subcell": sc, "cell_log2": cell, "ms": round(best, 4)})
-            if log:
-                log(f"autotune leaf 2^{leaf} subcell {sc}: {best:.3f} ms")
+            v.RebuildTopology(n, radius)
+            v.UpdateAtlas()
+            for block, pers in shapes:
+                v.SetLaunch(block, pers)
+                record_step(v, n, radius, sc)
+                ms = time_launches(v, reps)
+                v.GraphRelease()
+                timings.append({"leaf_log2": leaf, "subcell": sc, "block": block, "persistent": bool(pers),
+                                "cell_log2": cell, "ms": round(ms, 4)})
+                if log:
+                    log(f"autotune leaf 2^{leaf} subcell {sc} block {block or 'auto'} "
+                        f"{'persistent' if pers else 'grid'}: {ms:.3f} ms")
         finally:
             v.close()
     best = min(timings, key=lambda t: t["ms"])

@@ -99,6 +99,11 @@ SIGNATURES = {
     "gvdbx_dev_free": [_P, _P],
     "gvdbx_commit": [_P, _P, _P, _I64],
     "gvdbx_retrieve": [_P, _P, _P, _I64],
+    "gvdbx_set_launch": [_P, _I, _I],
+    "gvdbx_graph_begin": [_P],
+    "gvdbx_graph_end": [_P],
+    "gvdbx_graph_launch": [_P],
+    "gvdbx_graph_release": [_P],
     "gvdbx_kernel_timer": [_P, C.c_char_p, _I],
     "gvdbx_kernel_timer_read": [_P, C.POINTER(C.c_double), C.POINTER(_I64)],
 }

@@ -363,6 +363,26 @@ class VolumeGVDB:
             list_q16=grab(8, ln * 3, np.uint16, (ln, 3)), list_vq16=grab(9, ln * 3, np.uint16, (ln, 3)),
         )
 
+    # ------------------------------------------------------------ launch shape and graphs
+    def SetLaunch(self, block_threads=0, persistent=False):
+        """Threads per brick workgroup of the count, insert and density kernels (0 = chosen from
+        the items per brick) and the persistent brick-queue form."""
+        self._chk(self.L.gvdbx_set_launch(self.h, int(block_threads), int(bool(persistent))), "set_launch")
+
+    def GraphBegin(self):
+        """Record the following insert / gather / apron calls into one HIP graph (they run at
+        GraphLaunch); run the same calls once before recording."""
+        self._chk(self.L.gvdbx_graph_begin(self.h), "graph_begin")
+
+    def GraphEnd(self):
+        self._chk(self.L.gvdbx_graph_end(self.h), "graph_end")
+
+    def GraphLaunch(self):
+        self._chk(self.L.gvdbx_graph_launch(self.h), "graph_launch")
+
+    def GraphRelease(self):
+        self._chk(self.L.gvdbx_graph_release(self.h), "graph_release")
+
     # ------------------------------------------------------------ timing
     def kernel_timer(self, name, enable=True):
         self._chk(self.L.gvdbx_kernel_timer(self.h, name.encode() if name else None, int(enable)), "kernel_timer")

@@ -233,7 +233,8 @@ int gvdbx_copy_childlists(gvdbx_ctx* ctx, int lev, uint64_t* host_out, int64_t m
 int gvdbx_get_num_slots(const gvdbx_ctx* ctx, int64_t* slots, int* brick_res);
 int gvdbx_copy_channel(gvdbx_ctx* ctx, int chan, float* host_out, int64_t max_floats);
 int gvdbx_get_channel_ptr(gvdbx_ctx* ctx, int chan, void** dev_ptr, int64_t* floats);
-int gvdbx_get_subcell_info(const gvdbx_ctx* ctx, int64_t* num_subcells, int64_t* list_len, int* subcell);
+/* list_len after a graph launch (gvdbx_graph_launch) is read back from the device (a host wait). */
+int gvdbx_get_subcell_info(gvdbx_ctx* ctx, int64_t* num_subcells, int64_t* list_len, int* subcell);
 /* which: 0 map(int32 per leaf) 1 cnt 2 off 3 pos(int32x3) 4 nid 5 list pos(float x3) 6 list vel(float x3)
  *        7 list colour (uint32 = uchar4) 8 FP16 list pos (uint16 x3) 9 FP16 list vel (uint16 x3) */
 int gvdbx_copy_subcell_array(gvdbx_ctx* ctx, int which, void* host_out, int64_t max_bytes);
@@ -283,6 +284,33 @@ int gvdbx_dev_alloc(gvdbx_ctx* ctx, int64_t bytes, void** dev_ptr);
 int gvdbx_dev_free(gvdbx_ctx* ctx, void* dev_ptr);
 int gvdbx_commit(gvdbx_ctx* ctx, void* dev_dst, const void* host_src, int64_t bytes);
 int gvdbx_retrieve(gvdbx_ctx* ctx, void* host_dst, const void* dev_src, int64_t bytes);
+
+/* ---- launch shape of the per-brick kernels (InsertPointsSubcell's count and insert passes, the
+ *      density gather).  No reference counterpart (the reference launches fixed CUDA blocks).
+ *      block_threads: 64, 128 or 256 threads per brick workgroup, or 0 to choose from the items
+ *      per brick (one wave for sparse bricks, four otherwise).  persistent = 1: a grid of
+ *      resident workgroups takes bricks from a device queue, four at a time, instead of one
+ *      workgroup per brick.  Results are the same for every shape. */
+int gvdbx_set_launch(gvdbx_ctx* ctx, int block_threads, int persistent);
+
+/* ---- graphs: the calls made between gvdbx_graph_begin and gvdbx_graph_end are recorded on the
+ *      context stream as one HIP graph instead of running; gvdbx_graph_launch runs them all
+ *      with one launch.  No reference counterpart (the reference syncs inside its calls).
+ *      Recordable: gvdbx_insert_points_subcell, the gathers (density, level set, velocity,
+ *      density + velocity, colour), gvdbx_scatter_reduce_levelset, gvdbx_update_apron(_channels),
+ *      gvdbx_fill_channel(4), gvdbx_advance_points.  A recorded insert sizes its lists for the
+ *      largest possible length and leaves the length on the device (gvdbx_get_subcell_info
+ *      reads it back after a launch).  The calls must have run once with the same arguments
+ *      before recording, so that no buffer has to grow (GVDBX_ERR_STATE otherwise, as for any
+ *      call that waits for the device or changes the topology or bindings; such a failure voids
+ *      the recording, and gvdbx_graph_end returns it).  The host-side decisions of the recorded
+ *      calls are made at recording; their effects take place at each launch.  A launch is
+ *      refused (GVDBX_ERR_STATE) once the topology, atlas, bound points or any buffer the graph
+ *      uses has changed since the recording; record again then.  One graph per context. */
+int gvdbx_graph_begin(gvdbx_ctx* ctx);
+int gvdbx_graph_end(gvdbx_ctx* ctx);
+int gvdbx_graph_launch(gvdbx_ctx* ctx);
+int gvdbx_graph_release(gvdbx_ctx* ctx);
 
 /* ---- timing of one named kernel with HIP events on the context stream.
  *      kernel: "gather", "insert", "count", "apron", "keys", "sort" ... ; enable=0 stops. */
