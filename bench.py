@@ -64,6 +64,14 @@ def algorithmic_bytes(n, s_len, nsc, bricks, chans=4, vel=True):
 APRON_VOX = 10 ** 3 - 8 ** 3  # apron voxels of a padded 8^3 brick with apron 1
 
 
+def design_step_bytes(n, s_len, nsc, bricks, chans=4):
+    """HBM bytes this design must move per step: positions and velocities read once, the binned
+    positions written and read once (no velocity lists: the velocity means come from per-subcell
+    sums), per-subcell count and offset, the 64-bit velocity sums written and read, and every
+    channel's padded brick written once (the gather writes the apron too, so no apron copy)."""
+    return n * 24 + s_len * 24 + nsc * 8 + nsc * 48 + chans * bricks * 1000 * 4
+
+
 def gather_bytes(s_len, nsc, bricks, vel=True, apron=True):
     """Algorithmic bytes of one gather launch: list positions read, per-subcell
     count/offset/corner/leaf read, interior voxels written; with the velocity means in the same
@@ -416,15 +424,15 @@ def calibrate_copy():
     print(json.dumps({"calibration_copy_bytes": n * 4, "copies": 3}))
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of the roofline kernel from the committed counter summary
-    (profiles/pmc_traffic.json: FETCH_SIZE and WRITE_SIZE of the same workload, each scaled
-    by its reading for a copy of known size), or None."""
+def measured_traffic_ratio(kernel):
+    """HBM bytes over algorithmic bytes of one launch of the roofline kernel, from the committed
+    counter summary (profiles/pmc_traffic.json: the memory-side read and write requests of the
+    same workload's last step, by request size), or None."""
     f = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(f):
         return None
     e = json.load(open(f)).get("kernels", {}).get(kernel)
-    return int(e["bytes_per_launch"]) if e else None
+    return round(e["bytes"] / e["algorithmic_bytes"], 4) if e and e.get("algorithmic_bytes") else None
 
 
 def main():
@@ -441,6 +449,10 @@ def main():
     ap.add_argument("--batched-only", action="store_true",
                     help="only time the 256-cloud batch (8^3 bricks, subcell 4; no autotune, no per-cloud runs), "
                          "for kernel profiles")
+    ap.add_argument("--no-late", action="store_true", help="skip the late window (steps 40-59)")
+    ap.add_argument("--levelset-only", action="store_true",
+                    help="only time the 16M-point level set (rebuild, insert, ScatterReduceLevelSet, apron, "
+                         "queries), for kernel profiles")
     ap.add_argument("--kernel", default="gather", help="kernel family timed for the roofline")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the counter-calibration copy")
     args = ap.parse_args()
@@ -449,6 +461,9 @@ def main():
         return
     if args.batched_only:
         print(json.dumps({"batched": batched_bench(args.steps, tune=False, per_cloud=False)}))
+        return
+    if args.levelset_only:
+        print(json.dumps({"levelset": levelset_bench(args.steps)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -479,18 +494,18 @@ def main():
     # timed region: K steps, kernel-family events recorded on the library stream
     v.kernel_timer(args.kernel, True)
     t0 = time.perf_counter()
-    lens = []
+    shapes = []  # (list length, subcells, bricks) of every timed step: host-side values, no wait
     for k in range(args.warmup, total):
-        lens.append(run_step(v, n, True))
+        ln = run_step(v, n, True)
+        shapes.append((ln, v.subcell_info()[0], v.getNumUsedNodes(0)))
     v.synchronize()
     t1 = time.perf_counter()
     k_ms, k_launches = v.kernel_timer_read()
     v.kernel_timer(None, False)
-    nsc, s_len, _ = v.subcell_info()
-    bricks = v.getNumUsedNodes(0)
+    s_len, nsc, bricks = shapes[-1]
 
     late = None
-    if total <= LATE_FIRST:
+    if total <= LATE_FIRST and not args.no_late:
         for k in range(total, LATE_FIRST):
             run_step(v, n, True)
         v.synchronize()
@@ -508,11 +523,16 @@ def main():
 
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3) / 1e6
-    step_bytes = algorithmic_bytes(n, s_len, nsc, bricks)
-    kb = gather_bytes(s_len, nsc, bricks) if args.kernel == "gather" else None
-    kb_interior = gather_bytes(s_len, nsc, bricks, apron=False) if args.kernel == "gather" else None
+    # byte counts are averaged over the timed steps (the topology grows from step to step)
+    mean = lambda f: sum(f(*x) for x in shapes) / len(shapes)
+    step_bytes = mean(lambda sl, ns, b: algorithmic_bytes(n, sl, ns, b))
+    design_bytes = mean(lambda sl, ns, b: design_step_bytes(n, sl, ns, b))
+    gather = args.kernel == "gather"
+    kb = mean(gather_bytes) if gather else None
+    kb_interior = mean(lambda sl, ns, b: gather_bytes(sl, ns, b, apron=False)) if gather else None
     k_avg_ms = k_ms / max(k_launches, 1)
     achieved = kb / (k_avg_ms * 1e-3) / 1e9 if kb else None
+    ratio = measured_traffic_ratio("k_density_brick") if gather else None
 
     out = {
         "metric": "Mpoints/s insert+gather+apron (SPH step, 4M particles)",
@@ -533,10 +553,13 @@ def main():
                    "bricks_last_step": int(bricks), "subcell_entries_last_step": int(s_len)},
         "step_algorithmic_bytes": int(step_bytes),
         "step_hbm_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "step_design_bytes": int(design_bytes),
+        "step_design_hbm_frac": round(design_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": args.kernel,
                      "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": measured_traffic("k_density_brick") if args.kernel == "gather" else None,
+                     "traffic": int(kb * ratio) if ratio else None,
+                     "traffic_over_algorithmic": ratio,
                      "avg_launch_ms": round(k_avg_ms, 5), "launches": int(k_launches),
                      "algorithmic_bytes_per_launch": int(kb) if kb else None,
                      "algorithmic_bytes_interiors_only": int(kb_interior) if kb_interior else None},

@@ -202,6 +202,10 @@ struct gvdbx_ctx {
   int64_t sclen_hint = 0;        // the last list length known on the host
   bool graph_insert = false;     // the recording holds an insert (its length is graph_len)
   uint64_t buf_gen = 0;          // counts reallocations: a graph refers to the buffers of its recording
+  // buffers replaced by larger ones: kept until the context ends (or memory runs short), so a
+  // reallocation needs no wait for the work still using them
+  std::vector<void*> retired;
+  size_t retired_bytes = 0;
   uint64_t graph_buf_gen = 0;
 
   // named-kernel timer: event pairs recorded on the stream, summed when read
@@ -256,20 +260,46 @@ int launch_status(gvdbx_ctx* c, const char* what) {
 
 // Grow-only buffer; contents are not kept.  Work already enqueued may still read the old
 // buffer, so the stream is drained before it is freed (growth is rare in steady state).
+// Frees the retired buffers once the stream has finished the work that may use them.
+int release_retired(gvdbx_ctx* c) {
+  if (c->retired.empty()) return GVDBX_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (void* p : c->retired) (void)hipFree(p);
+  c->retired.clear();
+  c->retired_bytes = 0;
+  return GVDBX_OK;
+}
+
+// hipMalloc, retrying once after freeing the retired buffers
+hipError_t dev_malloc(gvdbx_ctx* c, void** p, size_t nb) {
+  hipError_t e = hipMalloc(p, nb);
+  if (e != hipSuccess && !c->retired.empty()) {
+    (void)hipGetLastError();
+    if (release_retired(c) == GVDBX_OK) e = hipMalloc(p, nb);
+  }
+  return e;
+}
+
+// The old buffer of a reallocation stays valid for the work already enqueued on the stream.
+int retire(gvdbx_ctx* c, void* p, size_t bytes) {
+  c->retired.push_back(p);
+  c->retired_bytes += bytes;
+  if (c->retired_bytes > c->list_budget) return release_retired(c);
+  return GVDBX_OK;
+}
+
+// Grow-only buffer; contents are not kept.
 int ensure(gvdbx_ctx* c, Buf& b, size_t bytes) {
   if (bytes <= b.cap) return GVDBX_OK;
   if (c->recording)
     return fail(c, GVDBX_ERR_STATE, "a buffer would grow while recording: run the same calls once before recording");
   size_t nb = b.cap ? b.cap : 256;
   while (nb < bytes) nb += nb / 2 + 256;
-  if (b.p) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipFree(b.p));
-  }
+  if (b.p) CHK(retire(c, b.p, b.cap));
   b.p = nullptr;
   b.cap = 0;
   c->buf_gen++;
-  hipError_t e = hipMalloc(&b.p, nb);
+  hipError_t e = dev_malloc(c, &b.p, nb);
   if (e != hipSuccess) {
     b.p = nullptr;
     (void)hipGetLastError();
@@ -286,19 +316,18 @@ int ensure_keep(gvdbx_ctx* c, Buf& b, size_t bytes, size_t keep) {
   size_t nb = b.cap ? b.cap : 256;
   while (nb < bytes) nb *= 2;
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, nb);
+  hipError_t e = dev_malloc(c, &p, nb);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return fail(c, GVDBX_ERR_NOMEM, "hipMalloc(%zu) failed: %s", nb, hipGetErrorString(e));
   }
   c->buf_gen++;
   if (keep && b.p) HIPCHK(c, hipMemcpyAsync(p, b.p, keep, hipMemcpyDeviceToDevice, c->stream));
-  if (b.p) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipFree(b.p));
-  }
+  void* old = b.p;
+  const size_t old_cap = b.cap;
   b.p = p;
   b.cap = nb;
+  if (old) CHK(retire(c, old, old_cap));  // the copy above still reads it
   return GVDBX_OK;
 }
 
@@ -1059,6 +1088,7 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
                  &c->cloud_off_dev, &c->queue, &c->graph_len};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
+  for (void* p : c->retired) (void)hipFree(p);
   for (int l = 0; l < kMaxLev; ++l) {
     if (c->nodes[l].p) (void)hipFree(c->nodes[l].p);
     if (c->clist[l].p) (void)hipFree(c->clist[l].p);

@@ -553,10 +553,14 @@ void sum_counts(const int* a, const int* b, int* out, int64_t n, hipStream_t s) 
   hipLaunchKernelGGL(k_sum_counts, dim3(nblocks(n, 256)), dim3(256), 0, s, a, b, out, n);
 }
 
-// Sorted copies of the points (coalesced reads for the per-brick kernels).
+// Sorted copies of the points (coalesced reads for the per-brick kernels).  Neighbouring
+// bricks' points share cache lines of the caller's arrays, so consecutive blocks of sorted
+// points go to one XCD (xcd_item), whose L2 then fetches each line once.
 __global__ void k_gather_sorted(PointsView P, const uint32_t* __restrict__ svals, int64_t n, float4* __restrict__ spos,
-                                float4* __restrict__ svel) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                float4* __restrict__ svel, int64_t blocks) {
+  const int64_t blk = xcd_item(blocks);
+  if (blk >= blocks) return;
+  const int64_t j = blk * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = svals[j];
   const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
@@ -569,7 +573,9 @@ __global__ void k_gather_sorted(PointsView P, const uint32_t* __restrict__ svals
 
 void gather_sorted(const PointsView& P, const uint32_t* svals, int64_t n, float4* spos, float4* svel, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_gather_sorted, dim3(nblocks(n, 256)), dim3(256), 0, s, P, svals, n, spos, svel);
+  const int64_t blocks = nblocks(n, 256);
+  hipLaunchKernelGGL(k_gather_sorted, dim3((unsigned)xcd_items(blocks)), dim3(256), 0, s, P, svals, n, spos, svel,
+                     blocks);
 }
 
 // Walks the bricks whose window reaches subcell g = n*D^3 + l: per axis the brick itself and,
@@ -579,8 +585,11 @@ void gather_sorted(const PointsView& P, const uint32_t* svals, int64_t n, float4
 // bases pass adds the block's prefix to a scan of its own counts (cnt = the counts, out = off).
 template <bool kBases, int kD>
 __global__ void __launch_bounds__(256) k_cell_walk(BinArgs A, int64_t nsc, int* __restrict__ cnt,
-                                                   int* __restrict__ tile, int* __restrict__ off_out) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                   int* __restrict__ tile, int* __restrict__ off_out, int64_t blocks) {
+  // consecutive blocks (neighbouring bricks, whose window counts each walk reads) on one XCD
+  const int64_t blk = xcd_item(blocks);
+  if (blk >= blocks) return;
+  const int64_t g = blk * blockDim.x + threadIdx.x;
   const bool live = g < nsc;
   const int D = kD ? kD : A.c.D, D3 = D * D * D, W = D + 2, W3 = W * W * W;
   // every lane reaches the one block scan of each pass (it holds barriers): the bases pass scans
@@ -589,7 +598,7 @@ __global__ void __launch_bounds__(256) k_cell_walk(BinArgs A, int64_t nsc, int* 
   if (kBases) {
     int tot;
     const int c0 = live ? cnt[g] : 0;
-    base = tile[blockIdx.x] + block_exclusive_scan_256(c0, &tot);
+    base = tile[blk] + block_exclusive_scan_256(c0, &tot);
     if (g == nsc - 1) off_out[nsc] = base + c0;  // the list length closes the offsets
   }
   int p = 0;
@@ -635,26 +644,28 @@ __global__ void __launch_bounds__(256) k_cell_walk(BinArgs A, int64_t nsc, int* 
     }
     int tot;
     (void)block_exclusive_scan_256(total, &tot);
-    if (threadIdx.x == 0) tile[blockIdx.x] = tot;
+    if (threadIdx.x == 0) tile[blk] = tot;
   }
 }
 
 void cell_totals(const BinArgs& A, int64_t nsc, int* cnt, int* tile, hipStream_t s) {
   if (nsc <= 0) return;
+  const int64_t blocks = nblocks(nsc, 256);
+  const dim3 grid((unsigned)xcd_items(blocks));
   if (A.c.D == 2)
-    hipLaunchKernelGGL((k_cell_walk<false, 2>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt, tile,
-                       (int*)nullptr);
+    hipLaunchKernelGGL((k_cell_walk<false, 2>), grid, dim3(256), 0, s, A, nsc, cnt, tile, (int*)nullptr, blocks);
   else
-    hipLaunchKernelGGL((k_cell_walk<false, 0>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt, tile,
-                       (int*)nullptr);
+    hipLaunchKernelGGL((k_cell_walk<false, 0>), grid, dim3(256), 0, s, A, nsc, cnt, tile, (int*)nullptr, blocks);
 }
 
 void cell_bases(const BinArgs& A, int64_t nsc, int* cnt, int* tile, int* off, hipStream_t s) {
   if (nsc <= 0) return;
+  const int64_t blocks = nblocks(nsc, 256);
+  const dim3 grid((unsigned)xcd_items(blocks));
   if (A.c.D == 2)
-    hipLaunchKernelGGL((k_cell_walk<true, 2>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt, tile, off);
+    hipLaunchKernelGGL((k_cell_walk<true, 2>), grid, dim3(256), 0, s, A, nsc, cnt, tile, off, blocks);
   else
-    hipLaunchKernelGGL((k_cell_walk<true, 0>), dim3(nblocks(nsc, 256)), dim3(256), 0, s, A, nsc, cnt, tile, off);
+    hipLaunchKernelGGL((k_cell_walk<true, 0>), grid, dim3(256), 0, s, A, nsc, cnt, tile, off, blocks);
 }
 
 // ------------------------------------------------------------------ per-point path
@@ -751,7 +762,7 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
   // LDS sized for the compile-time geometry when there is one (more bricks fit on a CU)
   constexpr int kVox = kRes ? (kRes + 2 * kApron) * (kRes + 2 * kApron) * (kRes + 2 * kApron) : kMaxPaddedVox;
   constexpr int kOff = kRes ? kD * kD * kD + 1 : 513;
-  __shared__ unsigned int d2[kVox];
+  __shared__ __align__(16) unsigned int d2[kVox];
   __shared__ int soff[kOff];
   __shared__ int nb[27];
   constexpr int kMeanCells = kVel ? (kRes ? (kD + 2) * (kD + 2) * (kD + 2) : kMaxWindowCells) : 1;
@@ -783,7 +794,12 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
   }
   const int nv = bw * bw * bw;
   const int64_t k0 = (int64_t)m * D3;
-  for (int t = threadIdx.x; t < nv; t += blockDim.x) d2[t] = 0x7F800000u;
+  if constexpr (kRes && kWithApron && (kVox & 3) == 0) {  // nv == kVox: 16-byte stores
+    for (int t4 = threadIdx.x; t4 < kVox / 4; t4 += blockDim.x)
+      reinterpret_cast<uint4*>(d2)[t4] = make_uint4(0x7F800000u, 0x7F800000u, 0x7F800000u, 0x7F800000u);
+  } else {
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) d2[t] = 0x7F800000u;
+  }
   for (int s = threadIdx.x; s <= D3; s += blockDim.x) soff[s] = off[k0 + s];  // off[nsc] = the list length
   if (threadIdx.x < 27) nb[threadIdx.x] = (kWithApron && slot >= 0) ? nbr[(int64_t)slot * 27 + threadIdx.x] : -1;
   __syncthreads();
@@ -884,6 +900,52 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
   __syncthreads();
   if (slot < 0) return;
   float* bptr = chan + (int64_t)slot * br * br * br;
+  if constexpr (kRes && kWithApron && (kVox & 3) == 0) {
+    // fixed geometry with the apron: the computed region is the whole padded brick, so voxels
+    // t..t+3 are consecutive in LDS and in the channel (16-byte loads and stores)
+    constexpr int kBw = kRes + 2 * kApron;
+    constexpr int lgsc = kRes ? 31 - __builtin_clz(kRes / kD) : 0;
+    constexpr int W = kD + 2;
+    for (int t4 = threadIdx.x; t4 < kVox / 4; t4 += blockDim.x) {
+      const uint4 u4 = reinterpret_cast<const uint4*>(d2)[t4];
+      const unsigned int u[4] = {u4.x, u4.y, u4.z, u4.w};
+      float val[4], vm[3][4];
+      // padded coordinates of voxel 4 t4, then one step along x (carrying into y, z) per voxel
+      const int t0 = 4 * t4, zy0 = t0 / kBw;
+      int px = t0 - zy0 * kBw, py = zy0 % kBw, pz = zy0 / kBw;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (e) {
+          if (++px == kBw) {
+            px = 0;
+            if (++py == kBw) {
+              py = 0;
+              ++pz;
+            }
+          }
+        }
+        const int x = px - kApron, y = py - kApron, z = pz - kApron;
+        const int ox = x < 0 ? -1 : (x >= kRes ? 1 : 0), oy = y < 0 ? -1 : (y >= kRes ? 1 : 0),
+                  oz = z < 0 ? -1 : (z >= kRes ? 1 : 0);
+        // apron voxel without a neighbour brick: boundary value 0
+        const bool missing = (ox | oy | oz) != 0 && nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] < 0;
+        val[e] = (missing || u[e] == 0x7F800000u) ? 0.0f : fmaxf(0.0f, c.r - sqrtf(__uint_as_float(u[e])));
+        if (kVel) {
+          const int w = (((z >> lgsc) + 1) * W + ((y >> lgsc) + 1)) * W + ((x >> lgsc) + 1);
+#pragma unroll
+          for (int q = 0; q < 3; ++q) vm[q][e] = missing ? 0.0f : vmean[w][q];
+        }
+      }
+      reinterpret_cast<float4*>(bptr)[t4] = make_float4(val[0], val[1], val[2], val[3]);
+      if (kVel) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          reinterpret_cast<float4*>(vo.ch[q] + (int64_t)slot * kVox)[t4] =
+              make_float4(vm[q][0], vm[q][1], vm[q][2], vm[q][3]);
+      }
+    }
+    return;
+  }
   for (int t = threadIdx.x; t < nv; t += blockDim.x) {
     const int zy = t / bw;
     const int x = t - zy * bw - a, z = zy / bw - a, y = zy - (zy / bw) * bw - a;

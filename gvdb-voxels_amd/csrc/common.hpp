@@ -58,6 +58,7 @@ inline int64_t xcd_items(int64_t n) { return ((n + kXcds - 1) / kXcds) * kXcds; 
 __device__ __forceinline__ int64_t xcd_item(int64_t n) {
   const int64_t w = item_index();
   const int64_t chunk = (n + kXcds - 1) / kXcds;
+  if (w >= kXcds * chunk) return n;  // the surplus of a grid rounded up further
   return (w % kXcds) * chunk + w / kXcds;
 }
 

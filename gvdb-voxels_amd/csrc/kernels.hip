@@ -141,17 +141,36 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
       int64_t cur[8];
       bool act[8], walk[8];
       bool any_walk = false;
+      // Per axis, candidate j (coordinate lo + j R0): inside the box and the root, and its offset
+      // into the lookup table (in_root and table_leaf, taken apart by axis).
+      const int Rr = T.lgR[T.depth], s0 = T.lgR[0];
+      bool ok[3][2], tin[3][2];
+      int64_t toff[3][2];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int64_t stride = a == 0 ? 1 : (a == 1 ? (int64_t)T.bm_n[0] : (int64_t)T.bm_n[0] * T.bm_n[1]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int c = lo[a] + j * R0;
+          ok[a][j] = c <= hi[a] && c >= T.root[a] && ((c - T.root[a]) >> Rr) == 0;
+          const unsigned t = (unsigned)((c >> s0) - T.bm_lo[a]);
+          tin[a][j] = t < (unsigned)T.bm_n[a];
+          toff[a][j] = (int64_t)t * stride;
+        }
+      }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        cx[k] = lo[0] + (k & 1) * R0;
-        cy[k] = lo[1] + ((k >> 1) & 1) * R0;
-        cz[k] = lo[2] + (k >> 2) * R0;
-        act[k] = cx[k] <= hi[0] && cy[k] <= hi[1] && cz[k] <= hi[2] && in_root(T, cx[k], cy[k], cz[k]);
+        const int jx = k & 1, jy = (k >> 1) & 1, jz = k >> 2;
+        cx[k] = lo[0] + jx * R0;
+        cy[k] = lo[1] + jy * R0;
+        cz[k] = lo[2] + jz * R0;
+        act[k] = ok[0][jx] && ok[1][jy] && ok[2][jz];
         cur[k] = 0;
         miss[k] = -1;
         // a leaf found in the lookup table needs no descent; a missing one still descends, to
         // find the level where its chain breaks
-        const int64_t lf = act[k] ? table_leaf(T, cx[k], cy[k], cz[k]) : -2;
+        const bool in_tab = T.bmap && tin[0][jx] && tin[1][jy] && tin[2][jz];
+        const int64_t lf = (act[k] && in_tab) ? T.bmap[toff[2][jz] + toff[1][jy] + toff[0][jx]] : -2;
         if (lf >= 0) cur[k] = lf;
         walk[k] = act[k] && lf < 0;
         any_walk = any_walk || walk[k];
@@ -844,9 +863,37 @@ __global__ void __launch_bounds__(256) k_advance(char* __restrict__ pos, int pos
   }
 }
 
+// Packed float3 arrays, both 16-byte aligned: the same arithmetic over m floats, four per lane.
+__global__ void __launch_bounds__(256) k_advance_flat(float* __restrict__ q, const float* __restrict__ v, int64_t m,
+                                                     float dt) {
+  const int64_t m4 = m / 4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m4; i += stride) {
+    float4 a = reinterpret_cast<float4*>(q)[i];
+    const float4 b = reinterpret_cast<const float4*>(v)[i];
+    a.x = __fadd_rn(a.x, __fmul_rn(b.x, dt));
+    a.y = __fadd_rn(a.y, __fmul_rn(b.y, dt));
+    a.z = __fadd_rn(a.z, __fmul_rn(b.z, dt));
+    a.w = __fadd_rn(a.w, __fmul_rn(b.w, dt));
+    reinterpret_cast<float4*>(q)[i] = a;
+  }
+  const int64_t i = 4 * m4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // the last m % 4 floats
+  if (i < m) q[i] = __fadd_rn(q[i], __fmul_rn(v[i], dt));
+}
+
 void advance_points(char* pos, int pos_off, int pos_stride, const char* vel, int vel_off, int vel_stride, int64_t n,
                     float dt, hipStream_t s) {
   if (n <= 0) return;
+  char* q = pos + pos_off;
+  const char* v = vel + vel_off;
+  if (pos_stride == 12 && vel_stride == 12 && ((uintptr_t)q & 15) == 0 && ((uintptr_t)v & 15) == 0) {
+    const int64_t m = 3 * n;
+    int64_t g = (m / 4 + 255) / 256;
+    if (g > 16384) g = 16384;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_advance_flat, dim3((unsigned)g), dim3(256), 0, s, (float*)q, (const float*)v, m, dt);
+    return;
+  }
   int64_t g = (n + 255) / 256;
   if (g > 65536) g = 65536;
   hipLaunchKernelGGL(k_advance, dim3((unsigned)g), dim3(256), 0, s, pos, pos_off, pos_stride, vel, vel_off,
