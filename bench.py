@@ -450,6 +450,9 @@ def main():
                     help="only time the 256-cloud batch (8^3 bricks, subcell 4; no autotune, no per-cloud runs), "
                          "for kernel profiles")
     ap.add_argument("--no-late", action="store_true", help="skip the late window (steps 40-59)")
+    ap.add_argument("--block", type=int, default=0,
+                    help="threads per brick workgroup of the count, insert and density kernels (0: chosen from "
+                         "the items per brick)")
     ap.add_argument("--levelset-only", action="store_true",
                     help="only time the 16M-point level set (rebuild, insert, ScatterReduceLevelSet, apron, "
                          "queries), for kernel profiles")
@@ -486,6 +489,7 @@ def main():
     dpos = v.to_device(p0)
     dvel = v.to_device(vel)
     v.SetPoints(dpos, dvel, n=n)
+    v.SetLaunch(args.block, False)
 
     for k in range(args.warmup):
         run_step(v, n, k > 0)
@@ -547,7 +551,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded jittered lattice, N(0,1) velocities; displaced on the device every step)",
-        "config": {"workload": "sph_step_4M", "points": n, "box": 256, "fill": 0.4, "dt": 0.5,
+        "config": {"workload": "sph_step_4M", "points": n, "box": 256, "fill": 0.4, "dt": 0.5, "block": args.block,
                    "radius": RADIUS, "subcell": SUBCELL, "log2dims": [3, 3, 3, 3, 5], "channels": 4,
                    "apron": 1, "rebuild": "reference state machine (incremental after 2 full)",
                    "bricks_last_step": int(bricks), "subcell_entries_last_step": int(s_len)},

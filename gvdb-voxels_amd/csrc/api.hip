@@ -152,6 +152,7 @@ struct gvdbx_ctx {
   Buf sc_map, sc_cnt, sc_off, sc_pos, sc_nid, list_pos, list_vel, map_leaf, vsum;
   bool sc_map_current = false;  // sc_map holds the scan map (the insert may have used the slots)
   bool spec_sorted = false;     // incre_launch also sorted the box keys (see there)
+  bool spec_scanned = true;     // incre_launch's box launch scanned its block counts (else it appended its keys)
   Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb, nbm, spos, svel;
   Buf sp_cnt, sp_start, sp_keyrank, sp_payload, sp_perm;  // splat: points counting-sorted by (brick, region)
   Buf scatter_acc;  // ScatterDensity: 64-bit fixed-point sums per padded voxel
@@ -463,11 +464,13 @@ int reserve_level(gvdbx_ctx* c, int l, int64_t add) {
   return GVDBX_OK;
 }
 
-// GetBoundingBox (V.cpp:5404-5467): launch; the result lands in meta[0..5]
-int bounding_box_launch(gvdbx_ctx* c, int64_t n, const float* tr) {
+// GetBoundingBox (V.cpp:5404-5467): launch; the result lands in meta[0..5].  The launch also
+// clears the other meta counters, and the first nmark markers when nmark > 0.
+int bounding_box_launch(gvdbx_ctx* c, int64_t n, const float* tr, int64_t nmark = 0) {
   CHK(ensure(c, c->bbox_part, (size_t)kBBoxParts * 6 * sizeof(float)));
   PointsView pv = points_of(c, n, tr);
-  bbox_two_pass(pv, P<float>(c->bbox_part), P<int>(c->meta), c->stream);
+  bbox_two_pass(pv, P<float>(c->bbox_part), P<int>(c->meta), c->stream, P<uint32_t>(c->meta), M_INTS,
+                nmark > 0 ? P<uint32_t>(c->markers) : nullptr, nmark);
   return GVDBX_OK;
 }
 
@@ -699,8 +702,10 @@ int box_geom(gvdbx_ctx* c, int64_t n, float r, float hi_mul, int d, size_t ksize
 // Enqueue the box kernel for points [start, start + cnt) and the scan of its slice counts
 // (total -> meta[M_COUNT]).
 template <typename K>
+// compact: the keys are also appended to c->keys (up to kSmallSortKeys; their count at M_COUNT)
+// instead of the block counts being scanned (the caller scans them when it needs the slices)
 int box_launch(gvdbx_ctx* c, int64_t start, int64_t cnt, const float* tr, float r, float hi_mul, int d, int* markers,
-               const BoxGeom& g) {
+               const BoxGeom& g, bool compact = false) {
   const int64_t nb = (cnt + 255) / 256;
   CHK(ensure(c, c->box_region, (size_t)nb * 256 * g.per_point * sizeof(K)));
   CHK(ensure(c, c->box_cnt, (size_t)(nb + 1) * 4));
@@ -710,10 +715,16 @@ int box_launch(gvdbx_ctx* c, int64_t start, int64_t cnt, const float* tr, float 
   PointsView pv = points_of(c, cnt, tr);
   pv.pos += start * (int64_t)pv.pos_stride;
   if (pv.cloud) pv.cloud += start;
-  box_keys<K>(tree_of(c), pv, key_fmt(c, d), r, hi_mul, P<K>(c->box_region), 256 * g.per_point, P<int>(c->box_cnt),
-              markers, c->stream);
-  exclusive_scan(P<int>(c->box_cnt), P<int>(c->box_off), nb, P<int>(c->tile_buf), P<int>(c->meta) + M_COUNT,
-                 c->stream);
+  if (compact) {
+    CHK(ensure(c, c->keys, (size_t)kSmallSortKeys * sizeof(K)));
+    box_keys<K>(tree_of(c), pv, key_fmt(c, d), r, hi_mul, P<K>(c->box_region), 256 * g.per_point, P<int>(c->box_cnt),
+                markers, c->stream, P<K>(c->keys), kSmallSortKeys, P<int>(c->meta) + M_COUNT);
+  } else {
+    box_keys<K>(tree_of(c), pv, key_fmt(c, d), r, hi_mul, P<K>(c->box_region), 256 * g.per_point, P<int>(c->box_cnt),
+                markers, c->stream);
+    exclusive_scan(P<int>(c->box_cnt), P<int>(c->box_off), nb, P<int>(c->tile_buf), P<int>(c->meta) + M_COUNT,
+                   c->stream);
+  }
   return GVDBX_OK;
 }
 
@@ -729,6 +740,9 @@ int box_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, float hi_mul, in
     if (!(prelaunched && start == 0)) {
       CHK(box_launch<K>(c, start, cnt, tr, r, hi_mul, d, markers, g));
       CHK(read_meta(c));
+    } else if (!c->spec_scanned) {  // the speculative launch appended its keys instead: scan the slices
+      const int64_t nb = (cnt + 255) / 256;
+      exclusive_scan(P<int>(c->box_cnt), P<int>(c->box_off), nb, P<int>(c->tile_buf), nullptr, c->stream);
     }
     const int64_t m = (uint32_t)c->h_meta[M_COUNT];
     if (total + m > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "more than 2^31 box-pass keys");
@@ -756,19 +770,21 @@ int extra_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d) {  // A
 // Markers and the incremental box pass (hi = trunc(q) + r); launched before the read-back
 // that also returns the bounding box.
 template <typename K>
-int incre_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool accum) {
+int incre_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool accum, bool markers_clear = false) {
   const int64_t total0 = c->cnt[0];
   CHK(ensure(c, c->markers, (size_t)total0 * 4));
   DevTree T = tree_of(c);
   if (accum)
     copy_flags_to_markers(T, total0, P<int>(c->markers), c->stream);
-  else
+  else if (!markers_clear)  // (otherwise the bounding-box launch cleared them)
     fill_u32(P<uint32_t>(c->markers), total0, 0u, c->stream);
   BoxGeom g;
   CHK(box_geom(c, n, r, 1.0f, d, sizeof(K), &g));
   c->spec_sorted = false;
-  if (n > 0) CHK(box_launch<K>(c, 0, g.chunk, tr, r, 1.0f, d, P<int>(c->markers), g));
-  if (n > 0 && g.chunk >= n) {
+  const bool small = n > 0 && g.chunk >= n;
+  if (n > 0) CHK(box_launch<K>(c, 0, g.chunk, tr, r, 1.0f, d, P<int>(c->markers), g, small));
+  c->spec_scanned = !small;
+  if (small) {
     // One box launch covers the points: compact its keys, sort and unique them in one workgroup
     // and count the marked leaves, all reading the key count on the device, so the read-back
     // that returns it also returns the level and used counts.  A count above the one-workgroup
@@ -778,12 +794,10 @@ int incre_launch(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool 
     CHK(ensure(c, c->keys_alt, (size_t)kSmallSortKeys * sizeof(K)));
     CHK(ensure(c, c->flags, (size_t)kSmallSortKeys * 4));
     CHK(ensure(c, c->rank, (size_t)kSmallSortKeys * 4));
-    const int* mdev = P<int>(c->meta) + M_COUNT;
-    compact_regions<K>(P<K>(c->box_region), 256 * g.per_point, P<int>(c->box_cnt), P<int>(c->box_off),
-                       (n + 255) / 256, P<K>(c->keys), kSmallSortKeys, c->stream);
+    // the box launch appended its keys to c->keys (count at M_COUNT); one workgroup sorts them
     sort_unique_small<K>(P<K>(c->keys), 0, key_fmt(c, d), d, P<K>(c->keys_alt), P<int>(c->flags), P<int>(c->rank),
-                         P<int>(c->meta) + M_UNIQUE, P<int>(c->meta) + M_LEV, P<int>(c->meta) + M_USED, mdev,
-                         c->stream);
+                         P<int>(c->meta) + M_UNIQUE, P<int>(c->meta) + M_LEV, P<int>(c->meta) + M_USED,
+                         P<int>(c->meta) + M_COUNT, c->stream);
     count_leaf_flags(T, total0, P<int>(c->markers), P<int>(c->meta) + M_USED, c->stream);
     c->spec_sorted = true;
   }
@@ -852,11 +866,15 @@ int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accum
     if (orig && (orig[0] != 0.f || orig[1] != 0.f || orig[2] != 0.f))
       return fail(c, GVDBX_ERR_ARG, "batched clouds take no translation");
   }
-  CHK(zero_meta(c));
-  if (!c->batch) CHK(bounding_box_launch(c, n, orig));
+  // the bounding-box launch clears the meta counters and, for an incremental pass without
+  // accumulation, its markers
+  const bool bbox_markers = spec && !c->batch && !accumulate;
+  if (bbox_markers) CHK(ensure(c, c->markers, (size_t)c->cnt[0] * 4));
+  if (!c->batch) CHK(bounding_box_launch(c, n, orig, bbox_markers ? c->cnt[0] : 0));
+  else CHK(zero_meta(c));
   if (spec) {
-    if (wide_keys(c, c->depth)) CHK(incre_launch<uint64_t>(c, n, orig, r, c->depth, accumulate));
-    else CHK(incre_launch<uint32_t>(c, n, orig, r, c->depth, accumulate));
+    if (wide_keys(c, c->depth)) CHK(incre_launch<uint64_t>(c, n, orig, r, c->depth, accumulate, bbox_markers));
+    else CHK(incre_launch<uint32_t>(c, n, orig, r, c->depth, accumulate, bbox_markers));
   }
   int root[3] = {0, 0, 0};
   int depth;
@@ -1612,8 +1630,7 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   const int64_t nblk = (nsc + 255) / 256;
   CHK(ensure(c, c->tile_buf, (size_t)(nblk + 16) * 4));
   cell_totals(A, nsc, P<int>(c->sc_cnt), P<int>(c->tile_buf), c->stream);
-  CHK(zero_meta(c));
-  scan_tile_sums(P<int>(c->tile_buf), nblk, P<int>(c->meta) + M_TOTAL, c->stream);
+  scan_tile_sums(P<int>(c->tile_buf), nblk, P<int>(c->meta) + M_TOTAL, c->stream);  // stores the total
   cell_bases(A, nsc, P<int>(c->sc_cnt), P<int>(c->tile_buf), P<int>(c->sc_off), c->stream);
   const bool keep_clr = c->clr != nullptr;  // gvdbInsertSubcell stores colours when bound (P.cuh:268)
   Buf& lpos = fp16 ? c->list_q16 : c->list_pos;
@@ -1638,7 +1655,10 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     if (keep_clr) CHK(ensure(c, c->list_clr, (size_t)(bound + 1) * 4));
   } else {
     const bool wide = bound >= ((int64_t)1 << 32);
-    if (wide) sum_wide(P<int>(c->sc_cnt), nsc, (unsigned long long*)(P<int>(c->meta) + M_WIDE), c->stream);
+    if (wide) {  // the 64-bit sum accumulates into a cleared counter
+      fill_u32(P<uint32_t>(c->meta) + M_WIDE, 2, 0u, c->stream);
+      sum_wide(P<int>(c->sc_cnt), nsc, (unsigned long long*)(P<int>(c->meta) + M_WIDE), c->stream);
+    }
     CHK(read_meta(c));
     if (wide) {
       unsigned long long w;

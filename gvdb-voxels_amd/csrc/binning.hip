@@ -39,7 +39,12 @@ __device__ __forceinline__ void subcell_range(const float q[3], float r, int lgs
 }
 
 // ------------------------------------------------------------------ bounding box
-__global__ void __launch_bounds__(256) k_bbox_partial(PointsView P, float* __restrict__ part) {
+// Also clears clear0[0, n0) and clear1[0, n1) (the step's counters and markers, one launch fewer).
+__global__ void __launch_bounds__(256) k_bbox_partial(PointsView P, float* __restrict__ part, uint32_t* clear0,
+                                                     int64_t n0, uint32_t* clear1, int64_t n1) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gid; i < n0; i += gstride) clear0[i] = 0u;
+  for (int64_t i = gid; i < n1; i += gstride) clear1[i] = 0u;
   float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * blockDim.x) {
     const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
@@ -87,9 +92,10 @@ __global__ void __launch_bounds__(64) k_bbox_final(const float* __restrict__ par
   else if (lane < 6) meta[lane] = float_to_ordered(mx[lane - 3]);
 }
 
-void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s) {
+void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s, uint32_t* clear0, int64_t n0,
+                   uint32_t* clear1, int64_t n1) {
   const int nb = nblocks(P.n, 256) < kBBoxParts ? nblocks(P.n, 256) : kBBoxParts;
-  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, s, P, part);
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, s, P, part, clear0, n0, clear1, n1);
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, part, nb, meta);
 }
 

@@ -7,8 +7,10 @@ namespace gvdbx {
 
 constexpr int kBBoxParts = 1024;
 
-// two-pass bounding box: per-block partials, then one block writes meta[0..5] (ordered ints)
-void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s);
+// two-pass bounding box: per-block partials, then one block writes meta[0..5] (ordered ints);
+// the partial pass also clears clear0[0, n0) and clear1[0, n1) first
+void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s, uint32_t* clear0 = nullptr,
+                   int64_t n0 = 0, uint32_t* clear1 = nullptr, int64_t n1 = 0);
 
 void batch_ids(const PointsView& P, const int64_t* off, int ncloud, float limit, uint32_t* cloud, int* flag,
                hipStream_t s);

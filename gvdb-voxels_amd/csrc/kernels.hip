@@ -97,7 +97,8 @@ void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const
 template <typename K, bool kTwo>
 __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFmt f, float r, float hi_mul,
                                                   K* __restrict__ region, int64_t cap_b, int* __restrict__ blockcnt,
-                                                  int* __restrict__ markers) {
+                                                  int* __restrict__ markers, K* __restrict__ compact, int compact_cap,
+                                                  int* __restrict__ counter) {
   __shared__ int bcount;
   if (threadIdx.x == 0) bcount = 0;
   __syncthreads();
@@ -215,21 +216,31 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
   }
   __syncthreads();
   if (threadIdx.x == 0) blockcnt[blockIdx.x] = bcount;
+  if (compact) {
+    // the keys also go to one packed array: a workgroup that found any claims its range with one
+    // device atomic (few do, the keys of an incremental step come from the points near new
+    // bricks), in no fixed order, which the sort that follows removes
+    __shared__ int base;
+    if (threadIdx.x == 0) base = bcount > 0 ? atomicAdd(counter, bcount) : 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < bcount; j += blockDim.x)
+      if (base + j < compact_cap) compact[base + j] = out[j];
+  }
 }
 
 template <typename K>
 void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi_mul, K* region, int64_t cap_b,
-              int* blockcnt, int* markers, hipStream_t s) {
+              int* blockcnt, int* markers, hipStream_t s, K* compact, int compact_cap, int* counter) {
   if (P.n <= 0) return;
   // a box [trunc(q) - r, trunc(q) + hi_mul r] spans at most 2 leaf nodes per axis when it is
   // shorter than one leaf node
   const bool two = (1.0 + (double)hi_mul) * (double)r + 1.0 < (double)(1 << T.lgR[0]);
   if (two)
     hipLaunchKernelGGL((k_box_keys<K, true>), dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, r, hi_mul, region,
-                       cap_b, blockcnt, markers);
+                       cap_b, blockcnt, markers, compact, compact_cap, counter);
   else
     hipLaunchKernelGGL((k_box_keys<K, false>), dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, f, r, hi_mul,
-                       region, cap_b, blockcnt, markers);
+                       region, cap_b, blockcnt, markers, compact, compact_cap, counter);
 }
 
 template <typename K>
@@ -271,27 +282,14 @@ void unique_flags(const K* k, int64_t m, KeyFmt f, int depth, int* flags, hipStr
 // workgroup (bitonic sort in LDS): the same outputs as radix sort + k_unique_flags + scan +
 // k_level_hist, in one launch instead of about a dozen for the few keys an incremental step
 // adds.  Writes meta_unique = number of unique keys and meta_lev[l] = unique keys of level l.
+// Bitonic sort of s[0, np) in LDS (np a power of two, keys past m set to ~0), then the unique
+// flags, ranks and per-level counts of the first m keys: the tail of both one-workgroup sorts.
 template <typename K>
-__global__ void __launch_bounds__(1024) k_sort_unique_small(const K* __restrict__ in, int m, KeyFmt f, int depth,
-                                                            K* __restrict__ out, int* __restrict__ flags,
-                                                            int* __restrict__ rank, int* __restrict__ meta_unique,
-                                                            int* __restrict__ meta_lev, int* __restrict__ meta_zero,
-                                                            const int* __restrict__ mdev) {
-  __shared__ K s[kSmallSortKeys];
-  __shared__ int lev_h[kMaxLev];
-  __shared__ int ws[16];
+__device__ __forceinline__ void sort_unique_lds(K* s, int m, int np, KeyFmt f, int depth, K* __restrict__ out,
+                                                int* __restrict__ flags, int* __restrict__ rank,
+                                                int* __restrict__ meta_unique, int* __restrict__ meta_lev,
+                                                int* __restrict__ meta_zero, int* lev_h, int* ws) {
   const int t = threadIdx.x;
-  if (mdev) m = *mdev;  // the key count a kernel before wrote (uniform over the workgroup)
-  if (m <= 0 || m > kSmallSortKeys) {  // nothing sorted: the caller sees meta_unique = -1
-    if (t == 0) {
-      *meta_unique = -1;
-      *meta_zero = 0;
-    }
-    return;
-  }
-  int np = 2;
-  while (np < m) np <<= 1;
-  for (int i = t; i < np; i += 1024) s[i] = i < m ? in[i] : ~(K)0;
   if (t < kMaxLev) lev_h[t] = 0;
   __syncthreads();
   for (int k = 2; k <= np; k <<= 1)
@@ -352,6 +350,30 @@ __global__ void __launch_bounds__(1024) k_sort_unique_small(const K* __restrict_
     *meta_zero = 0;  // a counter the caller accumulates next
   }
   if (t < kMaxLev) meta_lev[t] = lev_h[t];
+}
+
+template <typename K>
+__global__ void __launch_bounds__(1024) k_sort_unique_small(const K* __restrict__ in, int m, KeyFmt f, int depth,
+                                                            K* __restrict__ out, int* __restrict__ flags,
+                                                            int* __restrict__ rank, int* __restrict__ meta_unique,
+                                                            int* __restrict__ meta_lev, int* __restrict__ meta_zero,
+                                                            const int* __restrict__ mdev) {
+  __shared__ K s[kSmallSortKeys];
+  __shared__ int lev_h[kMaxLev];
+  __shared__ int ws[16];
+  const int t = threadIdx.x;
+  if (mdev) m = *mdev;  // the key count a kernel before wrote (uniform over the workgroup)
+  if (m <= 0 || m > kSmallSortKeys) {  // nothing sorted: the caller sees meta_unique = -1
+    if (t == 0) {
+      *meta_unique = -1;
+      *meta_zero = 0;
+    }
+    return;
+  }
+  int np = 2;
+  while (np < m) np <<= 1;
+  for (int i = t; i < np; i += 1024) s[i] = i < m ? in[i] : ~(K)0;
+  sort_unique_lds<K>(s, m, np, f, depth, out, flags, rank, meta_unique, meta_lev, meta_zero, lev_h, ws);
 }
 
 template <typename K>
@@ -828,9 +850,9 @@ template void expand_ancestors<uint32_t>(const DevTree&, const uint32_t*, const 
 template void expand_ancestors<uint64_t>(const DevTree&, const uint64_t*, const int*, const int*, int64_t, KeyFmt,
                                          uint64_t*, hipStream_t);
 template void box_keys<uint32_t>(const DevTree&, const PointsView&, KeyFmt, float, float, uint32_t*, int64_t, int*,
-                                 int*, hipStream_t);
+                                 int*, hipStream_t, uint32_t*, int, int*);
 template void box_keys<uint64_t>(const DevTree&, const PointsView&, KeyFmt, float, float, uint64_t*, int64_t, int*,
-                                 int*, hipStream_t);
+                                 int*, hipStream_t, uint64_t*, int, int*);
 template void compact_regions<uint32_t>(const uint32_t*, int64_t, const int*, const int*, int64_t, uint32_t*,
                                         int64_t, hipStream_t);
 template void compact_regions<uint64_t>(const uint64_t*, int64_t, const int*, const int*, int64_t, uint64_t*,

@@ -51,8 +51,11 @@ void expand_ancestors(const DevTree& T, const K* sorted, const int* flags, const
 // region[b * cap_b ...] and their number to blockcnt[b]; compact_regions packs the slices at
 // blockoff (exclusive scan of blockcnt).  cap_b must hold 256 * the per-point bound.
 template <typename K>
+// compact != null: the keys are also appended to compact[0, compact_cap) (any order), their count
+// added to *counter
 void box_keys(const DevTree& T, const PointsView& P, KeyFmt f, float r, float hi_mul, K* region, int64_t cap_b,
-              int* blockcnt, int* markers, hipStream_t s);
+              int* blockcnt, int* markers, hipStream_t s, K* compact = nullptr, int compact_cap = 0,
+              int* counter = nullptr);
 template <typename K>
 void compact_regions(const K* region, int64_t cap_b, const int* blockcnt, const int* blockoff, int64_t nblocks, K* out,
                      int64_t out_cap, hipStream_t s);
