@@ -494,12 +494,15 @@ size_t bin_count_smem(int W, int D, bool vel) {
 }
 
 // Threads per brick workgroup.  The per-brick kernels stride every loop by blockDim, so any
-// multiple of 64 is valid: one wave when bricks hold few items (the time then goes to each
-// brick's chain of dependent loads, and one-wave workgroups keep four times as many bricks in
-// flight), four waves otherwise.  A launch shape that names the block size overrides this.
-static inline int brick_threads(const LaunchCfg& L, int64_t items, int64_t bricks, int64_t per_brick) {
+// multiple of 64 is valid: as many waves as the average brick has items for (one wave for
+// sparse bricks, where the time goes to each brick's chain of dependent loads and one-wave
+// workgroups keep four times as many bricks in flight; up to four).  unit scales the items to
+// lanes (a point of the count pass, an entry of the gather).  A launch shape that names the
+// block size overrides this.
+static inline int brick_threads(const LaunchCfg& L, int64_t items, int64_t bricks, int64_t unit) {
   if (L.block) return L.block;
-  return items < per_brick * bricks ? 64 : 256;
+  const int64_t per = items / (bricks > 0 ? bricks : 1) * 64 / unit;
+  return per <= 64 ? 64 : (per <= 128 ? 128 : 256);
 }
 
 // Grid form: one workgroup per brick.  Persistent form: as many workgroups as fit on the device

@@ -3,11 +3,14 @@
 // 320-356): SetPoints, RebuildTopology, FinishTopology, UpdateAtlas, InsertPointsSubcell,
 // GatherDensity / GatherLevelSet, UpdateApron.
 //
-// usage: points_to_grid <points.f32> <n> <radius> <subcell> <levelset 0|1> <out.bin>
-// points.f32 holds n float3 positions; out.bin receives [nodes0 count][slots][brick res]
+// usage: points_to_grid <points.f32 | synth:SEED> <n> <radius> <subcell> <levelset 0|1> <out.bin>
+// points.f32 holds n float3 positions; synth:SEED takes n points of the shared seeded sphere
+// (gvdbx_synth_sphere: radius 20 about (64, 64, 64), sigma 1.5; the points of the committed
+// golden sphere fixture for SEED 0 and n 3000).  out.bin receives [nodes0 count][slots][brick res]
 // (int64 x3), the level-0 node pool (64 B each) and channel 0 (float, brick-linear).
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <vector>
 
@@ -23,12 +26,20 @@ int main(int argc, char** argv) {
   const int subcell = atoi(argv[4]);
   const bool levelset = atoi(argv[5]) != 0;
   std::vector<float> pts((size_t)n * 3);
-  FILE* f = fopen(argv[1], "rb");
-  if (!f || fread(pts.data(), sizeof(float), pts.size(), f) != pts.size()) {
-    fprintf(stderr, "cannot read %s\n", argv[1]);
-    return 2;
+  if (strncmp(argv[1], "synth:", 6) == 0) {
+    const float center[3] = {64.f, 64.f, 64.f};
+    if (gvdbx_synth_sphere(strtoull(argv[1] + 6, nullptr, 10), n, center, 20.f, 1.5f, pts.data()) != GVDBX_OK) {
+      fprintf(stderr, "bad synthetic point count\n");
+      return 2;
+    }
+  } else {
+    FILE* f = fopen(argv[1], "rb");
+    if (!f || fread(pts.data(), sizeof(float), pts.size(), f) != pts.size()) {
+      fprintf(stderr, "cannot read %s\n", argv[1]);
+      return 2;
+    }
+    fclose(f);
   }
-  fclose(f);
 
   gvdbx::VolumeGVDB gvdb;
   gvdb.SetCudaDevice(0);

@@ -100,6 +100,9 @@ SIGNATURES = {
     "gvdbx_commit": [_P, _P, _P, _I64],
     "gvdbx_retrieve": [_P, _P, _P, _I64],
     "gvdbx_set_launch": [_P, _I, _I],
+    "gvdbx_synth_uniform": [C.c_uint64, _I64, _P, _P, _P],
+    "gvdbx_synth_sphere": [C.c_uint64, _I64, _P, _F, _F, _P],
+    "gvdbx_synth_lattice": [C.c_uint64, _P, _F, _F, _F, _P, _P],
     "gvdbx_graph_begin": [_P],
     "gvdbx_graph_end": [_P],
     "gvdbx_graph_launch": [_P],

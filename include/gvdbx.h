@@ -279,6 +279,22 @@ int gvdbx_vbx_write(const char* path, const gvdbx_vbx_desc* desc);
 int gvdbx_vbx_probe(const char* path, gvdbx_vbx_desc* desc);
 int gvdbx_vbx_read(const char* path, gvdbx_vbx_desc* desc);
 
+/* ---- seeded synthetic point sets (host memory; no reference counterpart: the reference ships
+ *      no point fixture).  The same bits as gvdbx/synth.py's shared_* functions: draw k of stream
+ *      `seed` is the SplitMix64 finalizer of seed * 0xD1B54A32D192ED03 + (k + 1) *
+ *      0x9E3779B97F4A7C15, a uniform its top 53 bits times 2^-53, a normal the in-order sum of 12
+ *      uniforms minus 6; FP64 arithmetic without contraction, cast to FP32 at the end.
+ *      gvdbx_synth_uniform: point i coordinate a = lo[a] + (hi[a] - lo[a]) u(3i + a).
+ *      gvdbx_synth_sphere: point i = center + d (radius + sigma g3) / |d|, d = (g0, g1, g2), gj the
+ *        normal at draw 48 i + 12 j, |d| = sqrt((d0 d0 + d1 d1) + d2 d2).
+ *      gvdbx_synth_lattice: dims[0] dims[1] dims[2] points, i = (ix dims[1] + iy) dims[2] + iz, at
+ *        (c + 0.5) spacing + (u(39 i + a) - 0.5) jitter; velocity a = vel_sigma * normal at draw
+ *        39 i + 3 + 12 a (vel may be NULL). */
+int gvdbx_synth_uniform(uint64_t seed, int64_t n, const float lo[3], const float hi[3], float* out);
+int gvdbx_synth_sphere(uint64_t seed, int64_t n, const float center[3], float radius, float sigma, float* out);
+int gvdbx_synth_lattice(uint64_t seed, const int dims[3], float spacing, float jitter, float vel_sigma, float* pos,
+                        float* vel);
+
 /* ---- device memory helpers: AllocData/CommitData/RetrieveData (V.cpp:4822-4860, A.cpp:263-298) */
 int gvdbx_dev_alloc(gvdbx_ctx* ctx, int64_t bytes, void** dev_ptr);
 int gvdbx_dev_free(gvdbx_ctx* ctx, void* dev_ptr);

@@ -46,3 +46,28 @@ def test_cpp_facade_points_to_grid(oracle_mod, tmp_path, levelset):
     err = np.abs(chan.astype(np.float64) - want) / np.maximum(np.abs(want), 1.0)
     assert err.max() <= U.TOL, err.max()
     assert f"leaves {total}" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_facade_shared_generator_golden():
+    """The C++ caller generates the golden sphere itself (gvdbx_synth_sphere, the same bits as
+    gvdbx/synth.py) and must reproduce the committed fixture: the leaf pool byte for byte and the
+    density within 1e-5 of the stored field."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} not built")
+    U.log("case c++ facade, shared generator, golden sphere")
+    with np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sphere.npz")) as z:
+        want_nodes, want = z["nodes0"], z["density"]
+        n = len(z["points"])
+    fout = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"gvdbx_golden_{os.getpid()}.bin")
+    r = subprocess.run([EXE, "synth:0", str(n), "1.0", "4", "0", fout], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(fout, "rb") as f:
+        raw = f.read()
+    os.remove(fout)
+    total, slots, br = np.frombuffer(raw[:24], np.int64)
+    assert raw[24:24 + 64 * total] == want_nodes.tobytes()
+    chan = np.frombuffer(raw[24 + 64 * total:], np.float32).reshape(slots, br, br, br)
+    err = np.abs(chan.astype(np.float64) - want) / np.maximum(np.abs(want.astype(np.float64)), 1.0)
+    assert err.max() <= U.TOL, err.max()
