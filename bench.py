@@ -256,7 +256,17 @@ def adjoint_bench(reps):
 
 
 VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4  # wave64 FP32 VALU issue rate of MI355X (no packed math)
-LEVELSET_VALU_PER_PAIR = 6.5  # VALU wave-instructions per (list entry, 4^3 subcell) in the level-set kernel
+
+
+def levelset_valu_per_entry():
+    """VALU wave-instructions per list entry of the level-set kernel (k_levelset_sc4), from the
+    committed counter summary (profiles/pmc_levelset.json: SQ_INSTS_VALU of the same workload
+    over its list length), or None."""
+    f = os.path.join(ROOT, "profiles", "pmc_levelset.json")
+    if not os.path.exists(f):
+        return None
+    e = json.load(open(f)).get("kernels", {}).get("k_levelset_sc4")
+    return e["valu_per_entry"] if e else None
 
 
 def levelset_bench(reps):
@@ -304,11 +314,13 @@ def levelset_bench(reps):
     v.close()
     total = sum(best.values()) - best["sample"]
     sr = best["scatter_reduce"] * 1e-3
+    vpe = levelset_valu_per_entry()  # measured by SQ_INSTS_VALU (the kernel's instruction count)
     return {"workload": "levelset_16M (16,777,216 points on 64 spheres in 1024^3, r=2, sc=4)",
             "bricks": int(bricks), "subcell_entries": int(ln), "ms": {k: round(x, 4) for k, x in best.items()},
             "points_to_sdf_ms": round(total, 4), "Mpoints_s": round(n / (total * 1e-3) / 1e6, 1),
             "pair_evals_per_s": round(64 * ln / sr, 1),
-            "scatter_valu_frac": round(LEVELSET_VALU_PER_PAIR * ln / sr / VALU_WAVE_INSTR_PER_S, 4),
+            "scatter_valu_frac": (round(vpe * ln / sr / VALU_WAVE_INSTR_PER_S, 4) if vpe else None),
+            "scatter_valu_per_entry": vpe,
             "queries_Mper_s": round(len(qs) / (best["sample"] * 1e-3) / 1e6, 1)}
 
 

@@ -329,3 +329,38 @@ def test_density_velocity_one_pass(oracle_mod, ld):
         cur = (cur + vel * np.float32(0.5)).astype(np.float32)
     for v in vs:
         v.close()
+
+
+def test_small_velocities_absolute_resolution(oracle_mod):
+    """Velocity means are exact sums of 2^-24 fixed-point terms: each term is rounded once, so a
+    mean is within 2^-25 of the FP64 mean whatever the magnitudes (plus the FP32 cast).  With
+    velocities of order 1e-3 that absolute bound is the statement (a 1e-5 relative bound would not
+    hold there); the one-pass density + velocity gather meets the same bound."""
+    from gvdbx import synth
+    U.log("case small velocities")
+    p, vel = synth.sph_block(n=40_000, seed=21, box=48.0)
+    vel = (vel * np.float32(1e-3)).astype(np.float32)
+    v, t = U.new_volume(chans=(0, 1, 2, 3)), U.new_tree(oracle_mod, chans=(0, 1, 2, 3))
+    d, dv = v.to_device(p), v.to_device(vel)
+    v.SetPoints(d, dv, n=len(p))
+    v.RebuildTopology(len(p), 1.0)
+    t.rebuild_topology(p, 1.0)
+    v.UpdateAtlas()
+    t.update_atlas()
+    ln = v.InsertPointsSubcell(4, len(p), 1.0)
+    t.insert_points_subcell(4, p, 1.0)
+    t.gather_velocity(vel, (1, 2, 3))
+    a = v.apron
+    for one_pass in (False, True):
+        if one_pass:
+            v.GatherDensityVelocity(4, len(p), 1.0, (0, 0, 0), ln, 0, (1, 2, 3))
+        else:
+            v.GatherVelocity(1, 2, 3)
+        for c in (1, 2, 3):
+            g = v.get_channel(c)[:, a:-a, a:-a, a:-a].astype(np.float64)
+            o = t.channel(c)[:, a:-a, a:-a, a:-a]
+            err = np.abs(g - o)
+            bound = 2.0 ** -25 + np.abs(o) * 2.0 ** -24
+            assert (err <= bound).all(), (one_pass, c, float((err - bound).max()))
+            assert np.abs(o).max() > 1e-4  # the means are of the small velocities' order
+    v.close()
