@@ -1585,9 +1585,17 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     key_ranges(skeys, n, nmap, P<int>(c->ranges), c->stream);
   }
   const int* tail = P<int>(c->ranges) + nmap;  // points whose own leaf is missing, and invalid ones
-  gather_sorted(pv, svals, n, P<float4>(c->spos), with_vel ? P<float4>(c->svel) : nullptr, c->stream);
+  // Sorted copies of the points for the per-brick kernels: with bricks of 64 points or more on
+  // average the count pass gathers them itself (its bricks' points, through the permutation);
+  // sparse bricks leave the count pass too little work per point to hide that gather's latency,
+  // so k_gather_sorted makes the copies first.
+  const bool gather_in_count = n >= (int64_t)64 * nmap;
+  if (!gather_in_count)
+    gather_sorted(pv, svals, n, P<float4>(c->spos), with_vel ? P<float4>(c->svel) : nullptr, c->stream);
   BinArgs A;
   A.L = launch_of(c);
+  A.spos_out = gather_in_count ? P<float4>(c->spos) : nullptr;
+  A.svel_out = (gather_in_count && keep_vel_lists) ? P<float4>(c->svel) : nullptr;
   A.T = T;
   A.P = pv;
   A.c = cfg;

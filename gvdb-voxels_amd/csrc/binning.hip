@@ -287,14 +287,33 @@ __device__ __forceinline__ void bin_count_brick(const BinArgs& A, int m, int use
   __syncthreads();
   const SubcellCfg c = A.c;
   for (int j = beg + threadIdx.x; j < end; j += blockDim.x) {
-    const float4 p = A.spos[j];
-    const float q[3] = {p.x, p.y, p.z};
+    float q[3];
     long long fv[3] = {0, 0, 0};
-    if (vel) {
-      const float4 v = A.svel[j];
-      fv[0] = vel_fixed(v.x);
-      fv[1] = vel_fixed(v.y);
-      fv[2] = vel_fixed(v.z);
+    if (A.spos_out) {  // gather from the caller's arrays, and write the sorted copies
+      const uint32_t i = A.svals[j];
+      const float3 p = load_f3(A.P.pos, i, A.P.pos_off, A.P.pos_stride);
+      q[0] = p.x + A.P.tx;
+      q[1] = p.y + A.P.ty;
+      q[2] = p.z + A.P.tz;
+      A.spos_out[j] = make_float4(q[0], q[1], q[2], __uint_as_float(i));
+      if (vel || A.svel_out) {
+        const float3 v = load_f3(A.P.vel, i, A.P.vel_off, A.P.vel_stride);
+        if (A.svel_out) A.svel_out[j] = make_float4(v.x, v.y, v.z, 0.f);
+        fv[0] = vel_fixed(v.x);
+        fv[1] = vel_fixed(v.y);
+        fv[2] = vel_fixed(v.z);
+      }
+    } else {  // the sorted copies k_gather_sorted wrote
+      const float4 p = A.spos[j];
+      q[0] = p.x;
+      q[1] = p.y;
+      q[2] = p.z;
+      if (vel) {
+        const float4 v = A.svel[j];
+        fv[0] = vel_fixed(v.x);
+        fv[1] = vel_fixed(v.y);
+        fv[2] = vel_fixed(v.z);
+      }
     }
     int lo[3], hi[3];
     subcell_range(q, c.r, c.lgsc, lo, hi);
@@ -386,7 +405,10 @@ template <int kD, bool kPersist>
 __global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_bins) {
   extern __shared__ unsigned long long smem64[];
   __shared__ int nb[27];
-  for_each_brick<kPersist>(nmap, A.L.queue, [&](int m) { bin_count_brick<kD>(A, m, use_bins, smem64, nb); });
+  // when the count pass gathers its bricks' points from the caller's arrays, neighbouring bricks
+  // (whose points share cache lines there) run on one XCD
+  for_each_brick<kPersist>(
+      nmap, A.L.queue, [&](int m) { bin_count_brick<kD>(A, m, use_bins, smem64, nb); }, A.spos_out != nullptr);
 }
 
 // One list entry: the position (float3, or ushort3 quantized for the FP16 insert), the velocity
@@ -520,8 +542,8 @@ static void launch_bricks(void (*grid_k)(KArgs...), void (*pers_k)(KArgs...), co
     if (g > need) g = need;
     if (g < 1) g = 1;
     hipLaunchKernelGGL(pers_k, dim3((unsigned)g), dim3(bs), sm, s, args...);
-  } else {
-    hipLaunchKernelGGL(grid_k, item_grid(items), dim3(bs), sm, s, args...);
+  } else {  // room for for_each_brick's XCD-aware order (surplus items skip)
+    hipLaunchKernelGGL(grid_k, item_grid(xcd_items(items)), dim3(bs), sm, s, args...);
   }
 }
 
@@ -532,7 +554,7 @@ void bin_count(const BinArgs& A, int nmap, hipStream_t s) {
   const int bins = bs == 256 ? bin_uses_class_bins(A.c.D) : 0;
   const size_t sm = bin_count_smem(A.W, bins ? A.c.D : 0, A.vsum != nullptr && A.P.vel != nullptr);
   if (A.c.D == 2)
-    launch_bricks(k_bin_count<2, false>, k_bin_count<2, true>, A.L, bs, sm, nmap, s, A, nmap, bins);
+    launch_bricks(k_bin_count<2, false>, k_bin_count<2, true>, A.L, bs, sm, nmap, s, A, nmap, bins);  // gathers too
   else
     launch_bricks(k_bin_count<0, false>, k_bin_count<0, true>, A.L, bs, sm, nmap, s, A, nmap, bins);
 }

@@ -20,6 +20,11 @@ void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStrea
 
 struct BinArgs {
   LaunchCfg L;   // launch shape of the count and insert kernels
+  // non-null: the count kernel reads each brick's points through svals from the caller's arrays
+  // and writes the sorted copies the insert reads (positions (q, index); velocities when
+  // svel_out is given); null: k_gather_sorted wrote spos / svel before the count
+  float4* spos_out;
+  float4* svel_out;
   DevTree T;
   PointsView P;
   SubcellCfg c;

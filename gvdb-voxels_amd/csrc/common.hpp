@@ -70,15 +70,17 @@ struct LaunchCfg {
   int cus;      // compute units of the device (the persistent grid is a multiple of it)
 };
 
-// The bricks of a per-brick kernel: one per workgroup (grid form, m = the item index), or,
-// persistent, taken from the queue kBrickChunk at a time until it passes n.  The body may return
+// The bricks of a per-brick kernel: one per workgroup (grid form: m = the item index, or with
+// xcd xcd_item, so that bricks that are neighbours in slot order, whose points share cache lines
+// of the caller's arrays, run on one XCD; the grid holds xcd_items(n) items), or, persistent,
+// taken from the queue kBrickChunk at a time until it passes n.  The body may return
 // early only uniformly over the workgroup; a barrier follows every brick, so the body may reuse
 // its LDS from one brick to the next.  Every workgroup leaves once the queue passes n.
 constexpr int kBrickChunk = 4;
 template <bool kPersist, typename F>
-__device__ __forceinline__ void for_each_brick(int64_t n, int* queue, F&& body) {
+__device__ __forceinline__ void for_each_brick(int64_t n, int* queue, F&& body, bool xcd = false) {
   if (!kPersist) {
-    const int64_t m = item_index();
+    const int64_t m = xcd ? xcd_item(n) : item_index();
     if (m < n) body((int)m);
     return;
   }

@@ -364,3 +364,43 @@ def test_small_velocities_absolute_resolution(oracle_mod):
             assert (err <= bound).all(), (one_pass, c, float((err - bound).max()))
             assert np.abs(o).max() > 1e-4  # the means are of the small velocities' order
     v.close()
+
+
+def test_dense_bricks_velocity_lists(oracle_mod):
+    """Bricks of hundreds of points, where the count pass gathers the points itself and writes the
+    sorted copies (velocities too, when the velocity lists are kept), with orphan points (their
+    own brick missing) mixed in: lists with velocities as per-subcell multisets, density and
+    velocity means against the oracle."""
+    from gvdbx import synth
+    U.log("case dense bricks, velocity lists")
+    p, vel = synth.sph_block(n=120_000, seed=5, box=40.0)
+    extra = np.random.default_rng(6).uniform(2.0, 60.0, (3000, 3)).astype(np.float32)
+    allp = np.concatenate([p, extra])
+    allv = np.concatenate([vel, np.random.default_rng(7).standard_normal(extra.shape).astype(np.float32)])
+    v, t = U.new_volume(chans=(0, 1, 2, 3)), U.new_tree(oracle_mod, chans=(0, 1, 2, 3))
+    v.SetSubcellOptions(store_velocity_lists=True)
+    dp, dv = v.to_device(allp), v.to_device(allv)
+    v.SetPoints(dp, dv, n=len(p))
+    v.RebuildTopology(len(p), 1.0)
+    t.rebuild_topology(p, 1.0)
+    v.UpdateAtlas()
+    t.update_atlas()
+    U.compare_topology(v, t)
+    assert len(allp) >= 64 * v.num_slots()[0], "the bricks are not dense"
+    v.SetPoints(dp, dv, n=len(allp))
+    ln = v.InsertPointsSubcell(4, len(allp), 1.0)
+    assert ln == t.insert_points_subcell(4, allp, 1.0)
+    gs = U.compare_subcells(v, t)
+    o = t.subcells()
+    seg = np.repeat(np.arange(len(o["cnt"])), o["cnt"])
+    key_g = np.lexsort((gs["list_vel"][:, 2], gs["list_vel"][:, 1], gs["list_vel"][:, 0], seg))
+    want_vel = allv[o["list"]]
+    key_o = np.lexsort((want_vel[:, 2], want_vel[:, 1], want_vel[:, 0], seg))
+    assert np.array_equal(gs["list_vel"][key_g], want_vel[key_o])
+    v.GatherDensity(4, len(allp), 1.0, (0, 0, 0), ln, 0)
+    t.gather_density(1.0, 0)
+    v.GatherVelocity(1, 2, 3)
+    t.gather_velocity(allv, (1, 2, 3))
+    for c in range(4):
+        U.compare_channel(v, t, c, interior_only=True)
+    v.close()
