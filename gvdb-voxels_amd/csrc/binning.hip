@@ -651,7 +651,8 @@ __global__ void __launch_bounds__(256) k_cell_walk(BinArgs A, int64_t nsc, int* 
           const int w = ((lz - dz * D + 1) * W + (ly - dy * D + 1)) * W + (lx - dx * D + 1);
           int* cell = A.wgb + (int64_t)b * W3 + w;
           const int c = *cell;
-          if (kBases) *cell = base + p;
+          // the insert reads a cell's base only where its brick places entries (c > 0)
+          if (kBases && c > 0) *cell = base + p;
           p += c;
           if (vel && c > 0) {
             const unsigned long long* v = A.wvs + ((int64_t)b * W3 + w) * 3;
@@ -781,7 +782,36 @@ __device__ __forceinline__ float mean_of(const int* cnt, const unsigned long lon
 // kVel: the velocity means of GatherVelocity (non-accumulating, k_velocity_brick's rule) are
 // written in the same pass, as the reference computes its colour channel inside the density
 // gather (P.cuh:717-731): window-cell means in LDS, three more stores per voxel.
-template <int kRes, int kApron, int kD, bool kWithApron, bool kVel>
+//
+// kCorner (fixed geometry with the apron): each point corner takes max(0, r - sqrt(d2)) itself
+// and max-reduces its bits in LDS (non-negative floats order as their bits), so the output loop
+// only copies.  f(d) = max(0, r - sqrtf(d)) is monotone non-increasing with correctly rounded
+// sqrtf and subtraction, so max_p f(d_p) = f(min_p d_p): the same bits as the per-voxel form,
+// with one square root per point corner instead of one per voxel (sparse bricks have far fewer
+// corners than voxels; dense bricks the other way round).
+
+// The fixed geometry's padded voxels (8^3 brick, apron 1, 2^3 subcells): entry t of the 10^3
+// padded brick holds in bits 0-4 the neighbour index (oz+1)*9 + (oy+1)*3 + (ox+1) of the brick
+// the voxel lies in (13 = this one), in bits 8-13 its window cell ((wz*4 + wy)*4 + wx, each
+// w = floor(coord / 4) + 1), so the output loop decodes no coordinates.
+template <int kRes, int kApron, int kD>
+struct PadTable {
+  static constexpr int kBw = kRes + 2 * kApron, kVox = kBw * kBw * kBw, kW = kD + 2, kSc = kRes / kD;
+  unsigned short v[kVox];
+  static constexpr int side(int x) { return x < 0 ? 0 : (x >= kRes ? 2 : 1); }
+  static constexpr int cell(int x) { return (x < 0 ? -1 : x / kSc) + 1; }  // apron <= subcell size
+  constexpr PadTable() : v() {
+    for (int t = 0; t < kVox; ++t) {
+      const int x = t % kBw - kApron, y = (t / kBw) % kBw - kApron, z = t / (kBw * kBw) - kApron;
+      const int nbi = side(z) * 9 + side(y) * 3 + side(x);
+      const int w = (cell(z) * kW + cell(y)) * kW + cell(x);
+      v[t] = (unsigned short)(nbi | (w << 8));
+    }
+  }
+};
+__constant__ __align__(16) PadTable<8, 1, 2> kPad812 = PadTable<8, 1, 2>();
+
+template <int kRes, int kApron, int kD, bool kWithApron, bool kVel, bool kCorner>
 __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t m, int64_t nsc, int64_t len,
                                                    const int* __restrict__ map_leaf, const int* __restrict__ off,
                                                    const float* __restrict__ list_pos,
@@ -825,11 +855,13 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
   }
   const int nv = bw * bw * bw;
   const int64_t k0 = (int64_t)m * D3;
+  static_assert(!kCorner || (kRes == 8 && kApron == 1 && kD == 2 && kWithApron), "kCorner: fixed geometry only");
+  constexpr unsigned int kInit = kCorner ? 0u : 0x7F800000u;  // value 0, or squared distance +inf
   if constexpr (kRes && kWithApron && (kVox & 3) == 0) {  // nv == kVox: 16-byte stores
     for (int t4 = threadIdx.x; t4 < kVox / 4; t4 += blockDim.x)
-      reinterpret_cast<uint4*>(d2)[t4] = make_uint4(0x7F800000u, 0x7F800000u, 0x7F800000u, 0x7F800000u);
+      reinterpret_cast<uint4*>(d2)[t4] = make_uint4(kInit, kInit, kInit, kInit);
   } else {
-    for (int t = threadIdx.x; t < nv; t += blockDim.x) d2[t] = 0x7F800000u;
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) d2[t] = kInit;
   }
   for (int s = threadIdx.x; s <= D3; s += blockDim.x) soff[s] = off[k0 + s];  // off[nsc] = the list length
   if (threadIdx.x < 27) nb[threadIdx.x] = (kWithApron && slot >= 0) ? nbr[(int64_t)slot * 27 + threadIdx.x] : -1;
@@ -842,7 +874,12 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
       const int ox = sx < 0 ? -1 : (sx >= D ? 1 : 0), oy = sy < 0 ? -1 : (sy >= D ? 1 : 0),
                 oz = sz < 0 ? -1 : (sz >= D ? 1 : 0);
       const int b = (ox | oy | oz) == 0 ? (int)m : nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)];
-      if (b < 0) continue;  // slot order equals subcell-map order
+      if (b < 0) {  // apron voxels without a neighbour brick: boundary value 0
+#pragma unroll
+        for (int q = 0; q < 3; ++q) vmean[w][q] = 0.0f;
+        continue;
+      }
+      // slot order equals subcell-map order
       const int64_t k = (int64_t)b * D3 + ((sz - oz * D) * D + (sy - oy * D)) * D + (sx - ox * D);
 #pragma unroll
       for (int q = 0; q < 3; ++q) vmean[w][q] = mean_of(vo.cnt, vo.vsum, k, q, 0, 0.0f);
@@ -913,7 +950,12 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
             for (int dx = 0; dx < 2; ++dx) {
               if (!(vx[dx] && vy[dy] && vz[dz])) continue;
               const float d = fmaf(ez[dz], ez[dz], fmaf(ey[dy], ey[dy], px[dx]));
-              atomicMin(&d2[i0 + (dz * bw + dy) * bw + dx], __float_as_uint(d));
+              if (kCorner) {
+                const float val = fmaxf(0.0f, c.r - sqrtf(d));
+                if (val > 0.0f) atomicMax(&d2[i0 + (dz * bw + dy) * bw + dx], __float_as_uint(val));
+              } else {
+                atomicMin(&d2[i0 + (dz * bw + dy) * bw + dx], __float_as_uint(d));
+              }
             }
       } else {
         const int x0 = max(lx0, fx - R + 1), x1 = min(lx1, fx + R);
@@ -923,7 +965,13 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
           for (int y = y0; y <= y1; ++y)
             for (int x = x0; x <= x1; ++x) {
               const float d = dist2(gx - (float)(bx + x), gy - (float)(by + y), gz - (float)(bz + z));
-              atomicMin(&d2[((z + a) * bw + (y + a)) * bw + (x + a)], __float_as_uint(d));
+              unsigned int* cell = &d2[((z + a) * bw + (y + a)) * bw + (x + a)];
+              if (kCorner) {
+                const float val = fmaxf(0.0f, c.r - sqrtf(d));
+                if (val > 0.0f) atomicMax(cell, __float_as_uint(val));
+              } else {
+                atomicMin(cell, __float_as_uint(d));
+              }
             }
       }
     }
@@ -931,40 +979,30 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
   __syncthreads();
   if (slot < 0) return;
   float* bptr = chan + (int64_t)slot * br * br * br;
-  if constexpr (kRes && kWithApron && (kVox & 3) == 0) {
+  if constexpr (kRes == 8 && kApron == 1 && kD == 2 && kWithApron) {
     // fixed geometry with the apron: the computed region is the whole padded brick, so voxels
-    // t..t+3 are consecutive in LDS and in the channel (16-byte loads and stores)
-    constexpr int kBw = kRes + 2 * kApron;
-    constexpr int lgsc = kRes ? 31 - __builtin_clz(kRes / kD) : 0;
-    constexpr int W = kD + 2;
+    // t..t+3 are consecutive in LDS and in the channel (16-byte loads and stores); each voxel's
+    // neighbour index and window cell come from the padded-voxel table
+    // the missing neighbours as a bit mask (wave-uniform): their apron voxels get 0
+    const int ln = (int)(threadIdx.x & 63);
+    const unsigned int miss = (unsigned int)__ballot(ln < 27 && nb[ln < 27 ? ln : 0] < 0);
+    const uint2* tab = reinterpret_cast<const uint2*>(kPad812.v);
     for (int t4 = threadIdx.x; t4 < kVox / 4; t4 += blockDim.x) {
       const uint4 u4 = reinterpret_cast<const uint4*>(d2)[t4];
+      const uint2 tb = tab[t4];
       const unsigned int u[4] = {u4.x, u4.y, u4.z, u4.w};
+      const unsigned int ent[4] = {tb.x & 0xFFFFu, tb.x >> 16, tb.y & 0xFFFFu, tb.y >> 16};
       float val[4], vm[3][4];
-      // padded coordinates of voxel 4 t4, then one step along x (carrying into y, z) per voxel
-      const int t0 = 4 * t4, zy0 = t0 / kBw;
-      int px = t0 - zy0 * kBw, py = zy0 % kBw, pz = zy0 / kBw;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        if (e) {
-          if (++px == kBw) {
-            px = 0;
-            if (++py == kBw) {
-              py = 0;
-              ++pz;
-            }
-          }
-        }
-        const int x = px - kApron, y = py - kApron, z = pz - kApron;
-        const int ox = x < 0 ? -1 : (x >= kRes ? 1 : 0), oy = y < 0 ? -1 : (y >= kRes ? 1 : 0),
-                  oz = z < 0 ? -1 : (z >= kRes ? 1 : 0);
-        // apron voxel without a neighbour brick: boundary value 0
-        const bool missing = (ox | oy | oz) != 0 && nb[(oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)] < 0;
-        val[e] = (missing || u[e] == 0x7F800000u) ? 0.0f : fmaxf(0.0f, c.r - sqrtf(__uint_as_float(u[e])));
+        const bool missing = (miss >> (ent[e] & 31u)) & 1u;
+        const float f = __uint_as_float(u[e]);
+        // per voxel: f = min d2 (+inf when no point reached it, giving max(0, -inf) = 0)
+        val[e] = missing ? 0.0f : (kCorner ? f : fmaxf(0.0f, c.r - sqrtf(f)));
         if (kVel) {
-          const int w = (((z >> lgsc) + 1) * W + ((y >> lgsc) + 1)) * W + ((x >> lgsc) + 1);
+          const unsigned int w = ent[e] >> 8;
 #pragma unroll
-          for (int q = 0; q < 3; ++q) vm[q][e] = missing ? 0.0f : vmean[w][q];
+          for (int q = 0; q < 3; ++q) vm[q][e] = vmean[w][q];  // 0 for missing neighbours
         }
       }
       reinterpret_cast<float4*>(bptr)[t4] = make_float4(val[0], val[1], val[2], val[3]);
@@ -1002,7 +1040,7 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
   }
 }
 
-template <int kRes, int kApron, int kD, bool kWithApron, bool kVel, bool kPersist>
+template <int kRes, int kApron, int kD, bool kWithApron, bool kVel, bool kCorner, bool kPersist>
 __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len,
                                                       const int* __restrict__ map_leaf, const int* __restrict__ off,
                                                       const float* __restrict__ list_pos,
@@ -1012,7 +1050,7 @@ __global__ void __launch_bounds__(256) k_density_brick(SubcellCfg c, int nmap, i
                                                       const int* __restrict__ nbr, float* __restrict__ chan,
                                                       VelOut vo, int* queue) {
   for_each_brick<kPersist>(nmap, queue, [&](int m) {
-    density_brick_body<kRes, kApron, kD, kWithApron, kVel>(c, m, nsc, len, map_leaf, off, list_pos, leaf_slot,
+    density_brick_body<kRes, kApron, kD, kWithApron, kVel, kCorner>(c, m, nsc, len, map_leaf, off, list_pos, leaf_slot,
                                                            nodes0, slot_pos, apron_rt, nbr, chan, vo);
   });
 }
@@ -1024,20 +1062,26 @@ void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* 
   const int bs = brick_threads(L, len, nmap, 256);
   const bool fixed = c.res0 == 8 && c.D == 2 && apron == 1;
   const VelOut vo = vel ? *vel : VelOut{nullptr, nullptr, {nullptr, nullptr, nullptr}};
-#define GVDBX_DENSITY(R, A, D, AP, V)                                                                             \
-  launch_bricks(k_density_brick<R, A, D, AP, V, false>, k_density_brick<R, A, D, AP, V, true>, L, bs, 0, nmap, s, c, \
-                nmap, nsc, len, map_leaf, off, list_pos, leaf_slot, nodes0, slot_pos, apron, nbr, chan, vo, L.queue)
+  // square roots per point corner when the bricks hold few list entries (an entry visits about
+  // 2 corners at r = 1 and subcells of 4, a brick has 1000 padded voxels): the same bits either way
+  const bool corner = len < (int64_t)256 * nmap;
+#define GVDBX_DENSITY(R, A, D, AP, V, C)                                                                           \
+  launch_bricks(k_density_brick<R, A, D, AP, V, C, false>, k_density_brick<R, A, D, AP, V, C, true>, L, bs, 0, nmap, \
+                s, c, nmap, nsc, len, map_leaf, off, list_pos, leaf_slot, nodes0, slot_pos, apron, nbr, chan, vo,    \
+                L.queue)
   if (vel) {  // velocity means need the apron pass (a neighbour table)
-    if (fixed) GVDBX_DENSITY(8, 1, 2, true, true);
-    else GVDBX_DENSITY(0, 0, 0, true, true);
+    if (fixed && corner) GVDBX_DENSITY(8, 1, 2, true, true, true);
+    else if (fixed) GVDBX_DENSITY(8, 1, 2, true, true, false);
+    else GVDBX_DENSITY(0, 0, 0, true, true, false);
   } else if (fixed && nbr) {
-    GVDBX_DENSITY(8, 1, 2, true, false);
+    if (corner) GVDBX_DENSITY(8, 1, 2, true, false, true);
+    else GVDBX_DENSITY(8, 1, 2, true, false, false);
   } else if (fixed) {
-    GVDBX_DENSITY(8, 1, 2, false, false);
+    GVDBX_DENSITY(8, 1, 2, false, false, false);
   } else if (nbr) {
-    GVDBX_DENSITY(0, 0, 0, true, false);
+    GVDBX_DENSITY(0, 0, 0, true, false, false);
   } else {
-    GVDBX_DENSITY(0, 0, 0, false, false);
+    GVDBX_DENSITY(0, 0, 0, false, false, false);
   }
 #undef GVDBX_DENSITY
 }
