@@ -1574,12 +1574,12 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   uint32_t* svals;
   {
     TimedRegion tr(c, "home");
-    home_keys(T, pv, mapp, nmap, P<uint32_t>(c->hkeys), P<uint32_t>(c->hvals), c->stream);
     int bits = 1;
     while (((int64_t)1 << bits) <= nmap) ++bits;
-    const int par = radix_sort_u32(P<uint32_t>(c->hkeys), P<uint32_t>(c->hkeys_alt), P<uint32_t>(c->hvals),
-                                   P<uint32_t>(c->hvals_alt), n, 0, bits, P<int>(c->radix_counts),
-                                   P<int>(c->tile_buf), c->stream);
+    // the key launch also counts the first radix digit; the sort generates the point positions
+    home_keys_counted(T, pv, mapp, nmap, P<uint32_t>(c->hkeys), bits, P<int>(c->radix_counts), c->stream);
+    const int par = radix_sort_u32_positions(P<uint32_t>(c->hkeys), P<uint32_t>(c->hkeys_alt), P<uint32_t>(c->hvals),
+                                             P<uint32_t>(c->hvals_alt), n, bits, P<int>(c->radix_counts), c->stream);
     skeys = par ? P<uint32_t>(c->hkeys_alt) : P<uint32_t>(c->hkeys);
     svals = par ? P<uint32_t>(c->hvals_alt) : P<uint32_t>(c->hvals);
     key_ranges(skeys, n, nmap, P<int>(c->ranges), c->stream);

@@ -102,27 +102,43 @@ void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s, u
 // ------------------------------------------------------------------ home keys
 // Key = subcell-map index of the leaf holding trunc(q); points that are invalid, NOHIT or
 // whose own leaf is missing get the sentinel nmap and are binned by the per-point path.
-__global__ void __launch_bounds__(256) k_home_keys(DevTree T, PointsView P, const int* __restrict__ map, int nmap,
-                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
-  uint32_t key = (uint32_t)nmap;
-  if (p.z != kNoHit) {
-    const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
-    if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
-      const int3 o = cloud_origin(P, i);
-      const int64_t lf = find_leaf(T, (int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z);
-      if (lf >= 0) key = (uint32_t)map[lf];
+// The home keys (no values: the sort generates the positions), and per tile of kRadixTile points
+// the counts of the first radix digit (the low `nbits` bits), digit-major into
+// counts[digit * ntiles + tile] for all 256 digits, as k_radix_up would count them.
+__global__ void __launch_bounds__(256) k_home_keys_counted(DevTree T, PointsView P, const int* __restrict__ map,
+                                                          int nmap, uint32_t* __restrict__ keys, int nbits,
+                                                          int* __restrict__ counts, int ntiles) {
+  __shared__ int hist[256];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+  const uint32_t mask = (1u << nbits) - 1u;
+#pragma unroll 4
+  for (int r = 0; r < kRadixTile / 256; ++r) {
+    const int64_t i = base + r * 256 + threadIdx.x;
+    if (i >= P.n) continue;
+    const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+    uint32_t key = (uint32_t)nmap;
+    if (p.z != kNoHit) {
+      const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
+      if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
+        const int3 o = cloud_origin(P, i);
+        const int64_t lf = find_leaf(T, (int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z);
+        if (lf >= 0) key = (uint32_t)map[lf];
+      }
     }
+    keys[i] = key;
+    atomicAdd(&hist[key & mask], 1);
   }
-  keys[i] = key;
-  vals[i] = (uint32_t)i;
+  __syncthreads();
+  counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-void home_keys(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, uint32_t* vals,
-               hipStream_t s) {
-  hipLaunchKernelGGL(k_home_keys, dim3(nblocks(P.n, 256)), dim3(256), 0, s, T, P, map, nmap, keys, vals);
+void home_keys_counted(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, int bits,
+                       int* counts, hipStream_t s) {
+  const int ntiles = (int)((P.n + kRadixTile - 1) / kRadixTile);
+  hipLaunchKernelGGL(k_home_keys_counted, dim3(ntiles), dim3(256), 0, s, T, P, map, nmap, keys,
+                     radix_first_digit_bits(bits), counts, ntiles);
 }
 
 // Batched clouds: cloud index of every point (off = exclusive prefix of the cloud sizes) and

@@ -14,8 +14,10 @@ void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s, u
 
 void batch_ids(const PointsView& P, const int64_t* off, int ncloud, float limit, uint32_t* cloud, int* flag,
                hipStream_t s);
-void home_keys(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, uint32_t* vals,
-               hipStream_t s);
+// home keys with the first radix pass's tile counts over `bits` key bits (radix_sort_u32_positions
+// then sorts them; counts holds radix_count_ints(n) ints)
+void home_keys_counted(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, int bits,
+                       int* counts, hipStream_t s);
 void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStream_t s);
 
 struct BinArgs {

@@ -100,10 +100,47 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
                                                   int* __restrict__ markers, K* __restrict__ compact, int compact_cap,
                                                   int* __restrict__ counter) {
   __shared__ int bcount;
+  // kTwo: the box-node list of the workgroup (thread << 3 | k) and each thread's box corner
+  __shared__ uint16_t s_list[kTwo ? 256 * 8 : 1];
+  __shared__ int s_lo[kTwo ? 256 : 1][3];
   if (threadIdx.x == 0) bcount = 0;
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   K* out = region + (int64_t)blockIdx.x * cap_b;
+  const int R0 = 1 << T.lgR[0];
+  const int d = T.depth;
+  auto emit = [&](int x, int y, int z, int64_t cur, int miss) {
+    if (miss < 0) {
+      if (markers && markers[cur] == 0) markers[cur] = 1;
+      return;
+    }
+    const int k = atomicAdd(&bcount, miss + 1);
+    for (int l = miss; l >= 0; --l) {
+      const int sh = T.lgR[l];
+      out[k + miss - l] = pack_key<K>(f, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
+                                      (z >> sh) - (T.root[2] >> sh));
+    }
+  };
+  // the leaf of box node (x, y, z) from the lookup table, else by descent; then emit
+  auto visit = [&](int x, int y, int z) {
+    const int64_t lf = table_leaf(T, x, y, z);
+    if (lf >= 0) {  // found in the lookup table
+      emit(x, y, z, lf, -1);
+      return;
+    }
+    int64_t cur = 0;
+    int miss = -1;  // highest missing level along the chain
+    for (int l = d; l > 0; --l) {
+      const uint64_t e = T.clist[l][((uint64_t)cur << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, x, y, z)];
+      if (e == kUndef64) {
+        miss = l - 1;
+        break;
+      }
+      cur = (int64_t)(e >> 16);
+    }
+    emit(x, y, z, cur, miss);
+  };
+  unsigned box_mask = 0;
   float q[3] = {-1.f, -1.f, -1.f};
   if (i < P.n) {
     const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
@@ -113,7 +150,6 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
   }
   if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
     int lo[3], hi[3];
-    const int R0 = 1 << T.lgR[0];
     const int3 o = i < P.n ? cloud_origin(P, i) : make_int3(0, 0, 0);
     const int org[3] = {o.x, o.y, o.z};
 #pragma unroll
@@ -122,96 +158,42 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
       lo[a] = (int)(tq - r) / R0 * R0 + org[a];
       hi[a] = (int)(tq + hi_mul * r) / R0 * R0 + org[a];
     }
-    const int d = T.depth;
-    auto emit = [&](int x, int y, int z, int64_t cur, int miss) {
-      if (miss < 0) {
-        if (markers && markers[cur] == 0) markers[cur] = 1;
-        return;
-      }
-      const int k = atomicAdd(&bcount, miss + 1);
-      for (int l = miss; l >= 0; --l) {
-        const int sh = T.lgR[l];
-        out[k + miss - l] = pack_key<K>(f, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
-                                        (z >> sh) - (T.root[2] >> sh));
-      }
-    };
     if (kTwo) {
-      // at most two leaf nodes per axis: the 8 candidates descend level by level together, so the
-      // child-list loads of one level are independent and in flight at once
-      int cx[8], cy[8], cz[8], miss[8];
-      int64_t cur[8];
-      bool act[8], walk[8];
-      bool any_walk = false;
-      // Per axis, candidate j (coordinate lo + j R0): inside the box and the root, and its offset
-      // into the lookup table (in_root and table_leaf, taken apart by axis).
-      const int Rr = T.lgR[T.depth], s0 = T.lgR[0];
-      bool ok[3][2], tin[3][2];
-      int64_t toff[3][2];
+      // At most two leaf nodes per axis.  The point's box nodes (bit k = (jx, jy, jz) of mask:
+      // corner lo + j R0 per axis, inside the box and the root) are listed after those of the
+      // threads before it, and the workgroup then walks that list 256 entries at a time, so the
+      // lookups of a round are independent and no load slot goes to a corner outside the box.
+      const int Rr = T.lgR[T.depth];
+      bool ok[3][2];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const int64_t stride = a == 0 ? 1 : (a == 1 ? (int64_t)T.bm_n[0] : (int64_t)T.bm_n[0] * T.bm_n[1]);
+      for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           const int c = lo[a] + j * R0;
           ok[a][j] = c <= hi[a] && c >= T.root[a] && ((c - T.root[a]) >> Rr) == 0;
-          const unsigned t = (unsigned)((c >> s0) - T.bm_lo[a]);
-          tin[a][j] = t < (unsigned)T.bm_n[a];
-          toff[a][j] = (int64_t)t * stride;
         }
-      }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int jx = k & 1, jy = (k >> 1) & 1, jz = k >> 2;
-        cx[k] = lo[0] + jx * R0;
-        cy[k] = lo[1] + jy * R0;
-        cz[k] = lo[2] + jz * R0;
-        act[k] = ok[0][jx] && ok[1][jy] && ok[2][jz];
-        cur[k] = 0;
-        miss[k] = -1;
-        // a leaf found in the lookup table needs no descent; a missing one still descends, to
-        // find the level where its chain breaks
-        const bool in_tab = T.bmap && tin[0][jx] && tin[1][jy] && tin[2][jz];
-        const int64_t lf = (act[k] && in_tab) ? T.bmap[toff[2][jz] + toff[1][jy] + toff[0][jx]] : -2;
-        if (lf >= 0) cur[k] = lf;
-        walk[k] = act[k] && lf < 0;
-        any_walk = any_walk || walk[k];
-      }
-      for (int l = d; l > 0 && any_walk; --l) {
-        uint64_t e[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-          e[k] = (walk[k] && miss[k] < 0)
-                     ? T.clist[l][((uint64_t)cur[k] << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, cx[k], cy[k], cz[k])]
-                     : 0ull;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          if (!(walk[k] && miss[k] < 0)) continue;
-          if (e[k] == kUndef64) miss[k] = l - 1;
-          else cur[k] = (int64_t)(e[k] >> 16);
-        }
-      }
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if (act[k]) emit(cx[k], cy[k], cz[k], cur[k], miss[k]);
+        if (ok[0][k & 1] && ok[1][(k >> 1) & 1] && ok[2][k >> 2]) box_mask |= 1u << k;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) s_lo[threadIdx.x][a] = lo[a];
     } else {
       for (int x = lo[0]; x <= hi[0]; x += R0)
         for (int y = lo[1]; y <= hi[1]; y += R0)
-          for (int z = lo[2]; z <= hi[2]; z += R0) {
-            if (!in_root(T, x, y, z)) continue;
-            const int64_t lf = table_leaf(T, x, y, z);
-            if (lf >= 0) {  // found in the lookup table
-              emit(x, y, z, lf, -1);
-              continue;
-            }
-            int64_t cur = 0;
-            int miss = -1;  // highest missing level along the chain
-            for (int l = d; l > 0; --l) {
-              const uint64_t e = T.clist[l][((uint64_t)cur << (3 * T.ld[l])) + (uint64_t)child_bit(T, l, x, y, z)];
-              if (e == kUndef64) { miss = l - 1; break; }
-              cur = (int64_t)(e >> 16);
-            }
-            emit(x, y, z, cur, miss);
-          }
+          for (int z = lo[2]; z <= hi[2]; z += R0)
+            if (in_root(T, x, y, z)) visit(x, y, z);
+    }
+  }
+  if (kTwo) {
+    int total;
+    int at = block_exclusive_scan_256(__popc(box_mask), &total);
+    for (unsigned m = box_mask; m; m &= m - 1) s_list[at++] = (uint16_t)(threadIdx.x << 3 | (__ffs(m) - 1));
+    __syncthreads();
+    for (int base = 0; base < total; base += 256) {
+      const int t = base + (int)threadIdx.x;
+      if (t >= total) continue;
+      const int e = s_list[t], src = e >> 3, k = e & 7;
+      visit(s_lo[src][0] + (k & 1) * R0, s_lo[src][1] + ((k >> 1) & 1) * R0, s_lo[src][2] + (k >> 2) * R0);
     }
   }
   __syncthreads();

@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(256) k_radix_down(const K* __restrict__ kin, K
   for (int r = 0; r < kPer; ++r) {  // all loads in flight before the first use
     const int64_t i = wbase + r * 64 + lane;
     k[r] = i < n ? kin[i] : (K)0;
-    if (kHasVal) v[r] = i < n ? vin[i] : 0u;
+    if (kHasVal) v[r] = i < n ? (vin ? vin[i] : (uint32_t)i) : 0u;  // no input values: the positions
   }
   const int within = offsets[(int64_t)tid * ntiles + tile];
   const int dtot = totals[tid];
@@ -346,10 +346,15 @@ __global__ void __launch_bounds__(256) k_radix_down(const K* __restrict__ kin, K
 // the digit-major tile counts, then the 256 digit totals
 int64_t radix_count_ints(int64_t n) { return 256 * ((n + kRadixTile - 1) / kRadixTile) + 256; }
 
+int radix_first_digit_bits(int bits) { return bits - 8 * ((bits + 7) / 8 - 1); }
+
+// positions: vals is not read (the first pass generates the input positions, vals_alt receives
+// them); first_counted: the first pass's tile counts are in counts already
 template <typename K>
 static int radix_sort_impl(K* keys, K* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int begin_bit,
-                           int end_bit, int* counts, int* tile_buf, hipStream_t s) {
-  if (n <= 1) return 0;
+                           int end_bit, int* counts, int* tile_buf, hipStream_t s, bool positions = false,
+                           bool first_counted = false) {
+  if (n <= 1 && !positions) return 0;
   const int ntiles = (int)((n + kRadixTile - 1) / kRadixTile);
   const int64_t ncnt = 256LL * ntiles;
   int parity = 0;
@@ -363,12 +368,14 @@ static int radix_sort_impl(K* keys, K* keys_alt, uint32_t* vals, uint32_t* vals_
   const int bits = end_bit - begin_bit;
   const int passes = (bits + 7) / 8;
   for (int pass = 0, shift = begin_bit; pass < passes; ++pass) {
-    const int nbits = pass == 0 ? bits - 8 * (passes - 1) : 8;
-    hipLaunchKernelGGL(k_radix_up<K>, dim3(ntiles), dim3(256), 0, s, kin, n, shift, nbits, counts, ntiles);
+    const int nbits = pass == 0 ? radix_first_digit_bits(bits) : 8;
+    if (!(pass == 0 && first_counted))
+      hipLaunchKernelGGL(k_radix_up<K>, dim3(ntiles), dim3(256), 0, s, kin, n, shift, nbits, counts, ntiles);
     hipLaunchKernelGGL(k_digit_scan, dim3(256), dim3(256), 0, s, counts, ntiles, totals);
     if (vals)
-      hipLaunchKernelGGL((k_radix_down<K, true>), dim3(ntiles), dim3(256), 0, s, kin, kout, vin, vout, n, shift, nbits,
-                         counts, ntiles, (const int*)totals);
+      hipLaunchKernelGGL((k_radix_down<K, true>), dim3(ntiles), dim3(256), 0, s, kin, kout,
+                         (positions && pass == 0) ? (const uint32_t*)nullptr : vin, vout, n, shift, nbits, counts,
+                         ntiles, (const int*)totals);
     else
       hipLaunchKernelGGL((k_radix_down<K, false>), dim3(ntiles), dim3(256), 0, s, kin, kout, nullptr, nullptr, n, shift,
                          nbits, counts, ntiles, (const int*)totals);
@@ -387,6 +394,10 @@ int radix_sort_u32(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t*
 int radix_sort_u64(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int begin_bit,
                    int end_bit, int* counts, int* tile_buf, hipStream_t s) {
   return radix_sort_impl<uint64_t>(keys, keys_alt, vals, vals_alt, n, begin_bit, end_bit, counts, tile_buf, s);
+}
+int radix_sort_u32_positions(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n,
+                             int end_bit, int* counts, hipStream_t s) {
+  return radix_sort_impl<uint32_t>(keys, keys_alt, vals, vals_alt, n, 0, end_bit, counts, nullptr, s, true, true);
 }
 
 }  // namespace gvdbx

@@ -32,5 +32,13 @@ int radix_sort_u32(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t*
                    int end_bit, int* counts, int* tile_buf, hipStream_t s);
 int radix_sort_u64(uint64_t* keys, uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n, int begin_bit,
                    int end_bit, int* counts, int* tile_buf, hipStream_t s);
+// Digit width of the first pass of a sort over `bits` bits (the remainder; later passes take 8).
+int radix_first_digit_bits(int bits);
+// radix_sort_u32 over bits [0, end_bit) of (key, input position) pairs: the values are not read
+// but generated (vals_alt / vals receive the positions), and the first pass's per-tile digit
+// counts are already in counts (digit-major, tiles of kRadixTile keys, written by the kernel
+// that made the keys), so the sort starts with that pass's digit scan.
+int radix_sort_u32_positions(uint32_t* keys, uint32_t* keys_alt, uint32_t* vals, uint32_t* vals_alt, int64_t n,
+                             int end_bit, int* counts, hipStream_t s);
 
 }  // namespace gvdbx
