@@ -32,18 +32,16 @@ struct Buf {
 enum MetaSlot { M_BBOX = 0, M_UNIQUE = 8, M_COUNT = 9, M_USED = 10, M_TOTAL = 11, M_LEV = 16, M_WIDE = 32, M_INTS = 64 };
 
 // Scratch of gvdbx_splat_trilinear_noalloc, for callers that cannot allocate device memory: the
-// splat's counting sort (key counts and starts, scan tiles, per-point key/rank, sorted payload
-// and permutation) sized for kNoAllocPoints points and kNoAllocKeys keys (27 per brick + 1).
+// splat's sort (keys and values, both ping-pong halves, the radix tile counts, key starts and
+// the sorted payload) sized for kNoAllocPoints points and kNoAllocKeys keys (27 per brick + 1).
 constexpr int64_t kNoAllocPoints = (int64_t)1 << 20;
 constexpr int64_t kNoAllocKeys = (int64_t)1 << 20;
 constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
-constexpr size_t kScrCnt = align256((size_t)(kNoAllocKeys + 1) * 4);
-constexpr size_t kScrStart = align256((size_t)(kNoAllocKeys + 1) * 4);
-constexpr size_t kScrTiles = align256((size_t)((kNoAllocKeys + 1 + kScanTile - 1) / kScanTile + 16) * 4);
-constexpr size_t kScrKeyrank = align256((size_t)(kNoAllocPoints + 1) * 8);
+constexpr size_t kScrWord = align256((size_t)(kNoAllocPoints + 1) * 4);  // one of keys, keys_alt, vals, vals_alt
+constexpr size_t kScrCounts = align256((size_t)(256 * ((kNoAllocPoints + kRadixTile - 1) / kRadixTile) + 256) * 4);
+constexpr size_t kScrStart = align256((size_t)(kNoAllocKeys + 2) * 4);
 constexpr size_t kScrPayload = align256((size_t)(kNoAllocPoints + 1) * 16);
-constexpr size_t kScrPerm = align256((size_t)(kNoAllocPoints + 1) * 4);
-constexpr size_t kScratchBytes = kScrCnt + kScrStart + kScrTiles + kScrKeyrank + kScrPayload + kScrPerm;
+constexpr size_t kScratchBytes = 4 * kScrWord + kScrCounts + kScrStart + kScrPayload;
 
 }  // namespace
 
@@ -154,10 +152,11 @@ struct gvdbx_ctx {
   bool spec_sorted = false;     // incre_launch also sorted the box keys (see there)
   bool spec_scanned = true;     // incre_launch's box launch scanned its block counts (else it appended its keys)
   Buf hkeys, hkeys_alt, hvals, hvals_alt, ranges, bbox_part, cnt_in, ocnt, ocur, wgb, nbm, spos, svel;
-  Buf sp_cnt, sp_start, sp_keyrank, sp_payload, sp_perm;  // splat: points counting-sorted by (brick, region)
+  Buf sp_keys, sp_keys_alt, sp_vals, sp_vals_alt, sp_counts, sp_start, sp_payload;  // splat: points sorted by (brick, region)
   Buf scatter_acc;  // ScatterDensity: 64-bit fixed-point sums per padded voxel
   Buf scatter_clr;  // ScatterDensity colour: per interior voxel sums (avgColor) or point index
   Buf wvs;          // insert: per-brick window velocity sums
+  Buf om_home, om_box, om_ch, om_cb, om_sh, om_sb;  // full rebuild by occupancy maps: maps, group counts, scans
   Buf list_clr, csum, cmean;  // subcell list colours; per subcell colour sums and means
   // brick order of the last splat (a permutation of its n points), reused by the splat
   // backward on the same point buffer for locality
@@ -842,6 +841,95 @@ int incre_pass(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool ac
   return GVDBX_OK;
 }
 
+// Full rebuild (full_pass, then extra_pass when r >= 1) by occupancy maps (kernels.hip): one
+// marking launch over the points, a count and scan per map and one read-back, then the nodes are
+// written in key order.  The maps cover the nodes the two passes can reach: the valid points lie
+// in [posmin, posmax], their boxes reach r below and 2r above; clipped to the root.  *done stays
+// false -- and nothing in the tree has changed -- when the maps would hold more than
+// kOmapMaxNodes entries (points spread over a sparse box, batched clouds) or a node fell outside
+// them; the caller then sorts keys.
+constexpr int64_t kOmapMaxNodes = (int64_t)1 << 26;
+
+int omap_rebuild(gvdbx_ctx* c, int64_t n, const float* tr, float r, int d, bool* done) {
+  *done = false;
+  if (c->batch || d < 1 || d >= kMaxLev) return GVDBX_OK;
+  OmapLevels M;
+  memset(&M, 0, sizeof(M));
+  M.nlev = d;
+  const int64_t Rr = (int64_t)1 << c->lgR[d];
+  const int64_t reach = (int64_t)std::ceil((double)r) + 2;
+  int64_t vlo[3], vhi[3];
+  for (int a = 0; a < 3; ++a) {
+    vlo[a] = std::max<int64_t>(c->root[a], (int64_t)std::floor((double)c->posmin[a]) - reach);
+    vhi[a] = std::min<int64_t>(c->root[a] + Rr - 1, (int64_t)std::ceil((double)c->posmax[a]) + 2 * reach);
+    if (vhi[a] < vlo[a]) return GVDBX_OK;
+  }
+  int64_t words = 0;  // groups of 32 map entries
+  for (int l = 0; l < d; ++l) {
+    const int sh = c->lgR[l];
+    int64_t nodes = 1;
+    for (int a = 0; a < 3; ++a) {
+      const int64_t lo = (vlo[a] >> sh) - (c->root[a] >> sh), hi = (vhi[a] >> sh) - (c->root[a] >> sh);
+      M.lo[l][a] = (int)lo;
+      M.n[l][a] = (int)(hi - lo + 1);
+      nodes *= hi - lo + 1;
+      if (nodes > kOmapMaxNodes) return GVDBX_OK;
+    }
+    M.word0[l] = words;
+    words += (nodes + 31) / 32;
+    if (words * 32 > kOmapMaxNodes) return GVDBX_OK;
+  }
+  M.word0[d] = words;
+  const bool box = r >= 1;
+  CHK(ensure(c, c->om_home, (size_t)words * 32));
+  if (box) CHK(ensure(c, c->om_box, (size_t)words * 32));
+  CHK(ensure(c, c->om_ch, (size_t)(words + 1) * 4));
+  CHK(ensure(c, c->om_cb, (size_t)(words + 1) * 4));
+  CHK(ensure(c, c->om_sh, (size_t)(words + 1) * 4));
+  CHK(ensure(c, c->om_sb, (size_t)(words + 1) * 4));
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(words + 1) + 16) * 4));
+  uint8_t* home = P<uint8_t>(c->om_home);
+  uint8_t* boxmap = box ? P<uint8_t>(c->om_box) : nullptr;
+  CHK(zero_meta(c));
+  fill_u32((uint32_t*)home, words * 8, 0u, c->stream);
+  if (box) fill_u32((uint32_t*)boxmap, words * 8, 0u, c->stream);
+  {
+    TimedRegion trg(c, "keys");
+    omap_mark(tree_of(c), points_of(c, n, tr), M, r, box, home, boxmap, P<int>(c->meta) + M_COUNT, c->stream);
+  }
+  omap_counts(home, boxmap, words, P<int>(c->om_ch), P<int>(c->om_cb), c->stream);
+  exclusive_scan(P<int>(c->om_ch), P<int>(c->om_sh), words, P<int>(c->tile_buf), P<int>(c->om_sh) + words, c->stream);
+  exclusive_scan(P<int>(c->om_cb), P<int>(c->om_sb), words, P<int>(c->tile_buf), P<int>(c->om_sb) + words, c->stream);
+  omap_level_totals(M, P<int>(c->om_sh), P<int>(c->om_sb), P<int>(c->meta) + M_LEV, P<int>(c->meta) + M_LEV + 8,
+                    c->stream);
+  CHK(read_meta(c));
+  if (c->h_meta[M_COUNT]) return GVDBX_OK;  // a node outside the maps: sort instead
+  // the tree changes from here on (each level holds at most 2 kOmapMaxNodes < 2^31 nodes)
+  c->bm_valid = false;
+  OmapCounts C;
+  memset(&C, 0, sizeof(C));
+  CHK(reserve_level(c, d, 1));
+  c->cnt[d] = 1;
+  for (int l = 0; l < d; ++l) {
+    C.home[l] = (uint32_t)c->h_meta[M_LEV + l];
+    const int64_t cnt = C.home[l] + (uint32_t)c->h_meta[M_LEV + 8 + l];
+    CHK(reserve_level(c, l, cnt));
+    c->cnt[l] = cnt;
+    c->new_nodes[l] = cnt;
+  }
+  for (int l = d; l < kMaxLev; ++l) c->new_nodes[l] = 0;
+  DevTree T = tree_of(c);
+  setup_root(T, c->stream);
+  fill_u64(P<uint64_t>(c->clist[d]), clist_per(c, d), kUndef64, c->stream);
+  omap_nodes(T, M, home, boxmap, P<int>(c->om_sh), P<int>(c->om_sb), C, c->stream);
+  for (int l = 1; l < d; ++l)
+    if (c->cnt[l]) fill_u64(P<uint64_t>(c->clist[l]), c->cnt[l] * clist_per(c, l), kUndef64, c->stream);
+  link_all(c, d);
+  c->used0 = c->cnt[0];
+  *done = true;
+  return GVDBX_OK;
+}
+
 int check_points(gvdbx_ctx* c, int64_t n) {
   if (!c || !c->configured) return fail(c, GVDBX_ERR_ARG, "context not configured");
   if (!c->pos) return fail(c, GVDBX_ERR_STATE, "no points bound (gvdbx_set_points)");
@@ -914,10 +1002,12 @@ int rebuild_impl(gvdbx_ctx* c, int64_t n, float r, const float* orig, bool accum
   if (c->rebuild) {
     c->depth = depth;
     for (int a = 0; a < 3; ++a) c->root[a] = root[a];
-    if (wide_keys(c, depth)) {
+    bool done = false;
+    CHK(omap_rebuild(c, n, orig, r, depth, &done));
+    if (!done && wide_keys(c, depth)) {
       CHK(full_pass<uint64_t>(c, n, orig, depth));
       if (r >= 1) CHK(extra_pass<uint64_t>(c, n, orig, r, depth));
-    } else {
+    } else if (!done) {
       CHK(full_pass<uint32_t>(c, n, orig, depth));
       if (r >= 1) CHK(extra_pass<uint32_t>(c, n, orig, r, depth));
     }
@@ -1100,9 +1190,9 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
                  &c->sc_nid, &c->list_pos, &c->list_vel, &c->map_leaf, &c->vsum, &c->hkeys, &c->hkeys_alt,
                  &c->hvals, &c->hvals_alt, &c->ranges, &c->bbox_part, &c->cnt_in, &c->ocnt, &c->ocur, &c->wgb, &c->nbm, &c->spos, &c->svel, &c->keys, &c->keys_alt, &c->flags,
                  &c->rank, &c->radix_counts, &c->tile_buf, &c->markers, &c->meta, &c->box_region, &c->box_cnt,
-                 &c->box_off, &c->sp_cnt, &c->sp_start, &c->sp_keyrank, &c->sp_payload, &c->sp_perm, &c->scatter_acc, &c->wvs,
+                 &c->box_off, &c->sp_keys, &c->sp_keys_alt, &c->sp_vals, &c->sp_vals_alt, &c->sp_counts, &c->sp_start, &c->sp_payload, &c->scatter_acc, &c->wvs,
                  &c->list_clr, &c->csum, &c->cmean, &c->scatter_clr, &c->bmap, &c->list_q16, &c->list_vq16,
-                 &c->cloud_id,
+                 &c->cloud_id, &c->om_home, &c->om_box, &c->om_ch, &c->om_cb, &c->om_sh, &c->om_sb,
                  &c->cloud_off_dev, &c->queue, &c->graph_len};
   for (Buf* b : bufs)
     if (b->p) (void)hipFree(b->p);
@@ -1574,8 +1664,8 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   uint32_t* svals;
   {
     TimedRegion tr(c, "home");
-    int bits = 1;
-    while (((int64_t)1 << bits) <= nmap) ++bits;
+    int bits = 1;  // keys run up to the sentinel nmap + 1
+    while (((int64_t)1 << bits) <= (int64_t)nmap + 1) ++bits;
     // the key launch also counts the first radix digit; the sort generates the point positions
     home_keys_counted(T, pv, mapp, nmap, P<uint32_t>(c->hkeys), bits, P<int>(c->radix_counts), c->stream);
     const int par = radix_sort_u32_positions(P<uint32_t>(c->hkeys), P<uint32_t>(c->hkeys_alt), P<uint32_t>(c->hvals),
@@ -1584,7 +1674,8 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     svals = par ? P<uint32_t>(c->hvals_alt) : P<uint32_t>(c->hvals);
     key_ranges(skeys, n, nmap, P<int>(c->ranges), c->stream);
   }
-  const int* tail = P<int>(c->ranges) + nmap;  // points whose own leaf is missing, and invalid ones
+  // [tail[0], tail[1]): valid points whose own leaf is missing (invalid points sort after them)
+  const int* tail = P<int>(c->ranges) + nmap;
   // Sorted copies of the points for the per-brick kernels: with bricks of 64 points or more on
   // average the count pass gathers them itself (its bricks' points, through the permutation);
   // sparse bricks leave the count pass too little work per point to hide that gather's latency,
@@ -2012,15 +2103,16 @@ int field_checks(gvdbx_ctx* c, int chan, const void* q, int64_t n) {
   return GVDBX_OK;
 }
 
-// Scratch of one splat: key counts and starts, scan tiles, per-point (key, rank), the sorted
-// (q, m) payload and its permutation.
+// Scratch of one splat: the sort's keys and values (both ping-pong halves), the radix tile
+// counts, the key starts and the sorted (q, m) payload.
 struct SplatWork {
-  uint32_t* cnt;
+  uint32_t* keys;
+  uint32_t* keys_alt;
+  uint32_t* vals;
+  uint32_t* vals_alt;
+  int* counts;
   int* start;
-  int* tiles;
-  uint2* keyrank;
   float4* payload;
-  uint32_t* perm;
 };
 
 int splat_limits(gvdbx_ctx* c) {
@@ -2031,17 +2123,27 @@ int splat_limits(gvdbx_ctx* c) {
 }
 
 // Trilinear splat of (q, mass) into the interior of chan (sorted pull, see splat.hip), with the
-// sort's scratch in w (sized for n points and nslots * 27 + 1 keys).
+// sort's scratch in w (sized for n points and nslots * 27 + 1 keys); *perm receives the sorted
+// position -> point permutation (inside w).
 int splat_run(gvdbx_ctx* c, int chan, const float* q, const float* mass, int64_t n, int accum, const SplatWork& w,
-              bool timed) {
+              bool timed, const uint32_t** perm) {
   const uint32_t sentinel = (uint32_t)(c->nslots * 27);
-  const int64_t nkeys = (int64_t)sentinel + 1;
+  *perm = nullptr;
   {
     TimedRegion tr(c, "splat_sort", timed);
-    fill_u32(w.cnt, nkeys, 0u, c->stream);  // a fill kernel: the scratch may hold a previous call's counts
-    splat_count(tree_of(c), q, n, P<int>(c->leaf_slot), c->ld[0], sentinel, w.cnt, w.keyrank, c->stream);
-    exclusive_scan((const int*)w.cnt, w.start, nkeys, w.tiles, nullptr, c->stream);
-    splat_place(q, mass, n, w.keyrank, w.start, w.payload, w.perm, c->stream);
+    const uint32_t* skeys = w.keys;
+    if (n > 0) {
+      int bits = 1;  // keys run up to the sentinel
+      while (((int64_t)1 << bits) <= (int64_t)sentinel) ++bits;
+      // the key launch also counts the first radix digit; the sort generates the point positions
+      splat_keys(tree_of(c), q, n, P<int>(c->leaf_slot), c->ld[0], sentinel, w.keys, bits, w.counts, c->stream);
+      const int par = radix_sort_u32_positions(w.keys, w.keys_alt, w.vals, w.vals_alt, n, bits, w.counts, c->stream);
+      skeys = par ? w.keys_alt : w.keys;
+      *perm = par ? w.vals_alt : w.vals;
+      // the other value buffer takes the inverse permutation
+      splat_place(q, mass, n, *perm, par ? w.vals : w.vals_alt, w.payload, c->stream);
+    }
+    key_starts(skeys, n, (int)sentinel, w.start, c->stream);  // start[0 .. sentinel + 1]
   }
   {
     TimedRegion tr(c, "splat_brick", timed);
@@ -2056,18 +2158,20 @@ int splat_impl(gvdbx_ctx* c, int chan, const float* q, const float* mass, int64_
   CHK(splat_limits(c));
   if (c->nslots == 0) return GVDBX_OK;
   const int64_t nkeys = c->nslots * 27 + 1;
-  CHK(ensure(c, c->sp_cnt, (size_t)(nkeys + 1) * 4));
+  CHK(ensure(c, c->sp_keys, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->sp_keys_alt, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->sp_vals, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->sp_vals_alt, (size_t)(n + 1) * 4));
+  CHK(ensure(c, c->sp_counts, (size_t)radix_count_ints(n) * 4));
   CHK(ensure(c, c->sp_start, (size_t)(nkeys + 1) * 4));
-  CHK(ensure(c, c->sp_keyrank, (size_t)(n + 1) * 8));
   CHK(ensure(c, c->sp_payload, (size_t)(n + 1) * 16));
-  CHK(ensure(c, c->sp_perm, (size_t)(n + 1) * 4));
-  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(nkeys + 1) + 16) * 4));
-  const SplatWork w{P<uint32_t>(c->sp_cnt), P<int>(c->sp_start), P<int>(c->tile_buf),
-                    P<uint2>(c->sp_keyrank), P<float4>(c->sp_payload), P<uint32_t>(c->sp_perm)};
-  CHK(splat_run(c, chan, q, mass, n, accum, w, true));
+  const SplatWork w{P<uint32_t>(c->sp_keys), P<uint32_t>(c->sp_keys_alt), P<uint32_t>(c->sp_vals),
+                    P<uint32_t>(c->sp_vals_alt), P<int>(c->sp_counts), P<int>(c->sp_start), P<float4>(c->sp_payload)};
+  const uint32_t* perm = nullptr;
+  CHK(splat_run(c, chan, q, mass, n, accum, w, true, &perm));
   c->sp_order_q = n > 0 ? q : nullptr;
   c->sp_order_n = n;
-  c->sp_order = n > 0 ? P<uint32_t>(c->sp_perm) : nullptr;
+  c->sp_order = perm;
   return GVDBX_OK;
 }
 
@@ -2095,11 +2199,12 @@ int gvdbx_splat_trilinear_noalloc(gvdbx_ctx* c, int chan, const void* q, const v
     return fail(c, GVDBX_ERR_HIP, "no scratch array on device %d: %s", c->device, hipGetErrorString(c->scratch_err));
   if (c->nslots == 0) return GVDBX_OK;
   unsigned char* base = (unsigned char*)c->scratch;
-  const SplatWork w{(uint32_t*)base, (int*)(base + kScrCnt), (int*)(base + kScrCnt + kScrStart),
-                    (uint2*)(base + kScrCnt + kScrStart + kScrTiles),
-                    (float4*)(base + kScrCnt + kScrStart + kScrTiles + kScrKeyrank),
-                    (uint32_t*)(base + kScrCnt + kScrStart + kScrTiles + kScrKeyrank + kScrPayload)};
-  CHK(splat_run(c, chan, (const float*)q, (const float*)mass, n, accum, w, false));
+  const SplatWork w{(uint32_t*)base, (uint32_t*)(base + kScrWord), (uint32_t*)(base + 2 * kScrWord),
+                    (uint32_t*)(base + 3 * kScrWord), (int*)(base + 4 * kScrWord),
+                    (int*)(base + 4 * kScrWord + kScrCounts),
+                    (float4*)(base + 4 * kScrWord + kScrCounts + kScrStart)};
+  const uint32_t* perm = nullptr;
+  CHK(splat_run(c, chan, (const float*)q, (const float*)mass, n, accum, w, false, &perm));
   // the permutation lives in the shared scratch: the backward does not reuse it
   c->sp_order_q = nullptr;
   c->sp_order_n = 0;

@@ -100,44 +100,45 @@ void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s, u
 }
 
 // ------------------------------------------------------------------ home keys
-// Key = subcell-map index of the leaf holding trunc(q); points that are invalid, NOHIT or
-// whose own leaf is missing get the sentinel nmap and are binned by the per-point path.
+// Key = subcell-map index of the leaf holding trunc(q); valid points whose own leaf is missing
+// get the sentinel nmap and are binned by the per-point path; invalid and NOHIT points, which
+// enter no subcell, get nmap + 1 and sort last, outside every range a pass walks.
 // The home keys (no values: the sort generates the positions), and per tile of kRadixTile points
 // the counts of the first radix digit (the low `nbits` bits), digit-major into
 // counts[digit * ntiles + tile] for all 256 digits, as k_radix_up would count them.
-__global__ void __launch_bounds__(256) k_home_keys_counted(DevTree T, PointsView P, const int* __restrict__ map,
+__global__ void __launch_bounds__(kKeyBlock) k_home_keys_counted(DevTree T, PointsView P, const int* __restrict__ map,
                                                           int nmap, uint32_t* __restrict__ keys, int nbits,
                                                           int* __restrict__ counts, int ntiles) {
   __shared__ int hist[256];
-  hist[threadIdx.x] = 0;
+  if (threadIdx.x < 256) hist[threadIdx.x] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kRadixTile;
   const uint32_t mask = (1u << nbits) - 1u;
-#pragma unroll 4
-  for (int r = 0; r < kRadixTile / 256; ++r) {
-    const int64_t i = base + r * 256 + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kRadixTile / kKeyBlock; ++r) {
+    const int64_t i = base + r * kKeyBlock + threadIdx.x;
     if (i >= P.n) continue;
     const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
-    uint32_t key = (uint32_t)nmap;
+    uint32_t key = (uint32_t)nmap + 1u;
     if (p.z != kNoHit) {
       const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
       if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
         const int3 o = cloud_origin(P, i);
         const int64_t lf = find_leaf(T, (int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z);
-        if (lf >= 0) key = (uint32_t)map[lf];
+        key = lf >= 0 ? (uint32_t)map[lf] : (uint32_t)nmap;
       }
     }
     keys[i] = key;
     atomicAdd(&hist[key & mask], 1);
   }
   __syncthreads();
-  counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
+  if (threadIdx.x < 256) counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
 void home_keys_counted(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, int bits,
                        int* counts, hipStream_t s) {
   const int ntiles = (int)((P.n + kRadixTile - 1) / kRadixTile);
-  hipLaunchKernelGGL(k_home_keys_counted, dim3(ntiles), dim3(256), 0, s, T, P, map, nmap, keys,
+  hipLaunchKernelGGL(k_home_keys_counted, dim3(ntiles), dim3(kKeyBlock), 0, s, T, P, map, nmap, keys,
                      radix_first_digit_bits(bits), counts, ntiles);
 }
 
@@ -181,6 +182,36 @@ __global__ void k_ranges(const uint32_t* __restrict__ skeys, int64_t n, int nmap
 
 void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStream_t s) {
   hipLaunchKernelGGL(k_ranges, dim3(nblocks((int64_t)nmap + 2, 256)), dim3(256), 0, s, skeys, n, nmap, start);
+}
+
+// The same starts from the boundaries of the sorted keys (keys in [0, nmax]): sorted position j
+// is the start of every key in (skeys[j - 1], skeys[j]] (j = 0: [0, skeys[0]]; j = n: up to
+// nmax + 1), so each start is written once, by one pass over the keys -- cheaper than a search
+// per key when there are about as many keys as points.  A lane fills a short run of keys
+// itself; runs longer than a wave (a block of empty keys) are filled by the whole wave.
+__global__ void __launch_bounds__(256) k_key_starts(const uint32_t* __restrict__ skeys, int64_t n, int nmax,
+                                                    int* __restrict__ start) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = j <= n;
+  int64_t a = 0, b = -1;
+  if (live) {
+    a = j == 0 ? 0 : (int64_t)skeys[j - 1] + 1;
+    b = j == n ? (int64_t)nmax + 1 : (int64_t)skeys[j];
+  }
+  const bool wide = b - a >= 64;
+  if (!wide)
+    for (int64_t k = a; k <= b; ++k) start[k] = (int)j;
+  uint64_t pending = __ballot(wide);
+  while (pending) {
+    const int l = __ffsll((unsigned long long)pending) - 1;
+    const int64_t la = __shfl(a, l, 64), lb = __shfl(b, l, 64), lj = __shfl(j, l, 64);
+    for (int64_t k = la + lane_id(); k <= lb; k += 64) start[k] = (int)lj;
+    pending &= pending - 1;
+  }
+}
+
+void key_starts(const uint32_t* skeys, int64_t n, int nmax, int* start, hipStream_t s) {
+  hipLaunchKernelGGL(k_key_starts, dim3(nblocks(n + 1, 256)), dim3(256), 0, s, skeys, n, nmax, start);
 }
 
 // ------------------------------------------------------------------ per-brick binning
@@ -717,14 +748,14 @@ void cell_bases(const BinArgs& A, int64_t nsc, int* cnt, int* tile, int* off, hi
 }
 
 // ------------------------------------------------------------------ per-point path
-// Points from sorted position *begin on (the sentinel tail: points whose own leaf is missing,
-// and invalid ones) are binned one pair at a time with global atomics.
+// Sorted positions [begin[0], begin[1]) (the points whose own leaf is missing) are binned one
+// pair at a time with global atomics.
 template <bool kInsert>
 __global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __restrict__ begin) {
-  const int64_t b0 = *begin;
+  const int64_t b0 = begin[0], b1 = begin[1];
   const SubcellCfg c = A.c;
   const bool vel = A.vsum != nullptr && A.P.vel != nullptr;
-  for (int64_t j = b0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < A.P.n;
+  for (int64_t j = b0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < b1;
        j += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = (int64_t)A.svals[j];
     const float3 p = load_f3(A.P.pos, i, A.P.pos_off, A.P.pos_stride);
@@ -759,14 +790,16 @@ __global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __rest
   }
 }
 
+// Points whose own leaf is missing are rare (an insert of points the rebuild did not see), so a
+// small grid strides over them.
+static inline int bin_points_blocks(int64_t n) { return nblocks(n, 256) < 256 ? nblocks(n, 256) : 256; }
+
 void bin_points_count(const BinArgs& A, const int* begin, hipStream_t s) {
-  const int nb = nblocks(A.P.n, 256) < 2048 ? nblocks(A.P.n, 256) : 2048;
-  hipLaunchKernelGGL(k_bin_points<false>, dim3(nb), dim3(256), 0, s, A, begin);
+  hipLaunchKernelGGL(k_bin_points<false>, dim3(bin_points_blocks(A.P.n)), dim3(256), 0, s, A, begin);
 }
 
 void bin_points_insert(const BinArgs& A, const int* begin, hipStream_t s) {
-  const int nb = nblocks(A.P.n, 256) < 2048 ? nblocks(A.P.n, 256) : 2048;
-  hipLaunchKernelGGL(k_bin_points<true>, dim3(nb), dim3(256), 0, s, A, begin);
+  hipLaunchKernelGGL(k_bin_points<true>, dim3(bin_points_blocks(A.P.n)), dim3(256), 0, s, A, begin);
 }
 
 // ------------------------------------------------------------------ gathers

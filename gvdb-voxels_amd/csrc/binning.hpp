@@ -19,6 +19,8 @@ void batch_ids(const PointsView& P, const int64_t* off, int ncloud, float limit,
 void home_keys_counted(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, int bits,
                        int* counts, hipStream_t s);
 void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStream_t s);
+// the same start[0 .. nmax + 1] of sorted keys in [0, nmax], by one pass over the keys
+void key_starts(const uint32_t* skeys, int64_t n, int nmax, int* start, hipStream_t s);
 
 struct BinArgs {
   LaunchCfg L;   // launch shape of the count and insert kernels

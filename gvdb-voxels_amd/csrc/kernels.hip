@@ -432,6 +432,160 @@ __global__ void k_setup_root(DevTree T) {
 }
 void setup_root(const DevTree& T, hipStream_t s) { hipLaunchKernelGGL(k_setup_root, dim3(1), dim3(1), 0, s, T); }
 
+// ---------------------------------------------------------------- full rebuild by occupancy maps
+// The nodes of a full rebuild are, per level below the root, the sorted unique covering nodes of
+// the valid points (ActivateBricksGPU) followed by the sorted unique extra-pass box nodes that
+// are not among them (ActivateExtraBricksGPU).  Instead of sorting one key per point and level,
+// each level gets two occupancy maps over a box of its root-relative node coordinates, one byte
+// per node, laid out x-major like the compact keys, so map order is key order: `home` marks the
+// covering nodes, `box` the box nodes (a superset: a point's box holds its own voxel).  Marks are
+// plain byte stores of 1 (every writer stores the same value, and the caches write back only the
+// bytes they hold dirty), so no mark needs an atomic.  Each group of 32 nodes is then read as a
+// 32-bit mask; counting and ranking the set bits gives every node its pool index, in the order
+// the sorts give.
+
+__device__ __forceinline__ int64_t omap_index(const OmapLevels& M, int l, int xr, int yr, int zr) {
+  const unsigned x = (unsigned)(xr - M.lo[l][0]), y = (unsigned)(yr - M.lo[l][1]), z = (unsigned)(zr - M.lo[l][2]);
+  if (x >= (unsigned)M.n[l][0] || y >= (unsigned)M.n[l][1] || z >= (unsigned)M.n[l][2]) return -1;
+  return M.word0[l] * 32 + ((int64_t)x * M.n[l][1] + y) * M.n[l][2] + z;
+}
+
+// Marks node (x, y, z) (voxel corner, inside the root) and its ancestors below the root; the
+// plain read skips the store where the mark is there already (a stale read only costs a store of
+// the same value).  A node outside the map box sets the overflow flag (the host then sorts).
+__device__ __forceinline__ void omap_mark_chain(const DevTree& T, const OmapLevels& M, uint8_t* map, int x, int y,
+                                                int z, int* overflow) {
+  for (int l = 0; l < T.depth; ++l) {
+    const int sh = T.lgR[l];
+    const int64_t b = omap_index(M, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
+                                 (z >> sh) - (T.root[2] >> sh));
+    if (b < 0) {
+      *overflow = 1;
+      return;
+    }
+    if (map[b] == 0) map[b] = 1;
+  }
+}
+
+// Per valid point: its covering node at every level below the root (gvdbCalcBrickId,
+// P.cuh:352-382: trunc(q) >> log2 R for q >= 0) into `home`; with box, every leaf-level node of
+// its extra-pass box [trunc(trunc(q) - r), trunc(trunc(q) + 2r)] (the corners gvdbCalcExtraBrickId
+// enumerates, P.cuh:467-512, as in k_box_keys) inside the root, with its ancestors, into `box`.
+__global__ void __launch_bounds__(256) k_omap_mark(DevTree T, PointsView P, OmapLevels M, float r, int box,
+                                                   uint8_t* __restrict__ home, uint8_t* __restrict__ boxmap,
+                                                   int* __restrict__ overflow) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float3 p = load_f3(P.pos, i, P.pos_off, P.pos_stride);
+  const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
+  if (q[0] < 0 || q[1] < 0 || q[2] < 0) return;
+  const int iq[3] = {(int)q[0], (int)q[1], (int)q[2]};
+  omap_mark_chain(T, M, home, iq[0], iq[1], iq[2], overflow);
+  if (!box) return;
+  const int R0 = 1 << T.lgR[0];
+  int lo[3], hi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float tq = (float)iq[a];
+    lo[a] = (int)(tq - r) / R0 * R0;
+    hi[a] = (int)(tq + 2.0f * r) / R0 * R0;
+  }
+  for (int x = lo[0]; x <= hi[0]; x += R0)
+    for (int y = lo[1]; y <= hi[1]; y += R0)
+      for (int z = lo[2]; z <= hi[2]; z += R0)
+        if (in_root(T, x, y, z)) omap_mark_chain(T, M, boxmap, x, y, z, overflow);
+}
+
+void omap_mark(const DevTree& T, const PointsView& P, const OmapLevels& M, float r, bool box, uint8_t* home,
+               uint8_t* boxmap, int* overflow, hipStream_t s) {
+  if (P.n <= 0) return;
+  hipLaunchKernelGGL(k_omap_mark, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, M, r, box ? 1 : 0, home, boxmap,
+                     overflow);
+}
+
+// Bit j = (byte j of group w != 0), for the 32 bytes of group w.
+__device__ __forceinline__ uint32_t omap_group(const uint8_t* __restrict__ map, int64_t w) {
+  const uint4* g = (const uint4*)(map + w * 32);
+  uint32_t m = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint4 v = g[h];
+    const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if ((u[k] >> (8 * b)) & 0xFFu) m |= 1u << (h * 16 + k * 4 + b);
+  }
+  return m;
+}
+
+// Per group of 32 nodes: the home nodes (ch) and the box nodes that are not home nodes (cb).
+__global__ void k_omap_counts(const uint8_t* __restrict__ home, const uint8_t* __restrict__ boxmap, int64_t words,
+                              int* __restrict__ ch, int* __restrict__ cb) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= words) return;
+  const uint32_t h = omap_group(home, w);
+  ch[w] = __popc(h);
+  cb[w] = boxmap ? __popc(omap_group(boxmap, w) & ~h) : 0;
+}
+
+void omap_counts(const uint8_t* home, const uint8_t* boxmap, int64_t words, int* ch, int* cb, hipStream_t s) {
+  hipLaunchKernelGGL(k_omap_counts, dim3(blocks_for(words, 256)), dim3(256), 0, s, home, boxmap, words, ch, cb);
+}
+
+// Per level: the home and the other box node counts, from the exclusive scans of the group
+// counts (sh, sb hold words + 1 entries: the scan with the total appended).
+__global__ void k_omap_level_totals(OmapLevels M, const int* __restrict__ sh, const int* __restrict__ sb,
+                                    int* __restrict__ home_cnt, int* __restrict__ box_cnt) {
+  const int l = threadIdx.x;
+  if (l >= M.nlev) return;
+  home_cnt[l] = sh[M.word0[l + 1]] - sh[M.word0[l]];
+  box_cnt[l] = sb[M.word0[l + 1]] - sb[M.word0[l]];
+}
+
+void omap_level_totals(const OmapLevels& M, const int* sh, const int* sb, int* home_cnt, int* box_cnt,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_omap_level_totals, dim3(1), dim3(64), 0, s, M, sh, sb, home_cnt, box_cnt);
+}
+
+// Writes the node of every marked entry: level l's pool holds its home nodes in map order, then
+// its other box nodes in map order (C.home[l] = the level's home count).  Child lists of inner
+// nodes are filled by the caller, links by the link launches.
+__global__ void k_omap_nodes(DevTree T, OmapLevels M, const uint8_t* __restrict__ home,
+                             const uint8_t* __restrict__ boxmap, const int* __restrict__ sh,
+                             const int* __restrict__ sb, OmapCounts C) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= M.word0[M.nlev]) return;
+  int l = 0;
+  while (w >= M.word0[l + 1]) ++l;
+  const uint32_t h = omap_group(home, w), b = boxmap ? (omap_group(boxmap, w) & ~h) : 0u;
+  if (!h && !b) return;
+  const int sft = T.lgR[l];
+  const int64_t first = M.word0[l] * 32;
+  auto put = [&](int bitpos, int64_t idx) {
+    const int64_t e = w * 32 + bitpos - first;  // entry inside the level's box
+    const int nz = M.n[l][2], ny = M.n[l][1];
+    const int zr = (int)(e % nz) + M.lo[l][2], yr = (int)((e / nz) % ny) + M.lo[l][1],
+              xr = (int)(e / ((int64_t)nz * ny)) + M.lo[l][0];
+    const int x = ((T.root[0] >> sft) + xr) << sft, y = ((T.root[1] >> sft) + yr) << sft,
+              z = ((T.root[2] >> sft) + zr) << sft;
+    const uint64_t cl = l > 0 ? (1ull | ((uint64_t)l << 8) | ((uint64_t)idx << 16)) : kUndefL;
+    write_node(T.nodes[l] + idx, l, x, y, z, cl);
+  };
+  int64_t ih = sh[w] - sh[M.word0[l]];
+  for (uint32_t m = h; m; m &= m - 1) put(__ffs(m) - 1, ih++);
+  int64_t ib = C.home[l] + sb[w] - sb[M.word0[l]];
+  for (uint32_t m = b; m; m &= m - 1) put(__ffs(m) - 1, ib++);
+}
+
+void omap_nodes(const DevTree& T, const OmapLevels& M, const uint8_t* home, const uint8_t* boxmap, const int* sh,
+                const int* sb, const OmapCounts& C, hipStream_t s) {
+  const int64_t words = M.word0[M.nlev];
+  if (words <= 0) return;
+  hipLaunchKernelGGL(k_omap_nodes, dim3(blocks_for(words, 256)), dim3(256), 0, s, T, M, home, boxmap, sh, sb, C);
+}
+
 // gvdbLinkBricks (N.cuh:470-505): flagged node -> parent by descent, parent's child slot.
 __global__ void k_link_level(DevTree T, int lev, int64_t first, int64_t count) {
   const int64_t i = first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -605,14 +759,17 @@ void assign_slots(const DevTree& T, const int* flags, const int* rank, int64_t c
 __global__ void k_sc_positions(DevTree T, int64_t leaves, SubcellCfg c, const int* __restrict__ map,
                                int* __restrict__ sc_pos, int* __restrict__ sc_nid, int* __restrict__ map_leaf,
                                int* __restrict__ ocnt, unsigned long long* __restrict__ vsum) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= leaves) return;
-  const gvdbx_node* nd = T.nodes[0] + i;
-  if (nd->parent == kUndef64) return;
-  map_leaf[map[i]] = (int)i;
+  // one thread per (leaf, subcell), grid-strided
   const int per = c.D * c.D * c.D;
-  for (int s = 0; s < per; ++s) {
-    const int64_t k = (int64_t)map[i] * per + s;
+  const int64_t total = leaves * per;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / per;
+    const int s = (int)(t - i * per);
+    const gvdbx_node* nd = T.nodes[0] + i;
+    if (nd->parent == kUndef64) continue;
+    const int mi = map[i];
+    if (s == 0) map_leaf[mi] = (int)i;
+    const int64_t k = (int64_t)mi * per + s;
     sc_pos[3 * k + 0] = nd->pos[0] + (s % c.D) * c.sc;
     sc_pos[3 * k + 1] = nd->pos[1] + ((s / c.D) % c.D) * c.sc;
     sc_pos[3 * k + 2] = nd->pos[2] + ((s / c.D / c.D) % c.D) * c.sc;
@@ -624,7 +781,8 @@ __global__ void k_sc_positions(DevTree T, int64_t leaves, SubcellCfg c, const in
 void sc_positions(const DevTree& T, int64_t leaves, SubcellCfg c, const int* map, int* sc_pos, int* sc_nid,
                   int* map_leaf, int* ocnt, unsigned long long* vsum, hipStream_t s) {
   if (leaves <= 0) return;
-  hipLaunchKernelGGL(k_sc_positions, dim3(blocks_for(leaves, 256)), dim3(256), 0, s, T, leaves, c, map, sc_pos,
+  const int64_t total = leaves * c.D * c.D * c.D;
+  hipLaunchKernelGGL(k_sc_positions, dim3(blocks_for(total, 256, 65536)), dim3(256), 0, s, T, leaves, c, map, sc_pos,
                      sc_nid, map_leaf, ocnt, vsum);
 }
 

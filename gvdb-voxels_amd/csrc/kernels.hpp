@@ -72,6 +72,32 @@ template <typename K>
 void setup_nodes(const DevTree& T, const K* k, const int* flags, const int* rank, int64_t m, KeyFmt f,
                  const AllocPlan& plan, hipStream_t s, bool link_leaves = false);
 void setup_root(const DevTree& T, hipStream_t s);
+
+// Occupancy maps of a full rebuild: per level below the root, a box of root-relative node
+// coordinates, one byte per node (x-major order = key order), each level starting on a group of
+// 32 nodes ("word").
+struct OmapLevels {
+  int nlev;                     // levels below the root
+  int lo[kMaxLev][3];           // the box's first node coordinates
+  int n[kMaxLev][3];            // its size in nodes
+  int64_t word0[kMaxLev + 1];   // first group of each level; word0[nlev] = all groups
+};
+struct OmapCounts {
+  int64_t home[kMaxLev];  // home (covering) nodes per level
+};
+// marks the covering nodes of the valid points into home and, with box, their extra-pass box
+// nodes into boxmap; *overflow = 1 when a node lies outside the map box
+void omap_mark(const DevTree& T, const PointsView& P, const OmapLevels& M, float r, bool box, uint8_t* home,
+               uint8_t* boxmap, int* overflow, hipStream_t s);
+// per group: the home nodes (ch) and the box nodes that are not home nodes (cb); boxmap may be null
+void omap_counts(const uint8_t* home, const uint8_t* boxmap, int64_t words, int* ch, int* cb, hipStream_t s);
+// per level the node counts from the scans of ch / cb (words + 1 entries each)
+void omap_level_totals(const OmapLevels& M, const int* sh, const int* sb, int* home_cnt, int* box_cnt,
+                       hipStream_t s);
+// writes every level's nodes (home nodes, then the other box nodes, each in key order) into the
+// pools, which start empty
+void omap_nodes(const DevTree& T, const OmapLevels& M, const uint8_t* home, const uint8_t* boxmap, const int* sh,
+                const int* sb, const OmapCounts& C, hipStream_t s);
 // gvdbLinkBricks over the level-lev nodes [first, first + count)
 void link_level(const DevTree& T, int lev, int64_t first, int64_t count, hipStream_t s);
 // gvdbDelinkLeafBricks over the first count leaves; with markers, the flags of the first nmark

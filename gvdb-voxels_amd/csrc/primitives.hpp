@@ -6,6 +6,7 @@ namespace gvdbx {
 
 constexpr int kScanTile = 4096;
 constexpr int kRadixTile = 4096;
+constexpr int kKeyBlock = 1024;  // threads of the kernels that make sort keys with the first digit's tile counts
 
 void fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 void fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);

@@ -6,13 +6,14 @@
 // in no brick are skipped, as the sampler returns 0 there.
 //
 // Forward (deterministic, no device-scope atomics on the field):
-//   1. k_splat_count: key = slot(home brick) * 27 + region, where region records per axis whether
+//   1. k_splat_keys: key = slot(home brick) * 27 + region, where region records per axis whether
 //      the point's dual cell pokes into the lower (0) or upper (2) neighbour brick or neither (1);
-//      points in no brick get the last key.  Each point takes a rank inside its key from a
-//      wave-aggregated atomic counter (a counting sort: the order inside a key is free);
-//   2. exclusive scan of the counts = start[] of every key;
-//   3. k_splat_place: (q, m) of each point written as float4 at start[key] + rank, and the
-//      permutation (sorted position -> point) kept for the backward;
+//      points in no brick get the last key; the launch also counts the first radix digit;
+//   2. LSD radix sort of the keys with the point positions as values (the permutation sorted
+//      position -> point, kept for the backward), then start[] of every key from the boundaries
+//      of the sorted keys;
+//   3. k_splat_place: (q, m) of each point as float4 at its sorted position (the inverse
+//      permutation scatters them);
 //   4. k_splat_brick: one wave per brick pulls its own points plus exactly those points of the
 //      26 neighbours whose region reaches it, and sums m * w in LDS as 64-bit integers with a
 //      per-brick power-of-two scale taken from max |m| -- integer sums are order independent, so
@@ -53,43 +54,46 @@ __device__ __forceinline__ uint32_t splat_key(const DevTree& T, float px, float 
   return (uint32_t)slot * 27u + (uint32_t)(rz * 9 + ry * 3 + rx);
 }
 
-// key and rank of every point; lanes of a wave that share a key take one atomic together
-__global__ void __launch_bounds__(256) k_splat_count(DevTree T, const float* __restrict__ q, int64_t n,
-                                                     const int* __restrict__ leaf_slot, int lgres, uint32_t sentinel,
-                                                     uint32_t* __restrict__ cnt, uint2* __restrict__ keyrank) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = i < n;
-  const uint32_t key = live ? splat_key(T, q[3 * i], q[3 * i + 1], q[3 * i + 2], leaf_slot, lgres, sentinel) : 0u;
-  const int lane = lane_id();
-  uint64_t pending = __ballot(live);
-  int leader = 0, pos = 0, size = 0;
-  while (pending) {
-    const int l = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t kl = __shfl(key, l, 64);
-    const uint64_t grp = __ballot(live && key == kl) & pending;
-    if (live && key == kl) {
-      leader = l;
-      pos = __popcll(grp & lanemask_lt());
-      size = __popcll(grp);
-    }
-    pending &= ~grp;
+// Key of every point, and per tile of kRadixTile points the counts of the first radix digit
+// (the low nbits bits), digit-major into counts[digit * ntiles + tile], as the radix sort's
+// first pass would count them.
+__global__ void __launch_bounds__(kKeyBlock) k_splat_keys(DevTree T, const float* __restrict__ q, int64_t n,
+                                                    const int* __restrict__ leaf_slot, int lgres, uint32_t sentinel,
+                                                    uint32_t* __restrict__ keys, int nbits, int* __restrict__ counts,
+                                                    int ntiles) {
+  __shared__ int hist[256];
+  if (threadIdx.x < 256) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+  const uint32_t mask = (1u << nbits) - 1u;
+#pragma unroll
+  for (int r = 0; r < kRadixTile / kKeyBlock; ++r) {
+    const int64_t i = base + r * kKeyBlock + threadIdx.x;
+    if (i >= n) continue;
+    const uint32_t key = splat_key(T, q[3 * i], q[3 * i + 1], q[3 * i + 2], leaf_slot, lgres, sentinel);
+    keys[i] = key;
+    atomicAdd(&hist[key & mask], 1);
   }
-  uint32_t old = 0;
-  if (live && lane == leader) old = atomicAdd(&cnt[key], (uint32_t)size);
-  const uint32_t base = __shfl(old, leader, 64);
-  if (live) keyrank[i] = make_uint2(key, base + (uint32_t)pos);
+  __syncthreads();
+  if (threadIdx.x < 256) counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
+// inv[perm[j]] = j: the sorted position of every point
+__global__ void __launch_bounds__(256) k_splat_inverse(const uint32_t* __restrict__ perm, int64_t n,
+                                                       uint32_t* __restrict__ inv) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) inv[perm[j]] = (uint32_t)j;
+}
+
+// payload[inv[i]] = (q, m) of point i (m = 1 without masses): the points are read in their own
+// order and scattered, since a wave's scattered writes do not wait on memory as its scattered
+// reads would
 __global__ void __launch_bounds__(256) k_splat_place(const float* __restrict__ q, const float* __restrict__ mass,
-                                                     int64_t n, const uint2* __restrict__ keyrank,
-                                                     const int* __restrict__ start, float4* __restrict__ payload,
-                                                     uint32_t* __restrict__ perm) {
+                                                     int64_t n, const uint32_t* __restrict__ inv,
+                                                     float4* __restrict__ payload) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint2 kr = keyrank[i];
-  const int64_t dst = (int64_t)start[kr.x] + kr.y;
-  payload[dst] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], mass ? mass[i] : 1.f);
-  perm[dst] = (uint32_t)i;
+  payload[inv[i]] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], mass ? mass[i] : 1.f);
 }
 
 // Per axis, the 5 (neighbour offset, region) pairs whose points reach this brick.
@@ -300,18 +304,20 @@ __global__ void __launch_bounds__(256) k_sample_backward(DevTree T, const int* _
   dq[3 * i + 2] = (float)(d * gz);
 }
 
-void splat_count(const DevTree& T, const float* q, int64_t n, const int* leaf_slot, int lgres, uint32_t sentinel,
-                 uint32_t* cnt, uint2* keyrank, hipStream_t s) {
+void splat_keys(const DevTree& T, const float* q, int64_t n, const int* leaf_slot, int lgres, uint32_t sentinel,
+                uint32_t* keys, int bits, int* counts, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_splat_count, dim3(nblocks(n, 256)), dim3(256), 0, s, T, q, n, leaf_slot, lgres, sentinel, cnt,
-                     keyrank);
+  const int ntiles = (int)((n + kRadixTile - 1) / kRadixTile);
+  hipLaunchKernelGGL(k_splat_keys, dim3(ntiles), dim3(kKeyBlock), 0, s, T, q, n, leaf_slot, lgres, sentinel, keys,
+                     radix_first_digit_bits(bits), counts, ntiles);
 }
 
-void splat_place(const float* q, const float* mass, int64_t n, const uint2* keyrank, const int* start,
-                 float4* payload, uint32_t* perm, hipStream_t s) {
+void splat_place(const float* q, const float* mass, int64_t n, const uint32_t* perm, uint32_t* inv, float4* payload,
+                 hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_splat_place, dim3(nblocks(n, 256)), dim3(256), 0, s, q, mass, n, keyrank, start, payload,
-                     perm);
+  hipLaunchKernelGGL(k_splat_inverse, dim3(nblocks(n, 256)), dim3(256), 0, s, perm, n, inv);
+  hipLaunchKernelGGL(k_splat_place, dim3(nblocks(n, 256)), dim3(256), 0, s, q, mass, n, (const uint32_t*)inv,
+                     payload);
 }
 
 bool splat_brick_supported(int lgres) { return lgres >= 1 && lgres <= 4; }

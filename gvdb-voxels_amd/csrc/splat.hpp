@@ -4,13 +4,15 @@
 
 namespace gvdbx {
 
-// Counting sort of the points by key = slot * 27 + region of the home brick (sentinel = points
-// in no brick): cnt[sentinel + 1] must be zero on entry; keyrank[i] = (key, rank inside key).
-void splat_count(const DevTree& T, const float* q, int64_t n, const int* leaf_slot, int lgres, uint32_t sentinel,
-                 uint32_t* cnt, uint2* keyrank, hipStream_t s);
-// payload[start[key] + rank] = (q, m) (m = 1 without masses), perm[same] = point index
-void splat_place(const float* q, const float* mass, int64_t n, const uint2* keyrank, const int* start,
-                 float4* payload, uint32_t* perm, hipStream_t s);
+// Sort key of every point, key = slot * 27 + region of the home brick (sentinel = points in no
+// brick), with the first radix pass's tile counts over `bits` key bits (radix_sort_u32_positions
+// then sorts them; counts holds radix_count_ints(n) ints).
+void splat_keys(const DevTree& T, const float* q, int64_t n, const int* leaf_slot, int lgres, uint32_t sentinel,
+                uint32_t* keys, int bits, int* counts, hipStream_t s);
+// payload[j] = (q, m) of point perm[j] (m = 1 without masses), by way of inv (n words of
+// scratch: the inverse permutation)
+void splat_place(const float* q, const float* mass, int64_t n, const uint32_t* perm, uint32_t* inv, float4* payload,
+                 hipStream_t s);
 bool splat_brick_supported(int lgres);
 // start: [27 * nslots + 1] key starts (exclusive scan of the counts)
 void splat_brick(int64_t nslots, const int* start, const float4* payload, const int* nbr, const int4* slot_pos,

@@ -25,6 +25,17 @@ def test_small_random_pipeline(oracle_mod, seed, radius, sc):
     U.run_pipeline(v, t, p, radius, sc, levelset=False)
 
 
+@pytest.mark.parametrize("radius", [0.5, 1.0])
+def test_full_rebuild_sparse_extent_sorts(oracle_mod, radius):
+    """Two clouds 6,000 voxels apart: the occupancy bitmaps of a full rebuild would cover over
+    2^27 leaf bricks, so the rebuild sorts keys instead; the pools must still equal the oracle's."""
+    U.log(f"case sparse extent r={radius}")
+    g = np.random.default_rng(11)
+    p = np.concatenate([g.uniform(10, 60, (1500, 3)), g.uniform(6000, 6050, (1500, 3))]).astype(np.float32)
+    v, t = U.new_volume(), U.new_tree(oracle_mod)
+    U.run_pipeline(v, t, p, radius, 4)
+
+
 def test_levelset_small(oracle_mod):
     U.log("case levelset small")
     p = rand_points(4000, 7, 30, 120)
