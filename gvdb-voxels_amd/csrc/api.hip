@@ -288,13 +288,18 @@ int retire(gvdbx_ctx* c, void* p, size_t bytes) {
   return GVDBX_OK;
 }
 
-// Grow-only buffer; contents are not kept.
+// Grow-only buffer; contents are not kept.  A growing buffer takes twice what is asked (a quarter
+// more above 256 MiB): the topology of a moving point set grows by a few per cent per step, and
+// the several dozen buffers sized by it would otherwise reallocate every few steps, each
+// reallocation a hipMalloc on the host's path between the rebuild's read-back and the launches
+// that follow it.
 int ensure(gvdbx_ctx* c, Buf& b, size_t bytes) {
   if (bytes <= b.cap) return GVDBX_OK;
   if (c->recording)
     return fail(c, GVDBX_ERR_STATE, "a buffer would grow while recording: run the same calls once before recording");
-  size_t nb = b.cap ? b.cap : 256;
-  while (nb < bytes) nb += nb / 2 + 256;
+  const size_t big = (size_t)256 << 20;
+  size_t nb = bytes <= big ? 2 * bytes : bytes + bytes / 4;
+  nb = (nb + 255) & ~(size_t)255;
   if (b.p) CHK(retire(c, b.p, b.cap));
   b.p = nullptr;
   b.cap = 0;
@@ -342,6 +347,13 @@ LaunchCfg launch_of(gvdbx_ctx* c) {
 
 int res_of(const gvdbx_ctx* c, int l) { return 1 << c->ld[l]; }
 int64_t clist_per(const gvdbx_ctx* c, int l) { return (int64_t)1 << (3 * c->ld[l]); }
+
+// Leaves the per-leaf buffers are sized for: the leaf pool's capacity, which doubles when it
+// grows, so those buffers grow with the pool rather than with every leaf an incremental rebuild
+// appends (delinked leaves stay in the pool, so its count only grows).
+int64_t leaf_room(const gvdbx_ctx* c) {
+  return std::max<int64_t>(c->cnt[0], (int64_t)(c->nodes[0].cap / sizeof(gvdbx_node)));
+}
 
 DevTree tree_of(gvdbx_ctx* c) {
   DevTree T;
@@ -1513,13 +1525,13 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {
   NOT_RECORDING(c);  // UpdateAtlas (V.cpp:2630-2704)
   if (!c || !c->configured) return GVDBX_ERR_ARG;
   if (c->apron < 0) return fail(c, GVDBX_ERR_STATE, "UpdateAtlas needs channel 0 (AddChannel)");
-  const int64_t total = c->cnt[0];
-  CHK(ensure(c, c->flags, (size_t)(total + 1) * 4));
-  CHK(ensure(c, c->rank, (size_t)(total + 1) * 4));
-  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(total + 1) + 16) * 4));
-  CHK(ensure(c, c->leaf_slot, (size_t)(total + 1) * 4));
-  CHK(ensure(c, c->slot_leaf, (size_t)(total + 1) * 4));
-  CHK(ensure(c, c->slot_pos, (size_t)(total + 1) * 16));
+  const int64_t total = c->cnt[0], room = leaf_room(c);
+  CHK(ensure(c, c->flags, (size_t)(room + 1) * 4));
+  CHK(ensure(c, c->rank, (size_t)(room + 1) * 4));
+  CHK(ensure(c, c->tile_buf, (size_t)(scan_tile_count(room + 1) + 16) * 4));
+  CHK(ensure(c, c->leaf_slot, (size_t)(room + 1) * 4));
+  CHK(ensure(c, c->slot_leaf, (size_t)(room + 1) * 4));
+  CHK(ensure(c, c->slot_pos, (size_t)(room + 1) * 16));
   // the lookup table is cleared by the flag launch and filled by the slot launch
   BrickMapOut tab;
   bool use_tab = false;
@@ -1546,7 +1558,7 @@ int gvdbx_update_atlas(gvdbx_ctx* c) {
     c->bm_valid = true;
   }
   T = tree_of(c);
-  CHK(ensure(c, c->nbr, (size_t)(c->nslots * 27 + 1) * 4));
+  CHK(ensure(c, c->nbr, (size_t)(room * 27 + 1) * 4));
   neighbor_table(T, c->nslots, P<int4>(c->slot_pos), P<int>(c->leaf_slot), res_of(c, 0), P<int>(c->nbr), c->stream);
   CHK(reserve_channels(c, c->nslots));
   c->atlas_version = c->topo_version;
@@ -1608,11 +1620,13 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   const int64_t nsc = per * c->used0;
   const int nmap = (int)c->used0;
   if (nsc > INT32_MAX || total > INT32_MAX) return fail(c, GVDBX_ERR_CAPACITY, "subcell count exceeds 2^31");
-  // arrays sized for every pool leaf (an upper bound of the linked leaves indexed by map)
-  const int64_t nsc_cap = per * total;
-  CHK(ensure(c, c->sc_map, (size_t)(total + 1) * 4));
-  CHK(ensure(c, c->map_leaf, (size_t)(total + 1) * 4));
-  CHK(ensure(c, c->flags, (size_t)(total + 1) * 4));
+  // arrays sized for every pool leaf (an upper bound of the linked leaves indexed by map), up to
+  // the pool's capacity
+  const int64_t room = leaf_room(c);
+  const int64_t nsc_cap = per * room;
+  CHK(ensure(c, c->sc_map, (size_t)(room + 1) * 4));
+  CHK(ensure(c, c->map_leaf, (size_t)(room + 1) * 4));
+  CHK(ensure(c, c->flags, (size_t)(room + 1) * 4));
   CHK(ensure(c, c->sc_cnt, (size_t)(nsc_cap + 1) * 4));
   CHK(ensure(c, c->sc_off, (size_t)(nsc_cap + 1) * 4));
   CHK(ensure(c, c->sc_pos, (size_t)(nsc_cap + 1) * 12));
@@ -1622,10 +1636,10 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   CHK(ensure(c, c->hkeys_alt, (size_t)n * 4));
   CHK(ensure(c, c->hvals, (size_t)n * 4));
   CHK(ensure(c, c->hvals_alt, (size_t)n * 4));
-  CHK(ensure(c, c->ranges, (size_t)(total + 2) * 4));
+  CHK(ensure(c, c->ranges, (size_t)(room + 2) * 4));
   const int64_t ncnt = radix_count_ints(n);
   CHK(ensure(c, c->radix_counts, (size_t)ncnt * 4));
-  int64_t big = nsc_cap > total ? nsc_cap : total;
+  int64_t big = nsc_cap > room ? nsc_cap : room;
   if (ncnt > big) big = ncnt;
   // scan tiles of the map scan, and the 256-subcell block sums of the offsets scan
   int64_t tiles = scan_tile_count(big + 1);
@@ -1638,8 +1652,8 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   CHK(ensure(c, c->cnt_in, (size_t)(nsc_cap + 1) * 4));
   CHK(ensure(c, c->ocnt, (size_t)(nsc_cap + 1) * 4));
   CHK(ensure(c, c->ocur, (size_t)(nsc_cap + 1) * 4));
-  CHK(ensure(c, c->wgb, (size_t)((int64_t)nmap * W * W * W + 1) * 4));
-  CHK(ensure(c, c->nbm, (size_t)((int64_t)nmap * 27 + 1) * 4));
+  CHK(ensure(c, c->wgb, (size_t)(room * W * W * W + 1) * 4));
+  CHK(ensure(c, c->nbm, (size_t)(room * 27 + 1) * 4));
   CHK(ensure(c, c->spos, (size_t)(n + 1) * 16));
   if (with_vel) CHK(ensure(c, c->svel, (size_t)(n + 1) * 16));
 
@@ -1717,7 +1731,7 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     A.vqrange[a] = c->vel_range[a];
   }
   A.vsum = with_vel ? P<unsigned long long>(c->vsum) : nullptr;
-  if (with_vel) CHK(ensure(c, c->wvs, (size_t)((int64_t)nmap * W * W * W * 3 + 1) * 8));
+  if (with_vel) CHK(ensure(c, c->wvs, (size_t)(room * W * W * W * 3 + 1) * 8));
   A.wvs = with_vel ? P<unsigned long long>(c->wvs) : nullptr;
 
   {
@@ -1742,8 +1756,8 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   // length comes from a 64-bit sum.
   const int64_t per_axis = subcells_per_axis(c, r, sc);
   const int64_t bound = per_axis * per_axis * per_axis * n;
-  const size_t room = lpos.cap > c->list_budget ? lpos.cap : c->list_budget;
-  const bool early = bound <= INT32_MAX && (size_t)(bound + 1) * ebytes <= room;
+  const size_t list_room = lpos.cap > c->list_budget ? lpos.cap : c->list_budget;
+  const bool early = bound <= INT32_MAX && (size_t)(bound + 1) * ebytes <= list_room;
   if (c->recording && !early)
     return fail(c, GVDBX_ERR_CAPACITY, "the list bound (%lld entries) does not fit a recorded insert",
                 (long long)bound);

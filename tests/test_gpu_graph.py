@@ -149,19 +149,20 @@ def test_graph_refusals():
         v.synchronize()
     with pytest.raises(GvdbxError):
         v.GraphEnd()
-    # growth: more points than the run before the recording
+    # growth: more points than the buffers of the run before the recording hold (a growing
+    # buffer reserves twice its request, so 2.5 times the points)
     from gvdbx import synth
-    p2, vel2 = synth.sph_block(n=60_000, seed=14, box=96.0)
+    p2, vel2 = synth.sph_block(n=100_000, seed=14, box=96.0)
     w = U.new_volume(chans=(0, 1, 2, 3))
     dp2, dv2 = w.to_device(p2), w.to_device(vel2)
     w.SetPoints(dp2, dv2, n=40_000)
     w.RebuildTopology(40_000, 1.0)
     w.UpdateAtlas()
     step(w, 40_000)
-    w.SetPoints(dp2, dv2, n=60_000)
+    w.SetPoints(dp2, dv2, n=100_000)
     w.GraphBegin()
     with pytest.raises(GvdbxError):
-        w.InsertPointsSubcell(4, 60_000, 1.0)
+        w.InsertPointsSubcell(4, 100_000, 1.0)
     with pytest.raises(GvdbxError):
         w.GraphEnd()
     w.close()
