@@ -450,27 +450,28 @@ __device__ __forceinline__ int64_t omap_index(const OmapLevels& M, int l, int xr
   return M.word0[l] * 32 + ((int64_t)x * M.n[l][1] + y) * M.n[l][2] + z;
 }
 
-// Marks node (x, y, z) (voxel corner, inside the root) and its ancestors below the root; the
-// plain read skips the store where the mark is there already (a stale read only costs a store of
-// the same value).  A node outside the map box sets the overflow flag (the host then sorts).
-__device__ __forceinline__ void omap_mark_chain(const DevTree& T, const OmapLevels& M, uint8_t* map, int x, int y,
-                                                int z, int* overflow) {
-  for (int l = 0; l < T.depth; ++l) {
-    const int sh = T.lgR[l];
-    const int64_t b = omap_index(M, l, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh),
-                                 (z >> sh) - (T.root[2] >> sh));
-    if (b < 0) {
-      *overflow = 1;
-      return;
-    }
-    if (map[b] == 0) map[b] = 1;
+// Marks leaf-level node (x, y, z) (voxel corner, inside the root); the plain read skips the store
+// where the mark is there already (a stale read only costs a store of the same value).  A node
+// outside the map box sets the overflow flag (the host then sorts).
+__device__ __forceinline__ void omap_mark_leaf(const DevTree& T, const OmapLevels& M, uint8_t* map, int x, int y, int z,
+                                               int* overflow) {
+  const int sh = T.lgR[0];
+  const int64_t b =
+      omap_index(M, 0, (x >> sh) - (T.root[0] >> sh), (y >> sh) - (T.root[1] >> sh), (z >> sh) - (T.root[2] >> sh));
+  if (b < 0) {
+    *overflow = 1;
+    return;
   }
+  if (map[b] == 0) map[b] = 1;
 }
 
-// Per valid point: its covering node at every level below the root (gvdbCalcBrickId,
-// P.cuh:352-382: trunc(q) >> log2 R for q >= 0) into `home`; with box, every leaf-level node of
-// its extra-pass box [trunc(trunc(q) - r), trunc(trunc(q) + 2r)] (the corners gvdbCalcExtraBrickId
-// enumerates, P.cuh:467-512, as in k_box_keys) inside the root, with its ancestors, into `box`.
+// Per valid point: its covering leaf (gvdbCalcBrickId, P.cuh:352-382: trunc(q) >> log2 R0 for
+// q >= 0) into `home`; with box, every leaf-level node of its extra-pass box
+// [trunc(trunc(q) - r), trunc(trunc(q) + 2r)] (the corners gvdbCalcExtraBrickId enumerates,
+// P.cuh:467-512, as in k_box_keys) inside the root into `box`.  The covering and box nodes of the
+// inner levels are the ancestors of these (k_omap_lift).  kTwo: the box spans at most two leaf
+// nodes per axis, so its up to 8 corners are marked unrolled, their loads independent.
+template <bool kTwo>
 __global__ void __launch_bounds__(256) k_omap_mark(DevTree T, PointsView P, OmapLevels M, float r, int box,
                                                    uint8_t* __restrict__ home, uint8_t* __restrict__ boxmap,
                                                    int* __restrict__ overflow) {
@@ -480,7 +481,7 @@ __global__ void __launch_bounds__(256) k_omap_mark(DevTree T, PointsView P, Omap
   const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
   if (q[0] < 0 || q[1] < 0 || q[2] < 0) return;
   const int iq[3] = {(int)q[0], (int)q[1], (int)q[2]};
-  omap_mark_chain(T, M, home, iq[0], iq[1], iq[2], overflow);
+  omap_mark_leaf(T, M, home, iq[0], iq[1], iq[2], overflow);
   if (!box) return;
   const int R0 = 1 << T.lgR[0];
   int lo[3], hi[3];
@@ -490,17 +491,65 @@ __global__ void __launch_bounds__(256) k_omap_mark(DevTree T, PointsView P, Omap
     lo[a] = (int)(tq - r) / R0 * R0;
     hi[a] = (int)(tq + 2.0f * r) / R0 * R0;
   }
+  if (kTwo) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int x = lo[0] + (k & 1) * R0, y = lo[1] + ((k >> 1) & 1) * R0, z = lo[2] + (k >> 2) * R0;
+      if (x <= hi[0] && y <= hi[1] && z <= hi[2] && in_root(T, x, y, z)) omap_mark_leaf(T, M, boxmap, x, y, z, overflow);
+    }
+    return;
+  }
   for (int x = lo[0]; x <= hi[0]; x += R0)
     for (int y = lo[1]; y <= hi[1]; y += R0)
       for (int z = lo[2]; z <= hi[2]; z += R0)
-        if (in_root(T, x, y, z)) omap_mark_chain(T, M, boxmap, x, y, z, overflow);
+        if (in_root(T, x, y, z)) omap_mark_leaf(T, M, boxmap, x, y, z, overflow);
+}
+
+// Level l from level l - 1: a node is marked when one of its children is (the covering and box
+// nodes of a level are the ancestors of the leaf-level ones).  One thread per (node, row of
+// children along z), both maps; children outside level l - 1's box are not there to read.
+__global__ void __launch_bounds__(256) k_omap_lift(DevTree T, OmapLevels M, int l, uint8_t* __restrict__ home,
+                                                   uint8_t* __restrict__ boxmap) {
+  const int ld = T.ld[l], res = 1 << ld;
+  const int64_t nodes = (int64_t)M.n[l][0] * M.n[l][1] * M.n[l][2];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nodes * res * res) return;
+  const int64_t e = t / (res * res);
+  const int row = (int)(t - e * res * res);
+  const int nz = M.n[l][2], ny = M.n[l][1];
+  const int zr = (int)(e % nz) + M.lo[l][2], yr = (int)((e / nz) % ny) + M.lo[l][1],
+            xr = (int)(e / ((int64_t)nz * ny)) + M.lo[l][0];
+  const int cx = (xr << ld) + row / res, cy = (yr << ld) + row % res;
+  bool h = false, b = false;
+  for (int k = 0; k < res; ++k) {
+    const int64_t c = omap_index(M, l - 1, cx, cy, (zr << ld) + k);
+    if (c < 0) continue;
+    h = h || home[c] != 0;
+    b = b || (boxmap && boxmap[c] != 0);
+  }
+  const int64_t self = M.word0[l] * 32 + e;
+  if (h && home[self] == 0) home[self] = 1;
+  if (b && boxmap[self] == 0) boxmap[self] = 1;
 }
 
 void omap_mark(const DevTree& T, const PointsView& P, const OmapLevels& M, float r, bool box, uint8_t* home,
                uint8_t* boxmap, int* overflow, hipStream_t s) {
-  if (P.n <= 0) return;
-  hipLaunchKernelGGL(k_omap_mark, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, M, r, box ? 1 : 0, home, boxmap,
-                     overflow);
+  if (P.n > 0) {
+    // a box [trunc(q) - r, trunc(q) + 2r] spans at most 2 leaf nodes per axis when it is shorter
+    // than one leaf node
+    const bool two = 3.0 * (double)r + 1.0 < (double)(1 << T.lgR[0]);
+    if (two)
+      hipLaunchKernelGGL(k_omap_mark<true>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, M, r, box ? 1 : 0, home,
+                         boxmap, overflow);
+    else
+      hipLaunchKernelGGL(k_omap_mark<false>, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, T, P, M, r, box ? 1 : 0,
+                         home, boxmap, overflow);
+  }
+  for (int l = 1; l < M.nlev; ++l) {
+    const int64_t res = (int64_t)1 << T.ld[l];
+    const int64_t threads = (int64_t)M.n[l][0] * M.n[l][1] * M.n[l][2] * res * res;
+    hipLaunchKernelGGL(k_omap_lift, dim3(blocks_for(threads, 256)), dim3(256), 0, s, T, M, l, home, boxmap);
+  }
 }
 
 // Bit j = (byte j of group w != 0), for the 32 bytes of group w.
