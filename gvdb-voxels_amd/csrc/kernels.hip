@@ -938,40 +938,62 @@ __global__ void __launch_bounds__(256) k_levelset_sc4(SubcellCfg c, int64_t nsc,
   const float vx = (float)(bx + (lane & 3)), vy = (float)(by + ((lane >> 2) & 3)), vz = (float)(bz + (lane >> 4));
   const int n = cnt[k];
   const float* __restrict__ lp = list_pos + 3 * (int64_t)off[k];
-  float m0 = INFINITY, m1 = INFINITY;
-  int j = 0;
-  constexpr int kStep = 8;  // entries per step (24 floats in scalar registers, twice over)
+  // Lane (x, y, g): the voxel column (x, y) of the subcell for every z, over the entries j = g
+  // (mod 4).  |p - v|^2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) (dist2's operations, same bits),
+  // and its inner fmaf(dy, dy, dx * dx) is the same for the column's four voxels, so a pair
+  // costs about 4 VALU instructions instead of 7.  The four lane groups read four different
+  // entries per load (each address shared by 16 lanes); their minima are combined at the end.
+  const int g = lane >> 4;
+  const float vz0 = (float)bz;
+  float mz[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  auto take = [&](float px, float py, float pz) {
+    const float dx = px - vx, dy = py - vy;
+    const float t = fmaf(dy, dy, dx * dx);
+#pragma unroll
+    for (int z = 0; z < 4; ++z) {
+      const float dz = pz - (vz0 + (float)z);
+      mz[z] = fminf(mz[z], fmaf(dz, dz, t));
+    }
+  };
+  int jb = 0;
+  constexpr int kStep = 16;  // entries per step: four per lane group, loaded one step ahead
   if (n >= kStep) {
-    // the next step's entries are loaded before this step's distances, so the scalar load
-    // latency overlaps the arithmetic
-    float nx[3 * kStep];
+    float nx[12];
 #pragma unroll
-    for (int e = 0; e < 3 * kStep; ++e) nx[e] = lp[e];
-    for (; j + kStep <= n; j += kStep) {
-      float p[3 * kStep];
+    for (int e = 0; e < 4; ++e) {
+      const float* q = lp + 3 * (4 * e + g);
+      nx[3 * e] = q[0];
+      nx[3 * e + 1] = q[1];
+      nx[3 * e + 2] = q[2];
+    }
+    for (; jb + kStep <= n; jb += kStep) {
+      float p[12];
 #pragma unroll
-      for (int e = 0; e < 3 * kStep; ++e) p[e] = nx[e];
-      if (j + 2 * kStep <= n) {
-        const float* q = lp + 3 * (j + kStep);
+      for (int e = 0; e < 12; ++e) p[e] = nx[e];
+      if (jb + 2 * kStep <= n) {
 #pragma unroll
-        for (int e = 0; e < 3 * kStep; ++e) nx[e] = q[e];
+        for (int e = 0; e < 4; ++e) {
+          const float* q = lp + 3 * (jb + kStep + 4 * e + g);
+          nx[3 * e] = q[0];
+          nx[3 * e + 1] = q[1];
+          nx[3 * e + 2] = q[2];
+        }
       }
 #pragma unroll
-      for (int e = 0; e < kStep; e += 2) {
-        m0 = fminf(m0, dist2(p[3 * e] - vx, p[3 * e + 1] - vy, p[3 * e + 2] - vz));
-        m1 = fminf(m1, dist2(p[3 * e + 3] - vx, p[3 * e + 4] - vy, p[3 * e + 5] - vz));
-      }
+      for (int e = 0; e < 4; ++e) take(p[3 * e], p[3 * e + 1], p[3 * e + 2]);
     }
   }
-  for (; j + 4 <= n; j += 4) {
-    const float* p = lp + 3 * j;
-    m0 = fminf(m0, dist2(p[0] - vx, p[1] - vy, p[2] - vz));
-    m1 = fminf(m1, dist2(p[3] - vx, p[4] - vy, p[5] - vz));
-    m0 = fminf(m0, dist2(p[6] - vx, p[7] - vy, p[8] - vz));
-    m1 = fminf(m1, dist2(p[9] - vx, p[10] - vy, p[11] - vz));
+  for (; jb < n; jb += 4) {
+    const int j = jb + g;
+    if (j < n) take(lp[3 * j], lp[3 * j + 1], lp[3 * j + 2]);
   }
-  for (; j < n; ++j) m0 = fminf(m0, dist2(lp[3 * j] - vx, lp[3 * j + 1] - vy, lp[3 * j + 2] - vz));
-  const float md2 = fminf(m0, m1);
+#pragma unroll
+  for (int z = 0; z < 4; ++z) {
+    mz[z] = fminf(mz[z], __shfl_xor(mz[z], 16, 64));
+    mz[z] = fminf(mz[z], __shfl_xor(mz[z], 32, 64));
+  }
+  // lane (x, y, g) writes voxel (x, y, z = g), the lane order of the voxels above
+  const float md2 = g == 0 ? mz[0] : (g == 1 ? mz[1] : (g == 2 ? mz[2] : mz[3]));
   const int slot = leaf_slot[sc_nid[k]];
   if (slot < 0) return;
   const int br = c.res0 + 2 * apron, m = c.res0 - 1;
