@@ -1690,11 +1690,13 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   }
   // [tail[0], tail[1]): valid points whose own leaf is missing (invalid points sort after them)
   const int* tail = P<int>(c->ranges) + nmap;
-  // Sorted copies of the points for the per-brick kernels: with bricks of 64 points or more on
+  // Sorted copies of the points for the per-brick kernels: with bricks of 64 to 511 points on
   // average the count pass gathers them itself (its bricks' points, through the permutation);
   // sparse bricks leave the count pass too little work per point to hide that gather's latency,
-  // so k_gather_sorted makes the copies first.
-  const bool gather_in_count = n >= (int64_t)64 * nmap;
+  // and bricks of hundreds of points give each count thread several dependent gathers in turn
+  // (16.7M level-set points, 787 per brick: 226 + 115 µs apart against 406 µs in the count
+  // pass), so there k_gather_sorted makes the copies first.
+  const bool gather_in_count = n >= (int64_t)64 * nmap && n < (int64_t)512 * nmap;
   if (!gather_in_count)
     gather_sorted(pv, svals, n, P<float4>(c->spos), with_vel ? P<float4>(c->svel) : nullptr, c->stream);
   BinArgs A;
