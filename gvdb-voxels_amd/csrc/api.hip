@@ -9,6 +9,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -166,8 +168,11 @@ struct gvdbx_ctx {
 
   Buf keys, keys_alt, flags, rank, radix_counts, tile_buf, markers, meta;
   Buf box_region, box_cnt, box_off;  // per-workgroup slices of the box-pass keys
+  // host-coherent pinned copy of meta: [0, M_INTS) the counters, [M_INTS] the sequence number of
+  // the last export (k_meta_export); d_meta is its device address
   int* h_meta = nullptr;
-  hipEvent_t meta_ev = nullptr;  // marks an early meta read-back (read_meta_begin / _end)
+  int* d_meta = nullptr;
+  unsigned meta_seq = 0;
 
   int64_t topo_version = 0, atlas_version = -1;
   int64_t new_nodes[kMaxLev] = {};  // per level: nodes the last allocation appended
@@ -399,27 +404,44 @@ PointsView points_of(gvdbx_ctx* c, int64_t n, const float* tr) {
   return v;
 }
 
-int read_meta(gvdbx_ctx* c) {
-  NOT_RECORDING(c);
-  CHK(launch_status(c, "read-back"));
-  HIPCHK(c, hipMemcpyAsync(c->h_meta, c->meta.p, M_INTS * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return GVDBX_OK;
-}
+// Counter read-back in two halves: the export kernel is enqueued where the values are final and
+// more work can be enqueued behind it; the host then waits for the export's sequence number,
+// which k_meta_export stores into pinned memory after the counters with a system-scope release.
+// The stream runs its work in order, so every launch before the export has finished by then.
+// This needs no copy engine or runtime copy kernel, and the host sees the number within a
+// microsecond or so by spinning; after kMetaSpinUs it waits on the stream instead (long work
+// before the read-back, or a failed launch).
+constexpr int64_t kMetaSpinUs = 2000;
 
-// Read-back split in two: the copy is enqueued where the values are final, more work can be
-// enqueued behind it, and the host waits only for the copy.
 int read_meta_begin(gvdbx_ctx* c) {
   NOT_RECORDING(c);
   CHK(launch_status(c, "read-back"));
-  HIPCHK(c, hipMemcpyAsync(c->h_meta, c->meta.p, M_INTS * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipEventRecord(c->meta_ev, c->stream));
-  return GVDBX_OK;
+  ++c->meta_seq;
+  meta_export(P<int>(c->meta), c->d_meta, M_INTS, c->meta_seq, c->stream);
+  return launch_status(c, "read-back");
 }
 
 int read_meta_end(gvdbx_ctx* c) {
-  HIPCHK(c, hipEventSynchronize(c->meta_ev));
+  const volatile unsigned* seq = reinterpret_cast<const volatile unsigned*>(c->h_meta + M_INTS);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned k = 1;; ++k) {
+    if (*seq == c->meta_seq) {
+      std::atomic_thread_fence(std::memory_order_acquire);
+      return GVDBX_OK;
+    }
+    if ((k & 1023u) == 0 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kMetaSpinUs))
+      break;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (*seq != c->meta_seq) return fail(c, GVDBX_ERR_HIP, "the counter read-back did not arrive");
+  std::atomic_thread_fence(std::memory_order_acquire);
   return GVDBX_OK;
+}
+
+int read_meta(gvdbx_ctx* c) {
+  CHK(read_meta_begin(c));
+  return read_meta_end(c);
 }
 
 int zero_meta(gvdbx_ctx* c) {
@@ -1169,11 +1191,12 @@ int gvdbx_ctx_create(int device, void* hip_stream, gvdbx_ctx** out) {
   (void)hipGetLastError();
   if (ensure(c, c->meta, M_INTS * sizeof(int)) != GVDBX_OK || ensure(c, c->queue, 256) != GVDBX_OK ||
       ensure(c, c->graph_len, 256) != GVDBX_OK ||
-      hipHostMalloc((void**)&c->h_meta, M_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&c->meta_ev, hipEventDisableTiming) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_meta, (M_INTS + 64) * sizeof(int), hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&c->d_meta, c->h_meta, 0) != hipSuccess) {
     gvdbx_ctx_destroy(c);
     return GVDBX_ERR_HIP;
   }
+  memset(c->h_meta, 0, (M_INTS + 64) * sizeof(int));
   *out = c;
   return GVDBX_OK;
 }
@@ -1216,7 +1239,6 @@ int gvdbx_ctx_destroy(gvdbx_ctx* c) {
   for (int ch = 0; ch < 32; ++ch)
     if (c->chan[ch].p) (void)hipFree(c->chan[ch].p);
   if (c->h_meta) (void)hipHostFree(c->h_meta);
-  if (c->meta_ev) (void)hipEventDestroy(c->meta_ev);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

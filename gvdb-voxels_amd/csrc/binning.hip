@@ -1054,12 +1054,12 @@ __device__ __forceinline__ void density_brick_body(const SubcellCfg& c, int64_t 
           for (int q = 0; q < 3; ++q) vm[q][e] = vmean[w][q];  // 0 for missing neighbours
         }
       }
-      reinterpret_cast<float4*>(bptr)[t4] = make_float4(val[0], val[1], val[2], val[3]);
+      // streaming stores: nothing in the step reads the channel bricks back
+      store_stream4(bptr + 4 * t4, val[0], val[1], val[2], val[3]);
       if (kVel) {
 #pragma unroll
         for (int q = 0; q < 3; ++q)
-          reinterpret_cast<float4*>(vo.ch[q] + (int64_t)slot * kVox)[t4] =
-              make_float4(vm[q][0], vm[q][1], vm[q][2], vm[q][3]);
+          store_stream4(vo.ch[q] + (int64_t)slot * kVox + 4 * t4, vm[q][0], vm[q][1], vm[q][2], vm[q][3]);
       }
     }
     return;

@@ -149,6 +149,14 @@ __device__ __forceinline__ int64_t find_leaf(const DevTree& T, int x, int y, int
 // scatter-reduce) rounds identically.
 __device__ __forceinline__ float dist2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
 
+// 16-byte store marked non-temporal (the line is not kept in the caches): for output written
+// once and not read again by the kernels that follow
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_stream4(float* p, float a, float b, float c, float d) {
+  const f32x4_t v = {a, b, c, d};
+  __builtin_nontemporal_store(v, reinterpret_cast<f32x4_t*>(p));
+}
+
 // ---- wave64 helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }

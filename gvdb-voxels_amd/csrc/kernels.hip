@@ -111,7 +111,8 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
   const int d = T.depth;
   auto emit = [&](int x, int y, int z, int64_t cur, int miss) {
     if (miss < 0) {
-      if (markers && markers[cur] == 0) markers[cur] = 1;
+      // a plain store of the one value every writer stores: no read to wait for
+      if (markers) markers[cur] = 1;
       return;
     }
     const int k = atomicAdd(&bcount, miss + 1);
@@ -1112,6 +1113,18 @@ __global__ void __launch_bounds__(256) k_advance_flat(float* __restrict__ q, con
   }
   const int64_t i = 4 * m4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // the last m % 4 floats
   if (i < m) q[i] = __fadd_rn(q[i], __fmul_rn(v[i], dt));
+}
+
+__global__ void __launch_bounds__(64) k_meta_export(const int* __restrict__ meta, int* host, int n, unsigned seq) {
+  const int t = threadIdx.x;
+  if (t < n) host[t] = meta[t];
+  __threadfence_system();  // each lane's counter store completes before the barrier
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(reinterpret_cast<unsigned*>(host + n), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void meta_export(const int* meta, int* host, int n, unsigned seq, hipStream_t s) {
+  hipLaunchKernelGGL(k_meta_export, dim3(1), dim3(64), 0, s, meta, host, n, seq);
 }
 
 void advance_points(char* pos, int pos_off, int pos_stride, const char* vel, int vel_off, int vel_stride, int64_t n,

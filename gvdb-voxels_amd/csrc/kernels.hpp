@@ -151,6 +151,10 @@ void sample_trilinear(const DevTree& T, const int* leaf_slot, int res0, int apro
                       const float* q, int64_t n, float* out, hipStream_t s);
 
 // q += v * dt in place over n points of the given byte views (FP32 product, then FP32 sum)
+// Counters to the host: host[0, n) = meta[0, n) (n <= 64), then host[n] = seq with a
+// system-scope release, so a host thread that sees seq there reads the new counters.  host is
+// the device address of host-coherent pinned memory.
+void meta_export(const int* meta, int* host, int n, unsigned seq, hipStream_t s);
 void advance_points(char* pos, int pos_off, int pos_stride, const char* vel, int vel_off, int vel_stride, int64_t n,
                     float dt, hipStream_t s);
 
