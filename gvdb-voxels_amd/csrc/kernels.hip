@@ -156,8 +156,8 @@ __global__ void __launch_bounds__(256) k_box_keys(DevTree T, PointsView P, KeyFm
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const float tq = (float)(int)q[a];
-      lo[a] = (int)(tq - r) / R0 * R0 + org[a];
-      hi[a] = (int)(tq + hi_mul * r) / R0 * R0 + org[a];
+      lo[a] = trunc_multiple((int)(tq - r), T.lgR[0]) + org[a];
+      hi[a] = trunc_multiple((int)(tq + hi_mul * r), T.lgR[0]) + org[a];
     }
     if (kTwo) {
       // At most two leaf nodes per axis.  The point's box nodes (bit k = (jx, jy, jz) of mask:
@@ -489,8 +489,8 @@ __global__ void __launch_bounds__(256) k_omap_mark(DevTree T, PointsView P, Omap
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float tq = (float)iq[a];
-    lo[a] = (int)(tq - r) / R0 * R0;
-    hi[a] = (int)(tq + 2.0f * r) / R0 * R0;
+    lo[a] = trunc_multiple((int)(tq - r), T.lgR[0]);
+    hi[a] = trunc_multiple((int)(tq + 2.0f * r), T.lgR[0]);
   }
   if (kTwo) {
 #pragma unroll
