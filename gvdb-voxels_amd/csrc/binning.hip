@@ -1108,7 +1108,10 @@ void density_brick(SubcellCfg c, int nmap, int64_t nsc, int64_t len, const int* 
                    const float* list_pos, const int* leaf_slot, const gvdbx_node* nodes0, const int4* slot_pos,
                    int apron, const int* nbr, float* chan, const VelOut* vel, const LaunchCfg& L, hipStream_t s) {
   if (nmap <= 0) return;
-  const int bs = brick_threads(L, len, nmap, 256);
+  // a wave per 512 entries of the average brick: with fewer lanes per brick a CU holds more
+  // bricks, whose set-up loads, entry loop and channel writes then overlap (4M SPH points, 900
+  // entries per brick: 128 threads 126 us, 256 threads 137 us)
+  const int bs = brick_threads(L, len, nmap, 512);
   const bool fixed = c.res0 == 8 && c.D == 2 && apron == 1;
   const VelOut vo = vel ? *vel : VelOut{nullptr, nullptr, {nullptr, nullptr, nullptr}};
   // square roots per point corner when the bricks hold few list entries (an entry visits about
