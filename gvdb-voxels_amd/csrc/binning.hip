@@ -72,15 +72,21 @@ __global__ void __launch_bounds__(256) k_bbox_partial(PointsView P, float* __res
   }
 }
 
-__global__ void __launch_bounds__(64) k_bbox_final(const float* __restrict__ part, int nparts, int* meta) {
-  const int lane = threadIdx.x;
+// nparts <= kBBoxParts: each of the 256 threads loads its up to 4 partials at once (one memory
+// round trip), then the waves reduce by shuffles and LDS
+__global__ void __launch_bounds__(256) k_bbox_final(const float* __restrict__ part, int nparts, int* meta) {
+  const int t = threadIdx.x;
   float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = lane; b < nparts; b += 64)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      mn[a] = fminf(mn[a], part[b * 6 + a]);
-      mx[a] = fmaxf(mx[a], part[b * 6 + 3 + a]);
-    }
+  for (int k = 0; k < kBBoxParts / 256; ++k) {
+    const int b = t + 256 * k;
+    if (b < nparts)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = fminf(mn[a], part[b * 6 + a]);
+        mx[a] = fmaxf(mx[a], part[b * 6 + 3 + a]);
+      }
+  }
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -88,15 +94,26 @@ __global__ void __launch_bounds__(64) k_bbox_final(const float* __restrict__ par
       mn[a] = fminf(mn[a], __shfl_xor(mn[a], o, 64));
       mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o, 64));
     }
-  if (lane < 3) meta[lane] = float_to_ordered(mn[lane]);
-  else if (lane < 6) meta[lane] = float_to_ordered(mx[lane - 3]);
+  __shared__ float sm[4][6];
+  const int w = t >> 6;
+  if ((t & 63) == 0)
+    for (int a = 0; a < 3; ++a) {
+      sm[w][a] = mn[a];
+      sm[w][3 + a] = mx[a];
+    }
+  __syncthreads();
+  if (t < 6) {
+    float v = sm[0][t];
+    for (int k = 1; k < 4; ++k) v = t < 3 ? fminf(v, sm[k][t]) : fmaxf(v, sm[k][t]);
+    meta[t] = float_to_ordered(v);
+  }
 }
 
 void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s, uint32_t* clear0, int64_t n0,
                    uint32_t* clear1, int64_t n1) {
   const int nb = nblocks(P.n, 256) < kBBoxParts ? nblocks(P.n, 256) : kBBoxParts;
   hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, s, P, part, clear0, n0, clear1, n1);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, part, nb, meta);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, part, nb, meta);
 }
 
 // ------------------------------------------------------------------ home keys

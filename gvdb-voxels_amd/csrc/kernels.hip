@@ -752,7 +752,14 @@ __global__ void __launch_bounds__(256) k_count_leaf_flags(DevTree T, int64_t n, 
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     c += markers ? (markers[i] != 0) : (T.nodes[0][i].flags != 0);
   c = wave_sum(c);
-  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+  // one device atomic per workgroup (they all add to one counter)
+  __shared__ int ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int b = ws[0] + ws[1] + ws[2] + ws[3];
+    if (b) atomicAdd(out, b);
+  }
 }
 void count_leaf_flags(const DevTree& T, int64_t n, const int* markers, int* out, hipStream_t s) {
   if (n <= 0) return;

@@ -145,47 +145,65 @@ __global__ void __launch_bounds__(256) k_scan_down(const int* __restrict__ in, i
 // carry -- one launch instead of three.
 constexpr int64_t kScanSmall = 65536;
 
+// n <= kScanSmall, in batches of 8 chunks of 4096 values: a batch's chunks are loaded at once
+// (one memory round trip per batch), then scanned in turn
+constexpr int kScanBatch = 8;
+
 __global__ void __launch_bounds__(1024) k_scan_small(const int* __restrict__ in, int* __restrict__ out, int64_t n,
                                                      int* __restrict__ total) {
   __shared__ int ws[16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nchunks = (int)((n + 4095) / 4096);
   int carry = 0;
-  for (int64_t base = 0; base < n; base += 4096) {
-    const int64_t i = base + (int64_t)tid * 4;
-    int v[4];
-    if (i + 3 < n) {
-      const int4 q = *(const int4*)(in + i);
-      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-    } else {
-      for (int k = 0; k < 4; ++k) v[k] = (i + k < n) ? in[i + k] : 0;
-    }
-    const int t = v[0] + v[1] + v[2] + v[3];
-    int x = t;
+  for (int b0 = 0; b0 < nchunks; b0 += kScanBatch) {
+    int4 q[kScanBatch];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
+    for (int k = 0; k < kScanBatch; ++k) {
+      const int64_t i = (int64_t)(b0 + k) * 4096 + (int64_t)tid * 4;
+      q[k] = make_int4(0, 0, 0, 0);
+      if (b0 + k < nchunks) {
+        if (i + 3 < n) {
+          q[k] = *(const int4*)(in + i);
+        } else {
+          q[k].x = i < n ? in[i] : 0;
+          q[k].y = i + 1 < n ? in[i + 1] : 0;
+          q[k].z = i + 2 < n ? in[i + 2] : 0;
+        }
+      }
     }
-    if (lane == 63) ws[wave] = x;
-    __syncthreads();
-    int pre = carry, tot = 0;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
-      const int sw = ws[w];
-      pre += (w < wave) ? sw : 0;
-      tot += sw;
+    for (int k = 0; k < kScanBatch; ++k) {
+      if (b0 + k >= nchunks) continue;  // uniform over the workgroup
+      const int64_t i = (int64_t)(b0 + k) * 4096 + (int64_t)tid * 4;
+      const int v[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
+      const int t = v[0] + v[1] + v[2] + v[3];
+      int x = t;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (lane == 63) ws[wave] = x;
+      __syncthreads();
+      int pre = carry, tot = 0;
+#pragma unroll
+      for (int w = 0; w < 16; ++w) {
+        const int sw = ws[w];
+        pre += (w < wave) ? sw : 0;
+        tot += sw;
+      }
+      __syncthreads();
+      pre += x - t;
+      int o[4];
+      o[0] = pre; o[1] = pre + v[0]; o[2] = o[1] + v[1]; o[3] = o[2] + v[2];
+      if (i + 3 < n) {
+        *(int4*)(out + i) = make_int4(o[0], o[1], o[2], o[3]);
+      } else {
+        for (int e = 0; e < 4; ++e)
+          if (i + e < n) out[i + e] = o[e];
+      }
+      carry += tot;
     }
-    __syncthreads();
-    pre += x - t;
-    int o[4];
-    o[0] = pre; o[1] = pre + v[0]; o[2] = o[1] + v[1]; o[3] = o[2] + v[2];
-    if (i + 3 < n) {
-      *(int4*)(out + i) = make_int4(o[0], o[1], o[2], o[3]);
-    } else {
-      for (int k = 0; k < 4; ++k)
-        if (i + k < n) out[i + k] = o[k];
-    }
-    carry += tot;
   }
   if (tid == 0 && total) *total = carry;
 }
