@@ -929,8 +929,8 @@ void levelset_field_q16(SubcellCfg c, int64_t nsc, const int* sc_pos, const int*
 
 // Level set over 4^3 subcells (GatherLevelSet, P.cuh:736-773, and ScatterReduceLevelSet, which is
 // the same segmented min over the (point, subcell) pairs the insert sorted by subcell): one wave
-// per subcell, lane = voxel, the subcell's pair segment read with wave-uniform (scalar) loads.
-// Each lane keeps min |p - v|^2 and takes one square root at the end: sqrtf and the
+// per subcell, each lane a block of 16 voxels over a sixteenth of the subcell's pair segment.
+// Each voxel keeps min |p - v|^2 and takes one square root at the end: sqrtf and the
 // subtraction of r are monotone, so sqrtf(min d2) - r equals the per-pair min of
 // sqrtf(d2) - r bit for bit.
 __global__ void __launch_bounds__(256) k_levelset_sc4(SubcellCfg c, int64_t nsc, const int* __restrict__ sc_pos,
@@ -943,69 +943,84 @@ __global__ void __launch_bounds__(256) k_levelset_sc4(SubcellCfg c, int64_t nsc,
   if (k >= nsc) return;
   const int lane = threadIdx.x & 63;
   const int bx = sc_pos[3 * k] & c.cmask, by = sc_pos[3 * k + 1] & c.cmask, bz = sc_pos[3 * k + 2] & c.cmask;
-  const float vx = (float)(bx + (lane & 3)), vy = (float)(by + ((lane >> 2) & 3)), vz = (float)(bz + (lane >> 4));
   const int n = cnt[k];
   const float* __restrict__ lp = list_pos + 3 * (int64_t)off[k];
-  // Lane (x, y, g): the voxel column (x, y) of the subcell for every z, over the entries j = g
-  // (mod 4).  |p - v|^2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) (dist2's operations, same bits),
-  // and its inner fmaf(dy, dy, dx * dx) is the same for the column's four voxels, so a pair
-  // costs about 4 VALU instructions instead of 7.  The four lane groups read four different
-  // entries per load (each address shared by 16 lanes); their minima are combined at the end.
-  const int g = lane >> 4;
+  // Lane (b, g): the 2 x 2 block b of voxel columns (x, y) of the subcell, all four z, over the
+  // entries j = g (mod 16).  |p - v|^2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) (dist2's
+  // operations, same bits): per entry the lane forms dx and dx * dx for its two x, dy for its
+  // two y, the four inner fmaf(dy, dy, dx * dx) and dz for the four z, so its 16 voxels cost
+  // 2.4 VALU instructions each instead of 7.  The 16 lane groups read 16 different entries per
+  // load (each address shared by 4 lanes); their minima are combined at the end.
+  const int b = lane & 3, g = lane >> 2;
+  const int ix0 = bx + 2 * (b & 1), iy0 = by + 2 * (b >> 1);
+  const float vx0 = (float)ix0, vx1 = (float)(ix0 + 1), vy0 = (float)iy0, vy1 = (float)(iy0 + 1);
   const float vz0 = (float)bz;
-  float mz[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-  auto take = [&](float px, float py, float pz) {
-    const float dx = px - vx, dy = py - vy;
-    const float t = fmaf(dy, dy, dx * dx);
+  float mn[16];  // voxel (x0 + xo, y0 + yo, z0 + z) at ((yo * 2 + xo) * 4 + z)
 #pragma unroll
-    for (int z = 0; z < 4; ++z) {
-      const float dz = pz - (vz0 + (float)z);
-      mz[z] = fminf(mz[z], fmaf(dz, dz, t));
-    }
+  for (int v = 0; v < 16; ++v) mn[v] = INFINITY;
+  auto take = [&](float px, float py, float pz) {
+    const float dx0 = px - vx0, dx1 = px - vx1, dy0 = py - vy0, dy1 = py - vy1;
+    const float sx0 = dx0 * dx0, sx1 = dx1 * dx1;
+    const float t[4] = {fmaf(dy0, dy0, sx0), fmaf(dy0, dy0, sx1), fmaf(dy1, dy1, sx0), fmaf(dy1, dy1, sx1)};
+    float dz[4];
+#pragma unroll
+    for (int z = 0; z < 4; ++z) dz[z] = pz - (vz0 + (float)z);
+#pragma unroll
+    for (int cxy = 0; cxy < 4; ++cxy)
+#pragma unroll
+      for (int z = 0; z < 4; ++z) mn[cxy * 4 + z] = fminf(mn[cxy * 4 + z], fmaf(dz[z], dz[z], t[cxy]));
   };
   int jb = 0;
-  constexpr int kStep = 16;  // entries per step: four per lane group, loaded one step ahead
+  constexpr int kStep = 32;  // entries per step: two per lane group, loaded one step ahead
   if (n >= kStep) {
-    float nx[12];
+    float nx[6];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const float* q = lp + 3 * (4 * e + g);
+    for (int e = 0; e < 2; ++e) {
+      const float* q = lp + 3 * (16 * e + g);
       nx[3 * e] = q[0];
       nx[3 * e + 1] = q[1];
       nx[3 * e + 2] = q[2];
     }
     for (; jb + kStep <= n; jb += kStep) {
-      float p[12];
+      float p[6];
 #pragma unroll
-      for (int e = 0; e < 12; ++e) p[e] = nx[e];
+      for (int e = 0; e < 6; ++e) p[e] = nx[e];
       if (jb + 2 * kStep <= n) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const float* q = lp + 3 * (jb + kStep + 4 * e + g);
+        for (int e = 0; e < 2; ++e) {
+          const float* q = lp + 3 * (jb + kStep + 16 * e + g);
           nx[3 * e] = q[0];
           nx[3 * e + 1] = q[1];
           nx[3 * e + 2] = q[2];
         }
       }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) take(p[3 * e], p[3 * e + 1], p[3 * e + 2]);
+      take(p[0], p[1], p[2]);
+      take(p[3], p[4], p[5]);
     }
   }
-  for (; jb < n; jb += 4) {
+  for (; jb < n; jb += 16) {
     const int j = jb + g;
     if (j < n) take(lp[3 * j], lp[3 * j + 1], lp[3 * j + 2]);
   }
+  // combine the 16 entry phases (lane bits 2..5)
 #pragma unroll
-  for (int z = 0; z < 4; ++z) {
-    mz[z] = fminf(mz[z], __shfl_xor(mz[z], 16, 64));
-    mz[z] = fminf(mz[z], __shfl_xor(mz[z], 32, 64));
+  for (int v = 0; v < 16; ++v) {
+    mn[v] = fminf(mn[v], __shfl_xor(mn[v], 4, 64));
+    mn[v] = fminf(mn[v], __shfl_xor(mn[v], 8, 64));
+    mn[v] = fminf(mn[v], __shfl_xor(mn[v], 16, 64));
+    mn[v] = fminf(mn[v], __shfl_xor(mn[v], 32, 64));
   }
-  // lane (x, y, g) writes voxel (x, y, z = g), the lane order of the voxels above
-  const float md2 = g == 0 ? mz[0] : (g == 1 ? mz[1] : (g == 2 ? mz[2] : mz[3]));
+  // lane (b, g) writes voxel g of its block: xo = g & 1, yo = (g >> 1) & 1, z = g >> 2
+  const int xo = g & 1, yo = (g >> 1) & 1, zo = g >> 2;
+  const int sel = (yo * 2 + xo) * 4 + zo;
+  float md2 = mn[0];
+#pragma unroll
+  for (int v = 1; v < 16; ++v) md2 = sel == v ? mn[v] : md2;
   const int slot = leaf_slot[sc_nid[k]];
   if (slot < 0) return;
   const int br = c.res0 + 2 * apron, m = c.res0 - 1;
-  const int64_t vi = atlas_voxel(slot, br, ((int)vx & m) + apron, ((int)vy & m) + apron, ((int)vz & m) + apron);
+  const int64_t vi =
+      atlas_voxel(slot, br, ((ix0 + xo) & m) + apron, ((iy0 + yo) & m) + apron, ((bz + zo) & m) + apron);
   float val = accum ? chan[vi] : init;
   if (n > 0) val = fminf(val, sqrtf(md2) - c.r);
   chan[vi] = val;
