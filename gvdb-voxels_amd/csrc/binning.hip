@@ -23,9 +23,6 @@ static inline int nblocks(int64_t n, int bs) {
 constexpr float kVelScale = 16777216.0f;  // 2^24 fixed-point units
 __device__ __forceinline__ long long vel_fixed(float v) { return __float2ll_rn(v * kVelScale); }
 
-// C division of v by 2^lg (truncation toward zero), times 2^lg
-__device__ __forceinline__ int trunc_mult(int v, int lg) { return (v >= 0 ? v >> lg : -((-v) >> lg)) * (1 << lg); }
-
 // Subcell corner range of a point (the exact FP32 operation order of P.cuh:238-243); sc = 2^lgsc.
 __device__ __forceinline__ void subcell_range(const float q[3], float r, int lgsc, int lo[3], int hi[3]) {
 #pragma unroll
@@ -33,8 +30,8 @@ __device__ __forceinline__ void subcell_range(const float q[3], float r, int lgs
     const float lq = q[a] - r;
     float hq = q[a] + 1.0f;
     hq = hq + r;
-    lo[a] = trunc_mult((int)lq, lgsc);
-    hi[a] = trunc_mult((int)hq, lgsc);
+    lo[a] = trunc_multiple((int)lq, lgsc);
+    hi[a] = trunc_multiple((int)hq, lgsc);
   }
 }
 
@@ -500,10 +497,13 @@ __device__ __forceinline__ void put_entry(const BinArgs& A, int64_t k, const flo
   if (A.list_clr) A.list_clr[k] = clr;
 }
 
-// kTwo (2r + 2 <= sc): a point's subcell box [trunc(q - r), trunc(q + 1 + r)] spans under one
-// subcell per axis, so it reaches at most 2 subcells per axis, all inside its brick's window; the
-// up to 8 entries then take their list slots with independent LDS atomics issued back to back.
-template <int kD, bool kTwo>
+// kCorners = 2 (2r + 2 <= sc): a point's subcell box [trunc(q - r), trunc(q + 1 + r)] spans under
+// one subcell per axis, so it reaches at most 2 subcells per axis, all inside its brick's window;
+// the up to 8 entries then take their list slots with independent LDS atomics issued back to back.
+// kCorners = 3 (r + 1 <= sc, as the level set's r = 2 with subcells of 4): up to 3 subcells per
+// axis, the 27 candidates likewise (a point reaching more, or a subcell outside the window, takes
+// the general path's steps); 0: the general loop.
+template <int kD, int kCorners>
 __device__ __forceinline__ void bin_insert_brick(const BinArgs& A, int m, unsigned long long* smem64, int* nb) {
   BrickCtx B;
   const int beg = A.start[m], end = A.start[m + 1];  // in flight with the brick's set-up loads
@@ -526,7 +526,7 @@ __device__ __forceinline__ void bin_insert_brick(const BinArgs& A, int m, unsign
     const uint32_t clr = A.list_clr ? load_u32(A.P.clr, __float_as_uint(p.w), A.P.clr_off, A.P.clr_stride) : 0u;
     int lo[3], hi[3];
     subcell_range(q, c.r, c.lgsc, lo, hi);
-    if (kTwo) {
+    if (kCorners == 2) {
       int slot[8];
       bool act[8];
 #pragma unroll
@@ -541,6 +541,36 @@ __device__ __forceinline__ void bin_insert_brick(const BinArgs& A, int m, unsign
       for (int e = 0; e < 8; ++e) {
         if (!act[e]) continue;
         put_entry(A, slot[e], q, v.x, v.y, v.z, clr);
+      }
+      continue;
+    }
+    if (kCorners == 3 && hi[0] - lo[0] <= 2 * c.sc && hi[1] - lo[1] <= 2 * c.sc && hi[2] - lo[2] <= 2 * c.sc) {
+      int slot[27];
+      unsigned act = 0u, direct = 0u;  // bit e: candidate e takes a window slot / a one-by-one slot
+#pragma unroll
+      for (int e = 0; e < 27; ++e) {
+        const int x = lo[0] + (e % 3) * c.sc, y = lo[1] + ((e / 3) % 3) * c.sc, z = lo[2] + (e / 9) * c.sc;
+        slot[e] = 0;
+        if (x > hi[0] || y > hi[1] || z > hi[2]) continue;
+        const int w = B.window_of(x, y, z, c.lgsc);
+        if (w < 0) {
+          direct |= 1u << e;
+          continue;
+        }
+        const int base = lbase[w];
+        if (base < 0) continue;
+        act |= 1u << e;
+        slot[e] = base + atomicAdd(&lcnt[w], 1);
+      }
+#pragma unroll
+      for (int e = 0; e < 27; ++e)
+        if ((act >> e) & 1u) put_entry(A, slot[e], q, v.x, v.y, v.z, clr);
+      for (unsigned d = direct; d; d &= d - 1u) {
+        const int e = __ffs(d) - 1;
+        const int x = lo[0] + (e % 3) * c.sc, y = lo[1] + ((e / 3) % 3) * c.sc, z = lo[2] + (e / 9) * c.sc;
+        const int64_t g = direct_subcell(A, x, y, z, B.ox, B.oy, B.oz);
+        if (g < 0) continue;
+        put_entry(A, (int64_t)A.off[g] + A.cnt_in[g] + atomicAdd(&A.ocur[g], 1), q, v.x, v.y, v.z, clr);
       }
       continue;
     }
@@ -563,11 +593,11 @@ __device__ __forceinline__ void bin_insert_brick(const BinArgs& A, int m, unsign
   }
 }
 
-template <int kD, bool kTwo, bool kPersist>
+template <int kD, int kCorners, bool kPersist>
 __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
   extern __shared__ unsigned long long smem64[];
   __shared__ int nb[27];
-  for_each_brick<kPersist>(nmap, A.L.queue, [&](int m) { bin_insert_brick<kD, kTwo>(A, m, smem64, nb); });
+  for_each_brick<kPersist>(nmap, A.L.queue, [&](int m) { bin_insert_brick<kD, kCorners>(A, m, smem64, nb); });
 }
 
 int bin_uses_class_bins(int D) { return D <= 2 ? 1 : 0; }
@@ -628,14 +658,19 @@ void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
   const size_t sm = (size_t)2 * A.W * A.W * A.W * 4 + 8;
   const int bs = brick_threads(A.L, A.P.n, nmap, 64);
   const bool two = 2.0 * (double)A.c.r + 2.0 <= (double)A.c.sc;
-  if (A.c.D == 2 && two)
-    launch_bricks(k_bin_insert<2, true, false>, k_bin_insert<2, true, true>, A.L, bs, sm, nmap, s, A, nmap);
-  else if (A.c.D == 2)
-    launch_bricks(k_bin_insert<2, false, false>, k_bin_insert<2, false, true>, A.L, bs, sm, nmap, s, A, nmap);
-  else if (two)
-    launch_bricks(k_bin_insert<0, true, false>, k_bin_insert<0, true, true>, A.L, bs, sm, nmap, s, A, nmap);
-  else
-    launch_bricks(k_bin_insert<0, false, false>, k_bin_insert<0, false, true>, A.L, bs, sm, nmap, s, A, nmap);
+  const bool three = (double)A.c.r + 1.0 <= (double)A.c.sc;
+#define GVDBX_INSERT(D, K) \
+  launch_bricks(k_bin_insert<D, K, false>, k_bin_insert<D, K, true>, A.L, bs, sm, nmap, s, A, nmap)
+  if (A.c.D == 2) {
+    if (two) GVDBX_INSERT(2, 2);
+    else if (three) GVDBX_INSERT(2, 3);
+    else GVDBX_INSERT(2, 0);
+  } else {
+    if (two) GVDBX_INSERT(0, 2);
+    else if (three) GVDBX_INSERT(0, 3);
+    else GVDBX_INSERT(0, 0);
+  }
+#undef GVDBX_INSERT
 }
 
 __global__ void k_sum_counts(const int* __restrict__ a, const int* __restrict__ b, int* __restrict__ out, int64_t n) {
