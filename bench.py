@@ -255,7 +255,9 @@ def adjoint_bench(reps):
             "forward_hbm_frac": round(fbytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
-VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4  # wave64 FP32 VALU issue rate of MI355X (no packed math)
+# wave64 FP32 VALU instructions per second that MI355X's published FP32 vector peak implies:
+# 157.3 TFLOP/s / (64 lanes x 2 flops per FMA) = 256 CUs x 4 SIMDs x 2.4 GHz / 2 cycles
+VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 2
 
 
 def levelset_valu_per_entry():
