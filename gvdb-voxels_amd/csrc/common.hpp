@@ -149,8 +149,9 @@ __device__ __forceinline__ int64_t find_leaf(const DevTree& T, int x, int y, int
 // scatter-reduce) rounds identically.
 __device__ __forceinline__ float dist2(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
 
-// x / 2^s * 2^s with C's truncation toward zero (the box passes' integer division of a voxel
-// coordinate by the leaf range), by shifts: negative x is biased by 2^s - 1 first
+// x / 2^s * 2^s with C's truncation toward zero (the reference's integer division of a voxel
+// coordinate by the leaf range or the subcell size), by shifts: negative x is biased by 2^s - 1
+// first
 __device__ __forceinline__ int trunc_multiple(int x, int s) { return ((x + ((x >> 31) & ((1 << s) - 1))) >> s) << s; }
 
 // 16-byte store marked non-temporal (the line is not kept in the caches): for output written
