@@ -1710,8 +1710,8 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     svals = par ? P<uint32_t>(c->hvals_alt) : P<uint32_t>(c->hvals);
     key_ranges(skeys, n, nmap, P<int>(c->ranges), c->stream);
   }
-  // [tail[0], tail[1]): valid points whose own leaf is missing (invalid points sort after them)
-  const int* tail = P<int>(c->ranges) + nmap;
+  // [ranges[nmap], ranges[nmap + 1]): the valid points whose own leaf is missing, which the count
+  // and insert launches place one by one (invalid points sort after them)
   // Sorted copies of the points for the per-brick kernels: with bricks of 64 to 511 points on
   // average the count pass gathers them itself (its bricks' points, through the permutation);
   // sparse bricks leave the count pass too little work per point to hide that gather's latency,
@@ -1761,7 +1761,6 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   {
     TimedRegion tr(c, "count");
     bin_count(A, nmap, c->stream);
-    bin_points_count(A, tail, c->stream);
   }
   // subcell offsets: block sums from the totals pass, their scan, the block scans in the bases pass
   const int64_t nblk = (nsc + 255) / 256;
@@ -1822,7 +1821,6 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
   {
     TimedRegion tr(c, "insert");
     bin_insert(A, nmap, c->stream);
-    bin_points_insert(A, tail, c->stream);
   }
   if (early && !c->recording) {
     CHK(read_meta_end(c));  // returns while the insert kernels still run

@@ -462,16 +462,6 @@ __device__ __forceinline__ void bin_count_brick(const BinArgs& A, int m, int use
   }
 }
 
-template <int kD, bool kPersist>
-__global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_bins) {
-  extern __shared__ unsigned long long smem64[];
-  __shared__ int nb[27];
-  // when the count pass gathers its bricks' points from the caller's arrays, neighbouring bricks
-  // (whose points share cache lines there) run on one XCD
-  for_each_brick<kPersist>(
-      nmap, A.L.queue, [&](int m) { bin_count_brick<kD>(A, m, use_bins, smem64, nb); }, A.spos_out != nullptr);
-}
-
 // One list entry: the position (float3, or ushort3 quantized for the FP16 insert), the velocity
 // when velocity lists are kept, the colour when colours are bound.
 __device__ __forceinline__ void put_entry(const BinArgs& A, int64_t k, const float q[3], float vx, float vy, float vz,
@@ -496,6 +486,69 @@ __device__ __forceinline__ void put_entry(const BinArgs& A, int64_t k, const flo
   }
   if (A.list_clr) A.list_clr[k] = clr;
 }
+
+template <bool kInsert>
+__device__ __forceinline__ void bin_points_body(const BinArgs& A, const int* __restrict__ begin, int64_t blk,
+                                                int64_t nblk) {
+  const int64_t b0 = begin[0], b1 = begin[1];
+  const SubcellCfg c = A.c;
+  const bool vel = A.vsum != nullptr && A.P.vel != nullptr;
+  for (int64_t j = b0 + blk * blockDim.x + threadIdx.x; j < b1; j += nblk * blockDim.x) {
+    const int64_t i = (int64_t)A.svals[j];
+    const float3 p = load_f3(A.P.pos, i, A.P.pos_off, A.P.pos_stride);
+    if (p.z == kNoHit) continue;
+    const float q[3] = {p.x + A.P.tx, p.y + A.P.ty, p.z + A.P.tz};
+    if (q[0] < 0 || q[1] < 0 || q[2] < 0) continue;
+    const int3 o = cloud_origin(A.P, i);
+    float3 v = make_float3(0.f, 0.f, 0.f);
+    long long fv[3] = {0, 0, 0};
+    if (A.P.vel && (vel || A.list_vel || A.list_vq16)) {
+      v = load_f3(A.P.vel, i, A.P.vel_off, A.P.vel_stride);
+      fv[0] = vel_fixed(v.x);
+      fv[1] = vel_fixed(v.y);
+      fv[2] = vel_fixed(v.z);
+    }
+    const uint32_t clr = (kInsert && A.list_clr) ? load_u32(A.P.clr, i, A.P.clr_off, A.P.clr_stride) : 0u;
+    int lo[3], hi[3];
+    subcell_range(q, c.r, c.lgsc, lo, hi);
+    for (int x = lo[0]; x <= hi[0]; x += c.sc)
+      for (int y = lo[1]; y <= hi[1]; y += c.sc)
+        for (int z = lo[2]; z <= hi[2]; z += c.sc) {
+          const int64_t g = direct_subcell(A, x, y, z, o.x, o.y, o.z);
+          if (g < 0) continue;
+          if (!kInsert) {
+            atomicAdd(&A.ocnt[g], 1);
+            if (vel) add_vel_global(A.vsum, g, fv);
+            continue;
+          }
+          const int64_t k = (int64_t)A.off[g] + A.cnt_in[g] + atomicAdd(&A.ocur[g], 1);
+          put_entry(A, k, q, v.x, v.y, v.z, clr);
+        }
+  }
+}
+
+
+// The count and insert launches hold kOrphanItems more workgroups than bricks: they stride over
+// the sorted points whose own leaf is missing ([start[nmap], start[nmap + 1]), rare: an insert of
+// points the rebuild did not see), placing their pairs one by one with global atomics.  The
+// window slices of the bricks and these one-by-one slots are disjoint, so both run in one launch.
+constexpr int kOrphanItems = 256;
+
+template <int kD, bool kPersist>
+__global__ void __launch_bounds__(256) k_bin_count(BinArgs A, int nmap, int use_bins) {
+  extern __shared__ unsigned long long smem64[];
+  __shared__ int nb[27];
+  // when the count pass gathers its bricks' points from the caller's arrays, neighbouring bricks
+  // (whose points share cache lines there) run on one XCD
+  for_each_brick<kPersist>(
+      (int64_t)nmap + kOrphanItems, A.L.queue,
+      [&](int m) {
+        if (m < nmap) bin_count_brick<kD>(A, m, use_bins, smem64, nb);
+        else bin_points_body<false>(A, A.start + nmap, m - nmap, kOrphanItems);
+      },
+      A.spos_out != nullptr);
+}
+
 
 // kCorners = 2 (2r + 2 <= sc): a point's subcell box [trunc(q - r), trunc(q + 1 + r)] spans under
 // one subcell per axis, so it reaches at most 2 subcells per axis, all inside its brick's window;
@@ -597,7 +650,10 @@ template <int kD, int kCorners, bool kPersist>
 __global__ void __launch_bounds__(256) k_bin_insert(BinArgs A, int nmap) {
   extern __shared__ unsigned long long smem64[];
   __shared__ int nb[27];
-  for_each_brick<kPersist>(nmap, A.L.queue, [&](int m) { bin_insert_brick<kD, kCorners>(A, m, smem64, nb); });
+  for_each_brick<kPersist>((int64_t)nmap + kOrphanItems, A.L.queue, [&](int m) {
+    if (m < nmap) bin_insert_brick<kD, kCorners>(A, m, smem64, nb);
+    else bin_points_body<true>(A, A.start + nmap, m - nmap, kOrphanItems);
+  });
 }
 
 int bin_uses_class_bins(int D) { return D <= 2 ? 1 : 0; }
@@ -648,9 +704,11 @@ void bin_count(const BinArgs& A, int nmap, hipStream_t s) {
   const int bins = bs == 256 ? bin_uses_class_bins(A.c.D) : 0;
   const size_t sm = bin_count_smem(A.W, bins ? A.c.D : 0, A.vsum != nullptr && A.P.vel != nullptr);
   if (A.c.D == 2)
-    launch_bricks(k_bin_count<2, false>, k_bin_count<2, true>, A.L, bs, sm, nmap, s, A, nmap, bins);  // gathers too
+    launch_bricks(k_bin_count<2, false>, k_bin_count<2, true>, A.L, bs, sm, (int64_t)nmap + kOrphanItems, s, A, nmap,
+                  bins);  // gathers too
   else
-    launch_bricks(k_bin_count<0, false>, k_bin_count<0, true>, A.L, bs, sm, nmap, s, A, nmap, bins);
+    launch_bricks(k_bin_count<0, false>, k_bin_count<0, true>, A.L, bs, sm, (int64_t)nmap + kOrphanItems, s, A, nmap,
+                  bins);
 }
 
 void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
@@ -660,7 +718,8 @@ void bin_insert(const BinArgs& A, int nmap, hipStream_t s) {
   const bool two = 2.0 * (double)A.c.r + 2.0 <= (double)A.c.sc;
   const bool three = (double)A.c.r + 1.0 <= (double)A.c.sc;
 #define GVDBX_INSERT(D, K) \
-  launch_bricks(k_bin_insert<D, K, false>, k_bin_insert<D, K, true>, A.L, bs, sm, nmap, s, A, nmap)
+  launch_bricks(k_bin_insert<D, K, false>, k_bin_insert<D, K, true>, A.L, bs, sm, (int64_t)nmap + kOrphanItems, s, A, \
+                nmap)
   if (A.c.D == 2) {
     if (two) GVDBX_INSERT(2, 2);
     else if (three) GVDBX_INSERT(2, 3);
@@ -802,58 +861,6 @@ void cell_bases(const BinArgs& A, int64_t nsc, int* cnt, int* tile, int* off, hi
 // ------------------------------------------------------------------ per-point path
 // Sorted positions [begin[0], begin[1]) (the points whose own leaf is missing) are binned one
 // pair at a time with global atomics.
-template <bool kInsert>
-__global__ void __launch_bounds__(256) k_bin_points(BinArgs A, const int* __restrict__ begin) {
-  const int64_t b0 = begin[0], b1 = begin[1];
-  const SubcellCfg c = A.c;
-  const bool vel = A.vsum != nullptr && A.P.vel != nullptr;
-  for (int64_t j = b0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < b1;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = (int64_t)A.svals[j];
-    const float3 p = load_f3(A.P.pos, i, A.P.pos_off, A.P.pos_stride);
-    if (p.z == kNoHit) continue;
-    const float q[3] = {p.x + A.P.tx, p.y + A.P.ty, p.z + A.P.tz};
-    if (q[0] < 0 || q[1] < 0 || q[2] < 0) continue;
-    const int3 o = cloud_origin(A.P, i);
-    float3 v = make_float3(0.f, 0.f, 0.f);
-    long long fv[3] = {0, 0, 0};
-    if (A.P.vel && (vel || A.list_vel || A.list_vq16)) {
-      v = load_f3(A.P.vel, i, A.P.vel_off, A.P.vel_stride);
-      fv[0] = vel_fixed(v.x);
-      fv[1] = vel_fixed(v.y);
-      fv[2] = vel_fixed(v.z);
-    }
-    const uint32_t clr = (kInsert && A.list_clr) ? load_u32(A.P.clr, i, A.P.clr_off, A.P.clr_stride) : 0u;
-    int lo[3], hi[3];
-    subcell_range(q, c.r, c.lgsc, lo, hi);
-    for (int x = lo[0]; x <= hi[0]; x += c.sc)
-      for (int y = lo[1]; y <= hi[1]; y += c.sc)
-        for (int z = lo[2]; z <= hi[2]; z += c.sc) {
-          const int64_t g = direct_subcell(A, x, y, z, o.x, o.y, o.z);
-          if (g < 0) continue;
-          if (!kInsert) {
-            atomicAdd(&A.ocnt[g], 1);
-            if (vel) add_vel_global(A.vsum, g, fv);
-            continue;
-          }
-          const int64_t k = (int64_t)A.off[g] + A.cnt_in[g] + atomicAdd(&A.ocur[g], 1);
-          put_entry(A, k, q, v.x, v.y, v.z, clr);
-        }
-  }
-}
-
-// Points whose own leaf is missing are rare (an insert of points the rebuild did not see), so a
-// small grid strides over them.
-static inline int bin_points_blocks(int64_t n) { return nblocks(n, 256) < 256 ? nblocks(n, 256) : 256; }
-
-void bin_points_count(const BinArgs& A, const int* begin, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_points<false>, dim3(bin_points_blocks(A.P.n)), dim3(256), 0, s, A, begin);
-}
-
-void bin_points_insert(const BinArgs& A, const int* begin, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_points<true>, dim3(bin_points_blocks(A.P.n)), dim3(256), 0, s, A, begin);
-}
-
 // ------------------------------------------------------------------ gathers
 
 __device__ __forceinline__ int64_t atlas_index(int slot, int br, int lx, int ly, int lz) {

@@ -59,10 +59,10 @@ struct BinArgs {
   unsigned long long* wvs;  // [nmap][W^3][3]: the brick's window velocity sums (where its count > 0)
 };
 
+// the bricks' pairs, and those of the sorted points whose own leaf is missing ([start[nmap],
+// start[nmap + 1]), placed one by one), in one launch each
 void bin_count(const BinArgs& A, int nmap, hipStream_t s);
 void bin_insert(const BinArgs& A, int nmap, hipStream_t s);
-void bin_points_count(const BinArgs& A, const int* begin, hipStream_t s);
-void bin_points_insert(const BinArgs& A, const int* begin, hipStream_t s);
 void sum_counts(const int* a, const int* b, int* out, int64_t n, hipStream_t s);
 void gather_sorted(const PointsView& P, const uint32_t* svals, int64_t n, float4* spos, float4* svel, hipStream_t s);
 // per subcell: cnt_in = sum of the brick-window counts of the (up to 8) bricks that reach it,
