@@ -1532,7 +1532,7 @@ static int plan_brick_map(gvdbx_ctx* c, BrickMapOut* out, bool* use) {
     cells *= n[a];
     if (cells > kBrickMapMax) return GVDBX_OK;
   }
-  CHK(ensure(c, c->bmap, (size_t)cells * 4));
+  CHK(ensure(c, c->bmap, (size_t)cells * 8));  // (leaf, slot) per cell
   out->bmap = P<int>(c->bmap);
   out->lg0 = s;
   for (int a = 0; a < 3; ++a) {
@@ -1703,7 +1703,7 @@ static int insert_impl(gvdbx_ctx* c, int sc, int64_t n, float r, const float tra
     int bits = 1;  // keys run up to the sentinel nmap + 1
     while (((int64_t)1 << bits) <= (int64_t)nmap + 1) ++bits;
     // the key launch also counts the first radix digit; the sort generates the point positions
-    home_keys_counted(T, pv, mapp, nmap, P<uint32_t>(c->hkeys), bits, P<int>(c->radix_counts), c->stream);
+    home_keys_counted(T, pv, mapp, slot_map, nmap, P<uint32_t>(c->hkeys), bits, P<int>(c->radix_counts), c->stream);
     const int par = radix_sort_u32_positions(P<uint32_t>(c->hkeys), P<uint32_t>(c->hkeys_alt), P<uint32_t>(c->hvals),
                                              P<uint32_t>(c->hvals_alt), n, bits, P<int>(c->radix_counts), c->stream);
     skeys = par ? P<uint32_t>(c->hkeys_alt) : P<uint32_t>(c->hkeys);

@@ -120,9 +120,10 @@ void bbox_two_pass(const PointsView& P, float* part, int* meta, hipStream_t s, u
 // The home keys (no values: the sort generates the positions), and per tile of kRadixTile points
 // the counts of the first radix digit (the low `nbits` bits), digit-major into
 // counts[digit * ntiles + tile] for all 256 digits, as k_radix_up would count them.
+// slot_keys: map is leaf_slot (the slot map), so the key is the point's slot, one table load
 __global__ void __launch_bounds__(kKeyBlock) k_home_keys_counted(DevTree T, PointsView P, const int* __restrict__ map,
-                                                          int nmap, uint32_t* __restrict__ keys, int nbits,
-                                                          int* __restrict__ counts, int ntiles) {
+                                                          int slot_keys, int nmap, uint32_t* __restrict__ keys,
+                                                          int nbits, int* __restrict__ counts, int ntiles) {
   __shared__ int hist[256];
   if (threadIdx.x < 256) hist[threadIdx.x] = 0;
   __syncthreads();
@@ -138,8 +139,13 @@ __global__ void __launch_bounds__(kKeyBlock) k_home_keys_counted(DevTree T, Poin
       const float q[3] = {p.x + P.tx, p.y + P.ty, p.z + P.tz};
       if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0) {
         const int3 o = cloud_origin(P, i);
-        const int64_t lf = find_leaf(T, (int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z);
-        key = lf >= 0 ? (uint32_t)map[lf] : (uint32_t)nmap;
+        if (slot_keys) {
+          const int sl = find_slot(T, (int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z, map);
+          key = sl >= 0 ? (uint32_t)sl : (uint32_t)nmap;
+        } else {
+          const int64_t lf = find_leaf(T, (int)q[0] + o.x, (int)q[1] + o.y, (int)q[2] + o.z);
+          key = lf >= 0 ? (uint32_t)map[lf] : (uint32_t)nmap;
+        }
       }
     }
     keys[i] = key;
@@ -149,11 +155,11 @@ __global__ void __launch_bounds__(kKeyBlock) k_home_keys_counted(DevTree T, Poin
   if (threadIdx.x < 256) counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-void home_keys_counted(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, int bits,
-                       int* counts, hipStream_t s) {
+void home_keys_counted(const DevTree& T, const PointsView& P, const int* map, bool slot_keys, int nmap,
+                       uint32_t* keys, int bits, int* counts, hipStream_t s) {
   const int ntiles = (int)((P.n + kRadixTile - 1) / kRadixTile);
-  hipLaunchKernelGGL(k_home_keys_counted, dim3(ntiles), dim3(kKeyBlock), 0, s, T, P, map, nmap, keys,
-                     radix_first_digit_bits(bits), counts, ntiles);
+  hipLaunchKernelGGL(k_home_keys_counted, dim3(ntiles), dim3(kKeyBlock), 0, s, T, P, map, slot_keys ? 1 : 0, nmap,
+                     keys, radix_first_digit_bits(bits), counts, ntiles);
 }
 
 // Batched clouds: cloud index of every point (off = exclusive prefix of the cloud sizes) and
@@ -1467,8 +1473,7 @@ __global__ void k_neighbor_table(DevTree T, int64_t nslots, const int4* __restri
   const int k = (int)(i % 27);
   const int4 bp = slot_pos[s];
   const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
-  const int64_t lf = find_leaf(T, bp.x + dx * res0, bp.y + dy * res0, bp.z + dz * res0);
-  nbr[i] = lf < 0 ? -1 : leaf_slot[lf];
+  nbr[i] = find_slot(T, bp.x + dx * res0, bp.y + dy * res0, bp.z + dz * res0, leaf_slot);
 }
 
 void neighbor_table(const DevTree& T, int64_t nslots, const int4* slot_pos, const int* leaf_slot, int res0, int* nbr,

@@ -16,8 +16,9 @@ void batch_ids(const PointsView& P, const int64_t* off, int ncloud, float limit,
                hipStream_t s);
 // home keys with the first radix pass's tile counts over `bits` key bits (radix_sort_u32_positions
 // then sorts them; counts holds radix_count_ints(n) ints)
-void home_keys_counted(const DevTree& T, const PointsView& P, const int* map, int nmap, uint32_t* keys, int bits,
-                       int* counts, hipStream_t s);
+// slot_keys: map is leaf_slot (the keys are the points' slots, read from the lookup table)
+void home_keys_counted(const DevTree& T, const PointsView& P, const int* map, bool slot_keys, int nmap,
+                       uint32_t* keys, int bits, int* counts, hipStream_t s);
 void key_ranges(const uint32_t* skeys, int64_t n, int nmap, int* start, hipStream_t s);
 // the same start[0 .. nmax + 1] of sorted keys in [0, nmax], by one pass over the keys
 void key_starts(const uint32_t* skeys, int64_t n, int nmax, int* start, hipStream_t s);

@@ -29,9 +29,10 @@ struct DevTree {
   gvdbx_node* nodes[kMaxLev];
   uint64_t* clist[kMaxLev];
   // Leaf lookup table over a box of leaf-brick cells inside the root, or null: cell
-  // (bx, by, bz) (brick coordinates, global) at bmap[((bz - lo) * n1 + by - lo) * n0 + bx - lo]
-  // holds the leaf whose brick it is, -1 where none is linked.  Built from the slots, which are
-  // exactly the linked leaves, so a lookup inside the box equals the descent.
+  // (bx, by, bz) (brick coordinates, global), c = ((bz - lo) * n1 + by - lo) * n0 + bx - lo,
+  // holds at bmap[2c] the leaf whose brick it is and at bmap[2c + 1] that leaf's atlas slot
+  // (both -1 where none is linked), so one 8-byte load takes a point to its slot.  Built from
+  // the slots, which are exactly the linked leaves, so a lookup inside the box equals the descent.
   const int* bmap;
   int bm_lo[3], bm_n[3];
 };
@@ -127,15 +128,21 @@ __device__ __forceinline__ int64_t descend(const DevTree& T, int x, int y, int z
   return cur;
 }
 
-// The lookup table's leaf for voxel (x, y, z): the leaf index, -1 when no leaf is linked there,
-// -2 when there is no table or the voxel's brick lies outside its box.
-__device__ __forceinline__ int64_t table_leaf(const DevTree& T, int x, int y, int z) {
-  if (!T.bmap) return -2;
+// The lookup table's (leaf, slot) for voxel (x, y, z): (-1, -1) when no leaf is linked there,
+// (-2, -2) when there is no table or the voxel's brick lies outside its box.
+__device__ __forceinline__ int2 table_entry(const DevTree& T, int x, int y, int z) {
+  if (!T.bmap) return make_int2(-2, -2);
   const int s = T.lgR[0];
   const unsigned bx = (unsigned)((x >> s) - T.bm_lo[0]), by = (unsigned)((y >> s) - T.bm_lo[1]),
                  bz = (unsigned)((z >> s) - T.bm_lo[2]);
-  if (bx >= (unsigned)T.bm_n[0] || by >= (unsigned)T.bm_n[1] || bz >= (unsigned)T.bm_n[2]) return -2;
-  return T.bmap[((int64_t)bz * T.bm_n[1] + by) * T.bm_n[0] + bx];
+  if (bx >= (unsigned)T.bm_n[0] || by >= (unsigned)T.bm_n[1] || bz >= (unsigned)T.bm_n[2]) return make_int2(-2, -2);
+  return reinterpret_cast<const int2*>(T.bmap)[((int64_t)bz * T.bm_n[1] + by) * T.bm_n[0] + bx];
+}
+
+// The lookup table's leaf for voxel (x, y, z): the leaf index, -1 when no leaf is linked there,
+// -2 when there is no table or the voxel's brick lies outside its box.
+__device__ __forceinline__ int64_t table_leaf(const DevTree& T, int x, int y, int z) {
+  return table_entry(T, x, y, z).x;
 }
 
 // The leaf holding voxel (x, y, z), or -1: descend(T, x, y, z, 0), through the lookup table
@@ -143,6 +150,15 @@ __device__ __forceinline__ int64_t table_leaf(const DevTree& T, int x, int y, in
 __device__ __forceinline__ int64_t find_leaf(const DevTree& T, int x, int y, int z) {
   const int64_t lf = table_leaf(T, x, y, z);
   return lf != -2 ? lf : descend(T, x, y, z, 0);
+}
+
+// The atlas slot of the leaf holding voxel (x, y, z), or -1 (no leaf, or a leaf without slot):
+// one table load inside the table's box, else the descent and leaf_slot.
+__device__ __forceinline__ int find_slot(const DevTree& T, int x, int y, int z, const int* __restrict__ leaf_slot) {
+  const int2 e = table_entry(T, x, y, z);
+  if (e.x != -2) return e.y;
+  const int64_t lf = descend(T, x, y, z, 0);
+  return lf < 0 ? -1 : leaf_slot[lf];
 }
 
 // |d|^2 with a fixed operation order, so every kernel that reduces distances (gathers,

@@ -724,7 +724,7 @@ __global__ void k_leaf_flags(DevTree T, int64_t count, int by_parent, int* __res
   out[i] = by_parent ? (nd->parent != kUndef64 ? 1 : 0) : (nd->flags ? 1 : 0);
 }
 void leaf_flags(const DevTree& T, int64_t count, int by_parent, int* out, hipStream_t s, const BrickMapOut* clear) {
-  const int64_t nclear = clear ? (int64_t)clear->n[0] * clear->n[1] * clear->n[2] : 0;
+  const int64_t nclear = clear ? 2 * (int64_t)clear->n[0] * clear->n[1] * clear->n[2] : 0;  // (leaf, slot) pairs
   const int64_t items = count > nclear ? count : nclear;
   if (items <= 0) return;
   hipLaunchKernelGGL(k_leaf_flags, dim3(blocks_for(items, 256)), dim3(256), 0, s, T, count, by_parent, out,
@@ -795,7 +795,7 @@ __global__ void k_assign_slots(DevTree T, const int* __restrict__ flags, const i
                    by = (unsigned)((nd->pos[1] >> tab.lg0) - tab.lo[1]),
                    bz = (unsigned)((nd->pos[2] >> tab.lg0) - tab.lo[2]);
     if (bx < (unsigned)tab.n[0] && by < (unsigned)tab.n[1] && bz < (unsigned)tab.n[2])
-      tab.bmap[((int64_t)bz * tab.n[1] + by) * tab.n[0] + bx] = (int)i;
+      reinterpret_cast<int2*>(tab.bmap)[((int64_t)bz * tab.n[1] + by) * tab.n[0] + bx] = make_int2((int)i, rank[i]);
   }
 }
 
@@ -1049,8 +1049,7 @@ __global__ void k_sample(DevTree T, const int* __restrict__ leaf_slot, int res0,
   const float px = q[3 * i], py = q[3 * i + 1], pz = q[3 * i + 2];
   if (!(px >= 0.f && py >= 0.f && pz >= 0.f)) { out[i] = 0.f; return; }
   const int ix = (int)px, iy = (int)py, iz = (int)pz;
-  const int64_t lf = find_leaf(T, ix, iy, iz);
-  const int slot = lf < 0 ? -1 : leaf_slot[lf];
+  const int slot = find_slot(T, ix, iy, iz, leaf_slot);
   if (slot < 0) { out[i] = 0.f; return; }
   const int m = ~(res0 - 1);
   const float xb = px - (float)(ix & m) + (float)apron - 0.5f;

@@ -45,8 +45,7 @@ __device__ __forceinline__ uint32_t splat_key(const DevTree& T, float px, float 
                                               const int* __restrict__ leaf_slot, int lgres, uint32_t sentinel) {
   if (!(px >= 0.f && py >= 0.f && pz >= 0.f)) return sentinel;
   const int ix = (int)px, iy = (int)py, iz = (int)pz;
-  const int64_t lf = find_leaf(T, ix, iy, iz);
-  const int slot = lf < 0 ? -1 : leaf_slot[lf];
+  const int slot = find_slot(T, ix, iy, iz, leaf_slot);
   if (slot < 0) return sentinel;
   const int res = 1 << lgres, m = res - 1;
   const int rx = region_of(ix & m, px - (float)ix, res), ry = region_of(iy & m, py - (float)iy, res),
@@ -231,8 +230,7 @@ __global__ void __launch_bounds__(256) k_splat_backward(DevTree T, const int* __
   const double m = mass ? (double)mass[i] : 1.0;
   if (px >= 0.f && py >= 0.f && pz >= 0.f) {
     const int ix = (int)px, iy = (int)py, iz = (int)pz;
-    const int64_t lf = find_leaf(T, ix, iy, iz);
-    const int slot = lf < 0 ? -1 : leaf_slot[lf];
+    const int slot = find_slot(T, ix, iy, iz, leaf_slot);
     if (slot >= 0) {
       const int res = 1 << lgres, msk = ~(res - 1), br = res + 2 * apron;
       const int64_t bv = (int64_t)br * br * br;
@@ -277,8 +275,7 @@ __global__ void __launch_bounds__(256) k_sample_backward(DevTree T, const int* _
   double gx = 0, gy = 0, gz = 0;
   if (px >= 0.f && py >= 0.f && pz >= 0.f) {
     const int ix = (int)px, iy = (int)py, iz = (int)pz;
-    const int64_t lf = find_leaf(T, ix, iy, iz);
-    const int slot = lf < 0 ? -1 : leaf_slot[lf];
+    const int slot = find_slot(T, ix, iy, iz, leaf_slot);
     if (slot >= 0) {
       const int msk = ~(res0 - 1), br = res0 + 2 * apron;
       const double x = (double)px - (ix & msk) + apron - 0.5, y = (double)py - (iy & msk) + apron - 0.5,
@@ -379,9 +376,7 @@ __global__ void __launch_bounds__(256) k_scatter_density(DevTree T, PointsView P
         q[2] < 1073741824.f))
     return;
   const int iq[3] = {(int)q[0], (int)q[1], (int)q[2]};
-  const int64_t lf = find_leaf(T, iq[0], iq[1], iq[2]);
-  if (lf < 0) return;
-  const int slot = leaf_slot[lf];
+  const int slot = find_slot(T, iq[0], iq[1], iq[2], leaf_slot);
   if (slot < 0) return;
   const int res = 1 << lgres, br = res + 2 * apron;
   float pl[3];
